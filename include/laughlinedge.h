@@ -1,0 +1,156 @@
+/*
+ * laughlinedge.h -- C ABI of the MI355X edge-state Monte-Carlo engine.
+ *
+ * Plain C types only (no HIP or torch types).  Errors are returned as codes; the
+ * message of the last failure on the calling thread is le_last_error().  The engine
+ * owns all device memory; callers own every host buffer they pass in.  One engine
+ * per host thread; calls on one engine are serialised on its own HIP stream.
+ *
+ * Which reference interface each entry point replaces (Nardin94/LaughlinEdge):
+ *   le_engine_create / le_engine_destroy
+ *       the per-call setup of edgeMC::spectrum_compute: RNG states, partitions,
+ *       capacity checks and result buffers (modules/edge_mc/em_spectrum.cu:210-252),
+ *       with the sys_params constants (modules/sys_params.h:4-25) passed at run time
+ *       and monteCarloParameters (modules/edge_montecarlo.h:45-88) as le_mc_params.
+ *   le_basis_state
+ *       iparts::integer_partitions host partitions (modules/integer_partitions.h:29-70).
+ *   le_tune_step
+ *       edgeMC::optimal_step / mcmc_acceptanceBisection (modules/edge_mc/em_bisection.cu:7-153).
+ *   le_burnin, le_sample, le_run
+ *       the mcmc_hamiltonian launch and its host averaging
+ *       (modules/edge_mc/em_spectrum.cu:139-208, 256-359), for all sectors at once.
+ *   le_get_moments
+ *       the D2H copy + block means of fHg/fg/gg (modules/edge_mc/em_spectrum.cu:297-335).
+ *   le_geneig, le_engine_spectrum
+ *       ges::generalizedEigenproblem::extract_spectrum
+ *       (modules/eigensolver/generalized_eigenproblem.cpp:45-52), on the GPU in FP64.
+ *   le_accumulate_positions, le_get_positions, le_set_positions
+ *       no reference counterpart: fixed-sample accumulation and walker state I/O, so the
+ *       same configurations can be fed to the FP64 CPU oracle.
+ */
+#ifndef LAUGHLINEDGE_H
+#define LAUGHLINEDGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LE_OK 0
+#define LE_ERR_ARG 1        /* invalid argument */
+#define LE_ERR_CAPACITY 2   /* basis beyond the engine's capacity (reference: exit(1), em_spectrum.cu:235-252) */
+#define LE_ERR_HIP 3        /* HIP runtime error */
+#define LE_ERR_NOT_PD 4     /* metric not positive definite in the generalized eigensolve */
+#define LE_ERR_BISECTION 5  /* step tuning failed to bracket (reference: exit(1), em_bisection.cu:146-149) */
+#define LE_ERR_STATE 6      /* call out of order */
+#define LE_ERR_NOCONV 7     /* eigensolver did not converge */
+
+#define LE_PREC_FP32 0      /* FP32 sampler, FP32 MFMA tiles flushed to FP64 */
+#define LE_PREC_FP64 1      /* FP64 sampler, FP64 MFMA accumulation (the plain FP64 form) */
+
+/* sys_params.h, at run time.  V(r) = 0.5 w r^2 + eps r^4 + strength (r / RCl)^exponent,
+ * RCl = sqrt(2 m N) (system_parameters.cpp:25-32 plus the w, eps form). */
+typedef struct le_system_params {
+    int particles_number;      /* N  (sys_params::particlesNumber) */
+    int inverse_filling;       /* m  (sys_params::inverseFilling) */
+    double conf_w;
+    double conf_eps;
+    double conf_strength;      /* sys_params::confinementStrength */
+    int conf_exponent;         /* sys_params::confinementExponent */
+} le_system_params;
+
+/* edgeMC::monteCarloParameters, at run time (em_parameters.cu:9-31 defaults). */
+typedef struct le_mc_params {
+    int walkers;               /* threadsPerBlock * nBlocks in the reference */
+    int burnin_sweeps;         /* burnInMoves */
+    int samples;               /* MCSamples: kept samples per walker */
+    int thin;                  /* sweeps per kept sample (reference: 1) */
+    double step;               /* randomStep */
+    int tune_sweeps;           /* targetAcceptanceMoves */
+    double target_acceptance;  /* targetAcceptance */
+    double acceptance_tol;     /* acceptance_tol */
+    uint64_t seed;
+    int precision;             /* LE_PREC_* */
+    int lanes_per_walker;      /* 0: autotune; else 2,4,8,16,32,64 */
+    int samples_per_launch;    /* kept samples per sweep+feature+Gram round (0: auto) */
+    int n_blocks;              /* equal time blocks for error bars (>= 1) */
+    int trace_walkers;         /* walkers whose p_1..p_4, V series are recorded */
+} le_mc_params;
+
+typedef struct le_engine le_engine;
+
+typedef struct le_engine_info {
+    int D;                     /* basis size over all sectors */
+    int K;                     /* largest power sum degree */
+    int n_sectors;
+    int longest_partition;     /* longest compressed partition */
+    long block_elems;          /* sum_s d_s^2 */
+    int lanes_per_walker;      /* mapping in use */
+    int samples_per_launch;
+    int n_tiles;               /* 64x64 Gram tiles per weight */
+    long x_ld;                 /* real columns per sample row */
+} le_engine_info;
+
+typedef struct le_run_stats {
+    double step;               /* proposal width in use */
+    double acceptance;         /* accepted / proposed over the sampling sweeps */
+    long long sweeps;          /* sweeps per walker so far (burn-in included) */
+    long long samples;         /* kept samples accumulated (all walkers) */
+    double ms_sweep;           /* device time, sampling sweeps */
+    double ms_features;        /* device time, features */
+    double ms_gram;            /* device time, Gram accumulation */
+    double ms_sample_total;    /* wall time of le_sample calls */
+    long long gram_launches;
+    long long sweep_launches;
+} le_run_stats;
+
+void le_mc_params_default(le_mc_params* p);
+const char* le_last_error(void);
+
+int le_engine_create(const le_system_params* sys, const le_mc_params* mc, const int* sectors,
+                     int n_sectors, le_engine** out);
+int le_engine_destroy(le_engine* e);
+int le_engine_get_info(const le_engine* e, le_engine_info* info);
+/* sector dims (n_sectors) and basis offsets (n_sectors + 1) */
+int le_sector_dims(const le_engine* e, int* dims, int* offsets);
+/* state -> dL and compressed (part, multiplicity) pairs; pairs holds up to 2*8 ints */
+int le_basis_state(const le_engine* e, int state, int* dL, int* pairs, int* n_pairs);
+
+int le_init_walkers(le_engine* e);
+int le_tune_step(le_engine* e, double* step_out);
+int le_burnin(le_engine* e, int sweeps);
+/* sample `samples` kept samples per walker into the current block; closes the block */
+int le_sample(le_engine* e, int samples);
+/* init + tune (if tune_sweeps > 0) + burn-in + n_blocks x (samples / n_blocks) */
+int le_run(le_engine* e);
+int le_get_stats(const le_engine* e, le_run_stats* st);
+int le_n_blocks_done(const le_engine* e);
+/* Mean over closed blocks (block = -1) or one block's means.  S, H: sum_s d_s^2 complex
+ * values, interleaved re/im, per sector d x d row-major, S_ab = <conj(p_a) p_b>, power
+ * sums of z / sqrt(2 m N); H carries sum_i V(r_i). */
+int le_get_moments(const le_engine* e, int block, double* S, double* H);
+/* acceptance trace: per kept sample, per traced walker: Re/Im p_1..p_4, V (float) */
+int le_get_trace(const le_engine* e, float* out, long capacity, long* n_rows);
+
+int le_get_positions(le_engine* e, double* out);        /* walkers x N x 2 */
+int le_set_positions(le_engine* e, const double* in);
+/* fixed samples: n configurations (n x N x 2, FP64) accumulated into a fresh block */
+int le_accumulate_positions(le_engine* e, const double* pos, long n);
+int le_reset_moments(le_engine* e);
+
+/* FP64 generalized eigensolve on the GPU: d x d complex (interleaved) S, H.
+ * evals ascending; evecs (optional, d x d complex, column j = eigenvector j, S-normalised). */
+int le_geneig(int d, const double* S, const double* H, double* evals, double* evecs);
+/* per-sector eigenvalues of the engine's moments (block -1 = mean); evals has d_s entries */
+int le_engine_spectrum(le_engine* e, int sector, int block, double* evals);
+
+/* eigenvalues of every sector in one launch (block -1 = mean): evals has D entries in basis
+ * order; status (optional, n_sectors): 0 ok, 1 metric not positive definite */
+int le_engine_spectra(le_engine* e, int block, double* evals, int* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

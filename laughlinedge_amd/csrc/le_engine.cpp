@@ -1,0 +1,892 @@
+// C-ABI engine: owns device memory, drives the sampler / features / Gram / eigensolver
+// kernels on one HIP stream, and keeps per-block FP64 moments for error bars.
+//
+// Reference drivers this replaces (Nardin94/LaughlinEdge):
+//   edgeMC::spectrum_compute host flow .... modules/edge_mc/em_spectrum.cu:210-422
+//   edgeMC::optimal_step (step bisection) . modules/edge_mc/em_bisection.cu:49-153
+//   host block averaging .................. modules/edge_mc/em_spectrum.cu:318-335
+// The reference runs one chain per basis row f on |p_f Psi_L|^2; this engine samples
+// |Psi_L|^2 once for all rows and sectors and accumulates S = <conj(p_a) p_b> and
+// H = <conj(p_a) V p_b> as weighted Gram matrices of the sample matrix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/laughlinedge.h"
+#include "le_basis.hpp"
+#include "le_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(LE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec) -> G
+
+}  // namespace
+
+struct le_engine {
+    le_system_params sys{};
+    le_mc_params mc{};
+    le::Basis basis;
+    int prec = LE_PREC_FP32;
+    size_t rsz = 4;
+    int G = 8;
+    int spl = 1;                 // kept samples per launch round
+    double R = 1.0;
+    double step = 0.5;
+    hipStream_t stream = nullptr;
+
+    // device buffers
+    void* pos = nullptr;         // walkers x N x (float2|double2)
+    unsigned* acc = nullptr;
+    void* X = nullptr;           // rows x ldx
+    void* Vw = nullptr;
+    uint32_t* fac = nullptr;
+    int* col_re = nullptr;
+    int* col_dim = nullptr;
+    le::Tile* tiles = nullptr;
+    int ntiles = 0;
+    long ldx = 0;
+    double* CS = nullptr;
+    double* CH = nullptr;
+    int* sec_off_d = nullptr;
+    long* blk_off_d = nullptr;
+    double* blkS_d = nullptr;
+    double* blkH_d = nullptr;
+    float* trace_d = nullptr;
+    long trace_cap_rows = 0;
+    long trace_rows = 0;
+    double* eig_work = nullptr;
+    size_t eig_work_elems = 0;
+
+    // host state
+    std::vector<std::vector<double>> blkS, blkH;
+    std::vector<long long> blk_samples;
+    bool walkers_ready = false;
+    unsigned long long sweep_counter = 0;
+    long long sweeps_done = 0;
+    le_run_stats st{};
+    unsigned long long acc_base = 0;   // accepted moves before the sampling sweeps
+    long long acc_moves = 0;           // proposed moves during sampling sweeps
+    unsigned long long acc_sampling = 0;
+    std::vector<hipEvent_t> ev;
+
+    ~le_engine();
+    int setup();
+    int autotune_lanes();
+    int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions);
+    int close_block(long long samples);
+    int gram_rows(long rows);
+    double measure_acceptance(double trial_step, int attempt, int* rc);
+};
+
+le_engine::~le_engine()
+{
+    (void)hipFree(pos);
+    (void)hipFree(acc);
+    (void)hipFree(X);
+    (void)hipFree(Vw);
+    (void)hipFree(fac);
+    (void)hipFree(col_re);
+    (void)hipFree(col_dim);
+    (void)hipFree(tiles);
+    (void)hipFree(CS);
+    (void)hipFree(CH);
+    (void)hipFree(sec_off_d);
+    (void)hipFree(blk_off_d);
+    (void)hipFree(blkS_d);
+    (void)hipFree(blkH_d);
+    (void)hipFree(trace_d);
+    (void)hipFree(eig_work);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" void le_mc_params_default(le_mc_params* p)
+{
+    // em_parameters.cu:9-31 defaults, walkers = 32 x 128
+    std::memset(p, 0, sizeof(*p));
+    p->walkers = 4096;
+    p->burnin_sweeps = 1000;
+    p->samples = 50000;
+    p->thin = 1;
+    p->step = 0.8;
+    p->tune_sweeps = 100;
+    p->target_acceptance = 0.5;
+    p->acceptance_tol = 0.01;
+    p->seed = 1234;
+    p->precision = LE_PREC_FP32;
+    p->lanes_per_walker = 0;
+    p->samples_per_launch = 0;
+    p->n_blocks = 1;
+    p->trace_walkers = 0;
+}
+
+extern "C" const char* le_last_error(void) { return g_err.c_str(); }
+
+int le_engine::setup()
+{
+    const int N = sys.particles_number;
+    const int D = basis.D;
+    R = std::sqrt(2.0 * sys.inverse_filling * N);
+    step = mc.step;
+    rsz = (prec == LE_PREC_FP64) ? sizeof(double) : sizeof(float);
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+    // real-embedded columns: sector s occupies [2 o_s, 2 o_s + 2 d_s): Re then Im
+    ldx = std::max<long>(64, ((2L * D + 63) / 64) * 64);
+    std::vector<int> cre(D), cdim(D);
+    std::set<std::pair<int, int>> tl;
+    for (int s = 0; s < basis.nsec(); ++s) {
+        const int o = basis.sector_offset[s], d = basis.dim(s);
+        for (int t = 0; t < d; ++t) {
+            cre[o + t] = 2 * o + t;
+            cdim[o + t] = d;
+        }
+        if (d == 0) continue;
+        const int t0 = (2 * o) / 64, t1 = (2 * o + 2 * d - 1) / 64;
+        for (int a = t0; a <= t1; ++a)
+            for (int b = a; b <= t1; ++b) tl.insert({a, b});
+    }
+    std::vector<le::Tile> th;
+    for (auto& p : tl) th.push_back({p.first * 64, p.second * 64});
+    ntiles = (int)th.size();
+
+    // kept samples per launch round: ~128k rows, X under 1 GiB
+    const long row_bytes = ldx * (long)rsz;
+    long max_rows = std::max<long>(mc.walkers, (1L << 30) / row_bytes);
+    if (mc.samples_per_launch > 0) {
+        spl = mc.samples_per_launch;
+    } else {
+        long want = (131072 + mc.walkers - 1) / mc.walkers;
+        spl = (int)std::max<long>(1, std::min<long>(want, max_rows / mc.walkers));
+    }
+    const long rows = (long)spl * mc.walkers;
+
+    HIPCHK(hipMalloc(&pos, (size_t)mc.walkers * N * 2 * rsz));
+    HIPCHK(dalloc(&acc, (size_t)mc.walkers));
+    HIPCHK(hipMalloc(&X, (size_t)rows * ldx * rsz));
+    HIPCHK(hipMemsetAsync(X, 0, (size_t)rows * ldx * rsz, stream));
+    HIPCHK(hipMalloc(&Vw, (size_t)rows * rsz));
+    auto ft = basis.factor_table();
+    HIPCHK(dalloc(&fac, std::max<size_t>(1, ft.size())));
+    if (!ft.empty()) HIPCHK(hipMemcpy(fac, ft.data(), ft.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&col_re, (size_t)std::max(1, D)));
+    HIPCHK(dalloc(&col_dim, (size_t)std::max(1, D)));
+    if (D) {
+        HIPCHK(hipMemcpy(col_re, cre.data(), D * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(col_dim, cdim.data(), D * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(dalloc(&tiles, (size_t)std::max(1, ntiles)));
+    if (ntiles) HIPCHK(hipMemcpy(tiles, th.data(), ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&CS, (size_t)ldx * ldx));
+    HIPCHK(dalloc(&CH, (size_t)ldx * ldx));
+    HIPCHK(le::launch_zero(CS, ldx * ldx, stream));
+    HIPCHK(le::launch_zero(CH, ldx * ldx, stream));
+    std::vector<long> bo(basis.nsec() + 1);
+    for (int s = 0; s <= basis.nsec(); ++s) bo[s] = basis.block_offset(s);
+    HIPCHK(dalloc(&sec_off_d, basis.sector_offset.size()));
+    HIPCHK(hipMemcpy(sec_off_d, basis.sector_offset.data(), basis.sector_offset.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&blk_off_d, bo.size()));
+    HIPCHK(hipMemcpy(blk_off_d, bo.data(), bo.size() * sizeof(long), hipMemcpyHostToDevice));
+    const long be = basis.block_elems();
+    HIPCHK(dalloc(&blkS_d, (size_t)std::max<long>(2, 2 * be)));
+    HIPCHK(dalloc(&blkH_d, (size_t)std::max<long>(2, 2 * be)));
+    if (mc.trace_walkers > 0) {
+        const int tw = std::min(mc.trace_walkers, mc.walkers);
+        trace_cap_rows = (long)std::max(1, mc.samples) * tw;
+        HIPCHK(dalloc(&trace_d, (size_t)trace_cap_rows * 9));
+    }
+    ev.resize(2 * spl + 4);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipStreamSynchronize(stream));
+    st.step = step;
+    return LE_OK;
+}
+
+extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params* mc, const int* sectors,
+                                int n_sectors, le_engine** out)
+{
+    if (!sys || !mc || !out || (n_sectors > 0 && !sectors) || n_sectors <= 0)
+        return fail(LE_ERR_ARG, "le_engine_create: null argument or no sectors");
+    *out = nullptr;
+    if (sys->particles_number < 2 || sys->inverse_filling < 1)
+        return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
+    if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
+        return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
+    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64)
+        return fail(LE_ERR_ARG, "unknown precision");
+    if (mc->lanes_per_walker != 0 && !le::sweep_supported(mc->lanes_per_walker))
+        return fail(LE_ERR_ARG, "lanes_per_walker must be 0, 2, 4, 8, 16, 32 or 64");
+    std::vector<int> dls(sectors, sectors + n_sectors);
+    for (int dl : dls)
+        if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
+    auto e = new le_engine();
+    e->sys = *sys;
+    e->mc = *mc;
+    e->prec = mc->precision;
+    e->basis.build(sys->particles_number, dls);
+    // capacity checks (reference: em_spectrum.cu:235-252 exits); here they are codes
+    if (e->basis.K > 24) {
+        delete e;
+        return fail(LE_ERR_CAPACITY, "largest power-sum degree " + std::to_string(e->basis.K) + " exceeds 24");
+    }
+    if (e->basis.longest > le::kMaxFactors) {
+        delete e;
+        return fail(LE_ERR_CAPACITY, "compressed partition longer than 8");
+    }
+    if (e->basis.D > 4096) {
+        delete e;
+        return fail(LE_ERR_CAPACITY, "basis larger than 4096 states");
+    }
+    for (int s = 0; s < e->basis.nsec(); ++s)
+        if (e->basis.dim(s) > 640) {
+            delete e;
+            return fail(LE_ERR_CAPACITY, "sector larger than 640 states");
+        }
+    int rc = e->setup();
+    if (rc != LE_OK) {
+        std::string msg = g_err;
+        delete e;
+        g_err = msg;
+        return rc;
+    }
+    if (mc->lanes_per_walker) e->G = mc->lanes_per_walker;
+    *out = e;
+    return LE_OK;
+}
+
+extern "C" int le_engine_destroy(le_engine* e)
+{
+    delete e;
+    return LE_OK;
+}
+
+extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
+{
+    if (!e || !info) return fail(LE_ERR_ARG, "null argument");
+    info->D = e->basis.D;
+    info->K = e->basis.K;
+    info->n_sectors = e->basis.nsec();
+    info->longest_partition = e->basis.longest;
+    info->block_elems = e->basis.block_elems();
+    info->lanes_per_walker = e->G;
+    info->samples_per_launch = e->spl;
+    info->n_tiles = e->ntiles;
+    info->x_ld = e->ldx;
+    return LE_OK;
+}
+
+extern "C" int le_sector_dims(const le_engine* e, int* dims, int* offsets)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    for (int s = 0; s < e->basis.nsec(); ++s) {
+        if (dims) dims[s] = e->basis.dim(s);
+    }
+    if (offsets)
+        for (int s = 0; s <= e->basis.nsec(); ++s) offsets[s] = e->basis.sector_offset[s];
+    return LE_OK;
+}
+
+extern "C" int le_basis_state(const le_engine* e, int state, int* dL, int* pairs, int* n_pairs)
+{
+    if (!e || state < 0 || state >= e->basis.D) return fail(LE_ERR_ARG, "state out of range");
+    const auto& s = e->basis.states[state];
+    if (dL) *dL = s.dL;
+    if (n_pairs) *n_pairs = (int)s.pairs.size();
+    if (pairs)
+        for (size_t q = 0; q < s.pairs.size() && q < 8; ++q) {
+            pairs[2 * q] = s.pairs[q].number;
+            pairs[2 * q + 1] = s.pairs[q].repetitions;
+        }
+    return LE_OK;
+}
+
+int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* positions)
+{
+    le::SweepArgs a{};
+    a.pos = positions;
+    a.accepted = acc;
+    a.seed = mc.seed;
+    a.sweep0 = sweep0;
+    a.N = sys.particles_number;
+    a.m = sys.inverse_filling;
+    a.walkers = mc.walkers;
+    a.nsweeps = nsweeps;
+    a.step = step;
+    HIPCHK(le::launch_sweep(a, prec, G, stream));
+    return LE_OK;
+}
+
+// Time two sweeps per candidate mapping on a scratch copy of the walkers.
+int le_engine::autotune_lanes()
+{
+    const int N = sys.particles_number;
+    auto key = std::make_tuple(N, mc.walkers, prec);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned_lanes.find(key);
+        if (it != g_tuned_lanes.end()) {
+            G = it->second;
+            return LE_OK;
+        }
+    }
+    void* scratch = nullptr;
+    const size_t bytes = (size_t)mc.walkers * N * 2 * rsz;
+    HIPCHK(hipMalloc(&scratch, bytes));
+    unsigned* acc_save = acc;
+    unsigned* acc_tmp = nullptr;
+    HIPCHK(dalloc(&acc_tmp, (size_t)mc.walkers));
+    acc = acc_tmp;
+    float best = 1e30f;
+    int bestG = 8;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int cand : {4, 8, 16, 32, 64}) {
+        if ((N + cand - 1) / cand < 2 && cand > 4) continue;
+        if (le::sweep_lds_bytes(prec, cand, N) > 160 * 1024) continue;
+        G = cand;
+        HIPCHK(hipMemcpyAsync(scratch, pos, bytes, hipMemcpyDeviceToDevice, stream));
+        int rc = run_sweeps(1, 1ull << 46, scratch);   // warm-up, separate RNG range
+        if (rc) break;
+        HIPCHK(hipEventRecord(e0, stream));
+        rc = run_sweeps(2, (1ull << 46) + 8, scratch);
+        if (rc) break;
+        HIPCHK(hipEventRecord(e1, stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestG = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    (void)hipFree(acc_tmp);
+    acc = acc_save;
+    G = bestG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned_lanes[key] = G;
+    return LE_OK;
+}
+
+extern "C" int le_init_walkers(le_engine* e)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    HIPCHK(le::launch_init_positions(e->pos, e->prec, e->sys.particles_number, e->mc.walkers, e->R, e->mc.seed,
+                                     e->stream));
+    HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->walkers_ready = true;
+    e->sweep_counter = 0;
+    e->sweeps_done = 0;
+    if (e->mc.lanes_per_walker == 0) {
+        int rc = e->autotune_lanes();
+        if (rc) return rc;
+    }
+    return LE_OK;
+}
+
+double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
+{
+    // fresh positions and tune_sweeps sweeps, as mcmc_acceptanceBisection (em_bisection.cu:7-47)
+    *rc = LE_OK;
+    step = trial_step;
+    hipError_t h = le::launch_init_positions(pos, prec, sys.particles_number, mc.walkers, R, mc.seed, stream);
+    if (h == hipSuccess) h = hipMemsetAsync(acc, 0, sizeof(unsigned) * mc.walkers, stream);
+    if (h != hipSuccess) {
+        *rc = fail(LE_ERR_HIP, hipGetErrorString(h));
+        return 0;
+    }
+    *rc = run_sweeps(mc.tune_sweeps, (1ull << 47) + ((unsigned long long)attempt << 20), pos);
+    if (*rc) return 0;
+    std::vector<unsigned> h_acc(mc.walkers);
+    h = hipMemcpyAsync(h_acc.data(), acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream);
+    if (h == hipSuccess) h = hipStreamSynchronize(stream);
+    if (h != hipSuccess) {
+        *rc = fail(LE_ERR_HIP, hipGetErrorString(h));
+        return 0;
+    }
+    double tot = 0;
+    for (unsigned v : h_acc) tot += v;
+    return tot / ((double)mc.walkers * mc.tune_sweeps * sys.particles_number);
+}
+
+// Secant/bracketing search of em_bisection.cu:80-150, restated; the reference loops
+// until |acc - target| < tol and exits the process on a failed bracket.
+extern "C" int le_tune_step(le_engine* e, double* step_out)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (e->mc.tune_sweeps <= 0) return fail(LE_ERR_ARG, "tune_sweeps must be > 0");
+    const double target = e->mc.target_acceptance, tol = e->mc.acceptance_tol;
+    int rc = LE_OK, attempt = 0;
+    auto meas = [&](double s) { return e->measure_acceptance(s, attempt++, &rc); };
+    auto done = [&](double s) {
+        e->step = s;
+        e->st.step = s;
+        if (step_out) *step_out = s;
+        // leave fresh walkers for the burn-in
+        return le_init_walkers(e);
+    };
+    double s2 = e->step, s1;
+    double a2 = meas(s2), a1;
+    if (rc) return rc;
+    if (std::fabs(a2 - target) < tol) return done(s2);
+    s1 = s2;
+    for (int it = 0;; ++it) {
+        s1 *= (a2 > target) ? 2.0 : 0.5;
+        a1 = meas(s1);
+        if (rc) return rc;
+        if ((a2 - target) * (a1 - target) <= 0) break;
+        if (it > 40) return fail(LE_ERR_BISECTION, "step search failed to bracket the target acceptance");
+    }
+    if (std::fabs(a1 - target) < tol) return done(s1);
+    if (s2 < s1) {
+        std::swap(s1, s2);
+        std::swap(a1, a2);
+    }
+    for (int it = 0; it < 60; ++it) {
+        const double g = ((target - a1) * s2 + (a2 - target) * s1) / (a2 - a1);
+        const double ag = meas(g);
+        if (rc) return rc;
+        if (std::fabs(ag - target) < tol) return done(g);
+        if ((ag - target) * (a1 - target) < 0) {
+            a2 = ag;
+            s2 = g;
+        } else if ((ag - target) * (a2 - target) < 0) {
+            a1 = ag;
+            s1 = g;
+        } else {
+            return fail(LE_ERR_BISECTION, "step bisection lost its bracket");
+        }
+    }
+    return fail(LE_ERR_BISECTION, "step bisection did not converge in 60 steps");
+}
+
+extern "C" int le_burnin(le_engine* e, int sweeps)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (!e->walkers_ready) {
+        int rc = le_init_walkers(e);
+        if (rc) return rc;
+    }
+    if (sweeps <= 0) return LE_OK;
+    int rc = e->run_sweeps(sweeps, e->sweep_counter, e->pos);
+    if (rc) return rc;
+    e->sweep_counter += (unsigned long long)sweeps;
+    e->sweeps_done += sweeps;
+    e->st.sweeps = e->sweeps_done;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LE_OK;
+}
+
+int le_engine::gram_rows(long rows)
+{
+    le::GramArgs g{};
+    g.X = X;
+    g.ldx = ldx;
+    g.Vw = Vw;
+    g.rows = rows;
+    const long want_wg = 2048;
+    long splits = std::max<long>(1, (want_wg + ntiles - 1) / std::max(1, ntiles));
+    long rps = (rows + splits - 1) / splits;
+    rps = std::max<long>(rps, 1024);
+    rps = ((rps + 31) / 32) * 32;
+    splits = (rows + rps - 1) / rps;
+    g.rows_per_split = rps;
+    g.tiles = tiles;
+    g.ntiles = ntiles;
+    g.CS = CS;
+    g.CH = CH;
+    g.ldc = ldx;
+    HIPCHK(le::launch_gram(g, prec, (int)splits, stream));
+    return LE_OK;
+}
+
+int le_engine::close_block(long long samples)
+{
+    const long be = basis.block_elems();
+    le::ExtractArgs x{};
+    x.CS = CS;
+    x.CH = CH;
+    x.ldc = ldx;
+    x.nsec = basis.nsec();
+    x.sec_off = sec_off_d;
+    x.blk_off = blk_off_d;
+    x.scale = samples > 0 ? 1.0 / (double)samples : 0.0;
+    x.S = blkS_d;
+    x.H = blkH_d;
+    x.total = be;
+    HIPCHK(le::launch_extract(x, stream));
+    std::vector<double> hs(2 * be), hh(2 * be);
+    HIPCHK(hipMemcpyAsync(hs.data(), blkS_d, sizeof(double) * 2 * be, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hh.data(), blkH_d, sizeof(double) * 2 * be, hipMemcpyDeviceToHost, stream));
+    HIPCHK(le::launch_zero(CS, ldx * ldx, stream));
+    HIPCHK(le::launch_zero(CH, ldx * ldx, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    blkS.push_back(std::move(hs));
+    blkH.push_back(std::move(hh));
+    blk_samples.push_back(samples);
+    return LE_OK;
+}
+
+extern "C" int le_sample(le_engine* e, int samples)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (samples <= 0) return fail(LE_ERR_ARG, "samples must be > 0");
+    if (!e->walkers_ready) {
+        int rc = le_init_walkers(e);
+        if (rc) return rc;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
+    le::FeatureArgs f{};
+    f.pos = e->pos;
+    f.pos_is_double = 0;
+    f.X = e->X;
+    f.ldx = e->ldx;
+    f.Vw = e->Vw;
+    f.N = e->sys.particles_number;
+    f.walkers = e->mc.walkers;
+    f.D = e->basis.D;
+    f.K = e->basis.K;
+    f.fac = e->fac;
+    f.col_re = e->col_re;
+    f.col_dim = e->col_dim;
+    f.invR = 1.0 / e->R;
+    f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
+    const int tw = std::min(e->mc.trace_walkers, e->mc.walkers);
+    long long kept = 0;
+    for (int done = 0; done < samples;) {
+        const int n = std::min(e->spl, samples - done);
+        for (int q = 0; q < n; ++q) {
+            HIPCHK(hipEventRecord(e->ev[2 * q], e->stream));
+            int rc = e->run_sweeps(e->mc.thin, e->sweep_counter, e->pos);
+            if (rc) return rc;
+            e->sweep_counter += (unsigned long long)e->mc.thin;
+            HIPCHK(hipEventRecord(e->ev[2 * q + 1], e->stream));
+            f.row0 = (long)q * e->mc.walkers;
+            if (e->trace_d && e->trace_rows + tw <= e->trace_cap_rows) {
+                f.trace = e->trace_d;
+                f.trace_walkers = tw;
+                f.trace_row0 = e->trace_rows;
+                e->trace_rows += tw;
+            } else {
+                f.trace = nullptr;
+                f.trace_walkers = 0;
+            }
+            HIPCHK(le::launch_features(f, e->prec, e->stream));
+        }
+        HIPCHK(hipEventRecord(e->ev[2 * n], e->stream));
+        int rc = e->gram_rows((long)n * e->mc.walkers);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e->ev[2 * n + 1], e->stream));
+        HIPCHK(hipEventSynchronize(e->ev[2 * n + 1]));
+        for (int q = 0; q < n; ++q) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, e->ev[2 * q], e->ev[2 * q + 1]));
+            HIPCHK(hipEventElapsedTime(&b, e->ev[2 * q + 1], e->ev[2 * q + 2]));
+            e->st.ms_sweep += a;
+            e->st.ms_features += b;
+        }
+        float gm = 0;
+        HIPCHK(hipEventElapsedTime(&gm, e->ev[2 * n], e->ev[2 * n + 1]));
+        e->st.ms_gram += gm;
+        e->st.gram_launches += 1;
+        e->st.sweep_launches += n;
+        done += n;
+        kept += n;
+    }
+    e->sweeps_done += (long long)samples * e->mc.thin;
+    const long long nsamp = kept * e->mc.walkers;
+    int rc = e->close_block(nsamp);
+    if (rc) return rc;
+    std::vector<unsigned> h(e->mc.walkers);
+    HIPCHK(hipMemcpy(h.data(), e->acc, sizeof(unsigned) * e->mc.walkers, hipMemcpyDeviceToHost));
+    double tot = 0;
+    for (unsigned v : h) tot += v;
+    e->acc_sampling += (unsigned long long)tot;
+    e->acc_moves += (long long)samples * e->mc.thin * e->sys.particles_number * e->mc.walkers;
+    e->st.acceptance = (double)e->acc_sampling / (double)std::max<long long>(1, e->acc_moves);
+    e->st.samples += nsamp;
+    e->st.sweeps = e->sweeps_done;
+    e->st.step = e->step;
+    e->st.ms_sample_total +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return LE_OK;
+}
+
+extern "C" int le_run(le_engine* e)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    int rc = le_init_walkers(e);
+    if (rc) return rc;
+    if (e->mc.tune_sweeps > 0) {
+        rc = le_tune_step(e, nullptr);
+        if (rc) return rc;
+    }
+    rc = le_burnin(e, e->mc.burnin_sweeps);
+    if (rc) return rc;
+    const int nb = e->mc.n_blocks;
+    if (e->mc.samples < nb) return fail(LE_ERR_ARG, "samples must be >= n_blocks");
+    const int per = e->mc.samples / nb;
+    for (int b = 0; b < nb; ++b) {
+        rc = le_sample(e, per);
+        if (rc) return rc;
+    }
+    return LE_OK;
+}
+
+extern "C" int le_get_stats(const le_engine* e, le_run_stats* st)
+{
+    if (!e || !st) return fail(LE_ERR_ARG, "null argument");
+    *st = e->st;
+    return LE_OK;
+}
+
+extern "C" int le_n_blocks_done(const le_engine* e) { return e ? (int)e->blkS.size() : 0; }
+
+extern "C" int le_get_moments(const le_engine* e, int block, double* S, double* H)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    const long be2 = 2 * e->basis.block_elems();
+    const int nb = (int)e->blkS.size();
+    if (nb == 0) return fail(LE_ERR_STATE, "no closed sample blocks");
+    if (block >= nb || block < -1) return fail(LE_ERR_ARG, "block out of range");
+    if (block >= 0) {
+        if (S) std::memcpy(S, e->blkS[block].data(), sizeof(double) * be2);
+        if (H) std::memcpy(H, e->blkH[block].data(), sizeof(double) * be2);
+        return LE_OK;
+    }
+    // sample-weighted mean over blocks
+    long long tot = 0;
+    for (auto n : e->blk_samples) tot += n;
+    for (long q = 0; q < be2; ++q) {
+        double s = 0, h = 0;
+        for (int b = 0; b < nb; ++b) {
+            const double w = tot ? (double)e->blk_samples[b] / (double)tot : 1.0 / nb;
+            s += w * e->blkS[b][q];
+            h += w * e->blkH[b][q];
+        }
+        if (S) S[q] = s;
+        if (H) H[q] = h;
+    }
+    return LE_OK;
+}
+
+extern "C" int le_get_trace(const le_engine* e, float* out, long capacity, long* n_rows)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (n_rows) *n_rows = e->trace_rows;
+    if (!out) return LE_OK;
+    const long n = std::min(capacity, e->trace_rows);
+    if (n > 0) HIPCHK(hipMemcpy(out, e->trace_d, sizeof(float) * 9 * n, hipMemcpyDeviceToHost));
+    return LE_OK;
+}
+
+extern "C" int le_get_positions(le_engine* e, double* out)
+{
+    if (!e || !out) return fail(LE_ERR_ARG, "null argument");
+    const size_t n = (size_t)e->mc.walkers * e->sys.particles_number * 2;
+    if (e->prec == LE_PREC_FP64) {
+        HIPCHK(hipMemcpy(out, e->pos, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), e->pos, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; ++q) out[q] = tmp[q];
+    }
+    return LE_OK;
+}
+
+extern "C" int le_set_positions(le_engine* e, const double* in)
+{
+    if (!e || !in) return fail(LE_ERR_ARG, "null argument");
+    const size_t n = (size_t)e->mc.walkers * e->sys.particles_number * 2;
+    if (e->prec == LE_PREC_FP64) {
+        HIPCHK(hipMemcpy(e->pos, in, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t q = 0; q < n; ++q) tmp[q] = (float)in[q];
+        HIPCHK(hipMemcpy(e->pos, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    e->walkers_ready = true;
+    return LE_OK;
+}
+
+extern "C" int le_reset_moments(le_engine* e)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    e->blkS.clear();
+    e->blkH.clear();
+    e->blk_samples.clear();
+    HIPCHK(le::launch_zero(e->CS, e->ldx * e->ldx, e->stream));
+    HIPCHK(le::launch_zero(e->CH, e->ldx * e->ldx, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return LE_OK;
+}
+
+extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
+{
+    if (!e || !pos || n <= 0) return fail(LE_ERR_ARG, "null argument or n <= 0");
+    const int N = e->sys.particles_number;
+    const long chunk = (long)e->spl * e->mc.walkers;
+    double* stage = nullptr;
+    HIPCHK(dalloc(&stage, (size_t)std::min(chunk, n) * N * 2));
+    le::FeatureArgs f{};
+    f.pos = stage;
+    f.pos_is_double = 1;
+    f.X = e->X;
+    f.ldx = e->ldx;
+    f.Vw = e->Vw;
+    f.row0 = 0;
+    f.N = N;
+    f.D = e->basis.D;
+    f.K = e->basis.K;
+    f.fac = e->fac;
+    f.col_re = e->col_re;
+    f.col_dim = e->col_dim;
+    f.invR = 1.0 / e->R;
+    f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
+    int rc = LE_OK;
+    for (long s0 = 0; s0 < n && rc == LE_OK; s0 += chunk) {
+        const long m = std::min(chunk, n - s0);
+        hipError_t h = hipMemcpyAsync(stage, pos + (size_t)s0 * N * 2, sizeof(double) * m * N * 2,
+                                      hipMemcpyHostToDevice, e->stream);
+        if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
+        f.walkers = (int)m;
+        h = le::launch_features(f, e->prec, e->stream);
+        if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
+        rc = e->gram_rows(m);
+        if (rc == LE_OK) {
+            h = hipStreamSynchronize(e->stream);
+            if (h != hipSuccess) rc = fail(LE_ERR_HIP, hipGetErrorString(h));
+        }
+    }
+    (void)hipFree(stage);
+    if (rc) return rc;
+    return e->close_block(n);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 generalized eigensolves on the GPU
+// ---------------------------------------------------------------------------
+static int geneig_batch(const std::vector<int>& dims, const double* S, const double* H, double* evals,
+                        double* evecs, std::vector<int>* status_out)
+{
+    const int np = (int)dims.size();
+    std::vector<le::GenEigProblem> probs(np);
+    long in_off = 0, ev_off = 0, w_off = 0;
+    for (int p = 0; p < np; ++p) {
+        probs[p] = {dims[p], in_off, ev_off, w_off};
+        in_off += (long)dims[p] * dims[p];
+        ev_off += dims[p];
+        w_off += 4L * dims[p] * dims[p];
+    }
+    double *dS = nullptr, *dH = nullptr, *dW = nullptr, *dE = nullptr, *dV = nullptr;
+    le::GenEigProblem* dP = nullptr;
+    int* dSt = nullptr;
+    hipError_t h = hipSuccess;
+    auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
+    chk(dalloc(&dS, (size_t)std::max<long>(2, 2 * in_off)));
+    chk(dalloc(&dH, (size_t)std::max<long>(2, 2 * in_off)));
+    chk(dalloc(&dW, (size_t)std::max<long>(2, 2 * w_off)));
+    chk(dalloc(&dE, (size_t)std::max<long>(1, ev_off)));
+    if (evecs) chk(dalloc(&dV, (size_t)std::max<long>(2, 2 * in_off)));
+    chk(dalloc(&dP, (size_t)std::max(1, np)));
+    chk(dalloc(&dSt, (size_t)std::max(1, np)));
+    if (h == hipSuccess) {
+        chk(hipMemcpy(dS, S, sizeof(double) * 2 * in_off, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dH, H, sizeof(double) * 2 * in_off, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dP, probs.data(), sizeof(le::GenEigProblem) * np, hipMemcpyHostToDevice));
+        chk(hipMemset(dSt, 0xff, sizeof(int) * np));
+    }
+    if (h == hipSuccess) chk(le::launch_geneig(dS, dH, dW, dP, np, dE, dV, dSt, nullptr));
+    std::vector<int> st(np, -1);
+    if (h == hipSuccess) chk(hipDeviceSynchronize());
+    if (h == hipSuccess) {
+        chk(hipMemcpy(evals, dE, sizeof(double) * ev_off, hipMemcpyDeviceToHost));
+        if (evecs) chk(hipMemcpy(evecs, dV, sizeof(double) * 2 * in_off, hipMemcpyDeviceToHost));
+        chk(hipMemcpy(st.data(), dSt, sizeof(int) * np, hipMemcpyDeviceToHost));
+    }
+    (void)hipFree(dS);
+    (void)hipFree(dH);
+    (void)hipFree(dW);
+    (void)hipFree(dE);
+    (void)hipFree(dV);
+    (void)hipFree(dP);
+    (void)hipFree(dSt);
+    if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
+    if (status_out) *status_out = st;
+    for (int p = 0; p < np; ++p)
+        if (st[p] == 1) return fail(LE_ERR_NOT_PD, "metric is not positive definite (problem " + std::to_string(p) + ")");
+    return LE_OK;
+}
+
+extern "C" int le_geneig(int d, const double* S, const double* H, double* evals, double* evecs)
+{
+    if (d <= 0 || d > 640 || !S || !H || !evals) return fail(LE_ERR_ARG, "le_geneig: bad argument");
+    return geneig_batch({d}, S, H, evals, evecs, nullptr);
+}
+
+extern "C" int le_engine_spectrum(le_engine* e, int sector, int block, double* evals)
+{
+    if (!e || !evals || sector < 0 || sector >= e->basis.nsec()) return fail(LE_ERR_ARG, "bad argument");
+    const long be2 = 2 * e->basis.block_elems();
+    std::vector<double> S(be2), H(be2);
+    int rc = le_get_moments(e, block, S.data(), H.data());
+    if (rc) return rc;
+    const long off = 2 * e->basis.block_offset(sector);
+    return geneig_batch({e->basis.dim(sector)}, S.data() + off, H.data() + off, evals, nullptr, nullptr);
+}
+
+extern "C" int le_engine_spectra(le_engine* e, int block, double* evals, int* status)
+{
+    if (!e || !evals) return fail(LE_ERR_ARG, "bad argument");
+    const long be2 = 2 * e->basis.block_elems();
+    std::vector<double> S(be2), H(be2);
+    int rc = le_get_moments(e, block, S.data(), H.data());
+    if (rc) return rc;
+    std::vector<int> dims;
+    for (int s = 0; s < e->basis.nsec(); ++s) dims.push_back(e->basis.dim(s));
+    std::vector<int> st;
+    rc = geneig_batch(dims, S.data(), H.data(), evals, nullptr, &st);
+    if (status)
+        for (size_t q = 0; q < st.size(); ++q) status[q] = st[q];
+    if (rc == LE_ERR_NOT_PD && status) return LE_OK;   // per-sector status reported
+    return rc;
+}

@@ -1,0 +1,818 @@
+// Edge-state Monte-Carlo kernels for gfx950 (MI355X).
+//
+//  k_init_positions  uniform placement in the disc of radius R (Philox tag 1 stream)
+//  k_sweep<G, Real>  Metropolis sweeps on |Psi_L|^2; G lanes share one walker whose
+//                    positions live in LDS; the O(N) log-ratio of a single-particle move
+//                    is split over the G lanes and combined with a butterfly reduction.
+//  k_features<Real>  one wavefront per sample: FP64 power sums p_k of z/R, partition
+//                    monomials p_lambda, confinement V; written as a real-embedded row
+//                    [Re p(sector) | Im p(sector)] of the sample matrix X.
+//  k_gram<Real>      weighted Gram tiles C = X^T X and X^T diag(V) X on MFMA
+//                    (16x16x4 f32 / f64), upper block-triangle tiles only, FP32 tiles
+//                    flushed into FP64 registers every 256 samples, FP64 atomics out.
+//  k_extract         FP64 real-embedded tiles -> complex Hermitian S, H per sector.
+//  k_geneig          FP64 Cholesky + parallel cyclic Jacobi generalized eigensolver,
+//                    one workgroup per sector.
+//
+// Reference behaviour these kernels reproduce (Nardin94/LaughlinEdge):
+//  Metropolis move and ratio ........ modules/edge_mc/em_markovchain.cu:282-328
+//  power sums / monomials ........... modules/edge_mc/em_markovchain.cu:120-183
+//  confinement ...................... modules/edge_mc/em_markovchain.cu:92-99
+//  generalized eigenproblem ......... modules/eigensolver/generalized_eigenproblem.cpp:45-52
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "le_kernels.hpp"
+
+namespace le {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon et al., SC'11).  Stream layout:
+//   key = {seed_lo, seed_hi}; ctr = {particle, sweep_lo, sweep_hi16 | tag << 16, walker}
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+template <typename Real> struct V2T;
+template <> struct V2T<float> { using T = float2; };
+template <> struct V2T<double> { using T = double2; };
+
+// uniform in (0, 1]: (x + 1) 2^-32
+__device__ __forceinline__ double u01d(uint32_t x) { return ((double)x + 1.0) * 2.3283064365386963e-10; }
+__device__ __forceinline__ float u01f(uint32_t x) { return (float)(((double)x + 1.0) * 2.3283064365386963e-10); }
+
+__device__ __forceinline__ void gauss2(uint32_t a, uint32_t b, float& n1, float& n2)
+{
+    // sqrt(-2 ln u) with ln u = log2(u) ln 2; angle in revolutions for v_sin/v_cos
+    float r = __fsqrt_rn(-1.3862943611198906f * __builtin_amdgcn_logf(u01f(a)));
+    float t = u01f(b);
+    n1 = r * __builtin_amdgcn_cosf(t);
+    n2 = r * __builtin_amdgcn_sinf(t);
+}
+
+__device__ __forceinline__ void gauss2(uint32_t a, uint32_t b, double& n1, double& n2)
+{
+    double r = sqrt(-2.0 * log(u01d(a)));
+    double s, c;
+    sincos(6.283185307179586 * u01d(b), &s, &c);
+    n1 = r * c;
+    n2 = r * s;
+}
+
+// ---------------------------------------------------------------------------
+// initial positions
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ void k_init_positions(typename V2T<Real>::T* pos, int N, int walkers, double R,
+                                 unsigned long long seed)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)N * walkers) return;
+    int w = (int)(g / N), i = (int)(g % N);
+    uint32_t r[4];
+    philox((uint32_t)i, 0u, 1u << 16, (uint32_t)w, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    double rad = R * sqrt(u01d(r[0]));
+    double s, c;
+    sincos(6.283185307179586 * u01d(r[1]), &s, &c);
+    pos[g].x = (Real)(rad * c);
+    pos[g].y = (Real)(rad * s);
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis sweeps.  Each group of G lanes owns one walker; lane l handles particles
+// j = l + G s (s < S = ceil(N / G)).  While particle i is moved its LDS slot holds the
+// midpoint of old and proposed positions, so the i-i pair contributes equal factors
+// to numerator and denominator; pad slots j >= N hold the same midpoint.  Products of
+// eight squared distances are taken to log2 at a time (FP32 range), FP64 takes one log.
+// ---------------------------------------------------------------------------
+template <typename Real> struct LogOps;
+template <> struct LogOps<float> {
+    static constexpr int chunk = 8;
+    __device__ static float lg(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-30f)); }
+    static constexpr float to_ln = 0.69314718055994531f;
+};
+template <> struct LogOps<double> {
+    static constexpr int chunk = 32;
+    __device__ static double lg(double x) { return log(fmax(x, 1e-300)); }
+    static constexpr double to_ln = 1.0;
+};
+
+template <int G, typename Real>
+__global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V2* lds = reinterpret_cast<V2*>(smem_raw);
+    constexpr int WPB = 64 / G;
+    const int N = a.N;
+    const int S = (N + G - 1) / G;
+    const int NP = S * G;
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    const bool live = walker < a.walkers;
+    V2* P = lds + grp * NP;
+    V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)(live ? walker : 0) * N;
+    if (live)
+        for (int j = lane; j < N; j += G) P[j] = gpos[j];
+    __syncthreads();
+    if (!live) return;
+
+    const Real step = (Real)a.step;
+    const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const bool has_pad = (lane + G * (S - 1)) >= N;
+    const int pad_j = lane + G * (S - 1);
+    unsigned accepted = 0;
+
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        for (int base = 0; base < N; base += G) {
+            uint32_t r[4];
+            philox((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
+            Real g1, g2;
+            gauss2(r[0], r[1], g1, g2);
+            uint32_t ubits = r[2];
+            const int cnt = min(G, N - base);
+            for (int t = 0; t < cnt; ++t) {
+                const int i = base + t;
+                const Real d1 = __shfl(g1, t, G);
+                const Real d2 = __shfl(g2, t, G);
+                const uint32_t ub = __shfl(ubits, t, G);
+                const V2 zo = P[i];
+                const Real xn = zo.x + step * d1, yn = zo.y + step * d2;
+                V2 mid;
+                mid.x = (Real)0.5 * (zo.x + xn);
+                mid.y = (Real)0.5 * (zo.y + yn);
+                P[i] = mid;
+                if (has_pad) P[pad_j] = mid;
+                Real lsum = 0;
+                Real pn = 1, pd = 1;
+                int s = 0;
+                for (; s + LogOps<Real>::chunk <= S; s += LogOps<Real>::chunk) {
+#pragma unroll
+                    for (int u = 0; u < LogOps<Real>::chunk; ++u) {
+                        const V2 q = P[lane + G * (s + u)];
+                        const Real ax = xn - q.x, ay = yn - q.y;
+                        const Real bx = zo.x - q.x, by = zo.y - q.y;
+                        pn *= ax * ax + ay * ay;
+                        pd *= bx * bx + by * by;
+                    }
+                    lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                    pn = 1;
+                    pd = 1;
+                }
+                for (; s < S; ++s) {
+                    const V2 q = P[lane + G * s];
+                    const Real ax = xn - q.x, ay = yn - q.y;
+                    const Real bx = zo.x - q.x, by = zo.y - q.y;
+                    pn *= ax * ax + ay * ay;
+                    pd *= bx * bx + by * by;
+                }
+                lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+#pragma unroll
+                for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
+                const Real logA = mfac * lsum -
+                                  (Real)0.5 * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y));
+                bool acc;
+                if constexpr (sizeof(Real) == 4) {
+                    acc = (logA >= 0.f) || (__builtin_amdgcn_logf(u01f(ub)) * 0.69314718055994531f < logA);
+                } else {
+                    acc = (logA >= 0.0) || (log(u01d(ub)) < logA);
+                }
+                V2 nz;
+                nz.x = acc ? xn : zo.x;
+                nz.y = acc ? yn : zo.y;
+                P[i] = nz;
+                accepted += acc ? 1u : 0u;
+            }
+        }
+    }
+    for (int j = lane; j < N; j += G) gpos[j] = P[j];
+    if (lane == 0) a.accepted[walker] += accepted;
+}
+
+// ---------------------------------------------------------------------------
+// Features: one wavefront per sample.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double potential_r2(const Potential& p, double r2)
+{
+    double v = 0.5 * p.w * r2 + p.eps * r2 * r2;
+    if (p.strength != 0.0) v += p.strength * pow(sqrt(r2) / p.R, (double)p.expo);
+    return v;
+}
+
+template <typename Real, int KMAX>
+__global__ __launch_bounds__(256) void k_features(FeatureArgs a)
+{
+    constexpr int NQ = 2 * KMAX + 1;            // Re/Im p_1..p_K, V
+    __shared__ double pk[4][NQ];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int w = blockIdx.x * 4 + wv;
+    if (w >= a.walkers) return;                 // whole wavefront leaves together
+    const int K = a.K;
+
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int j = lane; j < a.N; j += 64) {
+        double x, y;
+        if (a.pos_is_double || sizeof(Real) == 8) {
+            double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
+            x = q.x; y = q.y;
+        } else {
+            float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
+            x = q.x; y = q.y;
+        }
+        acc[2 * KMAX] += potential_r2(a.pot, x * x + y * y);
+        const double ur = x * a.invR, ui = y * a.invR;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            acc[2 * k] += qr;
+            acc[2 * k + 1] += qi;
+            const double t = qr * ur - qi * ui;
+            qi = qr * ui + qi * ur;
+            qr = t;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+        acc[q] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) pk[wv][q] = acc[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const long row = a.row0 + w;
+    Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
+    for (int t = lane; t < a.D; t += 64) {
+        double vr = 1.0, vi = 0.0;
+        const uint32_t* f = a.fac + (size_t)t * 8;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t word = f[q];
+            if (word == 0u) break;
+            const int k = (int)(word & 0xffu), r = (int)(word >> 8);
+            const double br = pk[wv][2 * (k - 1)], bi = pk[wv][2 * (k - 1) + 1];
+            for (int e = 0; e < r; ++e) {
+                const double tr = vr * br - vi * bi;
+                vi = vr * bi + vi * br;
+                vr = tr;
+            }
+        }
+        const int c = a.col_re[t];
+        X[c] = (Real)vr;
+        X[c + a.col_dim[t]] = (Real)vi;
+    }
+    if (lane == 0) reinterpret_cast<Real*>(a.Vw)[row] = (Real)pk[wv][2 * KMAX];
+    if (a.trace && w < a.trace_walkers && lane < 9) {
+        float v;
+        if (lane < 8) v = (lane / 2 < K) ? (float)pk[wv][lane] : 0.f;
+        else v = (float)pk[wv][2 * KMAX];
+        a.trace[((size_t)a.trace_row0 + w) * 9 + lane] = v;
+    }
+    (void)K;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted Gram tiles on MFMA.  Workgroup = 4 waves = one 64x64 output tile over
+// columns [i0, i0+64) x [j0, j0+64); wave (wi, wj) owns a 32x32 quadrant as 2x2
+// 16x16x4 MFMA tiles.  Samples are the reduction dimension, staged 32 at a time.
+// MFMA 16x16x4 operands: A[i = l%16][k = l/16], B[k = l/16][j = l%16]; the result
+// register r of lane l holds C[4 (l/16) + r][l%16] for f32 and C[4 r + l/16][l%16] for f64.
+// ---------------------------------------------------------------------------
+template <typename Real> struct Mfma;
+template <> struct Mfma<float> {
+    using acc_t = floatx4;
+    __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+template <> struct Mfma<double> {
+    using acc_t = doublex4;
+    __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+
+__device__ __forceinline__ void load8(const float* p, float* o)
+{
+    const float4 a = reinterpret_cast<const float4*>(p)[0];
+    const float4 b = reinterpret_cast<const float4*>(p)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+}
+__device__ __forceinline__ void load8(const double* p, double* o)
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const double2 a = reinterpret_cast<const double2*>(p)[e];
+        o[2 * e] = a.x;
+        o[2 * e + 1] = a.y;
+    }
+}
+
+constexpr int kKC = 32;          // samples staged per step
+constexpr int kLdsStride = 80;   // padded LDS row (elements)
+constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_gram(GramArgs g)
+{
+    using acc_t = typename Mfma<Real>::acc_t;
+    __shared__ __align__(16) Real xi[kKC][kLdsStride];
+    __shared__ __align__(16) Real xj[kKC][kLdsStride];
+    __shared__ Real vs[kKC];
+
+    const Tile tile = g.tiles[blockIdx.x];
+    const long r_begin = (long)blockIdx.y * g.rows_per_split;
+    long r_end = r_begin + g.rows_per_split;
+    if (r_end > g.rows) r_end = g.rows;
+    const bool same = tile.i0 == tile.j0;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int l16 = lane & 15, k4 = lane >> 4;
+
+    acc_t accS[2][2], accH[2][2];
+    double shS[2][2][4], shH[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accS[a][b] = acc_t{0, 0, 0, 0};
+            accH[a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { shS[a][b][r] = 0.0; shH[a][b][r] = 0.0; }
+        }
+
+    const Real* X = reinterpret_cast<const Real*>(g.X);
+    const Real* Vw = reinterpret_cast<const Real*>(g.Vw);
+    int steps = 0;
+    for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
+        // stage: 32 rows x 64 columns per side; 256 threads x 8 elements
+        {
+            const int rr = tid >> 3;           // 0..31
+            const int cc = (tid & 7) * 8;      // 0..56
+            const long row = r0 + rr;
+            Real bufi[8], bufj[8];
+            if (row < r_end) {
+                const Real* src = X + (size_t)row * g.ldx;
+                load8(src + tile.i0 + cc, bufi);
+                if (!same) load8(src + tile.j0 + cc, bufj);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) { bufi[e] = 0; bufj[e] = 0; }
+            }
+            __syncthreads();   // previous step's readers are done
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xi[rr][cc + e] = bufi[e];
+            if (!same) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) xj[rr][cc + e] = bufj[e];
+            }
+            if (tid < kKC) vs[tid] = (r0 + tid < r_end) ? Vw[r0 + tid] : (Real)0;
+            __syncthreads();
+        }
+        const Real(*yj)[kLdsStride] = same ? xi : xj;
+#pragma unroll
+        for (int kk = 0; kk < kKC; kk += 4) {
+            const int kr = kk + k4;
+            const Real v = vs[kr];
+            Real av[2], avv[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                av[a] = xi[kr][wi * 32 + a * 16 + l16];
+                avv[a] = av[a] * v;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = yj[kr][wj * 32 + b * 16 + l16];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accS[a][b] = Mfma<Real>::op(av[a], bv[b], accS[a][b]);
+                    accH[a][b] = Mfma<Real>::op(avv[a], bv[b], accH[a][b]);
+                }
+        }
+        if (sizeof(Real) == 4 && ++steps == kFlushSteps) {
+            steps = 0;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        shS[a][b][r] += (double)accS[a][b][r];
+                        shH[a][b][r] += (double)accH[a][b][r];
+                    }
+                    accS[a][b] = acc_t{0, 0, 0, 0};
+                    accH[a][b] = acc_t{0, 0, 0, 0};
+                }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double s = shS[a][b][r] + (double)accS[a][b][r];
+                const double h = shH[a][b][r] + (double)accH[a][b][r];
+                // f32 16x16x4: reg r of lane l is row 4 (l/16) + r; f64 16x16x4: row 4 r + l/16
+                const int orow = (sizeof(Real) == 4) ? (4 * k4 + r) : (4 * r + k4);
+                const long row = tile.i0 + wi * 32 + a * 16 + orow;
+                const long col = tile.j0 + wj * 32 + b * 16 + l16;
+                atomicAdd(&g.CS[row * g.ldc + col], s);
+                atomicAdd(&g.CH[row * g.ldc + col], h);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 tiles -> complex sector blocks: S_ab = (C[a][b] + C[d+a][d+b]) + i (C[a][d+b] - C[d+a][b])
+// (only upper 64-tiles were accumulated; the lower ones are read by symmetry)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cget(const double* C, long ldc, long p, long q)
+{
+    return ((p >> 6) <= (q >> 6)) ? C[p * ldc + q] : C[q * ldc + p];
+}
+
+__global__ void k_extract(ExtractArgs e)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= e.total) return;
+    int s = 0;
+    while (s + 1 < e.nsec && e.blk_off[s + 1] <= g) ++s;
+    const long loc = g - e.blk_off[s];
+    const int d = e.sec_off[s + 1] - e.sec_off[s];
+    const int a = (int)(loc / d), b = (int)(loc % d);
+    const long base = 2L * e.sec_off[s];
+    const long pa = base + a, pb = base + b, qa = base + d + a, qb = base + d + b;
+    const double sr = cget(e.CS, e.ldc, pa, pb) + cget(e.CS, e.ldc, qa, qb);
+    const double si = cget(e.CS, e.ldc, pa, qb) - cget(e.CS, e.ldc, qa, pb);
+    const double hr = cget(e.CH, e.ldc, pa, pb) + cget(e.CH, e.ldc, qa, qb);
+    const double hi = cget(e.CH, e.ldc, pa, qb) - cget(e.CH, e.ldc, qa, pb);
+    e.S[2 * g] = sr * e.scale;
+    e.S[2 * g + 1] = si * e.scale;
+    e.H[2 * g] = hr * e.scale;
+    e.H[2 * g + 1] = hi * e.scale;
+}
+
+__global__ void k_zero(double* p, long n)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; g < n; g += stride) p[g] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 generalized eigensolver, one workgroup (256 threads) per problem.
+// ---------------------------------------------------------------------------
+struct cd {
+    double r, i;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cd cmulc(cd a, cd b) { return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }  // a conj(b)
+__device__ __forceinline__ cd cconj(cd a) { return {a.r, -a.i}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cd cscale(cd a, double s) { return {a.r * s, a.i * s}; }
+
+constexpr int kEigThreads = 256;
+constexpr int kEigMaxPairs = 320;     // d <= 640
+
+__device__ double block_sum(double v, double* red)
+{
+    const int tid = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kEigThreads / 64; ++w) t += red[w];
+    return t;
+}
+
+__global__ __launch_bounds__(kEigThreads) void k_geneig(const double* Sd, const double* Hd, double* workd,
+                                                       const GenEigProblem* probs, double* evals,
+                                                       double* evecsd, int* status)
+{
+    __shared__ double red[kEigThreads / 64];
+    __shared__ double rc[kEigMaxPairs], rs[kEigMaxPairs];
+    __shared__ cd rph[kEigMaxPairs];
+    __shared__ int rp[kEigMaxPairs], rq[kEigMaxPairs];
+
+    const GenEigProblem pr = probs[blockIdx.x];
+    const int d = pr.d;
+    const int tid = threadIdx.x;
+    const cd* S = reinterpret_cast<const cd*>(Sd) + pr.in_off;
+    const cd* H = reinterpret_cast<const cd*>(Hd) + pr.in_off;
+    cd* L = reinterpret_cast<cd*>(workd) + pr.work_off;
+    cd* C = L + (size_t)d * d;
+    cd* W = C + (size_t)d * d;
+    cd* T = W + (size_t)d * d;
+    cd* V = reinterpret_cast<cd*>(evecsd) + pr.in_off;
+    double* ev = evals + pr.eval_off;
+
+    for (long x = tid; x < (long)d * d; x += kEigThreads) L[x] = S[x];
+    __syncthreads();
+
+    // Cholesky S = L L^H (lower)
+    for (int j = 0; j < d; ++j) {
+        double part = 0.0;
+        for (int k = tid; k < j; k += kEigThreads) {
+            cd l = L[(size_t)j * d + k];
+            part += l.r * l.r + l.i * l.i;
+        }
+        double sdot = block_sum(part, red);
+        double sjj = L[(size_t)j * d + j].r - sdot;
+        if (!(sjj > 0.0)) {
+            if (tid == 0) status[blockIdx.x] = 1;
+            return;   // uniform across the block
+        }
+        const double ljj = sqrt(sjj);
+        __syncthreads();
+        for (int i = j + 1 + tid; i < d; i += kEigThreads) {
+            cd t = L[(size_t)i * d + j];
+            for (int k = 0; k < j; ++k) t = csub(t, cmulc(L[(size_t)i * d + k], L[(size_t)j * d + k]));
+            L[(size_t)i * d + j] = cscale(t, 1.0 / ljj);
+        }
+        __syncthreads();
+        if (tid == 0) L[(size_t)j * d + j] = {ljj, 0.0};
+        __syncthreads();
+    }
+    // T = L^-1 H (one thread per column)
+    for (int c = tid; c < d; c += kEigThreads)
+        for (int i = 0; i < d; ++i) {
+            cd t = H[(size_t)i * d + c];
+            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], T[(size_t)k * d + c]));
+            T[(size_t)i * d + c] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+        }
+    __syncthreads();
+    // W = L^-1 T^H ; C = W^H  (so C = L^-1 H L^-H)
+    for (int r = tid; r < d; r += kEigThreads)
+        for (int i = 0; i < d; ++i) {
+            cd t = cconj(T[(size_t)r * d + i]);
+            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], W[(size_t)k * d + r]));
+            W[(size_t)i * d + r] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+        }
+    __syncthreads();
+    for (long x = tid; x < (long)d * d; x += kEigThreads) {
+        const int i = (int)(x / d), j = (int)(x % d);
+        cd a = cconj(W[(size_t)j * d + i]);
+        cd b = W[(size_t)i * d + j];
+        cd c = cscale(cadd(a, b), 0.5);
+        if (i == j) c.i = 0.0;
+        C[x] = c;
+        T[x] = (i == j) ? cd{1.0, 0.0} : cd{0.0, 0.0};   // eigenvector accumulator
+    }
+    __syncthreads();
+
+    // parallel cyclic Jacobi, round-robin pairing over n = d rounded up to even
+    const int n = (d + 1) & ~1;
+    const int npair = n / 2;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (long x = tid; x < (long)d * d; x += kEigThreads) {
+            cd c = C[x];
+            double a2 = c.r * c.r + c.i * c.i;
+            tot += a2;
+            if ((x / d) != (x % d)) off += a2;
+        }
+        off = block_sum(off, red);
+        tot = block_sum(tot, red);
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int round = 0; round < n - 1; ++round) {
+            for (int k = tid; k < npair; k += kEigThreads) {
+                int p, q;
+                if (k == 0) {
+                    p = 0;
+                    q = 1 + (round + n - 2) % (n - 1);
+                } else {
+                    p = 1 + (round + k - 1) % (n - 1);
+                    q = 1 + (round + n - 2 - k) % (n - 1);
+                }
+                if (p > q) { int t = p; p = q; q = t; }
+                double cs = 1.0, sn = 0.0;
+                cd ph = {1.0, 0.0};
+                if (q < d) {
+                    cd apq = C[(size_t)p * d + q];
+                    double aabs = sqrt(apq.r * apq.r + apq.i * apq.i);
+                    if (aabs > 1e-300) {
+                        const double app = C[(size_t)p * d + p].r, aqq = C[(size_t)q * d + q].r;
+                        ph = {apq.r / aabs, apq.i / aabs};
+                        const double tau = (aqq - app) / (2.0 * aabs);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        cs = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * cs;
+                    }
+                }
+                rp[k] = p; rq[k] = q; rc[k] = cs; rs[k] = sn; rph[k] = ph;
+            }
+            __syncthreads();
+            // columns: C <- C G, T <- T G
+            for (long x = tid; x < (long)d * npair; x += kEigThreads) {
+                const int k = (int)(x % npair), row = (int)(x / npair);
+                const int p = rp[k], q = rq[k];
+                if (q >= d || rs[k] == 0.0) continue;
+                const double cs = rc[k], sn = rs[k];
+                const cd ph = rph[k];
+                cd ckp = C[(size_t)row * d + p], ckq = C[(size_t)row * d + q];
+                C[(size_t)row * d + p] = csub(cscale(ckp, cs), cscale(cmul(cconj(ph), ckq), sn));
+                C[(size_t)row * d + q] = cadd(cscale(cmul(ph, ckp), sn), cscale(ckq, cs));
+                cd tkp = T[(size_t)row * d + p], tkq = T[(size_t)row * d + q];
+                T[(size_t)row * d + p] = csub(cscale(tkp, cs), cscale(cmul(cconj(ph), tkq), sn));
+                T[(size_t)row * d + q] = cadd(cscale(cmul(ph, tkp), sn), cscale(tkq, cs));
+            }
+            __syncthreads();
+            // rows: C <- G^H C
+            for (long x = tid; x < (long)d * npair; x += kEigThreads) {
+                const int k = (int)(x % npair), col = (int)(x / npair);
+                const int p = rp[k], q = rq[k];
+                if (q >= d || rs[k] == 0.0) continue;
+                const double cs = rc[k], sn = rs[k];
+                const cd ph = rph[k];
+                cd cpk = C[(size_t)p * d + col], cqk = C[(size_t)q * d + col];
+                C[(size_t)p * d + col] = csub(cscale(cpk, cs), cscale(cmul(ph, cqk), sn));
+                C[(size_t)q * d + col] = cadd(cscale(cmul(cconj(ph), cpk), sn), cscale(cqk, cs));
+            }
+            __syncthreads();
+            for (int k = tid; k < npair; k += kEigThreads) {
+                const int p = rp[k], q = rq[k];
+                if (q < d && rs[k] != 0.0) {
+                    C[(size_t)p * d + q] = {0.0, 0.0};
+                    C[(size_t)q * d + p] = {0.0, 0.0};
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // ascending order: rank of each diagonal entry
+    for (int i = tid; i < d; i += kEigThreads) {
+        const double ei = C[(size_t)i * d + i].r;
+        int rank = 0;
+        for (int j = 0; j < d; ++j) {
+            const double ej = C[(size_t)j * d + j].r;
+            rank += (ej < ei || (ej == ei && j < i)) ? 1 : 0;
+        }
+        ev[rank] = ei;
+        reinterpret_cast<int*>(W)[rank] = i;    // W is free now: permutation
+    }
+    __syncthreads();
+    // eigenvectors v = L^-H w (back substitution), S-normalised
+    if (evecsd) {
+        const int* perm = reinterpret_cast<const int*>(W);
+        for (int c = tid; c < d; c += kEigThreads) {
+            const int src = perm[c];
+            for (int i = d - 1; i >= 0; --i) {
+                cd t = T[(size_t)i * d + src];
+                for (int k = i + 1; k < d; ++k) t = csub(t, cmul(cconj(L[(size_t)k * d + i]), V[(size_t)k * d + c]));
+                V[(size_t)i * d + c] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+            }
+        }
+    }
+    if (tid == 0) status[blockIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t sweep_lds_bytes(int prec, int G, int N)
+{
+    const int S = (N + G - 1) / G;
+    const size_t el = (prec == kFP64) ? sizeof(double2) : sizeof(float2);
+    return (size_t)(64 / G) * (size_t)S * G * el;
+}
+
+bool sweep_supported(int G) { return G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64; }
+
+hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
+                                 unsigned long long seed, hipStream_t s)
+{
+    long n = (long)N * walkers;
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (prec == kFP64)
+        hipLaunchKernelGGL(k_init_positions<double>, grid, dim3(256), 0, s, (double2*)pos, N, walkers, R, seed);
+    else
+        hipLaunchKernelGGL(k_init_positions<float>, grid, dim3(256), 0, s, (float2*)pos, N, walkers, R, seed);
+    return hipGetLastError();
+}
+
+template <int G, typename Real>
+static hipError_t sweep_g(const SweepArgs& a, hipStream_t s)
+{
+    const size_t lds = sweep_lds_bytes(sizeof(Real) == 8 ? kFP64 : kFP32, G, a.N);
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep<G, Real>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sweep<G, Real>), grid, dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t sweep_dispatch(const SweepArgs& a, int G, hipStream_t s)
+{
+    switch (G) {
+    case 2: return sweep_g<2, Real>(a, s);
+    case 4: return sweep_g<4, Real>(a, s);
+    case 8: return sweep_g<8, Real>(a, s);
+    case 16: return sweep_g<16, Real>(a, s);
+    case 32: return sweep_g<32, Real>(a, s);
+    case 64: return sweep_g<64, Real>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    return prec == kFP64 ? sweep_dispatch<double>(a, G, s) : sweep_dispatch<float>(a, G, s);
+}
+
+template <typename Real>
+static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.walkers + 3) / 4));
+    if (a.K <= 8)
+        hipLaunchKernelGGL((k_features<Real, 8>), grid, dim3(256), 0, s, a);
+    else if (a.K <= 16)
+        hipLaunchKernelGGL((k_features<Real, 16>), grid, dim3(256), 0, s, a);
+    else if (a.K <= 24)
+        hipLaunchKernelGGL((k_features<Real, 24>), grid, dim3(256), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    return prec == kFP64 ? features_t<double>(a, s) : features_t<float>(a, s);
+}
+
+hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
+{
+    if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
+    dim3 grid((unsigned)a.ntiles, (unsigned)splits);
+    if (prec == kFP64)
+        hipLaunchKernelGGL(k_gram<double>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_gram<float>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract(const ExtractArgs& a, hipStream_t s)
+{
+    if (a.total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_extract, dim3((unsigned)((a.total + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_geneig(const double* S, const double* H, double* work, const GenEigProblem* probs,
+                         int nprob, double* evals, double* evecs, int* status, hipStream_t s)
+{
+    if (nprob <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geneig, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
+                       evecs, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(double* p, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace le

@@ -1,0 +1,97 @@
+// Host-side launch interface of the edge-state Monte-Carlo kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace le {
+
+// One potential: V(r) = 0.5 w r^2 + eps r^4 + strength (r / R)^expo
+struct Potential {
+    double w, eps, strength;
+    int expo;
+    double R;
+};
+
+struct SweepArgs {
+    void* pos;               // [walkers][N] float2 or double2
+    unsigned* accepted;      // [walkers] accepted-move counters (added to)
+    unsigned long long seed;
+    unsigned long long sweep0;   // global sweep index of the first sweep (Philox counter)
+    int N, m, walkers, nsweeps;
+    double step;
+};
+
+struct FeatureArgs {
+    const void* pos;         // [walkers][N] float2 or double2 (or double2 for fixed samples)
+    int pos_is_double;       // fixed-sample path feeds FP64 positions to an FP32 engine
+    void* X;                 // [rows][ldx] float or double: real-embedded monomials
+    long ldx;
+    void* Vw;                // [rows] float or double: sum_i V(r_i)
+    long row0;               // first output row
+    int N, walkers, D, K;
+    const uint32_t* fac;     // [D][8] packed (k | r << 8)
+    const int* col_re;       // [D] column of Re p_t (Im at col_re + col_dim)
+    const int* col_dim;      // [D] d of the state's sector
+    double invR;
+    Potential pot;
+    float* trace;            // optional [trace_rows][9]: Re/Im p1..p4, V
+    int trace_walkers;
+    long trace_row0;
+};
+
+struct Tile {
+    int i0, j0;              // column origins (multiples of 64) of an upper-triangle tile
+};
+
+struct GramArgs {
+    const void* X;           // [rows][ldx]
+    long ldx;
+    const void* Vw;          // [rows]
+    long rows;
+    long rows_per_split;
+    const Tile* tiles;
+    int ntiles;
+    double* CS;              // [ldc][ldc] FP64 accumulators (upper tiles)
+    double* CH;
+    long ldc;
+};
+
+struct ExtractArgs {
+    const double* CS;
+    const double* CH;
+    long ldc;
+    int nsec;
+    const int* sec_off;      // nsec+1 basis offsets
+    const long* blk_off;     // nsec+1 offsets into the sector-block arrays (complex elements)
+    double scale;            // 1 / samples
+    double* S;               // interleaved complex blocks
+    double* H;
+    long total;              // sum d^2
+};
+
+struct GenEigProblem {
+    int d;
+    long in_off;             // complex-element offset of S, H, evecs blocks
+    long eval_off;
+    long work_off;           // complex-element offset into the work buffer (4 d^2)
+};
+
+// precision selector for templated paths
+enum Prec { kFP32 = 0, kFP64 = 1 };
+
+hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
+                                 unsigned long long seed, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
+hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);
+hipError_t launch_extract(const ExtractArgs& a, hipStream_t s);
+hipError_t launch_geneig(const double* S, const double* H, double* work,
+                         const GenEigProblem* probs, int nprob, double* evals, double* evecs,
+                         int* status, hipStream_t s);
+hipError_t launch_zero(double* p, long n, hipStream_t s);
+
+size_t sweep_lds_bytes(int prec, int lanes_per_walker, int N);
+bool sweep_supported(int lanes_per_walker);
+
+}  // namespace le

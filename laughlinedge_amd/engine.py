@@ -1,0 +1,245 @@
+"""ctypes binding of liblaughlinedge.so (the C ABI declared in include/laughlinedge.h).
+
+This is the host-side mirror used by the tests and the benchmark; the drop-in C++
+interface (edgeMC::spectrum_compute and friends) sits in include/edge_montecarlo.h and
+calls the same C ABI.  Nothing here computes: every call goes to the HIP library, and
+loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblaughlinedge.so")
+
+LE_OK = 0
+LE_ERR_ARG = 1
+LE_ERR_CAPACITY = 2
+LE_ERR_HIP = 3
+LE_ERR_NOT_PD = 4
+LE_ERR_BISECTION = 5
+LE_ERR_STATE = 6
+LE_ERR_NOCONV = 7
+PREC_FP32 = 0
+PREC_FP64 = 1
+
+EXPORTED = [
+    "le_mc_params_default", "le_last_error", "le_engine_create", "le_engine_destroy",
+    "le_engine_get_info", "le_sector_dims", "le_basis_state", "le_init_walkers",
+    "le_tune_step", "le_burnin", "le_sample", "le_run", "le_get_stats", "le_n_blocks_done",
+    "le_get_moments", "le_get_trace", "le_get_positions", "le_set_positions",
+    "le_accumulate_positions", "le_reset_moments", "le_geneig", "le_engine_spectrum",
+    "le_engine_spectra",
+]
+
+
+class LEError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"laughlinedge error {code}: {msg}")
+        self.code = code
+
+
+class SystemParams(ctypes.Structure):
+    _fields_ = [("particles_number", ctypes.c_int), ("inverse_filling", ctypes.c_int),
+                ("conf_w", ctypes.c_double), ("conf_eps", ctypes.c_double),
+                ("conf_strength", ctypes.c_double), ("conf_exponent", ctypes.c_int)]
+
+
+class MCParams(ctypes.Structure):
+    _fields_ = [("walkers", ctypes.c_int), ("burnin_sweeps", ctypes.c_int),
+                ("samples", ctypes.c_int), ("thin", ctypes.c_int), ("step", ctypes.c_double),
+                ("tune_sweeps", ctypes.c_int), ("target_acceptance", ctypes.c_double),
+                ("acceptance_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
+                ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
+                ("trace_walkers", ctypes.c_int)]
+
+
+class EngineInfo(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int), ("K", ctypes.c_int), ("n_sectors", ctypes.c_int),
+                ("longest_partition", ctypes.c_int), ("block_elems", ctypes.c_long),
+                ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
+                ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_double), ("acceptance", ctypes.c_double),
+                ("sweeps", ctypes.c_longlong), ("samples", ctypes.c_longlong),
+                ("ms_sweep", ctypes.c_double), ("ms_features", ctypes.c_double),
+                ("ms_gram", ctypes.c_double), ("ms_sample_total", ctypes.c_double),
+                ("gram_launches", ctypes.c_longlong), ("sweep_launches", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run python -m laughlinedge_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.le_last_error.restype = ctypes.c_char_p
+        L.le_n_blocks_done.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != LE_OK:
+        raise LEError(rc, lib().le_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_mc():
+    p = MCParams()
+    lib().le_mc_params_default(ctypes.byref(p))
+    return p
+
+
+def _cplx(flat):
+    return flat[0::2] + 1j * flat[1::2]
+
+
+class Engine:
+    """One engine = one system, one list of sectors, one set of walkers on the GPU."""
+
+    def __init__(self, N, m, sectors, w=0.0, eps=0.0, strength=0.0, exponent=0, **mc):
+        self.sys = SystemParams(int(N), int(m), float(w), float(eps), float(strength), int(exponent))
+        self.mc = default_mc()
+        for k, v in mc.items():
+            if not hasattr(self.mc, k):
+                raise TypeError(f"unknown Monte-Carlo parameter {k}")
+            setattr(self.mc, k, v)
+        self.sectors = [int(s) for s in sectors]
+        arr = (ctypes.c_int * len(self.sectors))(*self.sectors)
+        self._h = ctypes.c_void_p()
+        _check(lib().le_engine_create(ctypes.byref(self.sys), ctypes.byref(self.mc), arr,
+                                      len(self.sectors), ctypes.byref(self._h)))
+        self.info = EngineInfo()
+        _check(lib().le_engine_get_info(self._h, ctypes.byref(self.info)))
+        ns = self.info.n_sectors
+        dims = (ctypes.c_int * ns)()
+        offs = (ctypes.c_int * (ns + 1))()
+        _check(lib().le_sector_dims(self._h, dims, offs))
+        self.dims = list(dims)
+        self.offsets = list(offs)
+        self.N, self.m = int(N), int(m)
+        self.R = float(np.sqrt(2.0 * m * N))
+
+    def close(self):
+        if self._h:
+            lib().le_engine_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # basis --------------------------------------------------------------
+    def state(self, t):
+        dl = ctypes.c_int()
+        pairs = (ctypes.c_int * 16)()
+        n = ctypes.c_int()
+        _check(lib().le_basis_state(self._h, int(t), ctypes.byref(dl), pairs, ctypes.byref(n)))
+        return dl.value, [(pairs[2 * q], pairs[2 * q + 1]) for q in range(n.value)]
+
+    # sampling ------------------------------------------------------------
+    def init_walkers(self):
+        _check(lib().le_init_walkers(self._h))
+
+    def tune_step(self):
+        s = ctypes.c_double()
+        _check(lib().le_tune_step(self._h, ctypes.byref(s)))
+        return s.value
+
+    def burnin(self, sweeps):
+        _check(lib().le_burnin(self._h, int(sweeps)))
+
+    def sample(self, samples):
+        _check(lib().le_sample(self._h, int(samples)))
+
+    def run(self):
+        _check(lib().le_run(self._h))
+
+    def stats(self):
+        st = RunStats()
+        _check(lib().le_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in RunStats._fields_}
+
+    def n_blocks(self):
+        return lib().le_n_blocks_done(self._h)
+
+    def moments(self, block=-1):
+        """list over sectors of (S, H) complex d x d arrays"""
+        be = self.info.block_elems
+        S = np.zeros(2 * be)
+        H = np.zeros(2 * be)
+        _check(lib().le_get_moments(self._h, int(block), _dp(S), _dp(H)))
+        out, off = [], 0
+        Sc, Hc = _cplx(S), _cplx(H)
+        for d in self.dims:
+            out.append((Sc[off:off + d * d].reshape(d, d), Hc[off:off + d * d].reshape(d, d)))
+            off += d * d
+        return out
+
+    def trace(self):
+        n = ctypes.c_long()
+        _check(lib().le_get_trace(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(1, n.value), 9), dtype=np.float32)
+        _check(lib().le_get_trace(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def positions(self):
+        out = np.zeros((self.mc.walkers, self.N, 2))
+        _check(lib().le_get_positions(self._h, _dp(out)))
+        return out
+
+    def set_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        assert pos.shape == (self.mc.walkers, self.N, 2)
+        _check(lib().le_set_positions(self._h, _dp(pos)))
+
+    def accumulate_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.N, 2)
+        _check(lib().le_accumulate_positions(self._h, _dp(pos), ctypes.c_long(pos.shape[0])))
+
+    def reset_moments(self):
+        _check(lib().le_reset_moments(self._h))
+
+    def spectra(self, block=-1):
+        ev = np.zeros(max(1, self.info.D))
+        st = (ctypes.c_int * self.info.n_sectors)()
+        _check(lib().le_engine_spectra(self._h, int(block), _dp(ev), st))
+        out = []
+        for s, d in enumerate(self.dims):
+            o = self.offsets[s]
+            out.append(ev[o:o + d].copy() if st[s] == 0 else None)
+        return out
+
+
+def geneig(S, H, vectors=False):
+    """FP64 generalized Hermitian eigensolve on the GPU."""
+    d = S.shape[0]
+    pack = lambda A: np.ascontiguousarray(np.stack([A.real, A.imag], -1).reshape(-1))
+    ev = np.zeros(d)
+    vec = np.zeros(2 * d * d) if vectors else None
+    _check(lib().le_geneig(ctypes.c_int(d), _dp(pack(S)), _dp(pack(H)), _dp(ev),
+                           _dp(vec) if vectors else None))
+    if vectors:
+        return ev, _cplx(vec).reshape(d, d)
+    return ev

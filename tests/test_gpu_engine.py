@@ -1,0 +1,122 @@
+"""GPU parity tests: HIP path (through the C ABI) against the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_fro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _disc_positions(rng, n, N, R):
+    r = R * np.sqrt(rng.random((n, N)))
+    t = 2 * np.pi * rng.random((n, N))
+    return np.stack([r * np.cos(t), r * np.sin(t)], -1)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-12)])
+def test_fixed_sample_moments(oracle_lib, prec, tol):
+    """Same FP64 configurations through the GPU features+Gram path and the oracle."""
+    from laughlinedge_amd import Engine
+    N, m, sectors = 12, 3, [0, 1, 2, 3, 4, 5, 6]
+    w, eps = 0.05, 1e-4
+    print(f"case prec={prec} N={N} sectors={sectors}", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=w, eps=eps)
+    pos, _ = o.plain_positions(16, 50, 40, 0.6, 99)
+    pos = pos.reshape(-1, N, 2)
+    So, Ho, _ = o.accumulate_fixed(pos)
+    n = pos.shape[0]
+    with Engine(N, m, sectors, w=w, eps=eps, walkers=64, precision=prec, samples=1) as e:
+        e.accumulate_positions(pos)
+        mom = e.moments()
+    for s, (Sg, Hg) in enumerate(mom):
+        assert _rel_fro(Sg, So[s] / n) < tol, (s, _rel_fro(Sg, So[s] / n))
+        assert _rel_fro(Hg, Ho[s] / n) < tol, (s, _rel_fro(Hg, Ho[s] / n))
+
+
+def test_fixed_sample_large_tiles(oracle_lib):
+    """N=40, dL 10..12 spans several 64-column Gram tiles; FP32 path vs FP64 oracle."""
+    from laughlinedge_amd import Engine
+    N, m, sectors = 40, 3, [10, 11, 12]
+    print(f"case N={N} sectors={sectors}", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05, eps=5e-5)
+    rng = np.random.default_rng(5)
+    pos = _disc_positions(rng, 3000, N, o.R)
+    So, Ho, _ = o.accumulate_fixed(pos)
+    with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=256, samples=1) as e:
+        assert e.info.n_tiles > 3
+        e.accumulate_positions(pos)
+        mom = e.moments()
+    for s, (Sg, Hg) in enumerate(mom):
+        assert _rel_fro(Sg, So[s] / 3000) < 1e-6
+        assert _rel_fro(Hg, Ho[s] / 3000) < 1e-6
+
+
+def test_fp64_chain_matches_oracle(oracle_lib):
+    """FP64 sampler: same Philox streams as the oracle chain -> same positions."""
+    from laughlinedge_amd import Engine
+    N, m, W = 10, 3, 64
+    print("case fp64 chain N=10", flush=True)
+    o = oracle_lib.Oracle(N, m, [0])
+    ref, _ = o.plain_positions(W, 0, 1, 0.5, 1234, thin=6)
+    with Engine(N, m, [0], walkers=W, precision=1, step=0.5, seed=1234, lanes_per_walker=2) as e:
+        e.init_walkers()
+        e.burnin(6)
+        got = e.positions()
+    err = np.abs(got - ref[:, 0]).max(axis=(1, 2))
+    assert np.mean(err < 1e-9) >= 0.95, err
+
+
+def test_fp32_chain_tracks_oracle(oracle_lib):
+    """FP32 sampler follows the FP64 chain for a couple of sweeps (same random streams)."""
+    from laughlinedge_amd import Engine
+    N, m, W = 16, 3, 64
+    print("case fp32 chain N=16", flush=True)
+    o = oracle_lib.Oracle(N, m, [0])
+    ref, _ = o.plain_positions(W, 0, 1, 0.5, 77, thin=2)
+    for G in (2, 4, 8, 16):
+        with Engine(N, m, [0], walkers=W, precision=0, step=0.5, seed=77, lanes_per_walker=G) as e:
+            e.init_walkers()
+            e.burnin(2)
+            got = e.positions()
+        err = np.abs(got - ref[:, 0]).max(axis=(1, 2))
+        assert np.mean(err < 1e-3) >= 0.9, (G, err)
+
+
+def test_geneig_gpu_vs_oracle(oracle_lib):
+    rng = np.random.default_rng(3)
+    from laughlinedge_amd import geneig
+    for d in (1, 2, 5, 17, 64, 131):
+        print(f"case geneig d={d}", flush=True)
+        A = rng.standard_normal((d, 2 * d)) + 1j * rng.standard_normal((d, 2 * d))
+        S = A @ A.conj().T / (2 * d) + 0.1 * np.eye(d)
+        B = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        H = (B + B.conj().T) / 2
+        ev, vec = geneig(S, H, vectors=True)
+        ref = oracle_lib.geneig(S, H)
+        assert np.allclose(ev, ref, rtol=1e-9, atol=1e-9)
+        # residual and S-orthonormality
+        r = H @ vec - S @ vec * ev[None, :]
+        assert np.abs(r).max() < 1e-8 * max(1.0, np.abs(H).max())
+        g = vec.conj().T @ S @ vec
+        assert np.allclose(g, np.eye(d), atol=1e-8)
+
+
+def test_sampler_r2_sum_rule():
+    """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler."""
+    from laughlinedge_amd import Engine
+    N, m = 20, 3
+    L0 = m * N * (N - 1) // 2
+    print("case r2 sum rule N=20", flush=True)
+    # H with w = 2, eps = 0 is sum_i r_i^2; sector 0 has S = 1
+    with Engine(N, m, [0], w=2.0, walkers=2048, precision=0, step=0.6, samples=200,
+                burnin_sweeps=300, tune_sweeps=0, n_blocks=4) as e:
+        e.run()
+        blocks = [e.moments(b)[0][1][0, 0].real for b in range(4)]
+        st = e.stats()
+    mean = np.mean(blocks)
+    err = np.std(blocks, ddof=1) / 2
+    exact = 2 * (L0 + N)
+    assert abs(mean - exact) < 4 * err + 1e-4 * exact, (mean, exact, err)
+    assert 0.2 < st["acceptance"] < 0.9

@@ -12,8 +12,9 @@
  *       capacity checks and result buffers (modules/edge_mc/em_spectrum.cu:210-252),
  *       with the sys_params constants (modules/sys_params.h:4-25) passed at run time
  *       and monteCarloParameters (modules/edge_montecarlo.h:45-88) as le_mc_params.
- *   le_basis_state
- *       iparts::integer_partitions host partitions (modules/integer_partitions.h:29-70).
+ *   le_basis_state, le_basis_describe
+ *       iparts::integer_partitions host partitions (modules/integer_partitions.h:29-70,
+ *       modules/partitions/integer_partitions.cu:10-226).
  *   le_tune_step
  *       edgeMC::optimal_step / mcmc_acceptanceBisection (modules/edge_mc/em_bisection.cu:7-153).
  *   le_burnin, le_sample, le_run
@@ -80,6 +81,35 @@ typedef struct le_mc_params {
 
 typedef struct le_engine le_engine;
 
+/* Extra accumulated observables (all optional; pass NULL for S and H only).
+ *  harmonics: angular harmonics L; for each L the weights sum_i cos(L theta_i) and
+ *      sum_i sin(L theta_i) are accumulated between every sector pair (s1, s2) with
+ *      |dL_s2 - dL_s1| = L (reference: em_dsf.cu:113-134 for the DSF, em_edgeResponse.tpp
+ *      :133-163 and :456-470 for the density Fourier components and selection rule).
+ *  n_user_weights: per-sample weights supplied by le_set_weight_callback, accumulated
+ *      between every sector pair (reference: the excitation potential U,
+ *      em_edgeResponse.tpp:129-164). */
+typedef struct le_observables {
+    int n_harmonics;           /* <= 8 */
+    const int* harmonics;
+    int n_user_weights;        /* <= 8 */
+} le_observables;
+
+/* Weight identifiers for le_get_block */
+#define LE_W_ONE 0                      /* metric S */
+#define LE_W_V 1                        /* confinement H */
+#define LE_W_COS(L) (0x100 | (L))       /* sum_i cos(L theta_i) */
+#define LE_W_SIN(L) (0x200 | (L))       /* sum_i sin(L theta_i) */
+#define LE_W_USER(k) (0x300 | (k))      /* k-th user weight */
+
+/* User weight callback, called on the engine's stream after each kept-sample sweep:
+ * positions: device, walkers x N complex (float when positions_are_double == 0, else
+ * double); weights: device, n_user_weights rows of `walkers` values (float for an FP32
+ * engine, double for FP64) with row stride ld; stream: the engine's hipStream_t.
+ * Return 0 on success. */
+typedef int (*le_weight_fn)(void* ctx, const void* positions, int positions_are_double, int walkers,
+                            int N, void* weights, long ld, int weights_are_double, void* stream);
+
 typedef struct le_engine_info {
     int D;                     /* basis size over all sectors */
     int K;                     /* largest power sum degree */
@@ -109,13 +139,19 @@ void le_mc_params_default(le_mc_params* p);
 const char* le_last_error(void);
 
 int le_engine_create(const le_system_params* sys, const le_mc_params* mc, const int* sectors,
-                     int n_sectors, le_engine** out);
+                     int n_sectors, const le_observables* obs, le_engine** out);
+int le_set_weight_callback(le_engine* e, le_weight_fn fn, void* ctx);
 int le_engine_destroy(le_engine* e);
 int le_engine_get_info(const le_engine* e, le_engine_info* info);
 /* sector dims (n_sectors) and basis offsets (n_sectors + 1) */
 int le_sector_dims(const le_engine* e, int* dims, int* offsets);
 /* state -> dL and compressed (part, multiplicity) pairs; pairs holds up to 2*8 ints */
 int le_basis_state(const le_engine* e, int state, int* dL, int* pairs, int* n_pairs);
+
+/* Basis without an engine (no GPU call): D states, largest degree K; per state (up to
+ * `capacity`): dL, up to 8 (part, multiplicity) pairs at pairs[16 t ..], pair count. */
+int le_basis_describe(int N, const int* sectors, int n_sectors, int capacity, int* D, int* K,
+                      int* dls, int* pairs, int* n_pairs);
 
 int le_init_walkers(le_engine* e);
 int le_tune_step(le_engine* e, double* step_out);
@@ -130,6 +166,10 @@ int le_n_blocks_done(const le_engine* e);
  * values, interleaved re/im, per sector d x d row-major, S_ab = <conj(p_a) p_b>, power
  * sums of z / sqrt(2 m N); H carries sum_i V(r_i). */
 int le_get_moments(const le_engine* e, int block, double* S, double* H);
+/* Mean (block = -1) or one block of a weight's complex block between sectors s1 and s2:
+ * d_s1 x d_s2 complex, interleaved, out_ab = <conj(p_a) w p_b>.  s1 > s2 returns the
+ * conjugate transpose of (s2, s1).  Blocks that were not accumulated give LE_ERR_ARG. */
+int le_get_block(const le_engine* e, int block, int weight, int s1, int s2, double* out);
 /* acceptance trace: per kept sample, per traced walker: Re/Im p_1..p_4, V (float) */
 int le_get_trace(const le_engine* e, float* out, long capacity, long* n_rows);
 

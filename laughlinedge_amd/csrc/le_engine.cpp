@@ -72,27 +72,44 @@ struct le_engine {
     void* pos = nullptr;         // walkers x N x (float2|double2)
     unsigned* acc = nullptr;
     void* X = nullptr;           // rows x ldx
-    void* Vw = nullptr;
     uint32_t* fac = nullptr;
     int* col_re = nullptr;
     int* col_dim = nullptr;
-    le::Tile* tiles = nullptr;
-    int ntiles = 0;
     long ldx = 0;
-    double* CS = nullptr;
-    double* CH = nullptr;
-    int* sec_off_d = nullptr;
-    long* blk_off_d = nullptr;
-    double* blkS_d = nullptr;
-    double* blkH_d = nullptr;
+    // weights: row 0 = V, rows 1..2 nh = cos/sin harmonics, then user weights
+    void* Wb = nullptr;
+    long wrows = 0;
+    int nh = 0;
+    int harm[le::kMaxHarmonics] = {0};
+    int n_user = 0;
+    le_weight_fn wfn = nullptr;
+    void* wctx = nullptr;
+    struct Job {
+        int wid[2];                  // weight identifiers (LE_W_*)
+        int nw;
+        std::vector<std::pair<int, int>> blocks;   // sector pairs s1 <= s2
+        le::Tile* tiles = nullptr;
+        int ntiles = 0;
+        double* C[2] = {nullptr, nullptr};
+    };
+    std::vector<Job> jobs;
+    int ntiles = 0;                  // tiles of the S/H job
+    struct Item {
+        int wid, s1, s2;
+        long off;                    // complex offset in a block record
+    };
+    std::vector<Item> items;
+    long rec_elems = 0;              // complex elements per block record
+    le::ExtractItem* items_d = nullptr;
+    double* rec_d = nullptr;
     float* trace_d = nullptr;
     long trace_cap_rows = 0;
     long trace_rows = 0;
     double* eig_work = nullptr;
     size_t eig_work_elems = 0;
 
-    // host state
-    std::vector<std::vector<double>> blkS, blkH;
+    // host state: one record (all items) per closed block
+    std::vector<std::vector<double>> rec;
     std::vector<long long> blk_samples;
     bool walkers_ready = false;
     unsigned long long sweep_counter = 0;
@@ -104,7 +121,9 @@ struct le_engine {
     std::vector<hipEvent_t> ev;
 
     ~le_engine();
-    int setup();
+    int setup(const le_observables* obs);
+    const void* weight_ptr(int wid) const;
+    long item_offset(int wid, int s1, int s2) const;
     int autotune_lanes();
     int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions);
     int close_block(long long samples);
@@ -117,17 +136,17 @@ le_engine::~le_engine()
     (void)hipFree(pos);
     (void)hipFree(acc);
     (void)hipFree(X);
-    (void)hipFree(Vw);
     (void)hipFree(fac);
     (void)hipFree(col_re);
     (void)hipFree(col_dim);
-    (void)hipFree(tiles);
-    (void)hipFree(CS);
-    (void)hipFree(CH);
-    (void)hipFree(sec_off_d);
-    (void)hipFree(blk_off_d);
-    (void)hipFree(blkS_d);
-    (void)hipFree(blkH_d);
+    (void)hipFree(Wb);
+    for (auto& j : jobs) {
+        (void)hipFree(j.tiles);
+        (void)hipFree(j.C[0]);
+        (void)hipFree(j.C[1]);
+    }
+    (void)hipFree(items_d);
+    (void)hipFree(rec_d);
     (void)hipFree(trace_d);
     (void)hipFree(eig_work);
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -156,10 +175,33 @@ extern "C" void le_mc_params_default(le_mc_params* p)
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
 
-int le_engine::setup()
+const void* le_engine::weight_ptr(int wid) const
+{
+    const size_t rsz_ = rsz;
+    auto row = [&](long r) { return (const void*)((const char*)Wb + (size_t)r * wrows * rsz_); };
+    if (wid == LE_W_ONE) return nullptr;
+    if (wid == LE_W_V) return row(0);
+    const int kind = wid >> 8, par = wid & 0xff;
+    if (kind == 1 || kind == 2) {
+        for (int h = 0; h < nh; ++h)
+            if (harm[h] == par) return row(1 + 2 * h + (kind == 2 ? 1 : 0));
+    }
+    if (kind == 3 && par < n_user) return row(1 + 2 * nh + par);
+    return nullptr;
+}
+
+long le_engine::item_offset(int wid, int s1, int s2) const
+{
+    for (const auto& it : items)
+        if (it.wid == wid && it.s1 == s1 && it.s2 == s2) return it.off;
+    return -1;
+}
+
+int le_engine::setup(const le_observables* obs)
 {
     const int N = sys.particles_number;
     const int D = basis.D;
+    const int ns = basis.nsec();
     R = std::sqrt(2.0 * sys.inverse_filling * N);
     step = mc.step;
     rsz = (prec == LE_PREC_FP64) ? sizeof(double) : sizeof(float);
@@ -168,21 +210,83 @@ int le_engine::setup()
     // real-embedded columns: sector s occupies [2 o_s, 2 o_s + 2 d_s): Re then Im
     ldx = std::max<long>(64, ((2L * D + 63) / 64) * 64);
     std::vector<int> cre(D), cdim(D);
-    std::set<std::pair<int, int>> tl;
-    for (int s = 0; s < basis.nsec(); ++s) {
+    for (int s = 0; s < ns; ++s) {
         const int o = basis.sector_offset[s], d = basis.dim(s);
         for (int t = 0; t < d; ++t) {
             cre[o + t] = 2 * o + t;
             cdim[o + t] = d;
         }
-        if (d == 0) continue;
-        const int t0 = (2 * o) / 64, t1 = (2 * o + 2 * d - 1) / 64;
-        for (int a = t0; a <= t1; ++a)
-            for (int b = a; b <= t1; ++b) tl.insert({a, b});
     }
-    std::vector<le::Tile> th;
-    for (auto& p : tl) th.push_back({p.first * 64, p.second * 64});
-    ntiles = (int)th.size();
+    if (obs) {
+        nh = obs->n_harmonics;
+        for (int h = 0; h < nh; ++h) harm[h] = obs->harmonics[h];
+        n_user = obs->n_user_weights;
+    }
+
+    // Gram jobs: {1, V} on diagonal sector blocks; {cos L, sin L} on pairs |dL2 - dL1| = L;
+    // user weights (two per job) on every pair
+    auto add_job = [&](int w0, int w1, int nw, const std::vector<std::pair<int, int>>& blocks) {
+        Job j;
+        j.wid[0] = w0;
+        j.wid[1] = w1;
+        j.nw = nw;
+        j.blocks = blocks;
+        jobs.push_back(j);
+    };
+    std::vector<std::pair<int, int>> diag, all;
+    for (int s = 0; s < ns; ++s)
+        if (basis.dim(s) > 0) diag.push_back({s, s});
+    for (int a = 0; a < ns; ++a)
+        for (int b = a; b < ns; ++b)
+            if (basis.dim(a) > 0 && basis.dim(b) > 0) all.push_back({a, b});
+    add_job(LE_W_ONE, LE_W_V, 2, diag);
+    for (int h = 0; h < nh; ++h) {
+        std::vector<std::pair<int, int>> bl;
+        for (auto& pr : all)
+            if (std::abs(basis.sectors[pr.second] - basis.sectors[pr.first]) == harm[h]) bl.push_back(pr);
+        if (!bl.empty()) add_job(LE_W_COS(harm[h]), LE_W_SIN(harm[h]), 2, bl);
+    }
+    for (int k = 0; k < n_user; k += 2) {
+        if (k + 1 < n_user) add_job(LE_W_USER(k), LE_W_USER(k + 1), 2, all);
+        else add_job(LE_W_USER(k), 0, 1, all);
+    }
+    for (auto& j : jobs) {
+        std::set<std::pair<int, int>> tl;
+        for (auto& pr : j.blocks) {
+            const int c1 = 2 * basis.sector_offset[pr.first], e1 = c1 + 2 * basis.dim(pr.first) - 1;
+            const int c2 = 2 * basis.sector_offset[pr.second], e2 = c2 + 2 * basis.dim(pr.second) - 1;
+            for (int a = c1 / 64; a <= e1 / 64; ++a)
+                for (int b = c2 / 64; b <= e2 / 64; ++b)
+                    if (a <= b) tl.insert({a, b});
+        }
+        std::vector<le::Tile> th;
+        for (auto& pr : tl) th.push_back({pr.first * 64, pr.second * 64});
+        j.ntiles = (int)th.size();
+        HIPCHK(dalloc(&j.tiles, (size_t)std::max(1, j.ntiles)));
+        if (j.ntiles) HIPCHK(hipMemcpy(j.tiles, th.data(), j.ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
+        for (int q = 0; q < j.nw; ++q) {
+            HIPCHK(dalloc(&j.C[q], (size_t)ldx * ldx));
+            HIPCHK(le::launch_zero(j.C[q], ldx * ldx, stream));
+        }
+        for (int q = 0; q < j.nw; ++q)
+            for (auto& pr : j.blocks) {
+                items.push_back({j.wid[q], pr.first, pr.second, rec_elems});
+                rec_elems += (long)basis.dim(pr.first) * basis.dim(pr.second);
+            }
+    }
+    ntiles = jobs[0].ntiles;
+    std::vector<le::ExtractItem> xi;
+    for (auto& it : items) {
+        const double* C = nullptr;
+        for (auto& j : jobs)
+            for (int q = 0; q < j.nw; ++q)
+                if (j.wid[q] == it.wid) C = j.C[q];
+        xi.push_back({C, 2 * basis.sector_offset[it.s1], basis.dim(it.s1), 2 * basis.sector_offset[it.s2],
+                      basis.dim(it.s2), it.off});
+    }
+    HIPCHK(dalloc(&items_d, std::max<size_t>(1, xi.size())));
+    if (!xi.empty()) HIPCHK(hipMemcpy(items_d, xi.data(), xi.size() * sizeof(le::ExtractItem), hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&rec_d, (size_t)std::max<long>(2, 2 * rec_elems)));
 
     // kept samples per launch round: ~128k rows, X under 1 GiB
     const long row_bytes = ldx * (long)rsz;
@@ -194,12 +298,14 @@ int le_engine::setup()
         spl = (int)std::max<long>(1, std::min<long>(want, max_rows / mc.walkers));
     }
     const long rows = (long)spl * mc.walkers;
+    wrows = rows;
 
     HIPCHK(hipMalloc(&pos, (size_t)mc.walkers * N * 2 * rsz));
     HIPCHK(dalloc(&acc, (size_t)mc.walkers));
     HIPCHK(hipMalloc(&X, (size_t)rows * ldx * rsz));
     HIPCHK(hipMemsetAsync(X, 0, (size_t)rows * ldx * rsz, stream));
-    HIPCHK(hipMalloc(&Vw, (size_t)rows * rsz));
+    HIPCHK(hipMalloc(&Wb, (size_t)(1 + 2 * nh + n_user) * rows * rsz));
+    HIPCHK(hipMemsetAsync(Wb, 0, (size_t)(1 + 2 * nh + n_user) * rows * rsz, stream));
     auto ft = basis.factor_table();
     HIPCHK(dalloc(&fac, std::max<size_t>(1, ft.size())));
     if (!ft.empty()) HIPCHK(hipMemcpy(fac, ft.data(), ft.size() * 4, hipMemcpyHostToDevice));
@@ -209,21 +315,6 @@ int le_engine::setup()
         HIPCHK(hipMemcpy(col_re, cre.data(), D * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(col_dim, cdim.data(), D * 4, hipMemcpyHostToDevice));
     }
-    HIPCHK(dalloc(&tiles, (size_t)std::max(1, ntiles)));
-    if (ntiles) HIPCHK(hipMemcpy(tiles, th.data(), ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
-    HIPCHK(dalloc(&CS, (size_t)ldx * ldx));
-    HIPCHK(dalloc(&CH, (size_t)ldx * ldx));
-    HIPCHK(le::launch_zero(CS, ldx * ldx, stream));
-    HIPCHK(le::launch_zero(CH, ldx * ldx, stream));
-    std::vector<long> bo(basis.nsec() + 1);
-    for (int s = 0; s <= basis.nsec(); ++s) bo[s] = basis.block_offset(s);
-    HIPCHK(dalloc(&sec_off_d, basis.sector_offset.size()));
-    HIPCHK(hipMemcpy(sec_off_d, basis.sector_offset.data(), basis.sector_offset.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(dalloc(&blk_off_d, bo.size()));
-    HIPCHK(hipMemcpy(blk_off_d, bo.data(), bo.size() * sizeof(long), hipMemcpyHostToDevice));
-    const long be = basis.block_elems();
-    HIPCHK(dalloc(&blkS_d, (size_t)std::max<long>(2, 2 * be)));
-    HIPCHK(dalloc(&blkH_d, (size_t)std::max<long>(2, 2 * be)));
     if (mc.trace_walkers > 0) {
         const int tw = std::min(mc.trace_walkers, mc.walkers);
         trace_cap_rows = (long)std::max(1, mc.samples) * tw;
@@ -237,8 +328,15 @@ int le_engine::setup()
 }
 
 extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params* mc, const int* sectors,
-                                int n_sectors, le_engine** out)
+                                int n_sectors, const le_observables* obs, le_engine** out)
 {
+    if (obs) {
+        if (obs->n_harmonics < 0 || obs->n_harmonics > le::kMaxHarmonics || (obs->n_harmonics && !obs->harmonics))
+            return fail(LE_ERR_ARG, "n_harmonics must be in [0, 8] with a harmonics array");
+        for (int h = 0; h < obs->n_harmonics; ++h)
+            if (obs->harmonics[h] < 0 || obs->harmonics[h] > 255) return fail(LE_ERR_ARG, "harmonic L must be in [0, 255]");
+        if (obs->n_user_weights < 0 || obs->n_user_weights > 8) return fail(LE_ERR_ARG, "n_user_weights must be in [0, 8]");
+    }
     if (!sys || !mc || !out || (n_sectors > 0 && !sectors) || n_sectors <= 0)
         return fail(LE_ERR_ARG, "le_engine_create: null argument or no sectors");
     *out = nullptr;
@@ -276,7 +374,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
             delete e;
             return fail(LE_ERR_CAPACITY, "sector larger than 640 states");
         }
-    int rc = e->setup();
+    int rc = e->setup(obs);
     if (rc != LE_OK) {
         std::string msg = g_err;
         delete e;
@@ -285,6 +383,28 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     }
     if (mc->lanes_per_walker) e->G = mc->lanes_per_walker;
     *out = e;
+    return LE_OK;
+}
+
+extern "C" int le_basis_describe(int N, const int* sectors, int n_sectors, int capacity, int* D, int* K,
+                                 int* dls, int* pairs, int* n_pairs)
+{
+    if (N < 1 || n_sectors <= 0 || !sectors || !D) return fail(LE_ERR_ARG, "le_basis_describe: bad argument");
+    for (int q = 0; q < n_sectors; ++q)
+        if (sectors[q] < 0 || sectors[q] > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
+    le::Basis b;
+    b.build(N, std::vector<int>(sectors, sectors + n_sectors));
+    *D = b.D;
+    if (K) *K = b.K;
+    for (int t = 0; t < b.D && t < capacity; ++t) {
+        if (dls) dls[t] = b.states[t].dL;
+        if (n_pairs) n_pairs[t] = (int)b.states[t].pairs.size();
+        if (pairs)
+            for (size_t q = 0; q < b.states[t].pairs.size() && q < 8; ++q) {
+                pairs[16 * t + 2 * q] = b.states[t].pairs[q].number;
+                pairs[16 * t + 2 * q + 1] = b.states[t].pairs[q].repetitions;
+            }
+    }
     return LE_OK;
 }
 
@@ -517,50 +637,47 @@ extern "C" int le_burnin(le_engine* e, int sweeps)
 
 int le_engine::gram_rows(long rows)
 {
-    le::GramArgs g{};
-    g.X = X;
-    g.ldx = ldx;
-    g.Vw = Vw;
-    g.rows = rows;
-    const long want_wg = 2048;
-    long splits = std::max<long>(1, (want_wg + ntiles - 1) / std::max(1, ntiles));
-    long rps = (rows + splits - 1) / splits;
-    rps = std::max<long>(rps, 1024);
-    rps = ((rps + 31) / 32) * 32;
-    splits = (rows + rps - 1) / rps;
-    g.rows_per_split = rps;
-    g.tiles = tiles;
-    g.ntiles = ntiles;
-    g.CS = CS;
-    g.CH = CH;
-    g.ldc = ldx;
-    HIPCHK(le::launch_gram(g, prec, (int)splits, stream));
+    for (auto& j : jobs) {
+        le::GramArgs g{};
+        g.X = X;
+        g.ldx = ldx;
+        g.w0 = weight_ptr(j.wid[0]);
+        g.w1 = j.nw > 1 ? weight_ptr(j.wid[1]) : nullptr;
+        g.nw = j.nw;
+        g.rows = rows;
+        const long want_wg = 2048;
+        long splits = std::max<long>(1, (want_wg + j.ntiles - 1) / std::max(1, j.ntiles));
+        long rps = (rows + splits - 1) / splits;
+        rps = std::max<long>(rps, 1024);
+        rps = ((rps + 31) / 32) * 32;
+        splits = (rows + rps - 1) / rps;
+        g.rows_per_split = rps;
+        g.tiles = j.tiles;
+        g.ntiles = j.ntiles;
+        g.C0 = j.C[0];
+        g.C1 = j.C[1];
+        g.ldc = ldx;
+        HIPCHK(le::launch_gram(g, prec, (int)splits, stream));
+    }
     return LE_OK;
 }
 
 int le_engine::close_block(long long samples)
 {
-    const long be = basis.block_elems();
     le::ExtractArgs x{};
-    x.CS = CS;
-    x.CH = CH;
+    x.items = items_d;
+    x.nitems = (int)items.size();
+    x.total = rec_elems;
     x.ldc = ldx;
-    x.nsec = basis.nsec();
-    x.sec_off = sec_off_d;
-    x.blk_off = blk_off_d;
     x.scale = samples > 0 ? 1.0 / (double)samples : 0.0;
-    x.S = blkS_d;
-    x.H = blkH_d;
-    x.total = be;
+    x.out = rec_d;
     HIPCHK(le::launch_extract(x, stream));
-    std::vector<double> hs(2 * be), hh(2 * be);
-    HIPCHK(hipMemcpyAsync(hs.data(), blkS_d, sizeof(double) * 2 * be, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(hh.data(), blkH_d, sizeof(double) * 2 * be, hipMemcpyDeviceToHost, stream));
-    HIPCHK(le::launch_zero(CS, ldx * ldx, stream));
-    HIPCHK(le::launch_zero(CH, ldx * ldx, stream));
+    std::vector<double> h(2 * rec_elems);
+    HIPCHK(hipMemcpyAsync(h.data(), rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
+    for (auto& j : jobs)
+        for (int q = 0; q < j.nw; ++q) HIPCHK(le::launch_zero(j.C[q], ldx * ldx, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    blkS.push_back(std::move(hs));
-    blkH.push_back(std::move(hh));
+    rec.push_back(std::move(h));
     blk_samples.push_back(samples);
     return LE_OK;
 }
@@ -580,7 +697,10 @@ extern "C" int le_sample(le_engine* e, int samples)
     f.pos_is_double = 0;
     f.X = e->X;
     f.ldx = e->ldx;
-    f.Vw = e->Vw;
+    f.W = e->Wb;
+    f.wld = e->wrows;
+    f.nh = e->nh;
+    for (int h = 0; h < e->nh; ++h) f.harm[h] = e->harm[h];
     f.N = e->sys.particles_number;
     f.walkers = e->mc.walkers;
     f.D = e->basis.D;
@@ -611,6 +731,12 @@ extern "C" int le_sample(le_engine* e, int samples)
                 f.trace_walkers = 0;
             }
             HIPCHK(le::launch_features(f, e->prec, e->stream));
+            if (e->wfn && e->n_user) {
+                void* wdst = (char*)e->Wb + ((size_t)(1 + 2 * e->nh) * e->wrows + f.row0) * e->rsz;
+                if (e->wfn(e->wctx, e->pos, e->prec == LE_PREC_FP64, e->mc.walkers, e->sys.particles_number, wdst,
+                           e->wrows, e->prec == LE_PREC_FP64, (void*)e->stream) != 0)
+                    return fail(LE_ERR_STATE, "user weight callback failed");
+            }
         }
         HIPCHK(hipEventRecord(e->ev[2 * n], e->stream));
         int rc = e->gram_rows((long)n * e->mc.walkers);
@@ -679,33 +805,78 @@ extern "C" int le_get_stats(const le_engine* e, le_run_stats* st)
     return LE_OK;
 }
 
-extern "C" int le_n_blocks_done(const le_engine* e) { return e ? (int)e->blkS.size() : 0; }
+extern "C" int le_n_blocks_done(const le_engine* e) { return e ? (int)e->rec.size() : 0; }
+
+extern "C" int le_set_weight_callback(le_engine* e, le_weight_fn fn, void* ctx)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    e->wfn = fn;
+    e->wctx = ctx;
+    return LE_OK;
+}
+
+// copy `n` complex elements at `off` of a record (block >= 0) or the sample-weighted mean
+static int record_slice(const le_engine* e, int block, long off, long n, double* out)
+{
+    const int nb = (int)e->rec.size();
+    if (nb == 0) return fail(LE_ERR_STATE, "no closed sample blocks");
+    if (block >= nb || block < -1) return fail(LE_ERR_ARG, "block out of range");
+    if (block >= 0) {
+        std::memcpy(out, e->rec[block].data() + 2 * off, sizeof(double) * 2 * n);
+        return LE_OK;
+    }
+    long long tot = 0;
+    for (auto c : e->blk_samples) tot += c;
+    for (long q = 0; q < 2 * n; ++q) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) {
+            const double w = tot ? (double)e->blk_samples[b] / (double)tot : 1.0 / nb;
+            s += w * e->rec[b][2 * off + q];
+        }
+        out[q] = s;
+    }
+    return LE_OK;
+}
 
 extern "C" int le_get_moments(const le_engine* e, int block, double* S, double* H)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
-    const long be2 = 2 * e->basis.block_elems();
-    const int nb = (int)e->blkS.size();
-    if (nb == 0) return fail(LE_ERR_STATE, "no closed sample blocks");
-    if (block >= nb || block < -1) return fail(LE_ERR_ARG, "block out of range");
-    if (block >= 0) {
-        if (S) std::memcpy(S, e->blkS[block].data(), sizeof(double) * be2);
-        if (H) std::memcpy(H, e->blkH[block].data(), sizeof(double) * be2);
-        return LE_OK;
-    }
-    // sample-weighted mean over blocks
-    long long tot = 0;
-    for (auto n : e->blk_samples) tot += n;
-    for (long q = 0; q < be2; ++q) {
-        double s = 0, h = 0;
-        for (int b = 0; b < nb; ++b) {
-            const double w = tot ? (double)e->blk_samples[b] / (double)tot : 1.0 / nb;
-            s += w * e->blkS[b][q];
-            h += w * e->blkH[b][q];
+    long o = 0;
+    for (int s = 0; s < e->basis.nsec(); ++s) {
+        const long n = (long)e->basis.dim(s) * e->basis.dim(s);
+        if (n) {
+            if (S) {
+                int rc = record_slice(e, block, e->item_offset(LE_W_ONE, s, s), n, S + 2 * o);
+                if (rc) return rc;
+            }
+            if (H) {
+                int rc = record_slice(e, block, e->item_offset(LE_W_V, s, s), n, H + 2 * o);
+                if (rc) return rc;
+            }
         }
-        if (S) S[q] = s;
-        if (H) H[q] = h;
+        o += n;
     }
+    return LE_OK;
+}
+
+extern "C" int le_get_block(const le_engine* e, int block, int weight, int s1, int s2, double* out)
+{
+    if (!e || !out) return fail(LE_ERR_ARG, "null argument");
+    const int ns = e->basis.nsec();
+    if (s1 < 0 || s2 < 0 || s1 >= ns || s2 >= ns) return fail(LE_ERR_ARG, "sector out of range");
+    const int a = std::min(s1, s2), b = std::max(s1, s2);
+    const long off = e->item_offset(weight, a, b);
+    if (off < 0) return fail(LE_ERR_ARG, "this weight was not accumulated between these sectors");
+    const int da = e->basis.dim(a), db = e->basis.dim(b);
+    if (s1 <= s2) return record_slice(e, block, off, (long)da * db, out);
+    std::vector<double> tmp(2 * (size_t)da * db);
+    int rc = record_slice(e, block, off, (long)da * db, tmp.data());
+    if (rc) return rc;
+    for (int i = 0; i < db; ++i)       // out is db x da: conj transpose
+        for (int j = 0; j < da; ++j) {
+            out[2 * ((size_t)i * da + j)] = tmp[2 * ((size_t)j * db + i)];
+            out[2 * ((size_t)i * da + j) + 1] = -tmp[2 * ((size_t)j * db + i) + 1];
+        }
     return LE_OK;
 }
 
@@ -751,11 +922,10 @@ extern "C" int le_set_positions(le_engine* e, const double* in)
 extern "C" int le_reset_moments(le_engine* e)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
-    e->blkS.clear();
-    e->blkH.clear();
+    e->rec.clear();
     e->blk_samples.clear();
-    HIPCHK(le::launch_zero(e->CS, e->ldx * e->ldx, e->stream));
-    HIPCHK(le::launch_zero(e->CH, e->ldx * e->ldx, e->stream));
+    for (auto& j : e->jobs)
+        for (int q = 0; q < j.nw; ++q) HIPCHK(le::launch_zero(j.C[q], e->ldx * e->ldx, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return LE_OK;
 }
@@ -772,7 +942,10 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
     f.pos_is_double = 1;
     f.X = e->X;
     f.ldx = e->ldx;
-    f.Vw = e->Vw;
+    f.W = e->Wb;
+    f.wld = e->wrows;
+    f.nh = e->nh;
+    for (int h = 0; h < e->nh; ++h) f.harm[h] = e->harm[h];
     f.row0 = 0;
     f.N = N;
     f.D = e->basis.D;
@@ -791,6 +964,13 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
         f.walkers = (int)m;
         h = le::launch_features(f, e->prec, e->stream);
         if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
+        if (e->wfn && e->n_user) {
+            void* wdst = (char*)e->Wb + (size_t)(1 + 2 * e->nh) * e->wrows * e->rsz;
+            if (e->wfn(e->wctx, stage, 1, (int)m, N, wdst, e->wrows, e->prec == LE_PREC_FP64, (void*)e->stream) != 0) {
+                rc = fail(LE_ERR_STATE, "user weight callback failed");
+                break;
+            }
+        }
         rc = e->gram_rows(m);
         if (rc == LE_OK) {
             h = hipStreamSynchronize(e->stream);

@@ -224,13 +224,16 @@ __device__ __forceinline__ double potential_r2(const Potential& p, double r2)
 template <typename Real, int KMAX>
 __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
 {
-    constexpr int NQ = 2 * KMAX + 1;            // Re/Im p_1..p_K, V
+    constexpr int NQ = 2 * KMAX + 1 + 2 * kMaxHarmonics;   // Re/Im p_1..p_K, V, cos/sin sums
+    constexpr int QV = 2 * KMAX;
+    constexpr int QH = 2 * KMAX + 1;
     __shared__ double pk[4][NQ];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int w = blockIdx.x * 4 + wv;
     if (w >= a.walkers) return;                 // whole wavefront leaves together
     const int K = a.K;
+    const int nh = a.nh;
 
     double acc[NQ];
 #pragma unroll
@@ -244,7 +247,8 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
             float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
             x = q.x; y = q.y;
         }
-        acc[2 * KMAX] += potential_r2(a.pot, x * x + y * y);
+        const double r2 = x * x + y * y;
+        acc[QV] += potential_r2(a.pot, r2);
         const double ur = x * a.invR, ui = y * a.invR;
         double qr = ur, qi = ui;
 #pragma unroll
@@ -255,9 +259,35 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
             qi = qr * ui + qi * ur;
             qr = t;
         }
+        if (nh > 0) {
+            // e^{i L theta} = (z / |z|)^L by binary powering
+            const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+            const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
+#pragma unroll
+            for (int h = 0; h < kMaxHarmonics; ++h) {
+                if (h < nh) {
+                    unsigned e = (unsigned)a.harm[h];
+                    double br = er, bi = ei, pr = 1.0, pi = 0.0;
+                    while (e) {
+                        if (e & 1u) {
+                            const double t = pr * br - pi * bi;
+                            pi = pr * bi + pi * br;
+                            pr = t;
+                        }
+                        e >>= 1;
+                        const double t = br * br - bi * bi;
+                        bi = 2.0 * br * bi;
+                        br = t;
+                    }
+                    acc[QH + 2 * h] += pr;
+                    acc[QH + 2 * h + 1] += pi;
+                }
+            }
+        }
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        if (q >= QH + 2 * nh) continue;
         double v = acc[q];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
@@ -292,14 +322,15 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
         X[c] = (Real)vr;
         X[c + a.col_dim[t]] = (Real)vi;
     }
-    if (lane == 0) reinterpret_cast<Real*>(a.Vw)[row] = (Real)pk[wv][2 * KMAX];
+    Real* W = reinterpret_cast<Real*>(a.W);
+    if (lane == 0) W[row] = (Real)pk[wv][QV];
+    if (lane < 2 * nh) W[(size_t)(1 + lane) * a.wld + row] = (Real)pk[wv][QH + lane];
     if (a.trace && w < a.trace_walkers && lane < 9) {
         float v;
         if (lane < 8) v = (lane / 2 < K) ? (float)pk[wv][lane] : 0.f;
-        else v = (float)pk[wv][2 * KMAX];
+        else v = (float)pk[wv][QV];
         a.trace[((size_t)a.trace_row0 + w) * 9 + lane] = v;
     }
-    (void)K;
 }
 
 // ---------------------------------------------------------------------------
@@ -340,13 +371,13 @@ constexpr int kKC = 32;          // samples staged per step
 constexpr int kLdsStride = 80;   // padded LDS row (elements)
 constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 
-template <typename Real>
+template <typename Real, int NW>
 __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 {
     using acc_t = typename Mfma<Real>::acc_t;
     __shared__ __align__(16) Real xi[kKC][kLdsStride];
     __shared__ __align__(16) Real xj[kKC][kLdsStride];
-    __shared__ Real vs[kKC];
+    __shared__ Real vs[NW][kKC];
 
     const Tile tile = g.tiles[blockIdx.x];
     const long r_begin = (long)blockIdx.y * g.rows_per_split;
@@ -359,20 +390,21 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
     const int wi = wave >> 1, wj = wave & 1;
     const int l16 = lane & 15, k4 = lane >> 4;
 
-    acc_t accS[2][2], accH[2][2];
-    double shS[2][2][4], shH[2][2][4];
+    acc_t acc[NW][2][2];
+    double sh[NW][2][2][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < NW; ++q)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            accS[a][b] = acc_t{0, 0, 0, 0};
-            accH[a][b] = acc_t{0, 0, 0, 0};
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) { shS[a][b][r] = 0.0; shH[a][b][r] = 0.0; }
-        }
+            for (int b = 0; b < 2; ++b) {
+                acc[q][a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sh[q][a][b][r] = 0.0;
+            }
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
-    const Real* Vw = reinterpret_cast<const Real*>(g.Vw);
+    const Real* wp[2] = {reinterpret_cast<const Real*>(g.w0), reinterpret_cast<const Real*>(g.w1)};
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         // stage: 32 rows x 64 columns per side; 256 threads x 8 elements
@@ -396,61 +428,63 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) xj[rr][cc + e] = bufj[e];
             }
-            if (tid < kKC) vs[tid] = (r0 + tid < r_end) ? Vw[r0 + tid] : (Real)0;
+            if (tid < NW * kKC) {
+                const int q = tid / kKC, k = tid % kKC;
+                const long row2 = r0 + k;
+                vs[q][k] = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
+            }
             __syncthreads();
         }
         const Real(*yj)[kLdsStride] = same ? xi : xj;
 #pragma unroll
         for (int kk = 0; kk < kKC; kk += 4) {
             const int kr = kk + k4;
-            const Real v = vs[kr];
-            Real av[2], avv[2], bv[2];
+            Real av[2], bv[2];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                av[a] = xi[kr][wi * 32 + a * 16 + l16];
-                avv[a] = av[a] * v;
-            }
+            for (int a = 0; a < 2; ++a) av[a] = xi[kr][wi * 32 + a * 16 + l16];
 #pragma unroll
             for (int b = 0; b < 2; ++b) bv[b] = yj[kr][wj * 32 + b * 16 + l16];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int q = 0; q < NW; ++q) {
+                const Real v = vs[q][kr];
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    accS[a][b] = Mfma<Real>::op(av[a], bv[b], accS[a][b]);
-                    accH[a][b] = Mfma<Real>::op(avv[a], bv[b], accH[a][b]);
+                for (int a = 0; a < 2; ++a) {
+                    const Real aw = av[a] * v;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) acc[q][a][b] = Mfma<Real>::op(aw, bv[b], acc[q][a][b]);
                 }
+            }
         }
         if (sizeof(Real) == 4 && ++steps == kFlushSteps) {
             steps = 0;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int q = 0; q < NW; ++q)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int a = 0; a < 2; ++a)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        shS[a][b][r] += (double)accS[a][b][r];
-                        shH[a][b][r] += (double)accH[a][b][r];
+                    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) sh[q][a][b][r] += (double)acc[q][a][b][r];
+                        acc[q][a][b] = acc_t{0, 0, 0, 0};
                     }
-                    accS[a][b] = acc_t{0, 0, 0, 0};
-                    accH[a][b] = acc_t{0, 0, 0, 0};
-                }
         }
     }
+    double* Cs[2] = {g.C0, g.C1};
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < NW; ++q)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double s = shS[a][b][r] + (double)accS[a][b][r];
-                const double h = shH[a][b][r] + (double)accH[a][b][r];
-                // f32 16x16x4: reg r of lane l is row 4 (l/16) + r; f64 16x16x4: row 4 r + l/16
-                const int orow = (sizeof(Real) == 4) ? (4 * k4 + r) : (4 * r + k4);
-                const long row = tile.i0 + wi * 32 + a * 16 + orow;
-                const long col = tile.j0 + wj * 32 + b * 16 + l16;
-                atomicAdd(&g.CS[row * g.ldc + col], s);
-                atomicAdd(&g.CH[row * g.ldc + col], h);
-            }
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double v = sh[q][a][b][r] + (double)acc[q][a][b][r];
+                    // f32 16x16x4: reg r of lane l is row 4 (l/16) + r; f64 16x16x4: row 4 r + l/16
+                    const int orow = (sizeof(Real) == 4) ? (4 * k4 + r) : (4 * r + k4);
+                    const long row = tile.i0 + wi * 32 + a * 16 + orow;
+                    const long col = tile.j0 + wj * 32 + b * 16 + l16;
+                    atomicAdd(&Cs[q][row * g.ldc + col], v);
+                }
 }
 
 // ---------------------------------------------------------------------------
@@ -466,21 +500,21 @@ __global__ void k_extract(ExtractArgs e)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= e.total) return;
-    int s = 0;
-    while (s + 1 < e.nsec && e.blk_off[s + 1] <= g) ++s;
-    const long loc = g - e.blk_off[s];
-    const int d = e.sec_off[s + 1] - e.sec_off[s];
-    const int a = (int)(loc / d), b = (int)(loc % d);
-    const long base = 2L * e.sec_off[s];
-    const long pa = base + a, pb = base + b, qa = base + d + a, qb = base + d + b;
-    const double sr = cget(e.CS, e.ldc, pa, pb) + cget(e.CS, e.ldc, qa, qb);
-    const double si = cget(e.CS, e.ldc, pa, qb) - cget(e.CS, e.ldc, qa, pb);
-    const double hr = cget(e.CH, e.ldc, pa, pb) + cget(e.CH, e.ldc, qa, qb);
-    const double hi = cget(e.CH, e.ldc, pa, qb) - cget(e.CH, e.ldc, qa, pb);
-    e.S[2 * g] = sr * e.scale;
-    e.S[2 * g + 1] = si * e.scale;
-    e.H[2 * g] = hr * e.scale;
-    e.H[2 * g + 1] = hi * e.scale;
+    int lo = 0, hi = e.nitems - 1;
+    while (lo < hi) {   // last item with out_off <= g
+        const int mid = (lo + hi + 1) >> 1;
+        if (e.items[mid].out_off <= g) lo = mid; else hi = mid - 1;
+    }
+    const ExtractItem it = e.items[lo];
+    const long loc = g - it.out_off;
+    const int a = (int)(loc / it.d2), b = (int)(loc % it.d2);
+    const long pa = it.c1 + a, qa = it.c1 + it.d1 + a;
+    const long pb = it.c2 + b, qb = it.c2 + it.d2 + b;
+    const long ldc = e.ldc;
+    const double re = cget(it.C, ldc, pa, pb) + cget(it.C, ldc, qa, qb);
+    const double im = cget(it.C, ldc, pa, qb) - cget(it.C, ldc, qa, pb);
+    e.out[2 * g] = re * e.scale;
+    e.out[2 * g + 1] = im * e.scale;
 }
 
 __global__ void k_zero(double* p, long n)
@@ -783,10 +817,13 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
 {
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
     dim3 grid((unsigned)a.ntiles, (unsigned)splits);
-    if (prec == kFP64)
-        hipLaunchKernelGGL(k_gram<double>, grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_gram<float>, grid, dim3(256), 0, s, a);
+    if (prec == kFP64) {
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_gram<double, 1>), grid, dim3(256), 0, s, a);
+    } else {
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_gram<float, 1>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -22,12 +22,15 @@ struct SweepArgs {
     double step;
 };
 
+constexpr int kMaxHarmonics = 8;
+
 struct FeatureArgs {
     const void* pos;         // [walkers][N] float2 or double2 (or double2 for fixed samples)
     int pos_is_double;       // fixed-sample path feeds FP64 positions to an FP32 engine
     void* X;                 // [rows][ldx] float or double: real-embedded monomials
     long ldx;
-    void* Vw;                // [rows] float or double: sum_i V(r_i)
+    void* W;                 // [nweights][wld] float or double: row weights
+    long wld;                // stride between weight rows (= rows of X)
     long row0;               // first output row
     int N, walkers, D, K;
     const uint32_t* fac;     // [D][8] packed (k | r << 8)
@@ -35,6 +38,8 @@ struct FeatureArgs {
     const int* col_dim;      // [D] d of the state's sector
     double invR;
     Potential pot;
+    int nh;                  // angular harmonics L_h: weights sum_i cos(L theta_i), sum_i sin(L theta_i)
+    int harm[kMaxHarmonics];
     float* trace;            // optional [trace_rows][9]: Re/Im p1..p4, V
     int trace_walkers;
     long trace_row0;
@@ -44,30 +49,38 @@ struct Tile {
     int i0, j0;              // column origins (multiples of 64) of an upper-triangle tile
 };
 
+// One Gram job: up to two row weights (nullptr = the constant 1) sharing a tile list.
 struct GramArgs {
     const void* X;           // [rows][ldx]
     long ldx;
-    const void* Vw;          // [rows]
+    const void* w0;          // [rows] or nullptr
+    const void* w1;          // [rows] or nullptr (unused when nw == 1)
+    int nw;
     long rows;
     long rows_per_split;
     const Tile* tiles;
     int ntiles;
-    double* CS;              // [ldc][ldc] FP64 accumulators (upper tiles)
-    double* CH;
+    double* C0;              // [ldc][ldc] FP64 accumulators (upper tiles)
+    double* C1;
     long ldc;
 };
 
+// Complex block (s1, s2) of one weight's accumulator:
+// out_ab = (C[p_a][p_b] + C[q_a][q_b]) + i (C[p_a][q_b] - C[q_a][p_b]), scaled.
+struct ExtractItem {
+    const double* C;
+    int c1, d1;              // first Re column and size of the row sector
+    int c2, d2;              // same for the column sector
+    long out_off;            // complex-element offset of the block in the output
+};
+
 struct ExtractArgs {
-    const double* CS;
-    const double* CH;
+    const ExtractItem* items;
+    int nitems;
+    long total;              // complex elements over all items
     long ldc;
-    int nsec;
-    const int* sec_off;      // nsec+1 basis offsets
-    const long* blk_off;     // nsec+1 offsets into the sector-block arrays (complex elements)
-    double scale;            // 1 / samples
-    double* S;               // interleaved complex blocks
-    double* H;
-    long total;              // sum d^2
+    double scale;
+    double* out;             // interleaved complex
 };
 
 struct GenEigProblem {

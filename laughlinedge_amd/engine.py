@@ -30,8 +30,23 @@ EXPORTED = [
     "le_tune_step", "le_burnin", "le_sample", "le_run", "le_get_stats", "le_n_blocks_done",
     "le_get_moments", "le_get_trace", "le_get_positions", "le_set_positions",
     "le_accumulate_positions", "le_reset_moments", "le_geneig", "le_engine_spectrum",
-    "le_engine_spectra",
+    "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
 ]
+
+W_ONE = 0
+W_V = 1
+
+
+def W_COS(L):
+    return 0x100 | int(L)
+
+
+def W_SIN(L):
+    return 0x200 | int(L)
+
+
+def W_USER(k):
+    return 0x300 | int(k)
 
 
 class LEError(RuntimeError):
@@ -54,6 +69,11 @@ class MCParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int)]
+
+
+class Observables(ctypes.Structure):
+    _fields_ = [("n_harmonics", ctypes.c_int), ("harmonics", ctypes.POINTER(ctypes.c_int)),
+                ("n_user_weights", ctypes.c_int)]
 
 
 class EngineInfo(ctypes.Structure):
@@ -109,7 +129,8 @@ def _cplx(flat):
 class Engine:
     """One engine = one system, one list of sectors, one set of walkers on the GPU."""
 
-    def __init__(self, N, m, sectors, w=0.0, eps=0.0, strength=0.0, exponent=0, **mc):
+    def __init__(self, N, m, sectors, w=0.0, eps=0.0, strength=0.0, exponent=0, harmonics=(),
+                 n_user_weights=0, **mc):
         self.sys = SystemParams(int(N), int(m), float(w), float(eps), float(strength), int(exponent))
         self.mc = default_mc()
         for k, v in mc.items():
@@ -119,8 +140,11 @@ class Engine:
         self.sectors = [int(s) for s in sectors]
         arr = (ctypes.c_int * len(self.sectors))(*self.sectors)
         self._h = ctypes.c_void_p()
+        self.harmonics = [int(x) for x in harmonics]
+        self._harr = (ctypes.c_int * max(1, len(self.harmonics)))(*self.harmonics)
+        obs = Observables(len(self.harmonics), self._harr, int(n_user_weights))
         _check(lib().le_engine_create(ctypes.byref(self.sys), ctypes.byref(self.mc), arr,
-                                      len(self.sectors), ctypes.byref(self._h)))
+                                      len(self.sectors), ctypes.byref(obs), ctypes.byref(self._h)))
         self.info = EngineInfo()
         _check(lib().le_engine_get_info(self._h, ctypes.byref(self.info)))
         ns = self.info.n_sectors
@@ -196,6 +220,18 @@ class Engine:
             off += d * d
         return out
 
+    def block(self, weight, s1, s2, block=-1):
+        """d_s1 x d_s2 complex block <conj(p_a) w p_b> of one accumulated weight"""
+        d1, d2 = self.dims[s1], self.dims[s2]
+        out = np.zeros(2 * d1 * d2)
+        _check(lib().le_get_block(self._h, int(block), int(weight), int(s1), int(s2), _dp(out)))
+        return _cplx(out).reshape(d1, d2)
+
+    def set_weight_callback(self, fn_ptr, ctx=None):
+        """fn_ptr: address of a C function with the le_weight_fn signature"""
+        self._wfn = fn_ptr
+        _check(lib().le_set_weight_callback(self._h, ctypes.c_void_p(fn_ptr), ctypes.c_void_p(ctx)))
+
     def trace(self):
         n = ctypes.c_long()
         _check(lib().le_get_trace(self._h, None, 0, ctypes.byref(n)))
@@ -230,6 +266,23 @@ class Engine:
             o = self.offsets[s]
             out.append(ev[o:o + d].copy() if st[s] == 0 else None)
         return out
+
+
+def basis_describe(N, sectors):
+    """Product basis (C++ enumerator) without touching the GPU: list of (dL, pairs)."""
+    sec = (ctypes.c_int * len(sectors))(*[int(x) for x in sectors])
+    D = ctypes.c_int()
+    K = ctypes.c_int()
+    _check(lib().le_basis_describe(int(N), sec, len(sectors), 0, ctypes.byref(D), ctypes.byref(K),
+                                   None, None, None))
+    n = D.value
+    dls = (ctypes.c_int * max(1, n))()
+    pairs = (ctypes.c_int * max(1, 16 * n))()
+    npairs = (ctypes.c_int * max(1, n))()
+    _check(lib().le_basis_describe(int(N), sec, len(sectors), n, ctypes.byref(D), ctypes.byref(K),
+                                   dls, pairs, npairs))
+    return [(dls[t], [(pairs[16 * t + 2 * q], pairs[16 * t + 2 * q + 1]) for q in range(npairs[t])])
+            for t in range(n)]
 
 
 def geneig(S, H, vectors=False):

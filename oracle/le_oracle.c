@@ -199,6 +199,41 @@ int or_accumulate_fixed(const or_system *sys, const or_basis *b, const double *p
     return 0;
 }
 
+/* Weighted cross block between sectors s1 and s2: out_ab = sum_s w_s conj(p_a) p_b
+ * (d1 x d2 complex, interleaved, not divided by n).  The weights w are per sample. */
+int or_accumulate_weighted(const or_system *sys, const or_basis *b, const double *pos, long n,
+                           const double *w, int s1, int s2, double *out)
+{
+    int o1 = b->so[s1], d1 = b->so[s1 + 1] - o1;
+    int o2 = b->so[s2], d2 = b->so[s2 + 1] - o2;
+    memset(out, 0, sizeof(double) * 2 * (size_t)d1 * d2);
+#pragma omp parallel
+    {
+        double *loc = calloc(2 * (size_t)d1 * d2, sizeof(double));
+        cplx *pk = malloc(sizeof(cplx) * (b->K + 1));
+        cplx *pl = malloc(sizeof(cplx) * b->D);
+#pragma omp for schedule(static)
+        for (long s = 0; s < n; ++s) {
+            const double *p = pos + (size_t)s * sys->N * 2;
+            power_sums(sys->N, sys->R, p, b->K, pk);
+            for (int t = 0; t < d1; ++t) pl[o1 + t] = monomial(b, o1 + t, pk);
+            for (int t = 0; t < d2; ++t) pl[o2 + t] = monomial(b, o2 + t, pk);
+            for (int a = 0; a < d1; ++a) {
+                cplx ca = conj(pl[o1 + a]) * w[s];
+                for (int c = 0; c < d2; ++c) {
+                    cplx x = ca * pl[o2 + c];
+                    loc[2 * ((size_t)a * d2 + c)] += creal(x);
+                    loc[2 * ((size_t)a * d2 + c) + 1] += cimag(x);
+                }
+            }
+        }
+#pragma omp critical
+        for (size_t q = 0; q < 2 * (size_t)d1 * d2; ++q) out[q] += loc[q];
+        free(loc); free(pk); free(pl);
+    }
+    return 0;
+}
+
 /* Features of one configuration: p_lambda for all states (complex, interleaved) and V. */
 int or_features(const or_system *sys, const or_basis *b, const double *pos, double *plam, double *V)
 {

@@ -82,6 +82,17 @@ class Oracle:
                                   ctypes.c_long(n), _dp(S), _dp(H), ctypes.byref(v))
         return self.split_blocks(S), self.split_blocks(H), v.value
 
+    def accumulate_weighted(self, pos, w, s1, s2):
+        """sum_s w_s conj(p_a) p_b over samples for a in sector s1, b in sector s2"""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        d1, d2 = self.dims[s1], self.dims[s2]
+        out = np.zeros(2 * d1 * d2)
+        lib().or_accumulate_weighted(ctypes.byref(self.sys), ctypes.byref(self.cb), _dp(pos),
+                                     ctypes.c_long(pos.shape[0]), _dp(w), ctypes.c_int(s1),
+                                     ctypes.c_int(s2), _dp(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(d1, d2)
+
     def features(self, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         pl = np.zeros(2 * self.basis.D)
@@ -126,6 +137,12 @@ class Oracle:
                                 ctypes.byref(acc))
         c = lambda a: (a[0::2] + 1j * a[1::2]).reshape(d, d)
         return c(H), c(M), c(VC), c(VS), acc.value
+
+
+def harmonic_sums(pos, L):
+    """per-sample sum_i cos(L theta_i), sum_i sin(L theta_i)"""
+    th = np.arctan2(pos[..., 1], pos[..., 0])
+    return np.cos(L * th).sum(-1), np.sin(L * th).sum(-1)
 
 
 def potential(r2, w, eps, strength=0.0, expo=0, R=1.0):

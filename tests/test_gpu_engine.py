@@ -120,3 +120,59 @@ def test_sampler_r2_sum_rule():
     exact = 2 * (L0 + N)
     assert abs(mean - exact) < 4 * err + 1e-4 * exact, (mean, exact, err)
     assert 0.2 < st["acceptance"] < 0.9
+
+
+def test_harmonic_cross_blocks(oracle_lib):
+    """sum_i cos/sin(L theta_i)-weighted blocks between sectors |dL2 - dL1| = L."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import W_COS, W_SIN
+    N, m, sectors = 12, 3, [0, 2, 4]
+    print("case harmonics N=12 sectors=[0,2,4] L=[2,4]", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05)
+    pos, _ = o.plain_positions(16, 50, 40, 0.6, 5)
+    pos = pos.reshape(-1, N, 2)
+    n = pos.shape[0]
+    with Engine(N, m, sectors, w=0.05, walkers=64, samples=1, harmonics=[2, 4]) as e:
+        e.accumulate_positions(pos)
+        for L, pairs in ((2, [(0, 1), (1, 2)]), (4, [(0, 2)])):
+            c, s_ = oracle_lib.harmonic_sums(pos, L)
+            for (a, b) in pairs:
+                for wid, wv in ((W_COS(L), c), (W_SIN(L), s_)):
+                    ref = o.accumulate_weighted(pos, wv, a, b) / n
+                    got = e.block(wid, a, b)
+                    scale = np.sqrt(np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), a, a)) *
+                                    np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), b, b))) / n * N
+                    assert np.linalg.norm(got - ref) < 1e-6 * scale, (L, a, b)
+                    back = e.block(wid, b, a)
+                    assert np.allclose(back, got.conj().T)
+
+
+def test_user_weight_blocks(oracle_lib):
+    """user weights via the C callback, accumulated between every sector pair"""
+    import ctypes
+    import os
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import W_USER
+    N, m, sectors = 10, 3, [0, 1, 3]
+    print("case user weights N=10", flush=True)
+    helper = ctypes.CDLL(os.path.join(os.path.dirname(__file__), "cpp", "libuserweights.so"))
+    R = np.sqrt(2.0 * m * N)
+    helper.le_test_set_radius(ctypes.c_double(R))
+    fn = ctypes.cast(helper.le_test_weight_fn, ctypes.c_void_p).value
+    o = oracle_lib.Oracle(N, m, sectors)
+    pos, _ = o.plain_positions(8, 50, 50, 0.6, 9)
+    pos = pos.reshape(-1, N, 2)
+    n = pos.shape[0]
+    u0 = (pos[..., 0] ** 2).sum(-1) / R ** 2
+    u1 = (pos[..., 0] ** 2 - pos[..., 1] ** 2).sum(-1) / R ** 2
+    for prec in (0, 1):
+        with Engine(N, m, sectors, walkers=64, samples=1, n_user_weights=2, precision=prec) as e:
+            e.set_weight_callback(fn)
+            e.accumulate_positions(pos)
+            for k, u in ((0, u0), (1, u1)):
+                for a in range(3):
+                    for b in range(a, 3):
+                        ref = o.accumulate_weighted(pos, u, a, b) / n
+                        got = e.block(W_USER(k), a, b)
+                        tol = 1e-6 if prec == 0 else 1e-12
+                        assert np.linalg.norm(got - ref) <= tol * max(np.linalg.norm(ref), 1e-300) * 10, (prec, k, a, b)

@@ -1,0 +1,216 @@
+"""CPU tests that pin the FP64 oracle (no GPU).
+
+The reference ships no tests, golden vectors or fixtures, so each oracle function is
+pinned by something that does not restate it: published known-answer vectors
+(Philox), closed-form counts (p(n)), brute force, exact LLL algebra (small N) and
+textbook library routines (scipy.linalg.eigh).
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle.exact_lll import exact_blocks
+from oracle.partitions import Basis, accel_asc, compress, sector_basis
+
+P_COUNTS = [1, 1, 2, 3, 5, 7, 11, 15, 22, 30, 42, 56, 77, 101, 135, 176, 231, 297, 385, 490, 627]
+
+
+def test_philox_known_answers(oracle_lib):
+    # Random123 kat_vectors, philox4x32_10
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert oracle_lib.philox(ctr, key) == want
+
+
+def test_partition_counts():
+    assert [len(accel_asc(n)) for n in range(21)] == P_COUNTS
+
+
+def _brute_partitions(n):
+    out = set()
+
+    def rec(rem, mx, cur):
+        if rem == 0:
+            out.add(tuple(sorted(cur)))
+            return
+        for k in range(min(rem, mx), 0, -1):
+            rec(rem - k, k, cur + [k])
+    rec(n, n, [])
+    return out if n else {(0,)}
+
+
+@pytest.mark.parametrize("n", list(range(0, 13)))
+def test_accel_asc_is_every_partition_once(n):
+    got = [tuple(p) for p in accel_asc(n)]
+    assert len(got) == len(set(got))
+    assert set(got) == _brute_partitions(n)
+    assert all(list(p) == sorted(p) for p in got)
+
+
+def test_compression_and_order_dl4():
+    # accelAsc order + (part, multiplicity) pairs sorted by increasing multiplicity
+    assert sector_basis(10, 4) == [[(1, 4)], [(2, 1), (1, 2)], [(1, 1), (3, 1)], [(2, 2)], [(4, 1)]]
+    assert sector_basis(10, 0) == [[(0, 1)]]
+
+
+def test_length_cutoff():
+    # at most N parts: N = 3 drops only (1,1,1,1) from dL = 4
+    b = sector_basis(3, 4)
+    assert all(sum(r for _, r in p) <= 3 for p in b)
+    assert len(b) == 4 and [(1, 4)] not in b
+    for n in range(1, 12):
+        for N in range(1, 8):
+            assert len(sector_basis(N, n)) == sum(1 for p in _brute_partitions(n) if len(p) <= N)
+
+
+def test_compress_stability():
+    # equal multiplicities keep increasing part order
+    assert compress([1, 2, 2, 3, 3, 5]) == [(1, 1), (5, 1), (2, 2), (3, 2)]
+
+
+def test_exact_quadratic_closed_form():
+    """eps = 0: H = w (L0 + dL + N) S exactly (sum_i r_i^2 = 2 (L + N) in the LLL)."""
+    for N, m, dl in [(3, 3, 3), (3, 2, 4), (4, 2, 3)]:
+        L0 = m * N * (N - 1) // 2
+        b = Basis(N, [dl])
+        S, H = exact_blocks(N, m, b.states, 0.05, 0.0)
+        S, H = np.array(S), np.array(H)
+        assert np.allclose(H, 0.05 * (L0 + dl + N) * S, rtol=1e-12)
+
+
+def test_exact_radial_moments_by_quadrature():
+    """<r^2>_a = 2(a+1), <r^4>_a = 4(a+1)(a+2) for |z^a|^2 e^{-|z|^2/2}."""
+    from scipy.integrate import quad
+    for a in range(6):
+        w = lambda r, p: r ** (2 * a + p + 1) * math.exp(-r * r / 2)
+        n0 = quad(lambda r: w(r, 0), 0, 60)[0]
+        assert abs(quad(lambda r: w(r, 2), 0, 60)[0] / n0 - 2 * (a + 1)) < 1e-8
+        assert abs(quad(lambda r: w(r, 4), 0, 60)[0] / n0 - 4 * (a + 1) * (a + 2)) < 1e-7
+
+
+def _numpy_features(pos, N, m, states):
+    """independent FP64 evaluation of p_lambda (z / R power sums) by direct products"""
+    R = math.sqrt(2 * m * N)
+    z = (pos[..., 0] + 1j * pos[..., 1]) / R
+    out = []
+    for _, pairs in states:
+        v = np.ones(z.shape[0], dtype=complex)
+        for k, r in pairs:
+            if k == 0:
+                continue
+            v = v * np.sum(z ** k, axis=1) ** r
+        out.append(v)
+    return np.stack(out, 1)
+
+
+def test_fixed_accumulation_vs_numpy(oracle_lib):
+    N, m, sectors = 7, 3, [0, 2, 3, 5]
+    w, eps = 0.05, 1e-3
+    o = oracle_lib.Oracle(N, m, sectors, w=w, eps=eps)
+    rng = np.random.default_rng(1)
+    pos = rng.normal(size=(500, N, 2)) * 3
+    So, Ho, vs = o.accumulate_fixed(pos)
+    P = _numpy_features(pos, N, m, o.basis.states)
+    r2 = (pos ** 2).sum(-1)
+    V = (0.5 * w * r2 + eps * r2 ** 2).sum(1)
+    assert abs(vs - V.sum()) < 1e-9 * abs(V.sum())
+    off = 0
+    for s, d in enumerate(o.dims):
+        blk = P[:, off:off + d]
+        S = blk.conj().T @ blk
+        H = blk.conj().T @ (V[:, None] * blk)
+        assert np.allclose(So[s], S, rtol=1e-12, atol=1e-12 * np.abs(S).max())
+        assert np.allclose(Ho[s], H, rtol=1e-12, atol=1e-12 * np.abs(H).max())
+        off += d
+
+
+def test_geneig_vs_scipy(oracle_lib):
+    rng = np.random.default_rng(7)
+    for d in (1, 2, 3, 8, 25):
+        A = rng.normal(size=(d, d + 3)) + 1j * rng.normal(size=(d, d + 3))
+        S = A @ A.conj().T + 0.1 * np.eye(d)
+        B = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        H = B + B.conj().T
+        ev, vec = oracle_lib.geneig(S, H, vectors=True)
+        ref = scipy.linalg.eigh(H, S, eigvals_only=True)
+        assert np.allclose(ev, ref, rtol=1e-10, atol=1e-10)
+        assert np.allclose(H @ vec, S @ vec * ev[None, :], atol=1e-9 * np.abs(H).max())
+        assert np.allclose(vec.conj().T @ S @ vec, np.eye(d), atol=1e-9)
+    # a non-positive-definite metric is reported, not turned into NaN
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle_lib.geneig(np.diag([1.0, -1.0]).astype(complex), np.eye(2, dtype=complex))
+
+
+def test_dsf_weights_vs_scipy(oracle_lib):
+    rng = np.random.default_rng(11)
+    d = 6
+    A = rng.normal(size=(d + 1, d + 4)) + 1j * rng.normal(size=(d + 1, d + 4))
+    M = A @ A.conj().T
+    B = rng.normal(size=(d + 1, d + 1)) + 1j * rng.normal(size=(d + 1, d + 1))
+    H = B + B.conj().T
+    VC = rng.normal(size=(d + 1, d + 1)) + 1j * rng.normal(size=(d + 1, d + 1))
+    VS = rng.normal(size=(d + 1, d + 1)) + 1j * rng.normal(size=(d + 1, d + 1))
+    e, wts = oracle_lib.dsf_weights(H, M, VC, VS)
+    ev, vec = scipy.linalg.eigh(H[1:, 1:], M[1:, 1:])
+    v = VC[0, 1:] - 1j * VS[0, 1:]
+    assert np.allclose(e, ev)
+    assert np.allclose(wts, np.abs(v @ vec) ** 2, rtol=1e-9)
+
+
+def test_plain_chain_r2_sum_rule(oracle_lib):
+    """FP64 Metropolis chain: <sum_i r_i^2> = 2 (L0 + N) exactly."""
+    N, m = 6, 3
+    o = oracle_lib.Oracle(N, m, [0])
+    L0 = m * N * (N - 1) // 2
+    vals = []
+    for seed in range(4):
+        _, _, acc, r2, _ = o.plain_run(64, 200, 400, 0.8, 100 + seed)
+        vals.append(r2)
+    mean, err = np.mean(vals), np.std(vals, ddof=1) / 2
+    assert abs(mean - 2 * (L0 + N)) < 4 * err + 2e-3 * 2 * (L0 + N)
+
+
+def test_plain_chain_vs_exact_small_n(oracle_lib):
+    """N = 3, m = 3, dL = 2: MC eigenvalues and normalised metric vs the exact expansion."""
+    N, m, dl = 3, 3, 2
+    w, eps = 0.05, 2e-3
+    o = oracle_lib.Oracle(N, m, [dl], w=w, eps=eps)
+    Se, He = exact_blocks(N, m, o.basis.states, w, eps)
+    Se, He = np.array(Se, dtype=complex), np.array(He, dtype=complex)
+    exact = scipy.linalg.eigh(He, Se, eigvals_only=True)
+    ests = []
+    for seed in range(6):
+        Sb, Hb, _, _, _ = o.plain_run(128, 200, 1500, 0.9, 2000 + seed)
+        ests.append(oracle_lib.geneig(Sb[0], Hb[0]))
+    ests = np.array(ests)
+    mean = ests.mean(0)
+    err = ests.std(0, ddof=1) / np.sqrt(len(ests))
+    assert np.all(np.abs(mean - exact) < 5 * err + 1e-9), (mean, exact, err)
+
+
+def test_rowwise_estimator_small_n(oracle_lib):
+    """Reference row-wise estimator (|p_f Psi|^2 chains, g/f ratios, /sqrt<|g/f|^2>,
+    symmetrised) reproduces the exact normalised metric and the spectrum."""
+    N, m, dl = 3, 3, 2
+    w, eps = 0.05, 2e-3
+    o = oracle_lib.Oracle(N, m, [dl], w=w, eps=eps)
+    Se, He = exact_blocks(N, m, o.basis.states, w, eps)
+    Se, He = np.array(Se), np.array(He)
+    dn = np.sqrt(np.diag(Se))
+    Mexact = Se / np.outer(dn, dn)
+    exact = scipy.linalg.eigh(He, Se, eigvals_only=True)
+    H, M, _, _, acc = o.rowwise_sector(0, 64, 200, 1500, 0.9, 31)
+    assert np.allclose(np.diag(M).real, 1.0, atol=1e-12)      # exact unit diagonal
+    assert np.allclose(M, M.conj().T) and np.allclose(H, H.conj().T)
+    assert np.abs(M - Mexact).max() < 0.03
+    ev = oracle_lib.geneig(M, H)
+    assert np.allclose(ev, exact, rtol=0.01)
+    assert 0.2 < acc < 0.9

@@ -185,6 +185,18 @@ int le_geneig(int d, const double* S, const double* H, double* evals, double* ev
 /* per-sector eigenvalues of the engine's moments (block -1 = mean); evals has d_s entries */
 int le_engine_spectrum(le_engine* e, int sector, int block, double* evals);
 
+/* Time evolution (timeEvolution::edgeDensityResponse_compute, modules/dynamics/dynamics.cpp
+ * :149-309) of C(0) = e_0 under M dC/dt = -i (H + F(t) U_u) C for n_u perturbations at once,
+ * FP64 on the GPU.  M and H are D x D complex (only the diagonal blocks of sizes
+ * block_dims[] are used: angular momentum is conserved by them), U and rho are n_u and
+ * n_rho D x D complex matrices.  Observed at t_k = k save_step, k < ceil(end_time /
+ * save_step): out_rho[u][k][l] = C^H rho_l C (complex), out_norm[u][k] = C^H M C.
+ * Adaptive Cash-Karp 5(4) with the given tolerances (the reference: 1e-10, 1e-8). */
+typedef double (*le_time_fn)(void* ctx, double t);
+int le_evolve(int D, int n_blocks, const int* block_dims, const double* M, const double* H, int n_u, const double* U,
+              int n_rho, const double* rho, le_time_fn F, void* ctx, double end_time, double save_step,
+              double eps_abs, double eps_rel, double* out_rho, double* out_norm, long* n_steps);
+
 /* eigenvalues of every sector in one launch (block -1 = mean): evals has D entries in basis
  * order; status (optional, n_sectors): 0 ok, 1 metric not positive definite */
 int le_engine_spectra(le_engine* e, int block, double* evals, int* status);

@@ -24,6 +24,7 @@
 
 #include "../../include/laughlinedge.h"
 #include "le_basis.hpp"
+#include "le_evolve.hpp"
 #include "le_kernels.hpp"
 
 namespace {
@@ -1069,4 +1070,195 @@ extern "C" int le_engine_spectra(le_engine* e, int block, double* evals, int* st
         for (size_t q = 0; q < st.size(); ++q) status[q] = st[q];
     if (rc == LE_ERR_NOT_PD && status) return LE_OK;   // per-sector status reported
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 time evolution (interaction picture in the generalized eigenbasis)
+// ---------------------------------------------------------------------------
+extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const double* M, const double* H, int n_u,
+                         const double* U, int n_rho, const double* rho, le_time_fn F, void* ctx, double end_time,
+                         double save_step, double eps_abs, double eps_rel, double* out_rho, double* out_norm,
+                         long* n_steps)
+{
+    if (D <= 0 || n_blocks <= 0 || !block_dims || !M || !H || n_u <= 0 || !U || n_rho < 0 || (n_rho && !rho) || !F ||
+        !(save_step > 0) || !(end_time > 0) || !out_rho)
+        return fail(LE_ERR_ARG, "le_evolve: bad argument");
+    long dsum = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+        if (block_dims[b] <= 0 || block_dims[b] > 640) return fail(LE_ERR_ARG, "le_evolve: block size out of range");
+        dsum += block_dims[b];
+    }
+    if (dsum != D) return fail(LE_ERR_ARG, "le_evolve: block sizes do not add up to D");
+    const size_t DD = (size_t)D * D;
+    // 1. per-block generalized eigenproblems (FP64, GPU)
+    std::vector<double> Sb, Hb;
+    std::vector<int> dims(block_dims, block_dims + n_blocks), offs(n_blocks + 1, 0);
+    for (int b = 0; b < n_blocks; ++b) offs[b + 1] = offs[b] + dims[b];
+    for (int b = 0; b < n_blocks; ++b)
+        for (int i = 0; i < dims[b]; ++i)
+            for (int j = 0; j < dims[b]; ++j) {
+                const size_t q = (size_t)(offs[b] + i) * D + offs[b] + j;
+                Sb.push_back(M[2 * q]);
+                Sb.push_back(M[2 * q + 1]);
+                Hb.push_back(H[2 * q]);
+                Hb.push_back(H[2 * q + 1]);
+            }
+    std::vector<double> lam(D), vecs(Sb.size());
+    int rc = geneig_batch(dims, Sb.data(), Hb.data(), lam.data(), vecs.data(), nullptr);
+    if (rc) return rc;
+    std::vector<double> V(2 * DD, 0.0);     // block-diagonal, column j = eigenvector j
+    {
+        size_t o = 0;
+        for (int b = 0; b < n_blocks; ++b) {
+            const int d = dims[b];
+            for (int i = 0; i < d; ++i)
+                for (int j = 0; j < d; ++j) {
+                    const size_t q = (size_t)(offs[b] + i) * D + offs[b] + j;
+                    V[2 * q] = vecs[2 * (o + (size_t)i * d + j)];
+                    V[2 * q + 1] = vecs[2 * (o + (size_t)i * d + j) + 1];
+                }
+            o += (size_t)d * d;
+        }
+    }
+    // a(0) = V^H M e_0
+    std::vector<double> a0(2 * (size_t)D, 0.0);
+    for (int j = 0; j < D; ++j) {
+        double sr = 0, si = 0;
+        for (int k = 0; k < D; ++k) {
+            const double vr = V[2 * ((size_t)k * D + j)], vi = -V[2 * ((size_t)k * D + j) + 1];
+            const double mr = M[2 * ((size_t)k * D)], mi = M[2 * ((size_t)k * D) + 1];
+            sr += vr * mr - vi * mi;
+            si += vr * mi + vi * mr;
+        }
+        a0[2 * j] = sr;
+        a0[2 * j + 1] = si;
+    }
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const long nt = (long)std::ceil(end_time / save_step);
+    double *dV = nullptr, *dT = nullptr, *dX = nullptr, *dW = nullptr, *dR = nullptr, *dlam = nullptr, *db = nullptr,
+           *dbn = nullptr, *dks = nullptr, *dy = nullptr, *dcoef = nullptr, *dout = nullptr;
+    unsigned long long* demax = nullptr;
+    hipError_t h = hipSuccess;
+    auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
+    chk(dalloc(&dV, 2 * DD));
+    chk(dalloc(&dT, 2 * DD));
+    chk(dalloc(&dX, 2 * DD));
+    chk(dalloc(&dW, 2 * DD * n_u));
+    chk(dalloc(&dR, 2 * DD * std::max(1, n_rho)));
+    chk(dalloc(&dlam, (size_t)D));
+    chk(dalloc(&db, 2 * (size_t)D * n_u));
+    chk(dalloc(&dbn, 2 * (size_t)D * n_u));
+    chk(dalloc(&dks, 12 * (size_t)D * n_u));
+    chk(dalloc(&dy, 2 * (size_t)D * n_u));
+    chk(dalloc(&dcoef, 64));
+    chk(dalloc(&dout, 2 * (size_t)nt * n_u * (n_rho + 1)));
+    chk(dalloc(&demax, 1));
+    // 2. W_u = V^H U_u V, R_l = V^H rho_l V
+    if (h == hipSuccess) chk(hipMemcpy(dV, V.data(), sizeof(double) * 2 * DD, hipMemcpyHostToDevice));
+    auto sandwich = [&](const double* host, double* dst) {
+        chk(hipMemcpyAsync(dX, host, sizeof(double) * 2 * DD, hipMemcpyHostToDevice, st));
+        chk(le::launch_zgemm(dX, dV, dT, D, D, D, D, D, D, 0, st));     // T = X V
+        chk(le::launch_zgemm(dV, dT, dst, D, D, D, D, D, D, 1, st));    // V^H T
+        chk(hipStreamSynchronize(st));
+    };
+    for (int u = 0; u < n_u && h == hipSuccess; ++u) sandwich(U + 2 * DD * u, dW + 2 * DD * u);
+    for (int l = 0; l < n_rho && h == hipSuccess; ++l) sandwich(rho + 2 * DD * l, dR + 2 * DD * l);
+    // 3. Cash-Karp 5(4) tableau
+    static const double cc[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+    static const double aa[6][5] = {{0, 0, 0, 0, 0},
+                                    {0.2, 0, 0, 0, 0},
+                                    {3.0 / 40, 9.0 / 40, 0, 0, 0},
+                                    {0.3, -0.9, 1.2, 0, 0},
+                                    {-11.0 / 54, 2.5, -70.0 / 27, 35.0 / 27, 0},
+                                    {1631.0 / 55296, 175.0 / 512, 575.0 / 13824, 44275.0 / 110592, 253.0 / 4096}};
+    static const double b5[6] = {37.0 / 378, 0, 250.0 / 621, 125.0 / 594, 0, 512.0 / 1771};
+    static const double b4[6] = {2825.0 / 27648, 0, 18575.0 / 48384, 13525.0 / 55296, 277.0 / 14336, 0.25};
+    std::vector<double> coef(64, 0.0);
+    for (int sI = 0; sI < 6; ++sI)
+        for (int r = 0; r < 5; ++r) coef[8 * sI + r] = aa[sI][r];
+    for (int sI = 0; sI < 6; ++sI) {
+        coef[48 + sI] = b5[sI];
+        coef[56 + sI] = b5[sI] - b4[sI];
+    }
+    if (h == hipSuccess) {
+        chk(hipMemcpy(dcoef, coef.data(), sizeof(double) * 64, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dlam, lam.data(), sizeof(double) * D, hipMemcpyHostToDevice));
+        for (int u = 0; u < n_u; ++u)
+            chk(hipMemcpy(db + 2 * (size_t)D * u, a0.data(), sizeof(double) * 2 * D, hipMemcpyHostToDevice));
+        chk(hipMemset(dout, 0, sizeof(double) * 2 * (size_t)nt * n_u * (n_rho + 1)));
+    }
+    // 4. adaptive integration, observing at t_k (host drives the controller; F is a host callback)
+    const int n = D * n_u;
+    double t = 0.0, dt = save_step;
+    long steps = 0;
+    const long max_steps = 50000000;
+    for (long kt = 0; kt < nt && h == hipSuccess; ++kt) {
+        const double tk = kt * save_step;
+        while (t < tk - 1e-14 * (1.0 + std::fabs(tk)) && h == hipSuccess) {
+            double hs = dt;
+            bool last = false;
+            if (t + hs >= tk) {
+                hs = tk - t;
+                last = true;
+            }
+            for (int tries = 0;; ++tries) {
+                for (int sI = 0; sI < 6 && h == hipSuccess; ++sI) {
+                    const double ts = t + cc[sI] * hs;
+                    chk(le::launch_stage_prep(db, dks, sI, dcoef + 8 * sI, hs, ts, dlam, dy, D, n_u, st));
+                    chk(le::launch_stage_matvec(dW, dy, dks + 2 * (size_t)sI * n, F(ctx, ts), ts, dlam, D, n_u, st));
+                }
+                chk(hipMemsetAsync(demax, 0, sizeof(unsigned long long), st));
+                chk(le::launch_rk_combine(db, dks, dcoef + 48, dcoef + 56, hs, eps_abs, eps_rel, dbn, demax, n, st));
+                unsigned long long bits = 0;
+                chk(hipMemcpyAsync(&bits, demax, sizeof(bits), hipMemcpyDeviceToHost, st));
+                chk(hipStreamSynchronize(st));
+                if (h != hipSuccess) break;
+                double emax;
+                std::memcpy(&emax, &bits, sizeof(emax));
+                ++steps;
+                if (!(emax <= 1.0)) {
+                    if (tries > 200 || steps > max_steps) {
+                        h = hipErrorUnknown;
+                        break;
+                    }
+                    const double f = std::isfinite(emax) ? 0.9 * std::pow(emax, -1.0 / 3.0) : 0.2;
+                    hs *= std::max(f, 0.2);
+                    last = false;
+                    continue;
+                }
+                std::swap(db, dbn);
+                t += hs;
+                if (last) t = tk;
+                if (emax < 0.5) {
+                    const double e = std::max(emax, std::pow(5.0, -5.0));
+                    hs *= 0.9 * std::pow(e, -1.0 / 5.0);
+                }
+                if (!last || hs > dt) dt = hs;
+                break;
+            }
+        }
+        if (h == hipSuccess)
+            chk(le::launch_observe(db, dR, dlam, tk, D, n_u, n_rho, dout + 2 * (size_t)kt * n_u * (n_rho + 1), st));
+    }
+    std::vector<double> res(2 * (size_t)nt * n_u * (n_rho + 1));
+    if (h == hipSuccess) {
+        chk(hipMemcpyAsync(res.data(), dout, sizeof(double) * res.size(), hipMemcpyDeviceToHost, st));
+        chk(hipStreamSynchronize(st));
+    }
+    for (double* p : {dV, dT, dX, dW, dR, dlam, db, dbn, dks, dy, dcoef, dout}) (void)hipFree(p);
+    (void)hipFree(demax);
+    (void)hipStreamDestroy(st);
+    if (h != hipSuccess) return fail(LE_ERR_HIP, std::string("le_evolve: ") + hipGetErrorString(h));
+    for (int u = 0; u < n_u; ++u)
+        for (long kt = 0; kt < nt; ++kt) {
+            const size_t src = ((size_t)kt * n_u + u) * (n_rho + 1);
+            for (int l = 0; l < n_rho; ++l) {
+                out_rho[2 * (((size_t)u * nt + kt) * n_rho + l)] = res[2 * (src + l)];
+                out_rho[2 * (((size_t)u * nt + kt) * n_rho + l) + 1] = res[2 * (src + l) + 1];
+            }
+            if (out_norm) out_norm[(size_t)u * nt + kt] = res[2 * (src + n_rho)];
+        }
+    if (n_steps) *n_steps = steps;
+    return LE_OK;
 }

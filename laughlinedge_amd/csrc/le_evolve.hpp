@@ -1,0 +1,19 @@
+// Launchers of the FP64 time-evolution kernels (le_evolve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace le {
+
+hipError_t launch_zgemm(const double* A, const double* B, double* C, int M, int N, int K, int lda, int ldb, int ldc,
+                        int conjA, hipStream_t s);
+hipError_t launch_stage_prep(const double* b, const double* ks, int nstage, const double* coef, double h, double t,
+                             const double* lam, double* y, int D, int nu, hipStream_t s);
+hipError_t launch_stage_matvec(const double* W, const double* y, double* kout, double F, double t, const double* lam,
+                               int D, int nu, hipStream_t s);
+hipError_t launch_rk_combine(const double* b, const double* ks, const double* b5, const double* db, double h,
+                             double eps_abs, double eps_rel, double* bnew, unsigned long long* emax, int n,
+                             hipStream_t s);
+hipError_t launch_observe(const double* b, const double* R, const double* lam, double t, int D, int nu, int nrho,
+                          double* out, hipStream_t s);
+
+}  // namespace le

@@ -31,7 +31,10 @@ EXPORTED = [
     "le_get_moments", "le_get_trace", "le_get_positions", "le_set_positions",
     "le_accumulate_positions", "le_reset_moments", "le_geneig", "le_engine_spectrum",
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
+    "le_evolve",
 ]
+
+TIME_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
 
 W_ONE = 0
 W_V = 1
@@ -296,3 +299,28 @@ def geneig(S, H, vectors=False):
     if vectors:
         return ev, _cplx(vec).reshape(d, d)
     return ev
+
+
+def _pack(A):
+    return np.ascontiguousarray(np.stack([np.asarray(A).real, np.asarray(A).imag], -1).reshape(-1))
+
+
+def evolve(M, H, Us, rhos, F, end_time, save_step, block_dims, eps_abs=1e-10, eps_rel=1e-8):
+    """FP64 GPU time evolution: returns (rho[u][k][l] complex, norm[u][k], steps)."""
+    D = M.shape[0]
+    Us = list(Us)
+    rhos = list(rhos)
+    nt = int(np.ceil(end_time / save_step))
+    out = np.zeros(2 * len(Us) * nt * max(1, len(rhos)))
+    nrm = np.zeros(len(Us) * nt)
+    cb = TIME_FN(lambda ctx, t: float(F(t)))
+    dims = (ctypes.c_int * len(block_dims))(*[int(x) for x in block_dims])
+    U = np.concatenate([_pack(u) for u in Us])
+    R = np.concatenate([_pack(r) for r in rhos]) if rhos else np.zeros(2)
+    steps = ctypes.c_long()
+    _check(lib().le_evolve(ctypes.c_int(D), len(block_dims), dims, _dp(_pack(M)), _dp(_pack(H)), len(Us), _dp(U),
+                           len(rhos), _dp(R), cb, None, ctypes.c_double(end_time), ctypes.c_double(save_step),
+                           ctypes.c_double(eps_abs), ctypes.c_double(eps_rel), _dp(out), _dp(nrm),
+                           ctypes.byref(steps)))
+    rho = _cplx(out)[: len(Us) * nt * len(rhos)].reshape(len(Us), nt, len(rhos))
+    return rho, nrm.reshape(len(Us), nt), steps.value

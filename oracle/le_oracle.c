@@ -701,3 +701,170 @@ int or_dsf_weights(int dp1, const double *H, const double *M, const double *VC, 
     free(h); free(m); free(v);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Time evolution (modules/dynamics/dynamics.cpp:20-53, 73-125, 264-290):      */
+/* dC/dt = -i M^-1 (H + F(t) U) C, C(0) = e_0, integrated with an adaptive     */
+/* Cash-Karp 5(4) Runge-Kutta (controlled stepper: err_i = |xerr_i| /          */
+/* (eps_abs + eps_rel (|x_i| + dt |dxdt_i|)), reject if max > 1 with           */
+/* dt *= max(0.9 err^-1/3, 0.2), grow if max < 0.5 with dt *= 0.9 err^-1/5      */
+/* capped at 5x), observed at t_k = k save_step, k < ceil(T / save_step):       */
+/* rho_l(t) = C^H rho_l C and the norm C^H M C.                                */
+/* ------------------------------------------------------------------------ */
+typedef double (*or_time_fn)(void *ctx, double t);
+
+static void cmatvec(int D, const cplx *A, const cplx *x, cplx *y)
+{
+    for (int i = 0; i < D; ++i) {
+        cplx s = 0;
+        for (int j = 0; j < D; ++j) s += A[(size_t)i * D + j] * x[j];
+        y[i] = s;
+    }
+}
+
+/* dC = -i (Heff + F Ueff) C */
+static void rhs(int D, const cplx *Heff, const cplx *Ueff, double F, const cplx *C, cplx *out, cplx *tmp)
+{
+    cmatvec(D, Heff, C, out);
+    cmatvec(D, Ueff, C, tmp);
+    for (int i = 0; i < D; ++i) out[i] = -I * (out[i] + F * tmp[i]);
+}
+
+/* inverse of a complex matrix by Gauss-Jordan with partial pivoting (M^-1 of dynamics.cpp:25) */
+static int cinverse(int D, const cplx *A, cplx *Ainv)
+{
+    cplx *a = malloc(sizeof(cplx) * D * D);
+    memcpy(a, A, sizeof(cplx) * D * D);
+    for (int i = 0; i < D * D; ++i) Ainv[i] = 0;
+    for (int i = 0; i < D; ++i) Ainv[i * D + i] = 1;
+    for (int c = 0; c < D; ++c) {
+        int p = c;
+        for (int r = c + 1; r < D; ++r) if (cabs(a[r * D + c]) > cabs(a[p * D + c])) p = r;
+        if (cabs(a[p * D + c]) == 0.0) { free(a); return -1; }
+        if (p != c)
+            for (int k = 0; k < D; ++k) {
+                cplx t = a[c * D + k]; a[c * D + k] = a[p * D + k]; a[p * D + k] = t;
+                t = Ainv[c * D + k]; Ainv[c * D + k] = Ainv[p * D + k]; Ainv[p * D + k] = t;
+            }
+        cplx piv = a[c * D + c];
+        for (int k = 0; k < D; ++k) { a[c * D + k] /= piv; Ainv[c * D + k] /= piv; }
+        for (int r = 0; r < D; ++r) {
+            if (r == c) continue;
+            cplx f = a[r * D + c];
+            if (f == 0) continue;
+            for (int k = 0; k < D; ++k) { a[r * D + k] -= f * a[c * D + k]; Ainv[r * D + k] -= f * Ainv[c * D + k]; }
+        }
+    }
+    free(a);
+    return 0;
+}
+
+int or_evolve(int D, const double *Min, const double *Hin, const double *Uin, or_time_fn F, void *ctx,
+              double end_time, double save_step, double eps_abs, double eps_rel, int nrho, const double *rhoin,
+              double *out_rho, double *out_norm, long *n_steps)
+{
+    size_t DD = (size_t)D * D;
+    cplx *M = malloc(sizeof(cplx) * DD), *Mi = malloc(sizeof(cplx) * DD);
+    cplx *Heff = malloc(sizeof(cplx) * DD), *Ueff = malloc(sizeof(cplx) * DD), *A = malloc(sizeof(cplx) * DD);
+    for (size_t q = 0; q < DD; ++q) M[q] = Min[2 * q] + I * Min[2 * q + 1];
+    if (cinverse(D, M, Mi)) return -1;
+    for (size_t q = 0; q < DD; ++q) A[q] = Hin[2 * q] + I * Hin[2 * q + 1];
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            cplx s = 0;
+            for (int k = 0; k < D; ++k) s += Mi[(size_t)i * D + k] * A[(size_t)k * D + j];
+            Heff[(size_t)i * D + j] = s;
+        }
+    for (size_t q = 0; q < DD; ++q) A[q] = Uin[2 * q] + I * Uin[2 * q + 1];
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            cplx s = 0;
+            for (int k = 0; k < D; ++k) s += Mi[(size_t)i * D + k] * A[(size_t)k * D + j];
+            Ueff[(size_t)i * D + j] = s;
+        }
+    cplx *C = calloc(D, sizeof(cplx)), *k[6], *y = malloc(sizeof(cplx) * D), *tmp = malloc(sizeof(cplx) * D);
+    cplx *Cn = malloc(sizeof(cplx) * D), *err = malloc(sizeof(cplx) * D);
+    for (int s = 0; s < 6; ++s) k[s] = malloc(sizeof(cplx) * D);
+    C[0] = 1;
+    static const double c[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+    static const double a[6][5] = {{0, 0, 0, 0, 0},
+                                   {0.2, 0, 0, 0, 0},
+                                   {3.0 / 40, 9.0 / 40, 0, 0, 0},
+                                   {0.3, -0.9, 1.2, 0, 0},
+                                   {-11.0 / 54, 2.5, -70.0 / 27, 35.0 / 27, 0},
+                                   {1631.0 / 55296, 175.0 / 512, 575.0 / 13824, 44275.0 / 110592, 253.0 / 4096}};
+    static const double b5[6] = {37.0 / 378, 0, 250.0 / 621, 125.0 / 594, 0, 512.0 / 1771};
+    static const double b4[6] = {2825.0 / 27648, 0, 18575.0 / 48384, 13525.0 / 55296, 277.0 / 14336, 0.25};
+    long nt = (long)ceil(end_time / save_step);
+    double t = 0.0, dt = save_step;
+    long steps = 0;
+    cplx *rho = malloc(sizeof(cplx) * DD);
+    for (long kt = 0; kt < nt; ++kt) {
+        double tk = kt * save_step;
+        /* advance to tk */
+        while (t < tk - 1e-14 * (1.0 + fabs(tk))) {
+            double h = dt;
+            int last = 0;
+            if (t + h >= tk) { h = tk - t; last = 1; }
+            for (;;) {
+                rhs(D, Heff, Ueff, F(ctx, t), C, k[0], tmp);
+                for (int s = 1; s < 6; ++s) {
+                    for (int i = 0; i < D; ++i) {
+                        cplx v = C[i];
+                        for (int r = 0; r < s; ++r) v += h * a[s][r] * k[r][i];
+                        y[i] = v;
+                    }
+                    rhs(D, Heff, Ueff, F(ctx, t + c[s] * h), y, k[s], tmp);
+                }
+                double emax = 0.0;
+                for (int i = 0; i < D; ++i) {
+                    cplx v = C[i], e = 0;
+                    for (int s = 0; s < 6; ++s) { v += h * b5[s] * k[s][i]; e += h * (b5[s] - b4[s]) * k[s][i]; }
+                    Cn[i] = v;
+                    err[i] = e;
+                    double r = cabs(e) / (eps_abs + eps_rel * (cabs(C[i]) + fabs(h) * cabs(k[0][i])));
+                    if (r > emax) emax = r;
+                }
+                ++steps;
+                if (emax > 1.0) {
+                    double f = 0.9 * pow(emax, -1.0 / 3.0);
+                    h *= (f > 0.2 ? f : 0.2);
+                    last = 0;
+                    continue;
+                }
+                memcpy(C, Cn, sizeof(cplx) * D);
+                t += h;
+                if (last) t = tk;
+                if (emax < 0.5) {
+                    double e = emax > pow(5.0, -5.0) ? emax : pow(5.0, -5.0);
+                    h *= 0.9 * pow(e, -1.0 / 5.0);
+                }
+                if (!last || h > dt) dt = h;
+                break;
+            }
+        }
+        /* observe */
+        double nrm = 0;
+        for (int i = 0; i < D; ++i) {
+            cplx s = 0;
+            for (int j = 0; j < D; ++j) s += M[(size_t)i * D + j] * C[j];
+            nrm += creal(conj(C[i]) * s);
+        }
+        if (out_norm) out_norm[kt] = nrm;
+        for (int l = 0; l < nrho; ++l) {
+            for (size_t q = 0; q < DD; ++q) rho[q] = rhoin[2 * (l * DD + q)] + I * rhoin[2 * (l * DD + q) + 1];
+            cplx s = 0;
+            for (int i = 0; i < D; ++i) {
+                cplx r = 0;
+                for (int j = 0; j < D; ++j) r += rho[(size_t)i * D + j] * C[j];
+                s += conj(C[i]) * r;
+            }
+            out_rho[2 * (kt * nrho + l)] = creal(s);
+            out_rho[2 * (kt * nrho + l) + 1] = cimag(s);
+        }
+    }
+    if (n_steps) *n_steps = steps;
+    free(M); free(Mi); free(Heff); free(Ueff); free(A); free(C); free(y); free(tmp); free(Cn); free(err); free(rho);
+    for (int s = 0; s < 6; ++s) free(k[s]);
+    return 0;
+}

@@ -185,3 +185,26 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     lib().or_philox4x32_10(c, k, o)
     return list(o)
+
+
+_TIME_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
+
+
+def evolve(M, H, U, rhos, F, end_time, save_step, eps_abs=1e-10, eps_rel=1e-8):
+    """FP64 CPU RK-CK54 restatement of the reference's time evolution (one perturbation)."""
+    pack = lambda A: np.ascontiguousarray(np.stack([np.asarray(A).real, np.asarray(A).imag], -1).reshape(-1))
+    D = M.shape[0]
+    nt = int(np.ceil(end_time / save_step))
+    nr = len(rhos)
+    out = np.zeros(2 * nt * max(1, nr))
+    nrm = np.zeros(nt)
+    R = np.concatenate([pack(r) for r in rhos]) if rhos else np.zeros(2)
+    cb = _TIME_FN(lambda ctx, t: float(F(t)))
+    steps = ctypes.c_long()
+    rc = lib().or_evolve(ctypes.c_int(D), _dp(pack(M)), _dp(pack(H)), _dp(pack(U)), cb, None,
+                         ctypes.c_double(end_time), ctypes.c_double(save_step), ctypes.c_double(eps_abs),
+                         ctypes.c_double(eps_rel), ctypes.c_int(nr), _dp(R), _dp(out), _dp(nrm),
+                         ctypes.byref(steps))
+    if rc != 0:
+        raise np.linalg.LinAlgError("singular metric")
+    return (out[0::2] + 1j * out[1::2])[: nt * nr].reshape(nt, nr), nrm, steps.value

@@ -1,0 +1,34 @@
+// Drives the drop-in C++ interface (include/edge_montecarlo.h) the way the reference's
+// main.cu snippets do, with a fixed seed, and prints machine-readable results.
+#include <cstdio>
+
+#include "edge_montecarlo.h"
+
+int main()
+{
+    edgeMC::monteCarloParameters p;
+    p.setGPU_threads_blocks(64, 8);      // 512 walkers
+    p.setMCSamples(400);
+    p.setBurnIn(200);
+    p.seed = 1234;
+
+    std::vector<double> ev = edgeMC::spectrum_compute(p, 4, true, 7);
+    std::printf("SPECTRUM");
+    for (double e : ev) std::printf(" %.12e", e);
+    std::printf("\nSTEP %.9e\nDIMS %d %d\n", p.randomStep, p.subspaceDimension, p.maxDegree);
+
+    auto dsf = edgeMC::dsf_compute(p, 3, true, 7);
+    std::printf("DSF");
+    for (auto& x : dsf) std::printf(" %.12e %.12e", x.first, x.second);
+    std::printf("\n");
+
+    auto U = [] __device__(complex<float>* position) {
+        double u = 0;
+        for (int i = 0; i < sys_params::particlesNumber; i++) u += pow(re(position[i]) / sys_params::RCl, 2);
+        return u;
+    };
+    vector<int> sectors = {0, 2, 4};
+    edgeMC::excitationResponse_compute(p, sectors, U, 0);
+    std::printf("DONE\n");
+    return 0;
+}

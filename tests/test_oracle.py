@@ -214,3 +214,56 @@ def test_rowwise_estimator_small_n(oracle_lib):
     ev = oracle_lib.geneig(M, H)
     assert np.allclose(ev, exact, rtol=0.01)
     assert 0.2 < acc < 0.9
+
+
+def _block_system(rng, dims, E0=7.0, spread=0.3):
+    D = sum(dims)
+    M = np.zeros((D, D), complex)
+    H = np.zeros((D, D), complex)
+    o = 0
+    for d in dims:
+        A = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        Mb = A @ A.conj().T / d + np.eye(d)
+        B = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        Hb = E0 * Mb + spread * (B + B.conj().T) / 2
+        M[o:o + d, o:o + d] = Mb
+        H[o:o + d, o:o + d] = Hb
+        o += d
+    B = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    U = (B + B.conj().T) / 2
+    rhos = [rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D)) for _ in range(2)]
+    return M, H, U, rhos
+
+
+def test_evolve_oracle_free_closed_form(oracle_lib):
+    """F = 0: C(t) = sum_j e^{-i E_j t} v_j (v_j^H M e_0); norm conserved (dynamics.cpp:73-83)."""
+    rng = np.random.default_rng(2)
+    M, H, U, rhos = _block_system(rng, [1, 2, 3])
+    rho, nrm, _ = oracle_lib.evolve(M, H, U, rhos, lambda t: 0.0, 3.0, 0.25)
+    E, V = scipy.linalg.eigh(H, M)
+    a0 = V.conj().T @ M[:, 0]
+    for k, t in enumerate(np.arange(12) * 0.25):
+        C = V @ (np.exp(-1j * E * t) * a0)
+        for l, R in enumerate(rhos):
+            assert abs(rho[k, l] - C.conj() @ R @ C) < 1e-6 * max(1, abs(C.conj() @ R @ C))
+        assert abs(nrm[k] - M[0, 0].real) < 1e-6   # C^H M C is conserved (= M_00 at t = 0)
+
+
+def test_evolve_oracle_driven_vs_dense_expm(oracle_lib):
+    """Driven case against a fine-grained exact propagator (product of matrix exponentials
+    at midpoints, step 1e-3) - an independent integrator."""
+    rng = np.random.default_rng(4)
+    M, H, U, rhos = _block_system(rng, [1, 2, 2])
+    F = lambda t: 0.3 * np.exp(-(t - 1.0) ** 2 / (2 * 0.3 ** 2))
+    rho, nrm, _ = oracle_lib.evolve(M, H, U, rhos, F, 2.0, 0.5)
+    Mi = np.linalg.inv(M)
+    C = np.zeros(M.shape[0], complex)
+    C[0] = 1
+    t, h = 0.0, 1e-3
+    for k in range(4):
+        while t < 0.5 * k - 1e-12:
+            C = scipy.linalg.expm(-1j * h * Mi @ (H + F(t + h / 2) * U)) @ C
+            t += h
+        for l, R in enumerate(rhos):
+            ref = C.conj() @ R @ C
+            assert abs(rho[k, l] - ref) < 2e-5 * max(1.0, abs(ref))

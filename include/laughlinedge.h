@@ -73,7 +73,7 @@ typedef struct le_mc_params {
     double acceptance_tol;     /* acceptance_tol */
     uint64_t seed;
     int precision;             /* LE_PREC_* */
-    int lanes_per_walker;      /* 0: autotune; else 2,4,8,16,32,64 */
+    int lanes_per_walker;      /* 0: smallest power of two >= N/16; -1: timed autotune; else 2..64 */
     int samples_per_launch;    /* kept samples per sweep+feature+Gram round (0: auto) */
     int n_blocks;              /* equal time blocks for error bars (>= 1) */
     int trace_walkers;         /* walkers whose p_1..p_4, V series are recorded */
@@ -100,7 +100,7 @@ typedef struct le_observables {
 #define LE_W_V 1                        /* confinement H */
 #define LE_W_COS(L) (0x100 | (L))       /* sum_i cos(L theta_i) */
 #define LE_W_SIN(L) (0x200 | (L))       /* sum_i sin(L theta_i) */
-#define LE_W_USER(k) (0x300 | (k))      /* k-th user weight */
+#define LE_W_USER(k) (0x10000 + (k))    /* k-th user weight */
 
 /* User weight callback, called on the engine's stream after each kept-sample sweep:
  * positions: device, walkers x N complex (float when positions_are_double == 0, else
@@ -196,6 +196,15 @@ typedef double (*le_time_fn)(void* ctx, double t);
 int le_evolve(int D, int n_blocks, const int* block_dims, const double* M, const double* H, int n_u, const double* U,
               int n_rho, const double* rho, le_time_fn F, void* ctx, double end_time, double save_step,
               double eps_abs, double eps_rel, double* out_rho, double* out_norm, long* n_steps);
+
+/* Occupation numbers <L| a^dagger_l a_l |R> for all pairs L, R of sector dL and l < lcut
+ * (greenFunctionMC::occupation_numbers_compute, gf_occupation_numbers.cu:309-562): one
+ * two-sided chain on |Psi(z_0..z_{N-1})| |Psi(z_1..z_N)| (positions hold N + 1 particles)
+ * serves every pair.  le_run / le_sample / le_accumulate_positions work as for the edge
+ * engine; le_get_occupation returns the raw means O[a][b][l] (d x d x lcut complex). */
+int le_engine_create_occupation(const le_system_params* sys, const le_mc_params* mc, int dL, int lcut,
+                                le_engine** out);
+int le_get_occupation(const le_engine* e, int block, double* out);
 
 /* eigenvalues of every sector in one launch (block -1 = mean): evals has D entries in basis
  * order; status (optional, n_sectors): 0 ok, 1 metric not positive definite */

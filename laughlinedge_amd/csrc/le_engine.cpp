@@ -52,6 +52,15 @@ hipError_t dalloc(T** p, size_t count)
     return hipMalloc((void**)p, count * sizeof(T));
 }
 
+// Default walker mapping: the smallest power of two >= N / 16 lanes per walker (about 13-25
+// pair slots per lane), in [2, 64].  Deterministic, so a seed fixes the chain.
+int default_lanes(int N)
+{
+    int g = 2;
+    while (g < 64 && g * 16 < N) g <<= 1;
+    return g;
+}
+
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec) -> G
 
@@ -83,6 +92,11 @@ struct le_engine {
     int nh = 0;
     int harm[le::kMaxHarmonics] = {0};
     int n_user = 0;
+    bool two_sided = false;          // occupation-number chain (N + 1 coordinates)
+    int lcut = 0;
+    double* lnfact_d = nullptr;
+    int* side_d = nullptr;
+    int npos() const { return sys.particles_number + (two_sided ? 1 : 0); }
     le_weight_fn wfn = nullptr;
     void* wctx = nullptr;
     struct Job {
@@ -130,7 +144,33 @@ struct le_engine {
     int close_block(long long samples);
     int gram_rows(long rows);
     double measure_acceptance(double trial_step, int attempt, int* rc);
+    hipError_t features2(const void* positions, int pos_is_double, int nsamp, long row0);
 };
+
+hipError_t le_engine::features2(const void* positions, int pos_is_double, int nsamp, long row0)
+{
+    le::Features2Args f{};
+    f.pos = positions;
+    f.pos_is_double = pos_is_double;
+    f.X = X;
+    f.ldx = ldx;
+    f.W = (char*)Wb + (size_t)(1 + 2 * nh) * wrows * rsz;   // user rows: (Re w_l, Im w_l)
+    f.wld = wrows;
+    f.row0 = row0;
+    f.N = sys.particles_number;
+    f.m = sys.inverse_filling;
+    f.walkers = nsamp;
+    f.D = basis.D;
+    f.K = basis.K;
+    f.lcut = lcut;
+    f.fac = fac;
+    f.col_re = col_re;
+    f.col_dim = col_dim;
+    f.side = side_d;
+    f.ln2pifact = lnfact_d;
+    f.invR = 1.0 / R;
+    return le::launch_features2(f, prec, stream);
+}
 
 le_engine::~le_engine()
 {
@@ -148,6 +188,8 @@ le_engine::~le_engine()
     }
     (void)hipFree(items_d);
     (void)hipFree(rec_d);
+    (void)hipFree(lnfact_d);
+    (void)hipFree(side_d);
     (void)hipFree(trace_d);
     (void)hipFree(eig_work);
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -182,12 +224,15 @@ const void* le_engine::weight_ptr(int wid) const
     auto row = [&](long r) { return (const void*)((const char*)Wb + (size_t)r * wrows * rsz_); };
     if (wid == LE_W_ONE) return nullptr;
     if (wid == LE_W_V) return row(0);
+    if (wid >= LE_W_USER(0)) {
+        const int k = wid - LE_W_USER(0);
+        return k < n_user ? row(1 + 2 * nh + k) : nullptr;
+    }
     const int kind = wid >> 8, par = wid & 0xff;
     if (kind == 1 || kind == 2) {
         for (int h = 0; h < nh; ++h)
             if (harm[h] == par) return row(1 + 2 * h + (kind == 2 ? 1 : 0));
     }
-    if (kind == 3 && par < n_user) return row(1 + 2 * nh + par);
     return nullptr;
 }
 
@@ -240,14 +285,20 @@ int le_engine::setup(const le_observables* obs)
     for (int a = 0; a < ns; ++a)
         for (int b = a; b < ns; ++b)
             if (basis.dim(a) > 0 && basis.dim(b) > 0) all.push_back({a, b});
-    add_job(LE_W_ONE, LE_W_V, 2, diag);
-    for (int h = 0; h < nh; ++h) {
+    if (two_sided) {
+        // occupation weights (Re w_l, Im w_l) between the left (0) and right (1) copies
+        std::vector<std::pair<int, int>> lr = {{0, 1}};
+        for (int k = 0; k < n_user; k += 2) add_job(LE_W_USER(k), LE_W_USER(k + 1), 2, lr);
+    } else {
+        add_job(LE_W_ONE, LE_W_V, 2, diag);
+    }
+    for (int h = 0; h < nh && !two_sided; ++h) {
         std::vector<std::pair<int, int>> bl;
         for (auto& pr : all)
             if (std::abs(basis.sectors[pr.second] - basis.sectors[pr.first]) == harm[h]) bl.push_back(pr);
         if (!bl.empty()) add_job(LE_W_COS(harm[h]), LE_W_SIN(harm[h]), 2, bl);
     }
-    for (int k = 0; k < n_user; k += 2) {
+    for (int k = 0; k < n_user && !two_sided; k += 2) {
         if (k + 1 < n_user) add_job(LE_W_USER(k), LE_W_USER(k + 1), 2, all);
         else add_job(LE_W_USER(k), 0, 1, all);
     }
@@ -301,7 +352,17 @@ int le_engine::setup(const le_observables* obs)
     const long rows = (long)spl * mc.walkers;
     wrows = rows;
 
-    HIPCHK(hipMalloc(&pos, (size_t)mc.walkers * N * 2 * rsz));
+    HIPCHK(hipMalloc(&pos, (size_t)mc.walkers * npos() * 2 * rsz));
+    if (two_sided) {
+        std::vector<double> lf(lcut);
+        for (int l = 0; l < lcut; ++l) lf[l] = std::lgamma((double)l + 1.0) + std::log(2.0 * M_PI);
+        std::vector<int> side(D);
+        for (int t = 0; t < D; ++t) side[t] = t >= basis.sector_offset[1] ? 1 : 0;
+        HIPCHK(dalloc(&lnfact_d, (size_t)std::max(1, lcut)));
+        HIPCHK(hipMemcpy(lnfact_d, lf.data(), sizeof(double) * lcut, hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&side_d, (size_t)std::max(1, D)));
+        HIPCHK(hipMemcpy(side_d, side.data(), sizeof(int) * D, hipMemcpyHostToDevice));
+    }
     HIPCHK(dalloc(&acc, (size_t)mc.walkers));
     HIPCHK(hipMalloc(&X, (size_t)rows * ldx * rsz));
     HIPCHK(hipMemsetAsync(X, 0, (size_t)rows * ldx * rsz, stream));
@@ -347,8 +408,8 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
     if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64)
         return fail(LE_ERR_ARG, "unknown precision");
-    if (mc->lanes_per_walker != 0 && !le::sweep_supported(mc->lanes_per_walker))
-        return fail(LE_ERR_ARG, "lanes_per_walker must be 0, 2, 4, 8, 16, 32 or 64");
+    if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
+        return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -382,7 +443,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         g_err = msg;
         return rc;
     }
-    if (mc->lanes_per_walker) e->G = mc->lanes_per_walker;
+    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number);
     *out = e;
     return LE_OK;
 }
@@ -467,7 +528,8 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     a.walkers = mc.walkers;
     a.nsweeps = nsweeps;
     a.step = step;
-    HIPCHK(le::launch_sweep(a, prec, G, stream));
+    if (two_sided) HIPCHK(le::launch_sweep2(a, prec, G, stream));
+    else HIPCHK(le::launch_sweep(a, prec, G, stream));
     return LE_OK;
 }
 
@@ -485,7 +547,7 @@ int le_engine::autotune_lanes()
         }
     }
     void* scratch = nullptr;
-    const size_t bytes = (size_t)mc.walkers * N * 2 * rsz;
+    const size_t bytes = (size_t)mc.walkers * npos() * 2 * rsz;
     HIPCHK(hipMalloc(&scratch, bytes));
     unsigned* acc_save = acc;
     unsigned* acc_tmp = nullptr;
@@ -498,7 +560,8 @@ int le_engine::autotune_lanes()
     HIPCHK(hipEventCreate(&e1));
     for (int cand : {4, 8, 16, 32, 64}) {
         if ((N + cand - 1) / cand < 2 && cand > 4) continue;
-        if (le::sweep_lds_bytes(prec, cand, N) > 160 * 1024) continue;
+        if ((two_sided ? le::sweep2_lds_bytes(prec, cand, N) : le::sweep_lds_bytes(prec, cand, N)) > 160 * 1024)
+            continue;
         G = cand;
         HIPCHK(hipMemcpyAsync(scratch, pos, bytes, hipMemcpyDeviceToDevice, stream));
         int rc = run_sweeps(1, 1ull << 46, scratch);   // warm-up, separate RNG range
@@ -529,14 +592,13 @@ int le_engine::autotune_lanes()
 extern "C" int le_init_walkers(le_engine* e)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
-    HIPCHK(le::launch_init_positions(e->pos, e->prec, e->sys.particles_number, e->mc.walkers, e->R, e->mc.seed,
-                                     e->stream));
+    HIPCHK(le::launch_init_positions(e->pos, e->prec, e->npos(), e->mc.walkers, e->R, e->mc.seed, e->stream));
     HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->walkers_ready = true;
     e->sweep_counter = 0;
     e->sweeps_done = 0;
-    if (e->mc.lanes_per_walker == 0) {
+    if (e->mc.lanes_per_walker < 0) {
         int rc = e->autotune_lanes();
         if (rc) return rc;
     }
@@ -548,7 +610,7 @@ double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
     // fresh positions and tune_sweeps sweeps, as mcmc_acceptanceBisection (em_bisection.cu:7-47)
     *rc = LE_OK;
     step = trial_step;
-    hipError_t h = le::launch_init_positions(pos, prec, sys.particles_number, mc.walkers, R, mc.seed, stream);
+    hipError_t h = le::launch_init_positions(pos, prec, npos(), mc.walkers, R, mc.seed, stream);
     if (h == hipSuccess) h = hipMemsetAsync(acc, 0, sizeof(unsigned) * mc.walkers, stream);
     if (h != hipSuccess) {
         *rc = fail(LE_ERR_HIP, hipGetErrorString(h));
@@ -565,7 +627,7 @@ double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
     }
     double tot = 0;
     for (unsigned v : h_acc) tot += v;
-    return tot / ((double)mc.walkers * mc.tune_sweeps * sys.particles_number);
+    return tot / ((double)mc.walkers * mc.tune_sweeps * npos());
 }
 
 // Secant/bracketing search of em_bisection.cu:80-150, restated; the reference loops
@@ -731,8 +793,9 @@ extern "C" int le_sample(le_engine* e, int samples)
                 f.trace = nullptr;
                 f.trace_walkers = 0;
             }
-            HIPCHK(le::launch_features(f, e->prec, e->stream));
-            if (e->wfn && e->n_user) {
+            if (e->two_sided) HIPCHK(e->features2(e->pos, 0, e->mc.walkers, f.row0));
+            else HIPCHK(le::launch_features(f, e->prec, e->stream));
+            if (e->wfn && e->n_user && !e->two_sided) {
                 void* wdst = (char*)e->Wb + ((size_t)(1 + 2 * e->nh) * e->wrows + f.row0) * e->rsz;
                 if (e->wfn(e->wctx, e->pos, e->prec == LE_PREC_FP64, e->mc.walkers, e->sys.particles_number, wdst,
                            e->wrows, e->prec == LE_PREC_FP64, (void*)e->stream) != 0)
@@ -768,7 +831,7 @@ extern "C" int le_sample(le_engine* e, int samples)
     double tot = 0;
     for (unsigned v : h) tot += v;
     e->acc_sampling += (unsigned long long)tot;
-    e->acc_moves += (long long)samples * e->mc.thin * e->sys.particles_number * e->mc.walkers;
+    e->acc_moves += (long long)samples * e->mc.thin * e->npos() * e->mc.walkers;
     e->st.acceptance = (double)e->acc_sampling / (double)std::max<long long>(1, e->acc_moves);
     e->st.samples += nsamp;
     e->st.sweeps = e->sweeps_done;
@@ -894,7 +957,7 @@ extern "C" int le_get_trace(const le_engine* e, float* out, long capacity, long*
 extern "C" int le_get_positions(le_engine* e, double* out)
 {
     if (!e || !out) return fail(LE_ERR_ARG, "null argument");
-    const size_t n = (size_t)e->mc.walkers * e->sys.particles_number * 2;
+    const size_t n = (size_t)e->mc.walkers * e->npos() * 2;
     if (e->prec == LE_PREC_FP64) {
         HIPCHK(hipMemcpy(out, e->pos, n * sizeof(double), hipMemcpyDeviceToHost));
     } else {
@@ -908,7 +971,7 @@ extern "C" int le_get_positions(le_engine* e, double* out)
 extern "C" int le_set_positions(le_engine* e, const double* in)
 {
     if (!e || !in) return fail(LE_ERR_ARG, "null argument");
-    const size_t n = (size_t)e->mc.walkers * e->sys.particles_number * 2;
+    const size_t n = (size_t)e->mc.walkers * e->npos() * 2;
     if (e->prec == LE_PREC_FP64) {
         HIPCHK(hipMemcpy(e->pos, in, n * sizeof(double), hipMemcpyHostToDevice));
     } else {
@@ -934,7 +997,7 @@ extern "C" int le_reset_moments(le_engine* e)
 extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
 {
     if (!e || !pos || n <= 0) return fail(LE_ERR_ARG, "null argument or n <= 0");
-    const int N = e->sys.particles_number;
+    const int N = e->npos();
     const long chunk = (long)e->spl * e->mc.walkers;
     double* stage = nullptr;
     HIPCHK(dalloc(&stage, (size_t)std::min(chunk, n) * N * 2));
@@ -963,9 +1026,9 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
                                       hipMemcpyHostToDevice, e->stream);
         if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
         f.walkers = (int)m;
-        h = le::launch_features(f, e->prec, e->stream);
+        h = e->two_sided ? e->features2(stage, 1, (int)m, 0) : le::launch_features(f, e->prec, e->stream);
         if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
-        if (e->wfn && e->n_user) {
+        if (e->wfn && e->n_user && !e->two_sided) {
             void* wdst = (char*)e->Wb + (size_t)(1 + 2 * e->nh) * e->wrows * e->rsz;
             if (e->wfn(e->wctx, stage, 1, (int)m, N, wdst, e->wrows, e->prec == LE_PREC_FP64, (void*)e->stream) != 0) {
                 rc = fail(LE_ERR_STATE, "user weight callback failed");
@@ -1260,5 +1323,63 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
             if (out_norm) out_norm[(size_t)u * nt + kt] = res[2 * (src + n_rho)];
         }
     if (n_steps) *n_steps = steps;
+    return LE_OK;
+}
+
+extern "C" int le_engine_create_occupation(const le_system_params* sys, const le_mc_params* mc, int dL, int lcut,
+                                           le_engine** out)
+{
+    if (!sys || !mc || !out) return fail(LE_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (lcut < 1 || lcut > 4096) return fail(LE_ERR_ARG, "lcut must be in [1, 4096]");
+    if (sys->particles_number < 2 || sys->inverse_filling < 1)
+        return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
+    if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
+        return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
+    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64) return fail(LE_ERR_ARG, "unknown precision");
+    if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
+        return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
+    if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
+    auto e = new le_engine();
+    e->sys = *sys;
+    e->mc = *mc;
+    e->prec = mc->precision;
+    e->two_sided = true;
+    e->lcut = lcut;
+    e->n_user = 2 * lcut;
+    e->basis.build(sys->particles_number, {dL, dL});
+    if (e->basis.K > 24 || e->basis.longest > le::kMaxFactors || e->basis.dim(0) > 640) {
+        delete e;
+        return fail(LE_ERR_CAPACITY, "sector beyond the engine's capacity (degree 24, 8 factors, 640 states)");
+    }
+    int rc = e->setup(nullptr);
+    if (rc != LE_OK) {
+        std::string msg = g_err;
+        delete e;
+        g_err = msg;
+        return rc;
+    }
+    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number);
+    *out = e;
+    return LE_OK;
+}
+
+extern "C" int le_get_occupation(const le_engine* e, int block, double* out)
+{
+    if (!e || !out) return fail(LE_ERR_ARG, "null argument");
+    if (!e->two_sided) return fail(LE_ERR_STATE, "not an occupation-number engine");
+    const int d = e->basis.dim(0);
+    const size_t dd = (size_t)d * d;
+    std::vector<double> br(2 * dd), bi(2 * dd);
+    for (int l = 0; l < e->lcut; ++l) {
+        int rc = le_get_block(e, block, LE_W_USER(2 * l), 0, 1, br.data());
+        if (rc) return rc;
+        rc = le_get_block(e, block, LE_W_USER(2 * l + 1), 0, 1, bi.data());
+        if (rc) return rc;
+        for (size_t q = 0; q < dd; ++q) {   // O_l = B[Re w_l] + i B[Im w_l]
+            out[2 * (q * e->lcut + l)] = br[2 * q] - bi[2 * q + 1];
+            out[2 * (q * e->lcut + l) + 1] = br[2 * q + 1] + bi[2 * q];
+        }
+    }
     return LE_OK;
 }

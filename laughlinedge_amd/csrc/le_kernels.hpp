@@ -45,6 +45,27 @@ struct FeatureArgs {
     long trace_row0;
 };
 
+// Two-sided chain features (occupation numbers): positions [walkers][N+1]; bulk power
+// sums over z_1..z_{N-1}; left monomials (sector copy 0) use + z_0^k, right (copy 1)
+// + z_N^k; weights w_l = e^{i phi} phi_l(z_0) conj(phi_l(z_N)), l < lcut, as rows
+// (Re w_l, Im w_l) = (2 l, 2 l + 1) of W.
+struct Features2Args {
+    const void* pos;
+    int pos_is_double;
+    void* X;
+    long ldx;
+    void* W;
+    long wld;
+    long row0;
+    int N, m, walkers, D, K, lcut;
+    const uint32_t* fac;     // [D][8]
+    const int* col_re;
+    const int* col_dim;
+    const int* side;         // [D] 0: left power sums, 1: right
+    const double* ln2pifact; // [lcut] ln(2 pi l!)
+    double invR;
+};
+
 struct Tile {
     int i0, j0;              // column origins (multiples of 64) of an upper-triangle tile
 };
@@ -96,6 +117,9 @@ enum Prec { kFP32 = 0, kFP64 = 1 };
 hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
                                  unsigned long long seed, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+// two-sided chain: a.N is the number of particles per state (positions hold N + 1)
+hipError_t launch_sweep2(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t s);
@@ -105,6 +129,7 @@ hipError_t launch_geneig(const double* S, const double* H, double* work,
 hipError_t launch_zero(double* p, long n, hipStream_t s);
 
 size_t sweep_lds_bytes(int prec, int lanes_per_walker, int N);
+size_t sweep2_lds_bytes(int prec, int lanes_per_walker, int N);
 bool sweep_supported(int lanes_per_walker);
 
 }  // namespace le

@@ -1,0 +1,357 @@
+// Two-sided Metropolis chain and its features, for the occupation numbers
+// <L| a^dagger_l a_l |R> (reference: greenFunctionMC, modules/green_function/green_function.cu:199-373
+// and gf_occupation_numbers.cu:16-239).
+//
+// The reference runs one chain per (L, R) pair on |p_L Psi(z_0..z_{N-1})| |p_R Psi(z_1..z_N)|.
+// Here one chain samples |Psi(z_0..z_{N-1})| |Psi(z_1..z_N)| (Psi the Laughlin state) for
+// every pair at once; p_L, p_R and the phase enter the accumulated observable:
+//   O_ab(l) = < conj(p_a(z_0..z_{N-1})) p_b(z_1..z_N) e^{i phi} phi_l(z_0) conj(phi_l(z_N)) >,
+//   e^{i phi} = (q / |q|)^m,  q = prod_{i=1}^{N-1} (z_i - z_N) conj(z_0 - z_i),
+//   phi_l(z_0) conj(phi_l(z_N)) = (z_0 conj(z_N) / 2)^l / (2 pi l!) e^{-(|z_0|^2 + |z_N|^2)/4}.
+// Move order per sweep: particle 0, particles 1..N-1, particle N (gf_occupation_numbers.cu:53-201).
+// log pi = m sum_{0<=i<j<=N-1} log|z_i - z_j| + m sum_{1<=i<j<=N} log|z_i - z_j|
+//          - (1/4) sum_{i<N} |z_i|^2 - (1/4) sum_{i>0} |z_i|^2.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "le_kernels.hpp"
+
+namespace le {
+
+__device__ __forceinline__ void philox2(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                        uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+template <typename Real> struct V2T2;
+template <> struct V2T2<float> { using T = float2; };
+template <> struct V2T2<double> { using T = double2; };
+
+__device__ __forceinline__ double u01d2(uint32_t x) { return ((double)x + 1.0) * 2.3283064365386963e-10; }
+
+template <typename Real>
+__device__ __forceinline__ void normals(uint32_t a, uint32_t b, Real& n1, Real& n2)
+{
+    double r = sqrt(-2.0 * log(u01d2(a)));
+    double s, c;
+    sincos(6.283185307179586 * u01d2(b), &s, &c);
+    n1 = (Real)(r * c);
+    n2 = (Real)(r * s);
+}
+
+template <typename Real>
+__device__ __forceinline__ Real lg2(Real x)
+{
+    if constexpr (sizeof(Real) == 4) return __builtin_amdgcn_logf(fmaxf(x, 1e-30f));
+    else return log2(fmax(x, 1e-300));
+}
+
+// Bulk particles 1..N-1 sit in LDS slots 0..N-2 (lane l owns slots l + G s); z_0 and z_N
+// live in registers of every lane of the group.
+template <int G, typename Real>
+__global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
+{
+    using V2 = typename V2T2<Real>::T;
+    extern __shared__ __align__(16) unsigned char smem2[];
+    V2* lds = reinterpret_cast<V2*>(smem2);
+    constexpr int WPB = 64 / G;
+    const int N = a.N;
+    const int NB = N - 1;
+    const int S = (NB + G - 1) / G;
+    const int NP = S * G;
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    const bool live = walker < a.walkers;
+    V2* P = lds + grp * NP;
+    V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)(live ? walker : 0) * (N + 1);
+    if (live)
+        for (int j = lane; j < NB; j += G) P[j] = gpos[1 + j];
+    __syncthreads();
+    if (!live) return;
+    V2 z0 = gpos[0], zN = gpos[N];
+
+    const Real step = (Real)a.step;
+    const Real mln2 = (Real)a.m * (Real)0.69314718055994531;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const bool has_pad = (lane + G * (S - 1)) >= NB;
+    const int pad_j = lane + G * (S - 1);
+    unsigned accepted = 0;
+
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        for (int i = 0; i <= N; ++i) {
+            uint32_t r[4];
+            philox2((uint32_t)i, c1, c2, (uint32_t)walker, k0, k1, r);   // same numbers in every lane
+            Real g1, g2;
+            normals<Real>(r[0], r[1], g1, g2);
+            const bool bulk = (i > 0 && i < N);
+            const V2 zo = (i == 0) ? z0 : ((i == N) ? zN : P[i - 1]);
+            const Real xn = zo.x + step * g1, yn = zo.y + step * g2;
+            V2 mid;
+            mid.x = (Real)0.5 * (zo.x + xn);
+            mid.y = (Real)0.5 * (zo.y + yn);
+            if (bulk) P[i - 1] = mid;
+            if (has_pad) P[pad_j] = mid;
+            Real lsum = 0, pn = 1, pd = 1;
+            for (int s = 0; s < S; ++s) {
+                const V2 q = P[lane + G * s];
+                const Real ax = xn - q.x, ay = yn - q.y;
+                const Real bx = zo.x - q.x, by = zo.y - q.y;
+                pn *= ax * ax + ay * ay;
+                pd *= bx * bx + by * by;
+                if ((s & 7) == 7) {
+                    lsum += lg2<Real>(pn) - lg2<Real>(pd);
+                    pn = 1;
+                    pd = 1;
+                }
+            }
+            lsum += lg2<Real>(pn) - lg2<Real>(pd);
+#pragma unroll
+            for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
+            const Real dr2 = (xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y);
+            Real logA;
+            if (bulk) {
+                // bulk pairs count twice (both wavefunctions), the pairs with z_0 and z_N once
+                Real e = 0;
+                {
+                    Real ax = xn - z0.x, ay = yn - z0.y, bx = zo.x - z0.x, by = zo.y - z0.y;
+                    e += lg2<Real>(ax * ax + ay * ay) - lg2<Real>(bx * bx + by * by);
+                    ax = xn - zN.x; ay = yn - zN.y; bx = zo.x - zN.x; by = zo.y - zN.y;
+                    e += lg2<Real>(ax * ax + ay * ay) - lg2<Real>(bx * bx + by * by);
+                }
+                logA = mln2 * (lsum + (Real)0.5 * e) - (Real)0.5 * dr2;
+            } else {
+                logA = (Real)0.5 * mln2 * lsum - (Real)0.25 * dr2;
+            }
+            const bool acc = (logA >= 0) || ((Real)log(u01d2(r[2])) < logA);
+            V2 nz;
+            nz.x = acc ? xn : zo.x;
+            nz.y = acc ? yn : zo.y;
+            if (bulk) P[i - 1] = nz;
+            if (i == 0) z0 = nz;
+            if (i == N) zN = nz;
+            accepted += acc ? 1u : 0u;
+        }
+    }
+    for (int j = lane; j < NB; j += G) gpos[1 + j] = P[j];
+    if (lane == 0) {
+        gpos[0] = z0;
+        gpos[N] = zN;
+        a.accepted[walker] += accepted;
+    }
+}
+
+// One wavefront per sample.
+template <typename Real, int KMAX>
+__global__ __launch_bounds__(256) void k_features2(Features2Args a)
+{
+    constexpr int NQ = 2 * KMAX + 2;           // bulk Re/Im p_1..p_K, Re/Im q
+    __shared__ double pkL[4][2 * KMAX], pkR[4][2 * KMAX];
+    __shared__ double wsh[4][4];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int w = blockIdx.x * 4 + wv;
+    if (w >= a.walkers) return;
+    const int N = a.N;
+    auto load = [&](int j, double& x, double& y) {
+        if (a.pos_is_double || sizeof(Real) == 8) {
+            const double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * (N + 1) + j];
+            x = q.x; y = q.y;
+        } else {
+            const float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * (N + 1) + j];
+            x = q.x; y = q.y;
+        }
+    };
+    double x0, y0, xN, yN;
+    load(0, x0, y0);
+    load(N, xN, yN);
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < 2 * KMAX; ++q) acc[q] = 0.0;
+    double qr = 1.0, qi = 0.0;                  // unit-modulus running product
+    for (int j = 1 + lane; j < N; j += 64) {
+        double x, y;
+        load(j, x, y);
+        const double ur = x * a.invR, ui = y * a.invR;
+        double pr = ur, pi = ui;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            acc[2 * k] += pr;
+            acc[2 * k + 1] += pi;
+            const double t = pr * ur - pi * ui;
+            pi = pr * ui + pi * ur;
+            pr = t;
+        }
+        // (z_j - z_N) conj(z_0 - z_j), normalised
+        const double ar = x - xN, ai = y - yN, br = x0 - x, bi = -(y0 - y);
+        double tr = ar * br - ai * bi, ti = ar * bi + ai * br;
+        const double inv = 1.0 / sqrt(tr * tr + ti * ti);
+        tr *= inv;
+        ti *= inv;
+        const double nr = qr * tr - qi * ti;
+        qi = qr * ti + qi * tr;
+        qr = nr;
+    }
+    acc[2 * KMAX] = qr;
+    acc[2 * KMAX + 1] = qi;
+#pragma unroll
+    for (int q = 0; q < 2 * KMAX; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+        acc[q] = v;
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        const double orr = __shfl_xor(qr, off, 64), oi = __shfl_xor(qi, off, 64);
+        const double nr = qr * orr - qi * oi;
+        qi = qr * oi + qi * orr;
+        qr = nr;
+    }
+    if (lane == 0) {
+        // left adds z_0^k, right adds z_N^k
+        const double u0r = x0 * a.invR, u0i = y0 * a.invR, uNr = xN * a.invR, uNi = yN * a.invR;
+        double lr = u0r, li = u0i, rr = uNr, ri = uNi;
+        for (int k = 0; k < KMAX; ++k) {
+            pkL[wv][2 * k] = acc[2 * k] + lr;
+            pkL[wv][2 * k + 1] = acc[2 * k + 1] + li;
+            pkR[wv][2 * k] = acc[2 * k] + rr;
+            pkR[wv][2 * k + 1] = acc[2 * k + 1] + ri;
+            double t = lr * u0r - li * u0i;
+            li = lr * u0i + li * u0r;
+            lr = t;
+            t = rr * uNr - ri * uNi;
+            ri = rr * uNi + ri * uNr;
+            rr = t;
+        }
+        // e^{i phi} = (q/|q|)^m
+        double er = 1.0, ei = 0.0;
+        for (int e = 0; e < a.m; ++e) {
+            const double t = er * qr - ei * qi;
+            ei = er * qi + ei * qr;
+            er = t;
+        }
+        wsh[wv][0] = er;
+        wsh[wv][1] = ei;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const long row = a.row0 + w;
+    Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
+    for (int t = lane; t < a.D; t += 64) {
+        const double* pk = a.side[t] ? pkR[wv] : pkL[wv];
+        double vr = 1.0, vi = 0.0;
+        const uint32_t* f = a.fac + (size_t)t * 8;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t word = f[q];
+            if (word == 0u) break;
+            const int k = (int)(word & 0xffu), r = (int)(word >> 8);
+            const double br = pk[2 * (k - 1)], bi = pk[2 * (k - 1) + 1];
+            for (int e = 0; e < r; ++e) {
+                const double tr = vr * br - vi * bi;
+                vi = vr * bi + vi * br;
+                vr = tr;
+            }
+        }
+        const int c = a.col_re[t];
+        X[c] = (Real)vr;
+        X[c + a.col_dim[t]] = (Real)vi;
+    }
+    // w_l = e^{i phi} (z_0 conj(z_N) / 2)^l e^{-(|z_0|^2 + |z_N|^2)/4} / (2 pi l!)
+    const double xr = 0.5 * (x0 * xN + y0 * yN), xi = 0.5 * (y0 * xN - x0 * yN);
+    const double lx = 0.5 * log(xr * xr + xi * xi), ax = atan2(xi, xr);
+    const double g0 = -0.25 * (x0 * x0 + y0 * y0 + xN * xN + yN * yN);
+    const double er = wsh[wv][0], ei = wsh[wv][1];
+    Real* W = reinterpret_cast<Real*>(a.W);
+    for (int l = lane; l < a.lcut; l += 64) {
+        const double mag = exp(g0 + l * lx - a.ln2pifact[l]);
+        double s, c;
+        sincos(l * ax, &s, &c);
+        const double wr = mag * (c * er - s * ei), wi = mag * (c * ei + s * er);
+        W[(size_t)(2 * l) * a.wld + row] = (Real)wr;
+        W[(size_t)(2 * l + 1) * a.wld + row] = (Real)wi;
+    }
+}
+
+size_t sweep2_lds_bytes(int prec, int G, int N)
+{
+    const int S = (N - 1 + G - 1) / G;
+    const size_t el = (prec == kFP64) ? sizeof(double2) : sizeof(float2);
+    return (size_t)(64 / G) * (size_t)S * G * el;
+}
+
+template <int G, typename Real>
+static hipError_t sweep2_g(const SweepArgs& a, hipStream_t s)
+{
+    const size_t lds = sweep2_lds_bytes(sizeof(Real) == 8 ? kFP64 : kFP32, G, a.N);
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep2<G, Real>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sweep2<G, Real>), grid, dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t sweep2_dispatch(const SweepArgs& a, int G, hipStream_t s)
+{
+    switch (G) {
+    case 2: return sweep2_g<2, Real>(a, s);
+    case 4: return sweep2_g<4, Real>(a, s);
+    case 8: return sweep2_g<8, Real>(a, s);
+    case 16: return sweep2_g<16, Real>(a, s);
+    case 32: return sweep2_g<32, Real>(a, s);
+    case 64: return sweep2_g<64, Real>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sweep2(const SweepArgs& a, int prec, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    if (a.N < 2) return hipErrorInvalidValue;
+    return prec == kFP64 ? sweep2_dispatch<double>(a, G, s) : sweep2_dispatch<float>(a, G, s);
+}
+
+template <typename Real>
+static hipError_t features2_t(const Features2Args& a, hipStream_t s)
+{
+    dim3 grid((unsigned)((a.walkers + 3) / 4));
+    if (a.K <= 8)
+        hipLaunchKernelGGL((k_features2<Real, 8>), grid, dim3(256), 0, s, a);
+    else if (a.K <= 16)
+        hipLaunchKernelGGL((k_features2<Real, 16>), grid, dim3(256), 0, s, a);
+    else if (a.K <= 24)
+        hipLaunchKernelGGL((k_features2<Real, 24>), grid, dim3(256), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    return prec == kFP64 ? features2_t<double>(a, s) : features2_t<float>(a, s);
+}
+
+}  // namespace le

@@ -31,7 +31,7 @@ EXPORTED = [
     "le_get_moments", "le_get_trace", "le_get_positions", "le_set_positions",
     "le_accumulate_positions", "le_reset_moments", "le_geneig", "le_engine_spectrum",
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
-    "le_evolve",
+    "le_evolve", "le_engine_create_occupation", "le_get_occupation",
 ]
 
 TIME_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
@@ -49,7 +49,7 @@ def W_SIN(L):
 
 
 def W_USER(k):
-    return 0x300 | int(k)
+    return 0x10000 + int(k)
 
 
 class LEError(RuntimeError):
@@ -324,3 +324,42 @@ def evolve(M, H, Us, rhos, F, end_time, save_step, block_dims, eps_abs=1e-10, ep
                            ctypes.byref(steps)))
     rho = _cplx(out)[: len(Us) * nt * len(rhos)].reshape(len(Us), nt, len(rhos))
     return rho, nrm.reshape(len(Us), nt), steps.value
+
+
+class OccupationEngine(Engine):
+    """Two-sided chain for <L| a^dagger_l a_l |R>, all pairs of one sector at once."""
+
+    def __init__(self, N, m, dL, lcut, **mc):
+        self.sys = SystemParams(int(N), int(m), 0.0, 0.0, 0.0, 0)
+        self.mc = default_mc()
+        for k, v in mc.items():
+            if not hasattr(self.mc, k):
+                raise TypeError(f"unknown Monte-Carlo parameter {k}")
+            setattr(self.mc, k, v)
+        self._h = ctypes.c_void_p()
+        _check(lib().le_engine_create_occupation(ctypes.byref(self.sys), ctypes.byref(self.mc), int(dL), int(lcut),
+                                                 ctypes.byref(self._h)))
+        self.info = EngineInfo()
+        _check(lib().le_engine_get_info(self._h, ctypes.byref(self.info)))
+        dims = (ctypes.c_int * 2)()
+        offs = (ctypes.c_int * 3)()
+        _check(lib().le_sector_dims(self._h, dims, offs))
+        self.dims = list(dims)
+        self.offsets = list(offs)
+        self.N, self.m, self.lcut = int(N), int(m), int(lcut)
+        self.R = float(np.sqrt(2.0 * m * N))
+
+    def positions(self):
+        out = np.zeros((self.mc.walkers, self.N + 1, 2))
+        _check(lib().le_get_positions(self._h, _dp(out)))
+        return out
+
+    def accumulate_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.N + 1, 2)
+        _check(lib().le_accumulate_positions(self._h, _dp(pos), ctypes.c_long(pos.shape[0])))
+
+    def occupation(self, block=-1):
+        d = self.dims[0]
+        out = np.zeros(2 * d * d * self.lcut)
+        _check(lib().le_get_occupation(self._h, int(block), _dp(out)))
+        return _cplx(out).reshape(d, d, self.lcut)

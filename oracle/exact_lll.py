@@ -123,3 +123,33 @@ def exact_blocks(n, m, states, w, eps, strength=0.0, expo=0):
             S[i][j] = si / norm0 / scale
             H[i][j] = hi / norm0 / scale
     return S, H
+
+
+def exact_occupation(n, m, states, lcut):
+    """<a| sum_i |l><l|_i |b> / (Psi|Psi) for one sector (z / R power sums), l < lcut.
+
+    The one-body projector on orbital l is diagonal in the monomial basis with eigenvalue
+    n_l(alpha) = #{i : alpha_i = l}.  Returns O[a][b][l] (real)."""
+    R = (2.0 * m * n) ** 0.5
+    psi = laughlin(n, m)
+    norm0 = sum(c * c * _norm(a) for a, c in psi.items())
+    polys = [_mul(monomial_poly(n, p), psi) for _, p in states]
+    dls = [dl for dl, _ in states]
+    d = len(states)
+    O = [[[0.0] * lcut for _ in range(d)] for _ in range(d)]
+    for i in range(d):
+        for j in range(d):
+            acc = [0] * lcut
+            pi, pj = polys[i], polys[j]
+            for a, c in pi.items():
+                c2 = pj.get(a)
+                if c2 is None:
+                    continue
+                w = c * c2 * _norm(a)
+                for x in a:
+                    if x < lcut:
+                        acc[x] += w
+            scale = R ** (dls[i] + dls[j]) * norm0
+            for l in range(lcut):
+                O[i][j][l] = acc[l] / scale
+    return O

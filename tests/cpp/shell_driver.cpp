@@ -2,6 +2,7 @@
 // main.cu snippets do, with a fixed seed, and prints machine-readable results.
 #include <cstdio>
 
+#include "dynamics.h"
 #include "edge_montecarlo.h"
 
 int main()
@@ -29,6 +30,13 @@ int main()
     };
     vector<int> sectors = {0, 2, 4};
     edgeMC::excitationResponse_compute(p, sectors, U, 0);
+
+    auto F = [](double t) {
+        const double z = (t - 2.5) / 1.0;
+        return 0.1 * std::exp(-z * z);
+    };
+    timeEvolution::edgeDensityResponse_compute(sys_params::particlesNumber, sys_params::inverseFilling, {0, 2, 4}, F,
+                                               5.0, 0.5, 0);
     std::printf("DONE\n");
     return 0;
 }

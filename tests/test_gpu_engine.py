@@ -259,6 +259,21 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
     assert np.all(RC2[same] == 0)                    # selection rule |dL2 - dL1| = 2
     assert np.abs(RC2[0, 1:3]).max() > 0
     assert np.allclose(Ur, Ur.conj().T, atol=2e-6)
+    # time evolution from those files vs the CPU RK-CK54 oracle on the same (rounded) matrices
+    Hr, = _read_tsv(ex / "Hamiltonian" / "0.tsv")
+    rhos = []
+    for L in (0, 2, 4):
+        c, s_ = _read_tsv(ex / "EdgeDenstiy_FourierTransform" / f"dL={L}" / "0.tsv", cols=6)
+        rhos.append(c + 1j * s_)
+    F = lambda t: 0.1 * np.exp(-((t - 2.5) / 1.0) ** 2)
+    ref, nrm, _ = oracle_lib.evolve(Mr, Hr, Ur, rhos, F, 5.0, 0.5)
+    for k, L in enumerate((0, 2, 4)):
+        rows = [l.split("\t") for l in open(ex / "edgeDynamics" / f"dL={L}" / "0.tsv")]
+        assert len(rows) == 10
+        t = np.array([float(r[0]) for r in rows])
+        z = np.array([float(r[1]) + 1j * float(r[2]) for r in rows])
+        assert np.allclose(t, 0.5 * np.arange(10))
+        assert np.allclose(z, ref[:, k], atol=2e-6), (L, np.abs(z - ref[:, k]).max())
     assert "DONE" in lines
 
 
@@ -294,3 +309,79 @@ def test_evolve_gpu_vs_oracle(oracle_lib):
             assert abs(rho0[0, k, l] - C.conj() @ R @ C) < 1e-10 * max(1.0, abs(C.conj() @ R @ C))
     ref0, _, _ = oracle_lib.evolve(M, H, U, rhos, lambda t: 0.0, 5.0, 0.25)
     assert np.allclose(rho0[0], ref0, rtol=1e-6, atol=1e-6)
+
+
+def _occ_numpy(pos, N, m, states, lcut):
+    """independent FP64 evaluation of the two-sided estimator on given (N+1)-particle samples"""
+    from math import lgamma, pi
+    R = np.sqrt(2.0 * m * N)
+    z = pos[..., 0] + 1j * pos[..., 1]
+    zl, zr = z[:, :N] / R, z[:, 1:] / R
+
+    def mono(zz, pairs):
+        v = np.ones(zz.shape[0], complex)
+        for k, r in pairs:
+            if k:
+                v = v * np.sum(zz ** k, axis=1) ** r
+        return v
+    PL = np.stack([mono(zl, p) for _, p in states], 1)
+    PR = np.stack([mono(zr, p) for _, p in states], 1)
+    z0, zN, zb = z[:, 0], z[:, N], z[:, 1:N]
+    q = np.prod((zb - zN[:, None]) * np.conj(z0[:, None] - zb) /
+                np.abs((zb - zN[:, None]) * np.conj(z0[:, None] - zb)), axis=1)
+    ph = (q / np.abs(q)) ** m
+    x = 0.5 * z0 * np.conj(zN)
+    g = np.exp(-0.25 * (np.abs(z0) ** 2 + np.abs(zN) ** 2))
+    W = np.stack([ph * x ** l * g / (2 * pi * np.exp(lgamma(l + 1))) for l in range(lcut)], 1)
+    return np.einsum("sa,sb,sl->abl", PL.conj(), PR, W) / pos.shape[0]
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 2e-5), (1, 1e-11)])
+def test_occupation_fixed_samples(prec, tol):
+    """two-sided features + Gram on given configurations vs an independent numpy estimator"""
+    from laughlinedge_amd.engine import OccupationEngine
+    from oracle.partitions import Basis
+    N, m, dl, lcut = 6, 3, 3, 20
+    print(f"case occupation fixed prec={prec}", flush=True)
+    rng = np.random.default_rng(12)
+    R = np.sqrt(2.0 * m * N)
+    r = R * np.sqrt(rng.random((3000, N + 1)))
+    t = 2 * np.pi * rng.random((3000, N + 1))
+    pos = np.stack([r * np.cos(t), r * np.sin(t)], -1)
+    ref = _occ_numpy(pos, N, m, Basis(N, [dl]).states, lcut)
+    with OccupationEngine(N, m, dl, lcut, walkers=256, samples=1, precision=prec) as e:
+        e.accumulate_positions(pos)
+        got = e.occupation()
+    assert np.linalg.norm(got - ref) < tol * np.linalg.norm(ref), np.linalg.norm(got - ref) / np.linalg.norm(ref)
+
+
+def test_occupation_vs_exact_small_n():
+    """N = 3, m = 3, dL = 3: two-sided chain -> normalised <a|n_l|b>/(|a||b|) vs the exact
+    expansion, and the sum rule sum_l = N M_ab."""
+    from laughlinedge_amd.engine import OccupationEngine
+    from oracle.exact_lll import exact_occupation, exact_blocks
+    from oracle.partitions import Basis
+    N, m, dl = 3, 3, 3
+    lcut = m * (N - 1) + dl + 1
+    print("case occupation exact N=3", flush=True)
+    b = Basis(N, [dl])
+    Oe = np.array(exact_occupation(N, m, b.states, lcut))
+    Se, _ = exact_blocks(N, m, b.states, 0.0, 0.0)
+    Se = np.array(Se)
+    nrm = np.sqrt(np.diag(Se))
+    ref = Oe / np.outer(nrm, nrm)[:, :, None]
+    Mexact = Se / np.outer(nrm, nrm)
+    with OccupationEngine(N, m, dl, lcut, walkers=8192, samples=400, burnin_sweeps=200, step=0.9,
+                          tune_sweeps=50, n_blocks=4) as e:
+        e.run()
+        blocks = [e.occupation(k) for k in range(4)]
+        st = e.stats()
+    def normalise(O):
+        s = O.sum(-1)
+        return O / s[:, :, None] * N * Mexact[:, :, None]
+    vals = np.array([normalise(O) for O in blocks])
+    mean = vals.mean(0)
+    err = vals.std(0, ddof=1) / 2
+    assert 0.2 < st["acceptance"] < 0.95
+    assert np.allclose(mean.sum(-1), N * Mexact)
+    assert np.all(np.abs(mean - ref) < 5 * err + 2e-3), np.abs(mean - ref).max()

@@ -267,3 +267,18 @@ def test_evolve_oracle_driven_vs_dense_expm(oracle_lib):
         for l, R in enumerate(rhos):
             ref = C.conj() @ R @ C
             assert abs(rho[k, l] - ref) < 2e-5 * max(1.0, abs(ref))
+
+
+def test_exact_occupation_sum_rules():
+    """sum_l O_ab(l) = N S_ab and sum_l V_l O_ab(l) = H_ab with V_l = w(l+1) + 4 eps (l+1)(l+2)."""
+    from oracle.exact_lll import exact_occupation
+    N, m, dl = 3, 3, 3
+    b = Basis(N, [dl])
+    lcut = m * (N - 1) + dl + 1
+    O = np.array(exact_occupation(N, m, b.states, lcut))
+    S, H = exact_blocks(N, m, b.states, 0.05, 1e-3)
+    S, H = np.array(S), np.array(H)
+    l = np.arange(lcut)
+    V = 0.05 * (l + 1) + 4e-3 * (l + 1) * (l + 2)
+    assert np.allclose(O.sum(-1), N * S, rtol=1e-12)
+    assert np.allclose((O * V).sum(-1), H, rtol=1e-12)

@@ -4,6 +4,7 @@
 
 #include "dynamics.h"
 #include "edge_montecarlo.h"
+#include "green_function.h"
 
 int main()
 {
@@ -37,6 +38,18 @@ int main()
     };
     timeEvolution::edgeDensityResponse_compute(sys_params::particlesNumber, sys_params::inverseFilling, {0, 2, 4}, F,
                                                5.0, 0.5, 0);
+    // occupation numbers of dL = 2, normalised with the metric of a spectrum run
+    edgeMC::monteCarloParameters p2 = p;
+    p2.seed = 77;
+    std::vector<double> ev2 = edgeMC::spectrum_compute(p2, 2, true, 5);
+    cmatrix<double> M2 = edgeMC::metric_compute(p2, 2, false, 5);
+    greenFunctionMC::monteCarloParameters gp;
+    gp.setGPU_threads_blocks(64, 32);
+    gp.setMCSamples(400);
+    gp.setBurnIn(200);
+    gp.seed = 99;
+    greenFunctionMC::occupation_numbers_compute(gp, 2, M2, 5);
+    greenFunctionMC::occupation_numbers_compute(gp, 2, 5);
     std::printf("DONE\n");
     return 0;
 }

@@ -274,6 +274,25 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
         z = np.array([float(r[1]) + 1j * float(r[2]) for r in rows])
         assert np.allclose(t, 0.5 * np.arange(10))
         assert np.allclose(z, ref[:, k], atol=2e-6), (L, np.abs(z - ref[:, k]).max())
+    # occupation numbers: format, sum rule, and sum_l V_l O_ab(l) = H_ab (cross-module)
+    occ_dir = root / "dL=2" / "Statistics"
+    text = open(occ_dir / "OccupationNumbers_normalized" / "5.tsv").read()
+    blocks = [b for b in text.split("\n\n") if b.strip()]
+    assert len(blocks) == 4                          # d = 2: (a, b) pairs, blank line after each
+    lcut = 3 * 9 + 15
+    O = np.zeros((2, 2, lcut), complex)
+    for blk in blocks:
+        rows = [r.split("\t") for r in blk.strip().split("\n")]
+        assert len(rows) == lcut and all(len(r) == 5 for r in rows)
+        for a, b, l, re_, im_ in rows:
+            O[int(a), int(b), int(l)] = float(re_) + 1j * float(im_)
+    M2, = _read_tsv(occ_dir / "Metric" / "5.tsv")
+    H2, = _read_tsv(occ_dir / "Hamiltonian" / "5.tsv")
+    assert np.allclose(O.sum(-1), 10 * M2, atol=lcut * 1e-6 + 1e-5)
+    l = np.arange(lcut)
+    Vl = 0.05 * (l + 1) + 4e-4 * (l + 1) * (l + 2)
+    assert np.allclose((O * Vl).sum(-1), H2, rtol=0.02, atol=0.02 * np.abs(H2).max())
+    assert (occ_dir / "OccupationNumbers_not_normalized" / "5.tsv").exists()
     assert "DONE" in lines
 
 

@@ -206,6 +206,39 @@ int le_engine_create_occupation(const le_system_params* sys, const le_mc_params*
                                 le_engine** out);
 int le_get_occupation(const le_engine* e, int block, double* out);
 
+/* Learned proposal (no reference counterpart): a 3 -> 64 -> 64 -> 2 tanh MLP maps the
+ * features (|z|/R, nearest-neighbour distance, mean of the 8 residue-class nearest
+ * distances) of the particle being moved to radial/tangential Gaussian widths; moves are
+ * accepted with the exact Metropolis-Hastings ratio (widths re-evaluated at the proposed
+ * position), so |Psi_L|^2 stays the target.  Training maximises the expected squared jump
+ * distance on configurations of the plain chain with Adam; the trained network is
+ * tabulated on a table_size^3 grid that the sampler interpolates.  FP32 engines only. */
+#define LE_PROPOSAL_PLAIN 0
+#define LE_PROPOSAL_LEARNED 1
+typedef struct le_train_params {
+    int steps;                 /* Adam steps (2000) */
+    double lr;                 /* 1e-3 */
+    int batch;                 /* training triples per step (4096) */
+    int n_configs;             /* configurations drawn from the plain chain (1e5) */
+    int snapshot_sweeps;       /* plain sweeps between snapshots of the walkers */
+    uint64_t seed;             /* weights and training noise (99) */
+    double sigma0;             /* width scale; <= 0: the engine's current step */
+    int table_size;            /* grid points per feature (24) */
+} le_train_params;
+typedef struct le_train_report {
+    double loss_first, loss_last;      /* -<min(1, alpha) |dz|^2> */
+    double accept_first, accept_last;  /* mean min(1, alpha) */
+    double ms;                         /* wall time */
+} le_train_report;
+void le_train_params_default(le_train_params* p);
+int le_train_proposal(le_engine* e, const le_train_params* tp, le_train_report* rep);
+int le_set_proposal(le_engine* e, int kind);
+int le_get_proposal_params(const le_engine* e, float* out);          /* 4546 floats */
+int le_set_proposal_params(le_engine* e, const float* in, double sigma0, int table_size);
+/* loss and parameter gradient on given triples (configs: n_cfg x N x 2 float) */
+int le_proposal_loss_grad(le_engine* e, const float* configs, int n_cfg, const int* cfg, const int* part,
+                          const float* noise, int B, double sigma0, double* loss, float* grad);
+
 /* eigenvalues of every sector in one launch (block -1 = mean): evals has D entries in basis
  * order; status (optional, n_sectors): 0 ok, 1 metric not positive definite */
 int le_engine_spectra(le_engine* e, int block, double* evals, int* status);

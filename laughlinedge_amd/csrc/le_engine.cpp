@@ -26,6 +26,7 @@
 #include "le_basis.hpp"
 #include "le_evolve.hpp"
 #include "le_kernels.hpp"
+#include "le_learned.hpp"
 
 namespace {
 
@@ -97,6 +98,13 @@ struct le_engine {
     double* lnfact_d = nullptr;
     int* side_d = nullptr;
     int npos() const { return sys.particles_number + (two_sided ? 1 : 0); }
+    // learned proposal
+    int proposal = LE_PROPOSAL_PLAIN;
+    float* mlp = nullptr;            // kMlpParams
+    float* table = nullptr;          // T^3 x 2
+    int table_T = 0;
+    float table_lo[3] = {0.f, 0.f, 0.f}, table_hi[3] = {1.5f, 4.f, 8.f};
+    double sigma0 = 0.0;
     le_weight_fn wfn = nullptr;
     void* wctx = nullptr;
     struct Job {
@@ -190,6 +198,8 @@ le_engine::~le_engine()
     (void)hipFree(rec_d);
     (void)hipFree(lnfact_d);
     (void)hipFree(side_d);
+    (void)hipFree(mlp);
+    (void)hipFree(table);
     (void)hipFree(trace_d);
     (void)hipFree(eig_work);
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -528,8 +538,21 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     a.walkers = mc.walkers;
     a.nsweeps = nsweeps;
     a.step = step;
-    if (two_sided) HIPCHK(le::launch_sweep2(a, prec, G, stream));
-    else HIPCHK(le::launch_sweep(a, prec, G, stream));
+    if (two_sided) {
+        HIPCHK(le::launch_sweep2(a, prec, G, stream));
+    } else if (proposal == LE_PROPOSAL_LEARNED) {
+        le::LearnedTable lt;
+        lt.table = table;
+        lt.T = table_T;
+        for (int q = 0; q < 3; ++q) {
+            lt.lo[q] = table_lo[q];
+            lt.hi[q] = table_hi[q];
+        }
+        lt.invR = 1.0 / R;
+        HIPCHK(le::launch_sweep_learned(a, lt, std::max(G, 8), stream));
+    } else {
+        HIPCHK(le::launch_sweep(a, prec, G, stream));
+    }
     return LE_OK;
 }
 
@@ -1380,6 +1403,202 @@ extern "C" int le_get_occupation(const le_engine* e, int block, double* out)
             out[2 * (q * e->lcut + l)] = br[2 * q] - bi[2 * q + 1];
             out[2 * (q * e->lcut + l) + 1] = br[2 * q + 1] + bi[2 * q];
         }
+    }
+    return LE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned proposal
+// ---------------------------------------------------------------------------
+extern "C" void le_train_params_default(le_train_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->steps = 2000;
+    p->lr = 1e-3;
+    p->batch = 4096;
+    p->n_configs = 100000;
+    p->snapshot_sweeps = 4;
+    p->seed = 99;
+    p->sigma0 = 0.0;
+    p->table_size = 24;
+}
+
+static int build_table(le_engine* e, int T)
+{
+    if (T < 2 || T > 64) return fail(LE_ERR_ARG, "table_size must be in [2, 64]");
+    if (e->table_T != T) {
+        (void)hipFree(e->table);
+        e->table = nullptr;
+        HIPCHK(dalloc(&e->table, (size_t)2 * T * T * T));
+        e->table_T = T;
+    }
+    float* lohi = nullptr;
+    HIPCHK(dalloc(&lohi, 6));
+    float h[6] = {e->table_lo[0], e->table_lo[1], e->table_lo[2], e->table_hi[0], e->table_hi[1], e->table_hi[2]};
+    hipError_t x = hipMemcpy(lohi, h, sizeof(h), hipMemcpyHostToDevice);
+    if (x == hipSuccess) x = le::launch_mlp_table(e->mlp, T, lohi, lohi + 3, (float)e->sigma0, e->table, e->stream);
+    if (x == hipSuccess) x = hipStreamSynchronize(e->stream);
+    (void)hipFree(lohi);
+    if (x != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(x));
+    return LE_OK;
+}
+
+extern "C" int le_set_proposal(le_engine* e, int kind)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (kind == LE_PROPOSAL_PLAIN) {
+        e->proposal = kind;
+        return LE_OK;
+    }
+    if (kind != LE_PROPOSAL_LEARNED) return fail(LE_ERR_ARG, "unknown proposal");
+    if (!e->table) return fail(LE_ERR_STATE, "no trained proposal (le_train_proposal or le_set_proposal_params)");
+    if (e->prec != LE_PREC_FP32 || e->two_sided) return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine");
+    e->proposal = kind;
+    return LE_OK;
+}
+
+extern "C" int le_get_proposal_params(const le_engine* e, float* out)
+{
+    if (!e || !out) return fail(LE_ERR_ARG, "null argument");
+    if (!e->mlp) return fail(LE_ERR_STATE, "no proposal parameters");
+    HIPCHK(hipMemcpy(out, e->mlp, sizeof(float) * le::kMlpParams, hipMemcpyDeviceToHost));
+    return LE_OK;
+}
+
+extern "C" int le_set_proposal_params(le_engine* e, const float* in, double sigma0, int table_size)
+{
+    if (!e || !in || !(sigma0 > 0)) return fail(LE_ERR_ARG, "bad argument");
+    if (!e->mlp) HIPCHK(dalloc(&e->mlp, (size_t)le::kMlpParams));
+    HIPCHK(hipMemcpy(e->mlp, in, sizeof(float) * le::kMlpParams, hipMemcpyHostToDevice));
+    e->sigma0 = sigma0;
+    return build_table(e, table_size);
+}
+
+extern "C" int le_proposal_loss_grad(le_engine* e, const float* configs, int n_cfg, const int* cfg, const int* part,
+                                     const float* noise, int B, double sigma0, double* loss, float* grad)
+{
+    if (!e || !configs || !cfg || !part || !noise || B <= 0 || n_cfg <= 0 || !(sigma0 > 0))
+        return fail(LE_ERR_ARG, "bad argument");
+    if (!e->mlp) return fail(LE_ERR_STATE, "no proposal parameters");
+    const int N = e->sys.particles_number;
+    for (int b = 0; b < B; ++b)
+        if (cfg[b] < 0 || cfg[b] >= n_cfg || part[b] < 0 || part[b] >= N) return fail(LE_ERR_ARG, "sample index out of range");
+    float *dc = nullptr, *dn = nullptr, *drec = nullptr, *dg = nullptr;
+    int *dcfg = nullptr, *dpart = nullptr;
+    double* dst = nullptr;
+    hipError_t h = hipSuccess;
+    auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
+    chk(dalloc(&dc, (size_t)n_cfg * N * 2));
+    chk(dalloc(&dn, (size_t)2 * B));
+    chk(dalloc(&drec, (size_t)2 * B * le::kMlpRecord));
+    chk(dalloc(&dg, (size_t)le::kMlpParams));
+    chk(dalloc(&dcfg, (size_t)B));
+    chk(dalloc(&dpart, (size_t)B));
+    chk(dalloc(&dst, 4));
+    if (h == hipSuccess) {
+        chk(hipMemcpy(dc, configs, sizeof(float) * n_cfg * N * 2, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dn, noise, sizeof(float) * 2 * B, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dcfg, cfg, sizeof(int) * B, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dpart, part, sizeof(int) * B, hipMemcpyHostToDevice));
+        chk(hipMemset(dst, 0, sizeof(double) * 4));
+    }
+    if (h == hipSuccess)
+        chk(le::launch_mlp_sample(e->mlp, dc, N, e->sys.inverse_filling, (float)(1.0 / e->R), (float)sigma0, dcfg, dpart,
+                                  dn, B, drec, dst, e->stream));
+    if (h == hipSuccess) chk(le::launch_mlp_grad(drec, B, dg, e->stream));
+    double hs[4] = {0, 0, 0, 0};
+    if (h == hipSuccess) chk(hipStreamSynchronize(e->stream));
+    if (h == hipSuccess) {
+        chk(hipMemcpy(hs, dst, sizeof(hs), hipMemcpyDeviceToHost));
+        if (grad) chk(hipMemcpy(grad, dg, sizeof(float) * le::kMlpParams, hipMemcpyDeviceToHost));
+    }
+    for (void* q : {(void*)dc, (void*)dn, (void*)drec, (void*)dg, (void*)dcfg, (void*)dpart, (void*)dst}) (void)hipFree(q);
+    if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
+    if (loss) *loss = hs[0];
+    return LE_OK;
+}
+
+extern "C" int le_train_proposal(le_engine* e, const le_train_params* tp, le_train_report* rep)
+{
+    if (!e || !tp) return fail(LE_ERR_ARG, "null argument");
+    if (e->prec != LE_PREC_FP32 || e->two_sided) return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine");
+    if (tp->steps < 0 || tp->batch <= 0 || tp->n_configs <= 0 || tp->snapshot_sweeps < 0)
+        return fail(LE_ERR_ARG, "bad training parameters");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!e->walkers_ready) {
+        int rc = le_init_walkers(e);
+        if (rc) return rc;
+    }
+    const int N = e->sys.particles_number;
+    const int W = e->mc.walkers;
+    const int ncfg = tp->n_configs;
+    e->sigma0 = tp->sigma0 > 0 ? tp->sigma0 : e->step;
+    // configurations of the plain chain (the engine's own walkers, advanced between snapshots)
+    float* cfgs = nullptr;
+    HIPCHK(dalloc(&cfgs, (size_t)ncfg * N * 2));
+    const int saved = e->proposal;
+    e->proposal = LE_PROPOSAL_PLAIN;
+    int rc = LE_OK;
+    for (int have = 0; have < ncfg && rc == LE_OK;) {
+        if (tp->snapshot_sweeps > 0) {
+            rc = e->run_sweeps(tp->snapshot_sweeps, e->sweep_counter, e->pos);
+            e->sweep_counter += (unsigned long long)tp->snapshot_sweeps;
+            e->sweeps_done += tp->snapshot_sweeps;
+        }
+        const int take = std::min(W, ncfg - have);
+        hipError_t h = hipMemcpyAsync(cfgs + (size_t)have * N * 2, e->pos, sizeof(float) * take * N * 2,
+                                      hipMemcpyDeviceToDevice, e->stream);
+        if (h != hipSuccess) rc = fail(LE_ERR_HIP, hipGetErrorString(h));
+        have += take;
+    }
+    e->proposal = saved;
+    const int B = tp->batch;
+    float *M1 = nullptr, *M2 = nullptr, *Gd = nullptr, *recs = nullptr, *noise = nullptr;
+    int *cfg = nullptr, *part = nullptr;
+    double* stats = nullptr;
+    hipError_t h = hipSuccess;
+    auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
+    if (!e->mlp) chk(dalloc(&e->mlp, (size_t)le::kMlpParams));
+    chk(dalloc(&M1, (size_t)le::kMlpParams));
+    chk(dalloc(&M2, (size_t)le::kMlpParams));
+    chk(dalloc(&Gd, (size_t)le::kMlpParams));
+    chk(dalloc(&recs, (size_t)2 * B * le::kMlpRecord));
+    chk(dalloc(&noise, (size_t)2 * B));
+    chk(dalloc(&cfg, (size_t)B));
+    chk(dalloc(&part, (size_t)B));
+    chk(dalloc(&stats, (size_t)4 * (tp->steps + 1)));
+    if (h == hipSuccess && rc == LE_OK) {
+        chk(hipMemsetAsync(M1, 0, sizeof(float) * le::kMlpParams, e->stream));
+        chk(hipMemsetAsync(M2, 0, sizeof(float) * le::kMlpParams, e->stream));
+        chk(hipMemsetAsync(stats, 0, sizeof(double) * 4 * (tp->steps + 1), e->stream));
+        chk(le::launch_mlp_init(e->mlp, tp->seed, e->stream));
+        const float invR = (float)(1.0 / e->R);
+        for (int t = 0; t <= tp->steps && h == hipSuccess; ++t) {
+            chk(le::launch_train_samples(B, ncfg, N, tp->seed, t, cfg, part, noise, e->stream));
+            chk(le::launch_mlp_sample(e->mlp, cfgs, N, e->sys.inverse_filling, invR, (float)e->sigma0, cfg, part, noise,
+                                      B, recs, stats + 4 * t, e->stream));
+            if (t == tp->steps) break;      // last pass only measures
+            chk(le::launch_mlp_grad(recs, B, Gd, e->stream));
+            chk(le::launch_adam(e->mlp, M1, M2, Gd, (float)tp->lr, t + 1, e->stream));
+        }
+    }
+    std::vector<double> hs(4 * (tp->steps + 1), 0.0);
+    if (h == hipSuccess) chk(hipMemcpyAsync(hs.data(), stats, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, e->stream));
+    if (h == hipSuccess) chk(hipStreamSynchronize(e->stream));
+    for (void* q : {(void*)M1, (void*)M2, (void*)Gd, (void*)recs, (void*)noise, (void*)cfg, (void*)part, (void*)stats,
+                    (void*)cfgs})
+        (void)hipFree(q);
+    if (rc) return rc;
+    if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
+    rc = build_table(e, tp->table_size);
+    if (rc) return rc;
+    e->proposal = LE_PROPOSAL_LEARNED;
+    if (rep) {
+        rep->loss_first = hs[0];
+        rep->accept_first = hs[1];
+        rep->loss_last = hs[4 * tp->steps];
+        rep->accept_last = hs[4 * tp->steps + 1];
+        rep->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return LE_OK;
 }

@@ -1,0 +1,581 @@
+// Learned proposal: a 3 -> 64 -> 64 -> 2 tanh MLP predicting radial / tangential move
+// widths from per-particle features, trained on the GPU to maximise the expected squared
+// jump distance of an exact Metropolis-Hastings move, then tabulated on a 3-D grid that
+// the sampler interpolates.  No reference counterpart.
+//
+// Definitions (shared with the FP64 oracle):
+//   f1 = |z| / R;  m_c = min(8, min_{j != i, j = c mod 8} |z - z_j|);  f2 = min_c m_c;  f3 = mean_c m_c
+//   (s_r, s_t) = sigma0 exp(1.5 tanh(MLP(f)));  z' = z + e^{i arg z} (s_r n1 + i s_t n2)
+//   log alpha = m sum_j log(|z'-z_j|^2/|z-z_j|^2) - (|z'|^2-|z|^2)/2 + log q(z|x') - log q(z'|x)
+//   loss = -mean min(1, e^{log alpha}) |z' - z|^2
+// Parameter layout: W1 (64x3), b1 (64), W2 (64x64), b2 (64), W3 (2x64), b3 (2).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "le_learned.hpp"
+
+namespace le {
+
+constexpr int kH = 64;
+constexpr int kP = kH * 3 + kH + kH * kH + kH + 2 * kH + 2;   // 4546
+constexpr int oW1 = 0, oB1 = kH * 3, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + 2 * kH;
+constexpr float kDcap = 8.0f;
+constexpr float kWScale = 1.5f;
+// per (sample, evaluation) record: f(3) h1(64) h2(64) do(2) dz2(64) dz1(64) pad(3)
+constexpr int kRec = 264;
+constexpr int rF = 0, rH1 = 3, rH2 = 67, rDO = 131, rDZ2 = 133, rDZ1 = 197;
+
+__device__ __forceinline__ void philox_l(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                                         uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ float u01l(uint32_t x) { return (float)(((double)x + 1.0) * 2.3283064365386963e-10); }
+
+// features of particle i at (x, y) against configuration cfg (N particles) and d/d(x, y)
+__device__ void features_at(const float2* cfg, int N, int i, float x, float y, float invR, float f[3], float df[3][2])
+{
+    float mins[8], gx[8], gy[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { mins[c] = kDcap; gx[c] = 0.f; gy[c] = 0.f; }
+    for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const float dx = x - cfg[j].x, dy = y - cfg[j].y;
+        const float d = sqrtf(dx * dx + dy * dy);
+        const int c = j & 7;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == c && d < mins[q]) {
+                mins[q] = d;
+                gx[q] = d > 0.f ? dx / d : 0.f;
+                gy[q] = d > 0.f ? dy / d : 0.f;
+            }
+    }
+    const float r = sqrtf(x * x + y * y);
+    f[0] = r * invR;
+    df[0][0] = r > 0.f ? x / r * invR : 0.f;
+    df[0][1] = r > 0.f ? y / r * invR : 0.f;
+    int cm = 0;
+    float s = 0.f, sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (mins[q] < mins[cm]) cm = q;
+        s += mins[q];
+        sx += gx[q];
+        sy += gy[q];
+    }
+    float mm = mins[0], mgx = gx[0], mgy = gy[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q)
+        if (q == cm) { mm = mins[q]; mgx = gx[q]; mgy = gy[q]; }
+    f[1] = mm;
+    df[1][0] = mgx;
+    df[1][1] = mgy;
+    f[2] = 0.125f * s;
+    df[2][0] = 0.125f * sx;
+    df[2][1] = 0.125f * sy;
+}
+
+// forward: h1, h2 into the record, outputs o[2]
+__device__ void mlp_fwd(const float* P, const float f[3], float* rec, float o[2])
+{
+    float h1[kH];
+#pragma unroll 8
+    for (int k = 0; k < kH; ++k) {
+        const float z = P[oW1 + 3 * k] * f[0] + P[oW1 + 3 * k + 1] * f[1] + P[oW1 + 3 * k + 2] * f[2] + P[oB1 + k];
+        h1[k] = tanhf(z);
+        rec[rH1 + k] = h1[k];
+    }
+    float o0 = P[oB3], o1 = P[oB3 + 1];
+    for (int j = 0; j < kH; ++j) {
+        float z = P[oB2 + j];
+#pragma unroll 8
+        for (int k = 0; k < kH; ++k) z += P[oW2 + kH * j + k] * h1[k];
+        const float h = tanhf(z);
+        rec[rH2 + j] = h;
+        o0 += P[oW3 + j] * h;
+        o1 += P[oW3 + kH + j] * h;
+    }
+    rec[rF] = f[0];
+    rec[rF + 1] = f[1];
+    rec[rF + 2] = f[2];
+    o[0] = o0;
+    o[1] = o1;
+}
+
+// backward given d(loss)/d(o): stores do, dz2, dz1 in the record; returns d/df
+__device__ void mlp_bwd(const float* P, float* rec, const float dout[2], float df[3])
+{
+    rec[rDO] = dout[0];
+    rec[rDO + 1] = dout[1];
+    float dh1[kH];
+#pragma unroll 8
+    for (int k = 0; k < kH; ++k) dh1[k] = 0.f;
+    for (int j = 0; j < kH; ++j) {
+        const float h = rec[rH2 + j];
+        const float dz2 = (P[oW3 + j] * dout[0] + P[oW3 + kH + j] * dout[1]) * (1.f - h * h);
+        rec[rDZ2 + j] = dz2;
+#pragma unroll 8
+        for (int k = 0; k < kH; ++k) dh1[k] += P[oW2 + kH * j + k] * dz2;
+    }
+    df[0] = df[1] = df[2] = 0.f;
+#pragma unroll 8
+    for (int k = 0; k < kH; ++k) {
+        const float h = rec[rH1 + k];
+        const float dz1 = dh1[k] * (1.f - h * h);
+        rec[rDZ1 + k] = dz1;
+        df[0] += P[oW1 + 3 * k] * dz1;
+        df[1] += P[oW1 + 3 * k + 1] * dz1;
+        df[2] += P[oW1 + 3 * k + 2] * dz1;
+    }
+}
+
+// random training triples: configuration index, particle, noise
+__global__ void k_train_samples(int B, int ncfg, int N, unsigned long long seed, int step, int* cfg, int* part,
+                                float* noise)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint32_t r[4];
+    philox_l((uint32_t)b, (uint32_t)step, 2u << 16, 0xfffffffeu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    cfg[b] = (int)(r[0] % (uint32_t)ncfg);
+    part[b] = (int)(r[1] % (uint32_t)N);
+    const float rr = sqrtf(-2.f * logf(u01l(r[2])));
+    float s, c;
+    sincosf(6.28318530717958648f * u01l(r[3]), &s, &c);
+    noise[2 * b] = rr * c;
+    noise[2 * b + 1] = rr * s;
+}
+
+// one thread per training triple: both MLP evaluations, loss, backprop to the outputs
+__global__ __launch_bounds__(128) void k_mlp_sample(const float* Pg, const float2* cfgs, int N, int m, float invR,
+                                                   float sigma0, const int* cfg, const int* part, const float* noise,
+                                                   int B, float* recs, double* stats)
+{
+    __shared__ float P[kP];
+    for (int q = threadIdx.x; q < kP; q += blockDim.x) P[q] = Pg[q];
+    __syncthreads();
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float2* X = cfgs + (size_t)cfg[b] * N;
+    const int i = part[b];
+    const float n1 = noise[2 * b], n2 = noise[2 * b + 1];
+    float* rec1 = recs + (size_t)(2 * b) * kRec;
+    float* rec2 = rec1 + kRec;
+    const float x = X[i].x, y = X[i].y;
+    const float r = sqrtf(x * x + y * y);
+    const float c = r > 0.f ? x / r : 1.f, s = r > 0.f ? y / r : 0.f;
+    float f[3], df[3][2], o[2];
+    features_at(X, N, i, x, y, invR, f, df);
+    mlp_fwd(P, f, rec1, o);
+    float sg[2], dsg[2];
+    for (int q = 0; q < 2; ++q) {
+        const float t = tanhf(o[q]);
+        sg[q] = sigma0 * expf(kWScale * t);
+        dsg[q] = sg[q] * kWScale * (1.f - t * t);
+    }
+    const float dr = sg[0] * n1, dt = sg[1] * n2;
+    const float xp = x + c * dr - s * dt, yp = y + s * dr + c * dt;
+    float fp[3], dfp[3][2], op[2];
+    features_at(X, N, i, xp, yp, invR, fp, dfp);
+    mlp_fwd(P, fp, rec2, op);
+    float sgp[2], dsgp[2];
+    for (int q = 0; q < 2; ++q) {
+        const float t = tanhf(op[q]);
+        sgp[q] = sigma0 * expf(kWScale * t);
+        dsgp[q] = sgp[q] * kWScale * (1.f - t * t);
+    }
+    // energy change and its gradient at z'
+    float E = 0.f, gEx = 0.f, gEy = 0.f;
+    for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const float ax = xp - X[j].x, ay = yp - X[j].y, bx = x - X[j].x, by = y - X[j].y;
+        const float a2 = ax * ax + ay * ay, b2 = bx * bx + by * by;
+        E += logf(a2 / b2);
+        gEx += 2.f * ax / a2;
+        gEy += 2.f * ay / a2;
+    }
+    E = (float)m * E - 0.5f * ((xp * xp + yp * yp) - (x * x + y * y));
+    gEx = (float)m * gEx - xp;
+    gEy = (float)m * gEy - yp;
+    const float rp = sqrtf(xp * xp + yp * yp);
+    const float cp = rp > 0.f ? xp / rp : 1.f, sp = rp > 0.f ? yp / rp : 0.f;
+    const float wx = x - xp, wy = y - yp;
+    const float Dr = wx * cp + wy * sp, Dt = -wx * sp + wy * cp;
+    const float lq_rev = -logf(6.28318530717958648f * sgp[0] * sgp[1]) -
+                         0.5f * ((Dr / sgp[0]) * (Dr / sgp[0]) + (Dt / sgp[1]) * (Dt / sgp[1]));
+    const float lq_fwd = -logf(6.28318530717958648f * sg[0] * sg[1]) - 0.5f * (n1 * n1 + n2 * n2);
+    const float la = E + lq_rev - lq_fwd;
+    const float A = la < 0.f ? expf(la) : 1.f;
+    const float J = dr * dr + dt * dt;
+    const float invB = 1.f / (float)B;
+    const float dla = (la < 0.f ? -J * A : 0.f) * invB;
+    const float dJ = -A * invB;
+    // reverse widths -> second evaluation
+    float dop[2] = {dla * (-1.f / sgp[0] + Dr * Dr / (sgp[0] * sgp[0] * sgp[0])) * dsgp[0],
+                    dla * (-1.f / sgp[1] + Dt * Dt / (sgp[1] * sgp[1] * sgp[1])) * dsgp[1]};
+    float dfp_in[3];
+    mlp_bwd(P, rec2, dop, dfp_in);
+    // d/dz'
+    float dDr0, dDr1, dDt0, dDt1;
+    if (rp > 0.f) {
+        dDr0 = -cp - sp * Dt / rp;
+        dDr1 = -sp + cp * Dt / rp;
+        dDt0 = sp + sp * Dr / rp;
+        dDt1 = -cp - cp * Dr / rp;
+    } else {
+        dDr0 = -1.f; dDr1 = 0.f; dDt0 = 0.f; dDt1 = -1.f;
+    }
+    const float qr = Dr / (sgp[0] * sgp[0]), qt = Dt / (sgp[1] * sgp[1]);
+    float dzx = dla * (gEx - (qr * dDr0 + qt * dDt0));
+    float dzy = dla * (gEy - (qr * dDr1 + qt * dDt1));
+    for (int q = 0; q < 3; ++q) {
+        dzx += dfp[q][0] * dfp_in[q];
+        dzy += dfp[q][1] * dfp_in[q];
+    }
+    // forward widths -> first evaluation
+    float dsr = dla / sg[0] + dJ * 2.f * sg[0] * n1 * n1 + (dzx * c + dzy * s) * n1;
+    float dst = dla / sg[1] + dJ * 2.f * sg[1] * n2 * n2 + (-dzx * s + dzy * c) * n2;
+    float dout[2] = {dsr * dsg[0], dst * dsg[1]};
+    float dfi[3];
+    mlp_bwd(P, rec1, dout, dfi);
+    // batch statistics: loss, acceptance, squared jump
+    atomicAdd(&stats[0], (double)(-A * J) * invB);
+    atomicAdd(&stats[1], (double)A * invB);
+    atomicAdd(&stats[2], (double)J * invB);
+}
+
+// parameter gradients: sum over the 2B records
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* recs, int R2, float* grad)
+{
+    const int p = blockIdx.x;           // one parameter per block
+    float acc = 0.f;
+    int ra, rb;                         // record offsets of the two factors (rb < 0: factor 1)
+    if (p < oB1) { ra = rDZ1 + p / 3; rb = rF + p % 3; }
+    else if (p < oW2) { ra = rDZ1 + (p - oB1); rb = -1; }
+    else if (p < oB2) { const int q = p - oW2; ra = rDZ2 + q / kH; rb = rH1 + q % kH; }
+    else if (p < oW3) { ra = rDZ2 + (p - oB2); rb = -1; }
+    else if (p < oB3) { const int q = p - oW3; ra = rDO + q / kH; rb = rH2 + q % kH; }
+    else { ra = rDO + (p - oB3); rb = -1; }
+    for (int r = threadIdx.x; r < R2; r += blockDim.x) {
+        const float* rec = recs + (size_t)r * kRec;
+        acc += rec[ra] * (rb >= 0 ? rec[rb] : 1.f);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) grad[p] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_adam(float* P, float* M1, float* M2, const float* G, int n, float lr, float b1, float b2, float eps,
+                       int t)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float g = G[q];
+    const float m1 = b1 * M1[q] + (1.f - b1) * g;
+    const float m2 = b2 * M2[q] + (1.f - b2) * g * g;
+    M1[q] = m1;
+    M2[q] = m2;
+    const float mh = m1 / (1.f - powf(b1, (float)t)), vh = m2 / (1.f - powf(b2, (float)t));
+    P[q] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Xavier-uniform init from Philox (seed), zero biases, small output layer
+__global__ void k_mlp_init(float* P, unsigned long long seed)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= kP) return;
+    uint32_t r[4];
+    philox_l((uint32_t)q, 0u, 3u << 16, 0xfffffffdu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    const float u = 2.f * u01l(r[0]) - 1.f;
+    float v = 0.f;
+    if (q < oB1) v = u * sqrtf(6.f / (3 + kH));
+    else if (q >= oW2 && q < oB2) v = u * sqrtf(6.f / (2 * kH));
+    else if (q >= oW3 && q < oB3) v = 0.1f * u * sqrtf(6.f / (kH + 2));
+    P[q] = v;
+}
+
+// table[i1][i2][i3][2] = log widths at grid points f = lo + idx * h
+__global__ void k_mlp_table(const float* Pg, int T, const float* lo, const float* hi, float sigma0, float* table)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= T * T * T) return;
+    const int i3 = g % T, i2 = (g / T) % T, i1 = g / (T * T);
+    float f[3];
+    const int id[3] = {i1, i2, i3};
+    for (int q = 0; q < 3; ++q) f[q] = lo[q] + (hi[q] - lo[q]) * (float)id[q] / (float)(T - 1);
+    float h1[kH];
+    for (int k = 0; k < kH; ++k)
+        h1[k] = tanhf(Pg[oW1 + 3 * k] * f[0] + Pg[oW1 + 3 * k + 1] * f[1] + Pg[oW1 + 3 * k + 2] * f[2] + Pg[oB1 + k]);
+    float o0 = Pg[oB3], o1 = Pg[oB3 + 1];
+    for (int j = 0; j < kH; ++j) {
+        float z = Pg[oB2 + j];
+        for (int k = 0; k < kH; ++k) z += Pg[oW2 + kH * j + k] * h1[k];
+        const float h = tanhf(z);
+        o0 += Pg[oW3 + j] * h;
+        o1 += Pg[oW3 + kH + j] * h;
+    }
+    table[2 * g] = logf(sigma0) + kWScale * tanhf(o0);
+    table[2 * g + 1] = logf(sigma0) + kWScale * tanhf(o1);
+}
+
+// ---------------------------------------------------------------------------
+// Metropolis-Hastings sweeps with the tabulated widths.  G lanes per walker (G in
+// {8,16,32,64}, lane l owns particles j = l + G s, so its class is l mod 8).  Pass 1 (slot
+// i and pads hold a far sentinel) gives the class minima at z; pass 2 (slot i and pads hold
+// the midpoint) gives the log-ratio and, skipping those slots, the class minima at z'.
+// ---------------------------------------------------------------------------
+struct Table3 {
+    const float* t;
+    int T;
+    float lo[3], inv_h[3];
+};
+
+__device__ __forceinline__ void widths_lookup(const Table3& tb, const float f[3], float& lsr, float& lst)
+{
+    int i0[3];
+    float w[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float u = (f[q] - tb.lo[q]) * tb.inv_h[q];
+        u = fminf(fmaxf(u, 0.f), (float)(tb.T - 1) - 1e-3f);
+        const int k = (int)u;
+        i0[q] = k;
+        w[q] = u - (float)k;
+    }
+    float a = 0.f, b = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int d1 = c >> 2, d2 = (c >> 1) & 1, d3 = c & 1;
+        const float wt = (d1 ? w[0] : 1.f - w[0]) * (d2 ? w[1] : 1.f - w[1]) * (d3 ? w[2] : 1.f - w[2]);
+        const int g = ((i0[0] + d1) * tb.T + (i0[1] + d2)) * tb.T + (i0[2] + d3);
+        a += wt * tb.t[2 * g];
+        b += wt * tb.t[2 * g + 1];
+    }
+    lsr = a;
+    lst = b;
+}
+
+template <int G>
+__device__ __forceinline__ void class_features(float cmin2, float invR, float r, float f[3])
+{
+    // cmin2: this lane's minimum squared distance over its slots
+#pragma unroll
+    for (int off = 8; off < G; off <<= 1) cmin2 = fminf(cmin2, __shfl_xor(cmin2, off, G));
+    float d = fminf(sqrtf(cmin2), kDcap);
+    float mn = d, sm = d;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off, G));
+        sm += __shfl_xor(sm, off, G);
+    }
+    f[0] = r * invR;
+    f[1] = mn;
+    f[2] = 0.125f * sm;
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, float invR)
+{
+    extern __shared__ __align__(16) unsigned char smem3[];
+    float2* lds = reinterpret_cast<float2*>(smem3);
+    constexpr int WPB = 64 / G;
+    const int N = a.N;
+    const int S = (N + G - 1) / G;
+    const int NP = S * G;
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    const bool live = walker < a.walkers;
+    float2* P = lds + grp * NP;
+    float2* gpos = reinterpret_cast<float2*>(a.pos) + (size_t)(live ? walker : 0) * N;
+    if (live)
+        for (int j = lane; j < N; j += G) P[j] = gpos[j];
+    __syncthreads();
+    if (!live) return;
+    const float mfac = (float)a.m * 0.69314718055994531f;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const bool has_pad = (lane + G * (S - 1)) >= N;
+    const int pad_j = lane + G * (S - 1);
+    const float2 far = make_float2(1e4f, 1e4f);
+    unsigned accepted = 0;
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        for (int base = 0; base < N; base += G) {
+            uint32_t r[4];
+            philox_l((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
+            const float rr = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01l(r[0])));
+            const float tt = u01l(r[1]);
+            const float g1 = rr * __builtin_amdgcn_cosf(tt), g2 = rr * __builtin_amdgcn_sinf(tt);
+            const uint32_t ubits = r[2];
+            const int cnt = min(G, N - base);
+            for (int t = 0; t < cnt; ++t) {
+                const int i = base + t;
+                const float n1 = __shfl(g1, t, G), n2 = __shfl(g2, t, G);
+                const uint32_t ub = __shfl(ubits, t, G);
+                const float2 zo = P[i];
+                // pass 1: class minima at z
+                P[i] = far;
+                if (has_pad) P[pad_j] = far;
+                float c2min = 1e30f;
+                for (int s = 0; s < S; ++s) {
+                    const float2 q = P[lane + G * s];
+                    const float dx = zo.x - q.x, dy = zo.y - q.y;
+                    c2min = fminf(c2min, dx * dx + dy * dy);
+                }
+                const float r0 = sqrtf(zo.x * zo.x + zo.y * zo.y);
+                float f[3];
+                class_features<G>(c2min, invR, r0, f);
+                float lsr, lst;
+                widths_lookup(tb, f, lsr, lst);
+                const float sr = __expf(lsr), st = __expf(lst);
+                const float cth = r0 > 0.f ? zo.x / r0 : 1.f, sth = r0 > 0.f ? zo.y / r0 : 0.f;
+                const float dr = sr * n1, dt = st * n2;
+                const float xn = zo.x + cth * dr - sth * dt, yn = zo.y + sth * dr + cth * dt;
+                // pass 2: ratio (midpoint in slot i and pads) and class minima at z'
+                float2 mid;
+                mid.x = 0.5f * (zo.x + xn);
+                mid.y = 0.5f * (zo.y + yn);
+                P[i] = mid;
+                if (has_pad) P[pad_j] = mid;
+                const int skip_i = ((i % G) == lane) ? (i / G) : -1;
+                const int skip_p = has_pad ? (S - 1) : -1;
+                float lsum = 0.f, pn = 1.f, pd = 1.f, a2min = 1e30f;
+                for (int s = 0; s < S; ++s) {
+                    const float2 q = P[lane + G * s];
+                    const float ax = xn - q.x, ay = yn - q.y;
+                    const float bx = zo.x - q.x, by = zo.y - q.y;
+                    const float A2 = ax * ax + ay * ay;
+                    pn *= A2;
+                    pd *= bx * bx + by * by;
+                    a2min = (s == skip_i || s == skip_p) ? a2min : fminf(a2min, A2);
+                    if ((s & 7) == 7) {
+                        lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
+                        pn = 1.f;
+                        pd = 1.f;
+                    }
+                }
+                lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
+#pragma unroll
+                for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
+                const float rn = sqrtf(xn * xn + yn * yn);
+                float fp[3];
+                class_features<G>(a2min, invR, rn, fp);
+                float lsrp, lstp;
+                widths_lookup(tb, fp, lsrp, lstp);
+                const float srp = __expf(lsrp), stp = __expf(lstp);
+                const float cp = rn > 0.f ? xn / rn : 1.f, sp = rn > 0.f ? yn / rn : 0.f;
+                const float wx = zo.x - xn, wy = zo.y - yn;
+                const float Dr = wx * cp + wy * sp, Dt = -wx * sp + wy * cp;
+                const float lq = -(lsrp + lstp) - 0.5f * ((Dr / srp) * (Dr / srp) + (Dt / stp) * (Dt / stp)) +
+                                 (lsr + lst) + 0.5f * (n1 * n1 + n2 * n2);
+                const float logA = mfac * lsum - 0.5f * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y)) + lq;
+                const bool acc =
+                    (logA >= 0.f) || (__builtin_amdgcn_logf(u01l(ub)) * 0.69314718055994531f < logA);
+                float2 nz;
+                nz.x = acc ? xn : zo.x;
+                nz.y = acc ? yn : zo.y;
+                P[i] = nz;
+                accepted += acc ? 1u : 0u;
+            }
+        }
+    }
+    for (int j = lane; j < N; j += G) gpos[j] = P[j];
+    if (lane == 0) a.accepted[walker] += accepted;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_mlp_init(float* P, unsigned long long seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mlp_init, dim3((kP + 255) / 256), dim3(256), 0, s, P, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_train_samples(int B, int ncfg, int N, unsigned long long seed, int step, int* cfg, int* part,
+                                float* noise, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_train_samples, dim3((B + 255) / 256), dim3(256), 0, s, B, ncfg, N, seed, step, cfg, part, noise);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_sample(const float* P, const float* cfgs, int N, int m, float invR, float sigma0, const int* cfg,
+                             const int* part, const float* noise, int B, float* recs, double* stats, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mlp_sample, dim3((B + 127) / 128), dim3(128), 0, s, P, (const float2*)cfgs, N, m, invR, sigma0,
+                       cfg, part, noise, B, recs, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const float* recs, int B, float* grad, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mlp_grad, dim3(kP), dim3(256), 0, s, recs, 2 * B, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* P, float* M1, float* M2, const float* G, float lr, int t, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, dim3((kP + 255) / 256), dim3(256), 0, s, P, M1, M2, G, kP, lr, 0.9f, 0.999f, 1e-8f, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_table(const float* P, int T, const float* lo, const float* hi, float sigma0, float* table,
+                            hipStream_t s)
+{
+    const int n = T * T * T;
+    hipLaunchKernelGGL(k_mlp_table, dim3((n + 127) / 128), dim3(128), 0, s, P, T, lo, hi, sigma0, table);
+    return hipGetLastError();
+}
+
+size_t learned_lds_bytes(int G, int N) { return (size_t)(64 / G) * (size_t)((N + G - 1) / G) * G * sizeof(float2); }
+
+template <int G>
+static hipError_t sweep_learned_g(const SweepArgs& a, const LearnedTable& lt, hipStream_t s)
+{
+    Table3 tb;
+    tb.t = lt.table;
+    tb.T = lt.T;
+    for (int q = 0; q < 3; ++q) {
+        tb.lo[q] = lt.lo[q];
+        tb.inv_h[q] = (float)(lt.T - 1) / (lt.hi[q] - lt.lo[q]);
+    }
+    const size_t lds = learned_lds_bytes(G, a.N);
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep_learned<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sweep_learned<G>), grid, dim3(64), lds, s, a, tb, (float)lt.invR);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_learned(const SweepArgs& a, const LearnedTable& lt, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    switch (G) {
+    case 8: return sweep_learned_g<8>(a, lt, s);
+    case 16: return sweep_learned_g<16>(a, lt, s);
+    case 32: return sweep_learned_g<32>(a, lt, s);
+    case 64: return sweep_learned_g<64>(a, lt, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace le

@@ -31,8 +31,24 @@ EXPORTED = [
     "le_get_moments", "le_get_trace", "le_get_positions", "le_set_positions",
     "le_accumulate_positions", "le_reset_moments", "le_geneig", "le_engine_spectrum",
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
-    "le_evolve", "le_engine_create_occupation", "le_get_occupation",
+    "le_evolve", "le_engine_create_occupation", "le_get_occupation", "le_train_params_default",
+    "le_train_proposal", "le_set_proposal", "le_get_proposal_params", "le_set_proposal_params",
+    "le_proposal_loss_grad",
 ]
+PROPOSAL_PLAIN = 0
+PROPOSAL_LEARNED = 1
+N_MLP_PARAMS = 4546
+
+
+class TrainParams(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("lr", ctypes.c_double), ("batch", ctypes.c_int),
+                ("n_configs", ctypes.c_int), ("snapshot_sweeps", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("sigma0", ctypes.c_double), ("table_size", ctypes.c_int)]
+
+
+class TrainReport(ctypes.Structure):
+    _fields_ = [("loss_first", ctypes.c_double), ("loss_last", ctypes.c_double),
+                ("accept_first", ctypes.c_double), ("accept_last", ctypes.c_double), ("ms", ctypes.c_double)]
 
 TIME_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
 
@@ -234,6 +250,43 @@ class Engine:
         """fn_ptr: address of a C function with the le_weight_fn signature"""
         self._wfn = fn_ptr
         _check(lib().le_set_weight_callback(self._h, ctypes.c_void_p(fn_ptr), ctypes.c_void_p(ctx)))
+
+    def train_proposal(self, **kw):
+        tp = TrainParams()
+        lib().le_train_params_default(ctypes.byref(tp))
+        for k, v in kw.items():
+            if not hasattr(tp, k):
+                raise TypeError(f"unknown training parameter {k}")
+            setattr(tp, k, v)
+        rep = TrainReport()
+        _check(lib().le_train_proposal(self._h, ctypes.byref(tp), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in TrainReport._fields_}
+
+    def set_proposal(self, kind):
+        _check(lib().le_set_proposal(self._h, int(kind)))
+
+    def proposal_params(self):
+        out = np.zeros(N_MLP_PARAMS, dtype=np.float32)
+        _check(lib().le_get_proposal_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def set_proposal_params(self, p, sigma0, table_size=24):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().le_set_proposal_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            ctypes.c_double(sigma0), int(table_size)))
+
+    def proposal_loss_grad(self, configs, cfg, part, noise, sigma0):
+        configs = np.ascontiguousarray(configs, dtype=np.float32)
+        cfg = np.ascontiguousarray(cfg, dtype=np.int32)
+        part = np.ascontiguousarray(part, dtype=np.int32)
+        noise = np.ascontiguousarray(noise, dtype=np.float32)
+        loss = ctypes.c_double()
+        grad = np.zeros(N_MLP_PARAMS, dtype=np.float32)
+        fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        _check(lib().le_proposal_loss_grad(self._h, fp(configs), configs.shape[0], ip(cfg), ip(part), fp(noise),
+                                           len(cfg), ctypes.c_double(sigma0), ctypes.byref(loss), fp(grad)))
+        return loss.value, grad
 
     def trace(self):
         n = ctypes.c_long()

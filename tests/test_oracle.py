@@ -282,3 +282,29 @@ def test_exact_occupation_sum_rules():
     V = 0.05 * (l + 1) + 4e-3 * (l + 1) * (l + 2)
     assert np.allclose(O.sum(-1), N * S, rtol=1e-12)
     assert np.allclose((O * V).sum(-1), H, rtol=1e-12)
+
+
+def test_learned_oracle_gradient_by_finite_differences():
+    """the oracle's analytic loss gradient (through both MLP evaluations, the reverse
+    density and the features at z') against central finite differences"""
+    from oracle import learned
+    rng = np.random.default_rng(3)
+    N, m = 12, 3
+    R = np.sqrt(2 * m * N)
+    p = learned.init_params(rng)
+    p += rng.normal(size=p.shape) * 0.05
+    configs, idx, noise = [], [], []
+    for b in range(6):
+        r = R * np.sqrt(rng.random(N))
+        t = 2 * np.pi * rng.random(N)
+        configs.append(r * np.exp(1j * t))
+        idx.append(int(rng.integers(N)))
+        noise.append(rng.normal(size=2))
+    L, G = learned.batch_loss_grad(p, configs, idx, noise, m, R, 0.7)
+    for k in rng.choice(learned.NPARAM, 25, replace=False).tolist() + [0, 100, 4400, 4545]:
+        e = np.zeros_like(p)
+        e[k] = 1e-6
+        Lp, _ = learned.batch_loss_grad(p + e, configs, idx, noise, m, R, 0.7)
+        Lm, _ = learned.batch_loss_grad(p - e, configs, idx, noise, m, R, 0.7)
+        fd = (Lp - Lm) / 2e-6
+        assert abs(fd - G[k]) < 1e-5 + 1e-4 * abs(fd), (k, fd, G[k])

@@ -82,10 +82,21 @@ struct le_engine {
     // device buffers
     void* pos = nullptr;         // walkers x N x (float2|double2)
     unsigned* acc = nullptr;
-    void* X = nullptr;           // rows x ldx
+    void* X = nullptr;           // rows x ldx: Xb[0] (fixed-sample and two-sided paths)
+    void* Xb[2] = {nullptr, nullptr};     // double buffer: sampler fills one, Gram reads the other
+    void* Wbb[2] = {nullptr, nullptr};
+    hipStream_t gstream = nullptr;        // Gram accumulation stream
+    hipEvent_t filled[2] = {nullptr, nullptr}, drained[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    hipEvent_t next_event();
     uint32_t* fac = nullptr;
     int* col_re = nullptr;
     int* col_dim = nullptr;
+    unsigned long long* prod_d = nullptr;   // per-state product plan over the power table
+    int* pwk_d = nullptr;
+    int* pwr_d = nullptr;
+    int npw = 0;
     long ldx = 0;
     // weights: row 0 = V, rows 1..2 nh = cos/sin harmonics, then user weights
     void* Wb = nullptr;
@@ -145,12 +156,12 @@ struct le_engine {
 
     ~le_engine();
     int setup(const le_observables* obs);
-    const void* weight_ptr(int wid) const;
+    const void* weight_ptr(int wid, int buf = 0) const;
     long item_offset(int wid, int s1, int s2) const;
     int autotune_lanes();
     int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions);
     int close_block(long long samples);
-    int gram_rows(long rows);
+    int gram_rows(long rows, int buf = 0, hipStream_t s = nullptr);
     double measure_acceptance(double trial_step, int attempt, int* rc);
     hipError_t features2(const void* positions, int pos_is_double, int nsamp, long row0);
 };
@@ -184,11 +195,22 @@ le_engine::~le_engine()
 {
     (void)hipFree(pos);
     (void)hipFree(acc);
-    (void)hipFree(X);
+    (void)hipFree(Xb[0]);
+    (void)hipFree(Xb[1]);
     (void)hipFree(fac);
     (void)hipFree(col_re);
     (void)hipFree(col_dim);
-    (void)hipFree(Wb);
+    (void)hipFree(prod_d);
+    (void)hipFree(pwk_d);
+    (void)hipFree(pwr_d);
+    (void)hipFree(Wbb[0]);
+    (void)hipFree(Wbb[1]);
+    for (auto q : evpool) (void)hipEventDestroy(q);
+    for (int b = 0; b < 2; ++b) {
+        if (filled[b]) (void)hipEventDestroy(filled[b]);
+        if (drained[b]) (void)hipEventDestroy(drained[b]);
+    }
+    if (gstream) (void)hipStreamDestroy(gstream);
     for (auto& j : jobs) {
         (void)hipFree(j.tiles);
         (void)hipFree(j.C[0]);
@@ -228,10 +250,11 @@ extern "C" void le_mc_params_default(le_mc_params* p)
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
 
-const void* le_engine::weight_ptr(int wid) const
+const void* le_engine::weight_ptr(int wid, int buf) const
 {
     const size_t rsz_ = rsz;
-    auto row = [&](long r) { return (const void*)((const char*)Wb + (size_t)r * wrows * rsz_); };
+    const void* base = Wbb[buf];
+    auto row = [&](long r) { return (const void*)((const char*)base + (size_t)r * wrows * rsz_); };
     if (wid == LE_W_ONE) return nullptr;
     if (wid == LE_W_V) return row(0);
     if (wid >= LE_W_USER(0)) {
@@ -374,13 +397,54 @@ int le_engine::setup(const le_observables* obs)
         HIPCHK(hipMemcpy(side_d, side.data(), sizeof(int) * D, hipMemcpyHostToDevice));
     }
     HIPCHK(dalloc(&acc, (size_t)mc.walkers));
-    HIPCHK(hipMalloc(&X, (size_t)rows * ldx * rsz));
-    HIPCHK(hipMemsetAsync(X, 0, (size_t)rows * ldx * rsz, stream));
-    HIPCHK(hipMalloc(&Wb, (size_t)(1 + 2 * nh + n_user) * rows * rsz));
-    HIPCHK(hipMemsetAsync(Wb, 0, (size_t)(1 + 2 * nh + n_user) * rows * rsz, stream));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(hipMalloc(&Xb[b], (size_t)rows * ldx * rsz));
+        HIPCHK(hipMemsetAsync(Xb[b], 0, (size_t)rows * ldx * rsz, stream));
+        HIPCHK(hipMalloc(&Wbb[b], (size_t)(1 + 2 * nh + n_user) * rows * rsz));
+        HIPCHK(hipMemsetAsync(Wbb[b], 0, (size_t)(1 + 2 * nh + n_user) * rows * rsz, stream));
+        HIPCHK(hipEventCreateWithFlags(&filled[b], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&drained[b], hipEventDisableTiming));
+    }
+    X = Xb[0];
+    Wb = Wbb[0];
+    HIPCHK(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
     auto ft = basis.factor_table();
     HIPCHK(dalloc(&fac, std::max<size_t>(1, ft.size())));
     if (!ft.empty()) HIPCHK(hipMemcpy(fac, ft.data(), ft.size() * 4, hipMemcpyHostToDevice));
+    {
+        // power table p_k^r for k r <= K, and each state's plan of table indices
+        std::vector<int> pk, pr;
+        std::vector<int> off(basis.K + 2, 0);
+        for (int k = 1; k <= basis.K; ++k) {
+            off[k] = (int)pk.size();
+            for (int r = 1; k * r <= basis.K; ++r) {
+                pk.push_back(k);
+                pr.push_back(r);
+            }
+        }
+        npw = (int)pk.size();
+        if (npw > 128) return fail(LE_ERR_CAPACITY, "power table larger than 128 entries");
+        std::vector<unsigned long long> plan(std::max(1, D), 0ull);
+        for (int t = 0; t < D; ++t) {
+            int cnt = 0;
+            unsigned long long w = 0;
+            for (const auto& pp : basis.states[t].pairs) {
+                if (pp.number == 0) continue;
+                const unsigned long long idx = (unsigned long long)(off[pp.number] + pp.repetitions - 1);
+                w |= idx << (4 + 7 * cnt);
+                ++cnt;
+            }
+            plan[t] = w | (unsigned long long)cnt;
+        }
+        HIPCHK(dalloc(&prod_d, plan.size()));
+        HIPCHK(hipMemcpy(prod_d, plan.data(), plan.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&pwk_d, (size_t)std::max(1, npw)));
+        HIPCHK(dalloc(&pwr_d, (size_t)std::max(1, npw)));
+        if (npw) {
+            HIPCHK(hipMemcpy(pwk_d, pk.data(), npw * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(pwr_d, pr.data(), npw * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     HIPCHK(dalloc(&col_re, (size_t)std::max(1, D)));
     HIPCHK(dalloc(&col_dim, (size_t)std::max(1, D)));
     if (D) {
@@ -721,14 +785,25 @@ extern "C" int le_burnin(le_engine* e, int sweeps)
     return LE_OK;
 }
 
-int le_engine::gram_rows(long rows)
+hipEvent_t le_engine::next_event()
 {
+    if (evused == evpool.size()) {
+        hipEvent_t q = nullptr;
+        if (hipEventCreate(&q) != hipSuccess) return nullptr;
+        evpool.push_back(q);
+    }
+    return evpool[evused++];
+}
+
+int le_engine::gram_rows(long rows, int buf, hipStream_t s)
+{
+    if (!s) s = stream;
     for (auto& j : jobs) {
         le::GramArgs g{};
-        g.X = X;
+        g.X = Xb[buf];
         g.ldx = ldx;
-        g.w0 = weight_ptr(j.wid[0]);
-        g.w1 = j.nw > 1 ? weight_ptr(j.wid[1]) : nullptr;
+        g.w0 = weight_ptr(j.wid[0], buf);
+        g.w1 = j.nw > 1 ? weight_ptr(j.wid[1], buf) : nullptr;
         g.nw = j.nw;
         g.rows = rows;
         const long want_wg = 2048;
@@ -743,7 +818,7 @@ int le_engine::gram_rows(long rows)
         g.C0 = j.C[0];
         g.C1 = j.C[1];
         g.ldc = ldx;
-        HIPCHK(le::launch_gram(g, prec, (int)splits, stream));
+        HIPCHK(le::launch_gram(g, prec, (int)splits, s));
     }
     return LE_OK;
 }
@@ -794,18 +869,38 @@ extern "C" int le_sample(le_engine* e, int samples)
     f.fac = e->fac;
     f.col_re = e->col_re;
     f.col_dim = e->col_dim;
+    f.prod = e->prod_d;
+    f.pw_k = e->pwk_d;
+    f.pw_r = e->pwr_d;
+    f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
     const int tw = std::min(e->mc.trace_walkers, e->mc.walkers);
     long long kept = 0;
-    for (int done = 0; done < samples;) {
+    // Rounds alternate between two sample buffers: the sampler stream fills buffer b while
+    // the Gram stream accumulates the other one; events order the reuse of each buffer.
+    struct Span { hipEvent_t a, b; int kind; };   // 0 sweep, 1 features, 2 gram
+    std::vector<Span> spans;
+    e->evused = 0;
+    auto ev = [&]() {
+        hipEvent_t q = e->next_event();
+        return q;
+    };
+    int round = 0;
+    for (int done = 0; done < samples; ++round) {
         const int n = std::min(e->spl, samples - done);
+        const int b = round & 1;
+        HIPCHK(hipStreamWaitEvent(e->stream, e->drained[b], 0));
+        f.X = e->Xb[b];
+        f.W = e->Wbb[b];
         for (int q = 0; q < n; ++q) {
-            HIPCHK(hipEventRecord(e->ev[2 * q], e->stream));
+            hipEvent_t e0 = ev(), e1 = ev(), e2 = ev();
+            if (!e0 || !e1 || !e2) return fail(LE_ERR_HIP, "event creation failed");
+            HIPCHK(hipEventRecord(e0, e->stream));
             int rc = e->run_sweeps(e->mc.thin, e->sweep_counter, e->pos);
             if (rc) return rc;
             e->sweep_counter += (unsigned long long)e->mc.thin;
-            HIPCHK(hipEventRecord(e->ev[2 * q + 1], e->stream));
+            HIPCHK(hipEventRecord(e1, e->stream));
             f.row0 = (long)q * e->mc.walkers;
             if (e->trace_d && e->trace_rows + tw <= e->trace_cap_rows) {
                 f.trace = e->trace_d;
@@ -816,34 +911,48 @@ extern "C" int le_sample(le_engine* e, int samples)
                 f.trace = nullptr;
                 f.trace_walkers = 0;
             }
-            if (e->two_sided) HIPCHK(e->features2(e->pos, 0, e->mc.walkers, f.row0));
-            else HIPCHK(le::launch_features(f, e->prec, e->stream));
+            if (e->two_sided) {
+                e->X = e->Xb[b];
+                e->Wb = e->Wbb[b];
+                HIPCHK(e->features2(e->pos, 0, e->mc.walkers, f.row0));
+                e->X = e->Xb[0];
+                e->Wb = e->Wbb[0];
+            } else {
+                HIPCHK(le::launch_features(f, e->prec, e->stream));
+            }
             if (e->wfn && e->n_user && !e->two_sided) {
-                void* wdst = (char*)e->Wb + ((size_t)(1 + 2 * e->nh) * e->wrows + f.row0) * e->rsz;
+                void* wdst = (char*)e->Wbb[b] + ((size_t)(1 + 2 * e->nh) * e->wrows + f.row0) * e->rsz;
                 if (e->wfn(e->wctx, e->pos, e->prec == LE_PREC_FP64, e->mc.walkers, e->sys.particles_number, wdst,
                            e->wrows, e->prec == LE_PREC_FP64, (void*)e->stream) != 0)
                     return fail(LE_ERR_STATE, "user weight callback failed");
             }
+            HIPCHK(hipEventRecord(e2, e->stream));
+            spans.push_back({e0, e1, 0});
+            spans.push_back({e1, e2, 1});
         }
-        HIPCHK(hipEventRecord(e->ev[2 * n], e->stream));
-        int rc = e->gram_rows((long)n * e->mc.walkers);
+        HIPCHK(hipEventRecord(e->filled[b], e->stream));
+        HIPCHK(hipStreamWaitEvent(e->gstream, e->filled[b], 0));
+        hipEvent_t g0 = ev(), g1 = ev();
+        if (!g0 || !g1) return fail(LE_ERR_HIP, "event creation failed");
+        HIPCHK(hipEventRecord(g0, e->gstream));
+        int rc = e->gram_rows((long)n * e->mc.walkers, b, e->gstream);
         if (rc) return rc;
-        HIPCHK(hipEventRecord(e->ev[2 * n + 1], e->stream));
-        HIPCHK(hipEventSynchronize(e->ev[2 * n + 1]));
-        for (int q = 0; q < n; ++q) {
-            float a = 0, b = 0;
-            HIPCHK(hipEventElapsedTime(&a, e->ev[2 * q], e->ev[2 * q + 1]));
-            HIPCHK(hipEventElapsedTime(&b, e->ev[2 * q + 1], e->ev[2 * q + 2]));
-            e->st.ms_sweep += a;
-            e->st.ms_features += b;
-        }
-        float gm = 0;
-        HIPCHK(hipEventElapsedTime(&gm, e->ev[2 * n], e->ev[2 * n + 1]));
-        e->st.ms_gram += gm;
+        HIPCHK(hipEventRecord(g1, e->gstream));
+        HIPCHK(hipEventRecord(e->drained[b], e->gstream));
+        spans.push_back({g0, g1, 2});
         e->st.gram_launches += 1;
         e->st.sweep_launches += n;
         done += n;
         kept += n;
+    }
+    HIPCHK(hipStreamSynchronize(e->gstream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (auto& sp : spans) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, sp.a, sp.b));
+        if (sp.kind == 0) e->st.ms_sweep += ms;
+        else if (sp.kind == 1) e->st.ms_features += ms;
+        else e->st.ms_gram += ms;
     }
     e->sweeps_done += (long long)samples * e->mc.thin;
     const long long nsamp = kept * e->mc.walkers;
@@ -1040,6 +1149,10 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
     f.fac = e->fac;
     f.col_re = e->col_re;
     f.col_dim = e->col_dim;
+    f.prod = e->prod_d;
+    f.pw_k = e->pwk_d;
+    f.pw_r = e->pwr_d;
+    f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
     int rc = LE_OK;

@@ -21,6 +21,7 @@
 //  generalized eigenproblem ......... modules/eigensolver/generalized_eigenproblem.cpp:45-52
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "le_kernels.hpp"
@@ -114,6 +115,42 @@ template <> struct LogOps<double> {
     static constexpr double to_ln = 1.0;
 };
 
+// Sum over the G lanes of a group, identical in every lane.  Within a 16-lane DPP row the
+// pairing steps are quad xor 1, quad xor 2, half-row mirror and row mirror (each pairs
+// lanes that hold equal partial sums of disjoint lane sets, so the result is the same in
+// every lane); wider groups add ds_swizzle xor steps across rows.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int G, typename Real>
+__device__ __forceinline__ Real group_sum(Real v)
+{
+    if constexpr (sizeof(Real) == 4) {
+        if constexpr (G >= 2) v += dpp_f<0xB1>(v);
+        if constexpr (G >= 4) v += dpp_f<0x4E>(v);
+        if constexpr (G >= 8) v += dpp_f<0x141>(v);
+        if constexpr (G >= 16) v += dpp_f<0x140>(v);
+    } else {
+        if constexpr (G >= 2) v += dpp_d<0xB1>(v);
+        if constexpr (G >= 4) v += dpp_d<0x4E>(v);
+        if constexpr (G >= 8) v += dpp_d<0x141>(v);
+        if constexpr (G >= 16) v += dpp_d<0x140>(v);
+    }
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 template <int G, typename Real>
 __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 {
@@ -129,6 +166,12 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
     const int walker = blockIdx.x * WPB + grp;
     const bool live = walker < a.walkers;
     V2* P = lds + grp * NP;
+    // per-group random numbers of the current batch of G moves: (n1, n2, u bits)
+    struct Rnd {
+        Real n1, n2;
+        uint32_t ub, pad;
+    };
+    Rnd* rnd = reinterpret_cast<Rnd*>(lds + WPB * NP) + grp * G;
     V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)(live ? walker : 0) * N;
     if (live)
         for (int j = lane; j < N; j += G) P[j] = gpos[j];
@@ -151,13 +194,16 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
             philox((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
             Real g1, g2;
             gauss2(r[0], r[1], g1, g2);
-            uint32_t ubits = r[2];
+            rnd[lane] = Rnd{g1, g2, r[2], 0u};
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int cnt = min(G, N - base);
             for (int t = 0; t < cnt; ++t) {
                 const int i = base + t;
-                const Real d1 = __shfl(g1, t, G);
-                const Real d2 = __shfl(g2, t, G);
-                const uint32_t ub = __shfl(ubits, t, G);
+                const Rnd rn = rnd[t];
+                const Real d1 = rn.n1, d2 = rn.n2;
+                const uint32_t ub = rn.ub;
                 const V2 zo = P[i];
                 const Real xn = zo.x + step * d1, yn = zo.y + step * d2;
                 V2 mid;
@@ -189,8 +235,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                     pd *= bx * bx + by * by;
                 }
                 lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
-#pragma unroll
-                for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
+                lsum = group_sum<G, Real>(lsum);
                 const Real logA = mfac * lsum -
                                   (Real)0.5 * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y));
                 bool acc;
@@ -205,6 +250,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 P[i] = nz;
                 accepted += acc ? 1u : 0u;
             }
+            __builtin_amdgcn_wave_barrier();   // rnd is rewritten by the next batch
         }
     }
     for (int j = lane; j < N; j += G) gpos[j] = P[j];
@@ -221,115 +267,155 @@ __device__ __forceinline__ double potential_r2(const Potential& p, double r2)
     return v;
 }
 
-template <typename Real, int KMAX>
+// One wavefront per sample (grid-stride over samples).  Power sums in FP64 are reduced with
+// a butterfly; a per-sample power table p_k^r (k r <= K) is built in LDS by all lanes, and
+// each monomial is a product of at most 8 table entries following a packed plan.
+constexpr int kMaxPow = 128;
+constexpr int kMaxStatesLds = 4096;
+
+template <typename Real, int KMAX, bool HARM>
 __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
 {
-    constexpr int NQ = 2 * KMAX + 1 + 2 * kMaxHarmonics;   // Re/Im p_1..p_K, V, cos/sin sums
+    constexpr int NH = HARM ? kMaxHarmonics : 0;
+    constexpr int NQ = 2 * KMAX + 1 + 2 * NH;   // Re/Im p_1..p_K, V, cos/sin sums
     constexpr int QV = 2 * KMAX;
     constexpr int QH = 2 * KMAX + 1;
-    __shared__ double pk[4][NQ];
+    __shared__ double sq[4][NQ];
+    __shared__ double2 ptab[4][kMaxPow];
+    __shared__ unsigned long long plan[kMaxStatesLds];
+    __shared__ uint32_t cols[kMaxStatesLds];     // col_re | col_dim << 16
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int w = blockIdx.x * 4 + wv;
-    if (w >= a.walkers) return;                 // whole wavefront leaves together
-    const int K = a.K;
-    const int nh = a.nh;
-
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int j = lane; j < a.N; j += 64) {
-        double x, y;
-        if (a.pos_is_double || sizeof(Real) == 8) {
-            double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
-            x = q.x; y = q.y;
-        } else {
-            float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
-            x = q.x; y = q.y;
+    const int D = a.D;
+    const bool plan_lds = D <= kMaxStatesLds;
+    if (plan_lds)
+        for (int t = threadIdx.x; t < D; t += blockDim.x) {
+            plan[t] = a.prod[t];
+            cols[t] = (uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16);
         }
-        const double r2 = x * x + y * y;
-        acc[QV] += potential_r2(a.pot, r2);
-        const double ur = x * a.invR, ui = y * a.invR;
-        double qr = ur, qi = ui;
+    __syncthreads();
+    const int nh = HARM ? a.nh : 0;
+    const int waves = gridDim.x * 4;
+    for (int w = blockIdx.x * 4 + wv; w < a.walkers; w += waves) {
+        double acc[NQ];
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            acc[2 * k] += qr;
-            acc[2 * k + 1] += qi;
-            const double t = qr * ur - qi * ui;
-            qi = qr * ui + qi * ur;
-            qr = t;
-        }
-        if (nh > 0) {
-            // e^{i L theta} = (z / |z|)^L by binary powering
-            const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
-            const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
+        for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+        for (int j = lane; j < a.N; j += 64) {
+            double x, y;
+            if (a.pos_is_double || sizeof(Real) == 8) {
+                double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
+                x = q.x; y = q.y;
+            } else {
+                float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
+                x = q.x; y = q.y;
+            }
+            const double r2 = x * x + y * y;
+            acc[QV] += potential_r2(a.pot, r2);
+            const double ur = x * a.invR, ui = y * a.invR;
+            double qr = ur, qi = ui;
 #pragma unroll
-            for (int h = 0; h < kMaxHarmonics; ++h) {
-                if (h < nh) {
-                    unsigned e = (unsigned)a.harm[h];
-                    double br = er, bi = ei, pr = 1.0, pi = 0.0;
-                    while (e) {
-                        if (e & 1u) {
-                            const double t = pr * br - pi * bi;
-                            pi = pr * bi + pi * br;
-                            pr = t;
+            for (int k = 0; k < KMAX; ++k) {
+                acc[2 * k] += qr;
+                acc[2 * k + 1] += qi;
+                const double t = qr * ur - qi * ui;
+                qi = qr * ui + qi * ur;
+                qr = t;
+            }
+            if constexpr (HARM) {
+                // e^{i L theta} = (z / |z|)^L by binary powering
+                const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+                const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    if (h < nh) {
+                        unsigned e = (unsigned)a.harm[h];
+                        double br = er, bi = ei, pr = 1.0, pi = 0.0;
+                        while (e) {
+                            if (e & 1u) {
+                                const double t = pr * br - pi * bi;
+                                pi = pr * bi + pi * br;
+                                pr = t;
+                            }
+                            e >>= 1;
+                            const double t = br * br - bi * bi;
+                            bi = 2.0 * br * bi;
+                            br = t;
                         }
-                        e >>= 1;
-                        const double t = br * br - bi * bi;
-                        bi = 2.0 * br * bi;
-                        br = t;
+                        acc[QH + 2 * h] += pr;
+                        acc[QH + 2 * h + 1] += pi;
                     }
-                    acc[QH + 2 * h] += pr;
-                    acc[QH + 2 * h + 1] += pi;
                 }
             }
         }
-    }
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        if (q >= QH + 2 * nh) continue;
-        double v = acc[q];
+        for (int q = 0; q < NQ; ++q) {
+            double v = acc[q];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-        acc[q] = v;
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) pk[wv][q] = acc[q];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    const long row = a.row0 + w;
-    Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
-    for (int t = lane; t < a.D; t += 64) {
-        double vr = 1.0, vi = 0.0;
-        const uint32_t* f = a.fac + (size_t)t * 8;
-#pragma unroll 1
-        for (int q = 0; q < 8; ++q) {
-            const uint32_t word = f[q];
-            if (word == 0u) break;
-            const int k = (int)(word & 0xffu), r = (int)(word >> 8);
-            const double br = pk[wv][2 * (k - 1)], bi = pk[wv][2 * (k - 1) + 1];
-            for (int e = 0; e < r; ++e) {
-                const double tr = vr * br - vi * bi;
-                vi = vr * bi + vi * br;
-                vr = tr;
-            }
+            for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+            acc[q] = v;
         }
-        const int c = a.col_re[t];
-        X[c] = (Real)vr;
-        X[c + a.col_dim[t]] = (Real)vi;
-    }
-    Real* W = reinterpret_cast<Real*>(a.W);
-    if (lane == 0) W[row] = (Real)pk[wv][QV];
-    if (lane < 2 * nh) W[(size_t)(1 + lane) * a.wld + row] = (Real)pk[wv][QH + lane];
-    if (a.trace && w < a.trace_walkers && lane < 9) {
-        float v;
-        if (lane < 8) v = (lane / 2 < K) ? (float)pk[wv][lane] : 0.f;
-        else v = (float)pk[wv][QV];
-        a.trace[((size_t)a.trace_row0 + w) * 9 + lane] = v;
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) sq[wv][q] = acc[q];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // power table: entry e = p_k^r, one lane per entry (binary powering)
+        for (int e = lane; e < a.npw; e += 64) {
+            const int k = a.pw_k[e];
+            unsigned r = (unsigned)a.pw_r[e];
+            double br = sq[wv][2 * (k - 1)], bi = sq[wv][2 * (k - 1) + 1], pr = 1.0, pi = 0.0;
+            while (r) {
+                if (r & 1u) {
+                    const double t = pr * br - pi * bi;
+                    pi = pr * bi + pi * br;
+                    pr = t;
+                }
+                r >>= 1;
+                if (r) {
+                    const double t = br * br - bi * bi;
+                    bi = 2.0 * br * bi;
+                    br = t;
+                }
+            }
+            ptab[wv][e] = make_double2(pr, pi);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        const long row = a.row0 + w;
+        Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
+#pragma unroll 2
+        for (int t = lane; t < D; t += 64) {
+            const unsigned long long pl = plan_lds ? plan[t] : a.prod[t];
+            const uint32_t cw = plan_lds ? cols[t] : ((uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16));
+            const int cnt = (int)(pl & 0xfull);
+            double vr = 1.0, vi = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q < cnt) {
+                    const double2 f = ptab[wv][(pl >> (4 + 7 * q)) & 0x7full];
+                    const double tr = vr * f.x - vi * f.y;
+                    vi = vr * f.y + vi * f.x;
+                    vr = tr;
+                }
+            }
+            const int c = (int)(cw & 0xffffu);
+            X[c] = (Real)vr;
+            X[c + (int)(cw >> 16)] = (Real)vi;
+        }
+        Real* W = reinterpret_cast<Real*>(a.W);
+        if (lane == 0) W[row] = (Real)sq[wv][QV];
+        if (HARM && lane < 2 * nh) W[(size_t)(1 + lane) * a.wld + row] = (Real)sq[wv][QH + lane];
+        if (a.trace && w < a.trace_walkers && lane < 9) {
+            float v;
+            if (lane < 8) v = (lane / 2 < a.K) ? (float)sq[wv][lane] : 0.f;
+            else v = (float)sq[wv][QV];
+            a.trace[((size_t)a.trace_row0 + w) * 9 + lane] = v;
+        }
+        __builtin_amdgcn_wave_barrier();   // sq / ptab are reused by the next sample
     }
 }
 
@@ -740,7 +826,8 @@ size_t sweep_lds_bytes(int prec, int G, int N)
 {
     const int S = (N + G - 1) / G;
     const size_t el = (prec == kFP64) ? sizeof(double2) : sizeof(float2);
-    return (size_t)(64 / G) * (size_t)S * G * el;
+    const size_t rnd = (prec == kFP64) ? 24 : 16;   // per-lane random numbers of a move batch
+    return (size_t)(64 / G) * (size_t)S * G * el + 64 * rnd;
 }
 
 bool sweep_supported(int G) { return G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64; }
@@ -792,19 +879,28 @@ hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
     return prec == kFP64 ? sweep_dispatch<double>(a, G, s) : sweep_dispatch<float>(a, G, s);
 }
 
-template <typename Real>
-static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
+template <typename Real, bool HARM>
+static hipError_t features_h(const FeatureArgs& a, hipStream_t s)
 {
-    dim3 grid((unsigned)((a.walkers + 3) / 4));
+    // enough waves to fill the chip, each looping over samples
+    const long blocks = std::min<long>((a.walkers + 3) / 4, 2048);
+    dim3 grid((unsigned)blocks);
     if (a.K <= 8)
-        hipLaunchKernelGGL((k_features<Real, 8>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_features<Real, 8, HARM>), grid, dim3(256), 0, s, a);
     else if (a.K <= 16)
-        hipLaunchKernelGGL((k_features<Real, 16>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_features<Real, 16, HARM>), grid, dim3(256), 0, s, a);
     else if (a.K <= 24)
-        hipLaunchKernelGGL((k_features<Real, 24>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_features<Real, 24, HARM>), grid, dim3(256), 0, s, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
+{
+    if (a.npw > kMaxPow) return hipErrorInvalidValue;
+    return a.nh > 0 ? features_h<Real, true>(a, s) : features_h<Real, false>(a, s);
 }
 
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s)

@@ -36,6 +36,10 @@ struct FeatureArgs {
     const uint32_t* fac;     // [D][8] packed (k | r << 8)
     const int* col_re;       // [D] column of Re p_t (Im at col_re + col_dim)
     const int* col_dim;      // [D] d of the state's sector
+    const unsigned long long* prod;  // [D] packed product plan: count (4 bits) + 8 x 7-bit power-table indices
+    const int* pw_k;         // [npw] power table entry e = p_{pw_k[e]}^{pw_r[e]}
+    const int* pw_r;
+    int npw;                 // <= 128
     double invR;
     Potential pot;
     int nh;                  // angular harmonics L_h: weights sum_i cos(L theta_i), sum_i sin(L theta_i)

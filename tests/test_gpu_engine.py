@@ -75,7 +75,7 @@ def test_fp32_chain_tracks_oracle(oracle_lib):
     print("case fp32 chain N=16", flush=True)
     o = oracle_lib.Oracle(N, m, [0])
     ref, _ = o.plain_positions(W, 0, 1, 0.5, 77, thin=2)
-    for G in (2, 4, 8, 16):
+    for G in (2, 4, 8, 16, 32, 64):
         with Engine(N, m, [0], walkers=W, precision=0, step=0.5, seed=77, lanes_per_walker=G) as e:
             e.init_walkers()
             e.burnin(2)

@@ -1,14 +1,21 @@
 """Benchmark of the edge-state Monte-Carlo hot path on one MI355X.
 
-Workload (N=200 Laughlin droplet, nu=1/3, all sectors dL=1..16 at once):
-  32768 walkers, seed 42, V(r) = 0.5*0.05*r^2 + 5e-5*r^4, FP32 sampler + FP32 MFMA tiles.
-A step is one Metropolis sweep of every walker followed by one kept sample per walker
-accumulated into all sector S and H matrices (one "sample-update" per walker).
+Workload (N=200 Laughlin droplet, nu=1/3, all sectors dL=1..16 at once, D=914):
+  32768 walkers, seed 42, V(r) = 0.5*0.05*r^2 + 5e-5*r^4, initial positions uniform in the
+  disc of radius R = sqrt(2 m N), step tuned to 50% acceptance, then burn-in.
+A step is one Metropolis sweep of every walker followed by one kept sample per walker,
+accumulated into every sector's S and H (one "sample-update" per walker and step).
 
-Prints ONE JSON line: metric/value = S,H sample-updates/s, plus ESS/s, the roofline
-of the dominant kernel (Gram accumulation on MFMA, FP32), the sampler's VALU rate, the
-FP32-vs-FP64 precision on a fixed sample set, and a CPU baseline (the FP64 oracle port,
-OpenMP on the host cores) on a bounded sample of the same workload.
+Prints ONE JSON line.  value = S,H sample-updates/s of the production form (FP32 sampler,
+FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
+  roofline      the Gram accumulation kernel (MFMA bound), live HIP-event timing
+  kernels       sampler (VALU FLOP/s), features (sample-matrix bytes/s), Gram (MFMA FLOP/s)
+  ess           tau_int (Madras-Sokal) and ESS/s over Re/Im p_1..p_4, V and rotation-
+                invariant S entries (|p_1|^2, |p_2|^2, Re conj(p_2) p_1^2, ...)
+  forms         plain FP64 / plain FP32 / learned FP32 at their best thinning: ESS/s,
+                ratios to the plain FP64 form, spectra vs the FP64 form in MC sigma
+  precision     FP32 GPU moments vs the FP64 oracle on one fixed sample set
+  cpu_baseline  the FP64 oracle port (OpenMP, host cores) on a bounded sample
 """
 import argparse
 import json
@@ -26,8 +33,71 @@ SECTORS = list(range(1, 17))
 W_CONF, EPS_CONF = 0.05, 5e-5
 WALKERS = 32768
 SEED = 42
-PEAK_FP32 = 157.3e12       # MI355X FP32 (MFMA and VALU), dense
+PEAK_FP32 = 157.3e12       # MI355X FP32 dense (MFMA; VALU counting packed FMA)
+PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+METRIC = "ESS/s and S,H sample-updates/s at N=200, DeltaL<=16 on 1 GPU; eigenvalue err"
+TRACE_NAMES = ["Re p1", "Im p1", "Re p2", "Im p2", "Re p3", "Im p3", "Re p4", "Im p4", "V"]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def observables(tr):
+    """(T, walkers, 9) trace -> dict name -> (T, walkers) series, including S entries of
+    sectors 1..4 that do not change under a global rotation z -> e^{i phi} z."""
+    p = [tr[:, :, 2 * k].astype(np.float64) + 1j * tr[:, :, 2 * k + 1] for k in range(4)]
+    obs = {nm: tr[:, :, q].astype(np.float64) for q, nm in enumerate(TRACE_NAMES)}
+    obs["|p1|^2"] = np.abs(p[0]) ** 2
+    obs["|p2|^2"] = np.abs(p[1]) ** 2
+    obs["|p1|^4"] = np.abs(p[0]) ** 4
+    obs["Re conj(p2) p1^2"] = (np.conj(p[1]) * p[0] ** 2).real
+    obs["|p3|^2"] = np.abs(p[2]) ** 2
+    obs["Re conj(p3) p1 p2"] = (np.conj(p[2]) * p[0] * p[1]).real
+    obs["|p4|^2"] = np.abs(p[3]) ** 2
+    return obs
+
+
+def taus_of(eng, n_samples, tw):
+    from laughlinedge_amd.stats import tau_int
+    tr = eng.trace()
+    T = min(tr.shape[0] // tw, n_samples)
+    if T < 8:
+        return None
+    tr = tr[tr.shape[0] - T * tw:].reshape(T, tw, 9)
+    return {k: float(tau_int(v)) for k, v in observables(tr).items()}
+
+
+def spectra_blocks(eng, dims):
+    from laughlinedge_amd.stats import batch_sigma
+    blocks = [eng.spectra(b) for b in range(eng.n_blocks())]
+    mean = eng.spectra(-1)
+    sig = []
+    for s in range(len(dims)):
+        vals = [bl[s] for bl in blocks if bl[s] is not None]
+        sig.append(batch_sigma(np.array(vals)) if len(vals) >= 2 else None)
+    return mean, sig
+
+
+def make_engine(walkers, precision, samples, n_blocks, tw):
+    from laughlinedge_amd import Engine
+    return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
+                  precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
+                  trace_walkers=tw)
+
+
+def timed_sampling(eng, steps, nblk):
+    eng.reset_moments()
+    st0 = eng.stats()
+    per = steps // nblk
+    t0 = time.perf_counter()
+    for b in range(nblk):
+        eng.sample(per if b < nblk - 1 else steps - per * (nblk - 1))
+    wall = time.perf_counter() - t0
+    st1 = eng.stats()
+    return wall, {k: st1[k] - st0[k] for k in ("ms_sweep", "ms_features", "ms_gram", "gram_launches",
+                                               "sweep_launches")}
 
 
 def main():
@@ -38,135 +108,183 @@ def main():
     ap.add_argument("--burnin", type=int, default=300)
     ap.add_argument("--walkers", type=int, default=WALKERS)
     ap.add_argument("--trace-walkers", type=int, default=512)
+    ap.add_argument("--form-steps", type=int, default=96, help="kept samples per (form, thinning)")
+    ap.add_argument("--thins", default="1,2,4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="copy this many MiB device-to-device first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(walker sharding across GPUs is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: walkers are "
+              "independent, so N GPUs would each run walkers/N chains and sum their FP64 moments "
+              "in one all-reduce, which is not built here", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mb > 0:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
+        del a, b
 
-    from laughlinedge_amd import Engine
-    from laughlinedge_amd.stats import tau_int, batch_sigma
-
-    steps, warm = args.steps, args.warmup
+    steps, warm, tw = args.steps, args.warmup, min(args.trace_walkers, args.walkers)
     nblk = 4 if steps >= 4 else 1
-    eng = Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=args.walkers, seed=SEED,
-                 precision=0, samples=steps, n_blocks=nblk, tune_sweeps=50, step=0.5,
-                 trace_walkers=args.trace_walkers)
-    info = eng.info
-    dims = eng.dims
+    eng = make_engine(args.walkers, 0, steps + warm, nblk, tw)
+    info, dims = eng.info, eng.dims
     sum_d2 = sum(d * d for d in dims)
-    print(f"[bench] N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} "
-          f"walkers={args.walkers}", file=sys.stderr, flush=True)
+    log(f"N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} walkers={args.walkers}")
     eng.init_walkers()
     step = eng.tune_step()
     eng.burnin(args.burnin)
-    print(f"[bench] step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} burn-in done",
-          file=sys.stderr, flush=True)
+    log(f"step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} burn-in done")
     if warm > 0:
         eng.sample(warm)
-    eng.reset_moments()
-    st0 = eng.stats()
-    # timed region: inputs (walkers) resident in HBM, steps x (sweep + features + Gram)
-    per = steps // nblk
-    t0 = time.perf_counter()
-    for b in range(nblk):
-        eng.sample(per if b < nblk - 1 else steps - per * (nblk - 1))
-    t1 = time.perf_counter()
-    st1 = eng.stats()
-    wall = t1 - t0
+    # timed region: walkers resident in HBM, steps x (sweep + features + Gram)
+    wall, d = timed_sampling(eng, steps, nblk)
     updates = args.walkers * steps
     value = updates / wall
-    ms_step = 1e3 * wall / steps
+    st = eng.stats()
+    log(f"timed: {value:.4e} sample-updates/s, {1e3 * wall / steps:.3f} ms/step")
 
-    # kernel-level rates from HIP events recorded on the engine's stream
-    gram_ms = st1["ms_gram"] - st0["ms_gram"]
-    gram_launches = st1["gram_launches"] - st0["gram_launches"]
-    sweep_ms = st1["ms_sweep"] - st0["ms_sweep"]
-    feat_ms = st1["ms_features"] - st0["ms_features"]
-    gram_flop_per_sample = 16.0 * sum_d2
-    gram_rate = gram_flop_per_sample * updates / (gram_ms * 1e-3)
-    sweep_flop = 12.0 * N * (N - 1) * updates
-    sweep_rate = sweep_flop / (sweep_ms * 1e-3)
-    row_bytes = info.x_ld * 4 + 4
-    stream_gbs = updates * (8 * info.D + 4) / (feat_ms * 1e-3) / 1e9
-
-    # ESS from the traced walkers: Re/Im p1..p4 and V
-    tr = eng.trace()
-    tw = min(args.trace_walkers, args.walkers)
-    ess_per_s = None
-    taus = {}
-    if tr.shape[0] >= tw * 8:
-        T = tr.shape[0] // tw
-        tr = tr[-T * tw:].reshape(T, tw, 9)
-        # the warmup samples precede the timed ones in the trace; use the last `steps`
-        tr = tr[-min(T, steps):]
-        names = ["Re p1", "Im p1", "Re p2", "Im p2", "Re p3", "Im p3", "Re p4", "Im p4", "V"]
-        for q, nm in enumerate(names):
-            taus[nm] = float(tau_int(tr[:, :, q]))
-        tmax = max(taus.values())
-        ess_per_s = updates / (2.0 * tmax) / wall
-
-    # spectra and their batch errors (FP64 eigensolve on the GPU)
-    blocks = [eng.spectra(b) for b in range(eng.n_blocks())]
-    mean_spec = eng.spectra(-1)
-    sig = []
-    for s in range(len(dims)):
-        vals = [bl[s] for bl in blocks if bl[s] is not None]
-        sig.append(batch_sigma(np.array(vals)) if len(vals) >= 2 else None)
+    gram_flop = 16.0 * sum_d2 * updates
+    gram_rate = gram_flop / (d["ms_gram"] * 1e-3)
+    launches = max(1, d["gram_launches"])
+    sweep_rate = 12.0 * N * (N - 1) * updates / (d["ms_sweep"] * 1e-3)
+    stream_bytes = (8 * info.D + 4) * updates
+    taus = taus_of(eng, steps, tw)
+    tau_max = max(taus.values()) if taus else None
+    mean_spec, sig = spectra_blocks(eng, dims)
 
     out = {
-        "metric": "S,H sample-updates/s at N=200, DeltaL<=16 (ESS/s reported alongside)",
+        "metric": METRIC,
         "value": value,
         "unit": "sample-updates/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": warm,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded Metropolis walkers, no dataset)",
-        "config": {"workload": "laughlin-edge N=200 nu=1/3 dL=1..16 grouped (D=914), "
-                               "V=0.025 r^2+5e-5 r^4",
-                   "walkers": args.walkers, "seed": SEED, "burnin_sweeps": args.burnin,
-                   "thin": 1, "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
+        "data": "synthetic (seeded Metropolis walkers on |Psi_L|^2, no dataset)",
+        "config": {"workload": "laughlin-edge N=200 nu=1/3 dL=1..16 grouped (D=914), V=0.025 r^2+5e-5 r^4",
+                   "walkers": args.walkers, "seed": SEED, "burnin_sweeps": args.burnin, "thin": 1,
+                   "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles},
         "roofline": {"bound": "mfma", "kernel": "k_gram<float>",
                      "achieved": gram_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": gram_rate / PEAK_FP32, "traffic": None,
-                     "algorithmic_flop_per_launch": gram_flop_per_sample * updates / max(1, gram_launches),
-                     "avg_launch_ms": gram_ms / max(1, gram_launches)},
-        "sampler": {"kernel": "k_sweep<G,float>", "achieved_tflops": sweep_rate / 1e12,
-                    "peak_tflops": PEAK_FP32 / 1e12, "frac": sweep_rate / PEAK_FP32,
-                    "flop_per_sweep_per_walker": 12 * N * (N - 1),
-                    "ms_per_sweep": sweep_ms / steps},
-        "features": {"ms_per_step": feat_ms / steps, "stream_gb_per_s": stream_gbs,
-                     "bytes_per_sample": 8 * info.D + 4},
-        "breakdown_ms_per_step": {"sweep": sweep_ms / steps, "features": feat_ms / steps,
-                                  "gram": gram_ms / steps},
-        "acceptance": st1["acceptance"],
-        "ess_per_s": ess_per_s,
-        "tau_int": taus,
+                     "algorithmic_flop_per_launch": gram_flop / launches,
+                     "avg_launch_ms": d["ms_gram"] / launches},
+        "kernels": {
+            "sampler": {"kernel": f"k_sweep<{eng.info.lanes_per_walker},float>", "bound": "valu",
+                        "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
+                        "frac": sweep_rate / PEAK_FP32, "flop_per_walker_sweep": 12 * N * (N - 1),
+                        "ms_per_sweep": d["ms_sweep"] / steps},
+            "features": {"kernel": "k_features<float,16>", "bound": "hbm",
+                         "ms_per_step": d["ms_features"] / steps,
+                         "sample_matrix_gb_per_s": stream_bytes / (d["ms_features"] * 1e-3) / 1e9,
+                         "bytes_per_sample": 8 * info.D + 4},
+            "gram": {"ms_per_step": d["ms_gram"] / steps, "launches": d["gram_launches"],
+                     "sample_matrix_read_gb_per_s": stream_bytes / (d["ms_gram"] * 1e-3) / 1e9},
+            "note": "sampler and features share one stream; Gram runs on a second stream on the other "
+                    "sample buffer, so ms_per_step < sum of the three",
+        },
+        "acceptance": st["acceptance"],
+        "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max,
+                "ess_per_s": updates / (2.0 * tau_max) / wall if tau_max else None,
+                "traced_walkers": tw},
         "spectrum_lowest": {str(SECTORS[s]): (float(mean_spec[s][0]) if mean_spec[s] is not None else None)
                             for s in range(len(dims))},
     }
-
     if not args.no_precision:
+        log("precision check (fixed samples vs FP64 oracle)")
         out["precision"] = precision_check(eng, sig)
     eng.close()
+    if not args.no_forms:
+        out["forms"] = forms(args, [int(x) for x in args.thins.split(",")])
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
+
+
+def forms(args, thins):
+    """Plain FP64, plain FP32 and learned FP32 forms at each thinning; ESS/s at equal code."""
+    res = {}
+    tw = min(args.trace_walkers, args.walkers)
+    fs = args.form_steps
+    nblk = 8
+    spec_ref = None
+    for name, prec, learned in (("plain_fp64", 1, False), ("plain_fp32", 0, False), ("learned_fp32", 0, True)):
+        eng = make_engine(args.walkers, prec, len(thins) * (fs + 4), nblk, tw)
+        eng.init_walkers()
+        step = eng.tune_step()
+        eng.burnin(args.burnin)
+        r = {"step": step}
+        if learned:
+            t0 = time.perf_counter()
+            rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)
+            r["training"] = dict(rep, wall_s=time.perf_counter() - t0)
+            eng.burnin(50)
+        runs = []
+        for thin in thins:
+            eng.set_thin(thin)
+            eng.sample(4)
+            wall, d = timed_sampling(eng, fs, nblk)
+            taus = taus_of(eng, fs, tw)
+            tmax = max(taus.values())
+            ess = args.walkers * fs / (2.0 * tmax) / wall
+            spec, sig = spectra_blocks(eng, eng.dims)
+            runs.append({"thin": thin, "ms_per_kept_sample": 1e3 * wall / fs,
+                         "ms_sweep_per_sweep": d["ms_sweep"] / (fs * thin),
+                         "tau_max": tmax, "tau_max_obs": max(taus, key=taus.get),
+                         "ess_per_s": ess, "acceptance": eng.stats()["acceptance"],
+                         "_spec": spec, "_sig": sig})
+            log(f"form {name} thin {thin}: {1e3 * wall / fs:.3f} ms/sample tau_max {tmax:.2f} ESS/s {ess:.3e}")
+        eng.close()
+        best = max(runs, key=lambda x: x["ess_per_s"])
+        r["best_thin"] = best["thin"]
+        r["ess_per_s"] = best["ess_per_s"]
+        r["_spec"], r["_sig"] = best["_spec"], best["_sig"]
+        for x in runs:
+            x.pop("_spec")
+            x.pop("_sig")
+        r["runs"] = runs
+        res[name] = r
+    ref = res["plain_fp64"]
+    for name in ("plain_fp32", "learned_fp32"):
+        r = res[name]
+        r["ess_ratio_vs_plain_fp64"] = r["ess_per_s"] / ref["ess_per_s"]
+        dev = []
+        for s in range(len(SECTORS)):
+            a, b, sa, sb = r["_spec"][s], ref["_spec"][s], r["_sig"][s], ref["_sig"][s]
+            if a is None or b is None or sa is None or sb is None:
+                continue
+            dev.append(np.abs(a - b) / np.sqrt(sa ** 2 + sb ** 2 + 1e-300))
+        dev = np.concatenate(dev) if dev else np.zeros(0)
+        r["spectra_vs_plain_fp64"] = {"eigenvalues": int(dev.size),
+                                      "max_dev_in_combined_sigma": float(dev.max()) if dev.size else None,
+                                      "frac_within_1_sigma": float(np.mean(dev <= 1)) if dev.size else None,
+                                      "frac_within_2_sigma": float(np.mean(dev <= 2)) if dev.size else None}
+    for r in res.values():
+        r.pop("_spec")
+        r.pop("_sig")
+    res["note"] = ("independent chains per form: spectra differences are statistical, so about 68% "
+                   "(95%) of eigenvalues are expected within 1 (2) combined sigma; the systematic "
+                   "FP32 error on identical samples is in 'precision'")
+    return res
 
 
 def precision_check(eng, sig, n_fixed=8192):
     """FP32 GPU moments vs the FP64 CPU oracle on identical configurations."""
     from oracle.oracle import Oracle, geneig
     pos = eng.positions()[: n_fixed]
-    # decorrelate a little: take configurations from two sweeps apart
     o = Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
     t0 = time.perf_counter()
     So, Ho, _ = o.accumulate_fixed(pos)
@@ -194,7 +312,7 @@ def precision_check(eng, sig, n_fixed=8192):
             eig_sig.append(float(np.max(np.abs(spec32[s] - e64) / np.maximum(sig[s], 1e-300))))
     return {"fixed_samples": int(n), "rel_fro_S_max": max(rs), "rel_fro_H_max": max(rh),
             "eig_rel_err_max": max(eig_rel) if eig_rel else None,
-            "eig_err_in_mc_sigma_max": max(eig_sig) if eig_sig else None,
+            "eig_shift_in_mc_sigma_max": max(eig_sig) if eig_sig else None,
             "oracle_seconds": t_cpu}
 
 
@@ -207,7 +325,6 @@ def cpu_baseline():
     t0 = time.perf_counter()
     o.plain_run(walkers, burn, samples, 0.5, SEED, threads=threads)
     dt = time.perf_counter() - t0
-    # burn-in sweeps cost the same as sampling sweeps minus the accumulation
     rate = walkers * samples / dt
     return {"value": rate, "unit": "sample-updates/s", "cores": threads, "kind": "port",
             "sample": f"{walkers} walkers x ({burn} burn-in + {samples} kept sweeps) at N=200, "

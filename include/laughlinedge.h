@@ -178,6 +178,10 @@ int le_set_positions(le_engine* e, const double* in);
 /* fixed samples: n configurations (n x N x 2, FP64) accumulated into a fresh block */
 int le_accumulate_positions(le_engine* e, const double* pos, long n);
 int le_reset_moments(le_engine* e);
+/* Sweeps per kept sample from the next le_sample on (>= 1).  No reference counterpart: the
+ * reference keeps one sample per sweep (em_spectrum.cu:49-115); thinning trades sample
+ * rate for less correlated samples. */
+int le_set_thin(le_engine* e, int thin);
 
 /* FP64 generalized eigensolve on the GPU: d x d complex (interleaved) S, H.
  * evals ascending; evecs (optional, d x d complex, column j = eigenvector j, S-normalised). */

@@ -1126,6 +1126,14 @@ extern "C" int le_reset_moments(le_engine* e)
     return LE_OK;
 }
 
+extern "C" int le_set_thin(le_engine* e, int thin)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (thin < 1) return fail(LE_ERR_ARG, "thin must be >= 1");
+    e->mc.thin = thin;
+    return LE_OK;
+}
+
 extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
 {
     if (!e || !pos || n <= 0) return fail(LE_ERR_ARG, "null argument or n <= 0");

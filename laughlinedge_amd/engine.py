@@ -33,7 +33,7 @@ EXPORTED = [
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
     "le_evolve", "le_engine_create_occupation", "le_get_occupation", "le_train_params_default",
     "le_train_proposal", "le_set_proposal", "le_get_proposal_params", "le_set_proposal_params",
-    "le_proposal_loss_grad",
+    "le_proposal_loss_grad", "le_set_thin",
 ]
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
@@ -312,6 +312,10 @@ class Engine:
 
     def reset_moments(self):
         _check(lib().le_reset_moments(self._h))
+
+    def set_thin(self, thin):
+        _check(lib().le_set_thin(self._h, int(thin)))
+        self.mc.thin = int(thin)
 
     def spectra(self, block=-1):
         ev = np.zeros(max(1, self.info.D))

@@ -80,11 +80,11 @@ def spectra_blocks(eng, dims):
     return mean, sig
 
 
-def make_engine(walkers, precision, samples, n_blocks, tw):
+def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5):
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
-                  trace_walkers=tw)
+                  trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01)
 
 
 def timed_sampling(eng, steps, nblk):
@@ -215,18 +215,22 @@ def main():
 
 
 def forms(args, thins):
-    """Plain FP64, plain FP32 and learned FP32 forms at each thinning; ESS/s at equal code."""
+    """Plain FP64, plain FP32 and learned FP32 forms at each thinning; ESS/s at equal code.
+
+    The plain forms tune the step to the reference's 50% acceptance; plain_fp32_wide tunes it
+    to 30%, which moves particles further per sweep (larger jump distance, shorter tau_int)."""
     res = {}
     tw = min(args.trace_walkers, args.walkers)
     fs = args.form_steps
     nblk = 8
     spec_ref = None
-    for name, prec, learned in (("plain_fp64", 1, False), ("plain_fp32", 0, False), ("learned_fp32", 0, True)):
-        eng = make_engine(args.walkers, prec, len(thins) * (fs + 4), nblk, tw)
+    for name, prec, learned, target in (("plain_fp64", 1, False, 0.5), ("plain_fp32", 0, False, 0.5),
+                                        ("plain_fp32_wide", 0, False, 0.3), ("learned_fp32", 0, True, 0.5)):
+        eng = make_engine(args.walkers, prec, len(thins) * (fs + 4), nblk, tw, target)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
-        r = {"step": step}
+        r = {"step": step, "target_acceptance": target}
         if learned:
             t0 = time.perf_counter()
             rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)
@@ -258,7 +262,7 @@ def forms(args, thins):
         r["runs"] = runs
         res[name] = r
     ref = res["plain_fp64"]
-    for name in ("plain_fp32", "learned_fp32"):
+    for name in ("plain_fp32", "plain_fp32_wide", "learned_fp32"):
         r = res[name]
         r["ess_ratio_vs_plain_fp64"] = r["ess_per_s"] / ref["ess_per_s"]
         dev = []

@@ -73,7 +73,7 @@ typedef struct le_mc_params {
     double acceptance_tol;     /* acceptance_tol */
     uint64_t seed;
     int precision;             /* LE_PREC_* */
-    int lanes_per_walker;      /* 0: smallest power of two >= N/16; -1: timed autotune; else 2..64 */
+    int lanes_per_walker;      /* 0: smallest power of two >= N/16 (N/32 for FP64); -1: timed autotune; else 2..64 */
     int samples_per_launch;    /* kept samples per sweep+feature+Gram round (0: auto) */
     int n_blocks;              /* equal time blocks for error bars (>= 1) */
     int trace_walkers;         /* walkers whose p_1..p_4, V series are recorded */

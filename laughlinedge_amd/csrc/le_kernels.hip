@@ -199,24 +199,33 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int cnt = min(G, N - base);
+            // the next move's old position and random numbers are read ahead: move t
+            // writes only slot base + t and the pads, so slot base + t + 1 is final
+            V2 zo = P[base];
+            Rnd rn = rnd[0];
             for (int t = 0; t < cnt; ++t) {
                 const int i = base + t;
-                const Rnd rn = rnd[t];
                 const Real d1 = rn.n1, d2 = rn.n2;
                 const uint32_t ub = rn.ub;
-                const V2 zo = P[i];
                 const Real xn = zo.x + step * d1, yn = zo.y + step * d2;
                 V2 mid;
                 mid.x = (Real)0.5 * (zo.x + xn);
                 mid.y = (Real)0.5 * (zo.y + yn);
                 P[i] = mid;
                 if (has_pad) P[pad_j] = mid;
+                V2 zo_next = zo;
+                Rnd rn_next = rn;
+                if (t + 1 < cnt) {
+                    zo_next = P[i + 1];
+                    rn_next = rnd[t + 1];
+                }
                 Real lsum = 0;
                 Real pn = 1, pd = 1;
+                constexpr int C = LogOps<Real>::chunk;
                 int s = 0;
-                for (; s + LogOps<Real>::chunk <= S; s += LogOps<Real>::chunk) {
+                for (; s + C <= S; s += C) {
 #pragma unroll
-                    for (int u = 0; u < LogOps<Real>::chunk; ++u) {
+                    for (int u = 0; u < C; ++u) {
                         const V2 q = P[lane + G * (s + u)];
                         const Real ax = xn - q.x, ay = yn - q.y;
                         const Real bx = zo.x - q.x, by = zo.y - q.y;
@@ -227,12 +236,34 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                     pn = 1;
                     pd = 1;
                 }
-                for (; s < S; ++s) {
-                    const V2 q = P[lane + G * s];
-                    const Real ax = xn - q.x, ay = yn - q.y;
-                    const Real bx = zo.x - q.x, by = zo.y - q.y;
-                    pn *= ax * ax + ay * ay;
-                    pd *= bx * bx + by * by;
+                // remainder (< C slots): blocks of 4, then 2 and 1, whose loads issue together
+                for (; s + 4 <= S; s += 4) {
+                    V2 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = P[lane + G * (s + u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const Real ax = xn - q[u].x, ay = yn - q[u].y;
+                        const Real bx = zo.x - q[u].x, by = zo.y - q[u].y;
+                        pn *= ax * ax + ay * ay;
+                        pd *= bx * bx + by * by;
+                    }
+                }
+#pragma unroll
+                for (int blk = 2; blk >= 1; blk >>= 1) {
+                    if (S - s >= blk) {
+                        V2 q[blk];
+#pragma unroll
+                        for (int u = 0; u < blk; ++u) q[u] = P[lane + G * (s + u)];
+#pragma unroll
+                        for (int u = 0; u < blk; ++u) {
+                            const Real ax = xn - q[u].x, ay = yn - q[u].y;
+                            const Real bx = zo.x - q[u].x, by = zo.y - q[u].y;
+                            pn *= ax * ax + ay * ay;
+                            pd *= bx * bx + by * by;
+                        }
+                        s += blk;
+                    }
                 }
                 lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
                 lsum = group_sum<G, Real>(lsum);
@@ -249,6 +280,8 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 nz.y = acc ? yn : zo.y;
                 P[i] = nz;
                 accepted += acc ? 1u : 0u;
+                zo = zo_next;
+                rn = rn_next;
             }
             __builtin_amdgcn_wave_barrier();   // rnd is rewritten by the next batch
         }

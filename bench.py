@@ -12,9 +12,10 @@ FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
   kernels       sampler (VALU FLOP/s), features (sample-matrix bytes/s), Gram (MFMA FLOP/s)
   ess           tau_int (Madras-Sokal) and ESS/s over Re/Im p_1..p_4, V and rotation-
                 invariant S entries (|p_1|^2, |p_2|^2, Re conj(p_2) p_1^2, ...)
-  forms         plain FP64 / plain FP32 / learned FP32 at their best thinning: ESS/s,
-                ratios to the plain FP64 form, spectra vs the FP64 form in MC sigma
-  precision     FP32 GPU moments vs the FP64 oracle on one fixed sample set
+  forms         plain FP64 / plain FP32 / wide-step FP32 and BF16-split / learned FP32 at
+                their best thinning: ESS/s, ratios to the plain FP64 form, spectra vs the
+                FP64 form in MC sigma
+  precision     FP32 and BF16-split GPU moments vs the FP64 oracle on one fixed sample set
   cpu_baseline  the FP64 oracle port (OpenMP, host cores) on a bounded sample
 """
 import argparse
@@ -215,22 +216,24 @@ def main():
 
 
 def forms(args, thins):
-    """Plain FP64, plain FP32 and learned FP32 forms at each thinning; ESS/s at equal code.
+    """Sampler/accumulation forms at each thinning: ESS/s, and spectra at the best thinning.
 
-    The plain forms tune the step to the reference's 50% acceptance; plain_fp32_wide tunes it
-    to 30%, which moves particles further per sweep (larger jump distance, shorter tau_int)."""
+    The plain forms tune the step to the reference's 50% acceptance; the *_wide forms tune it
+    to 30%, which moves particles further per sweep (larger jump distance, shorter tau_int).
+    learned_fp32 trains the MLP proposal first (2000 Adam steps, lr 1e-3, 1e5 configurations,
+    seed 99) and samples with it; its training time is reported, not charged to ESS/s."""
     res = {}
     tw = min(args.trace_walkers, args.walkers)
     fs = args.form_steps
     nblk = 8
-    spec_ref = None
-    for name, prec, learned, target in (("plain_fp64", 1, False, 0.5), ("plain_fp32", 0, False, 0.5),
-                                        ("plain_fp32_wide", 0, False, 0.3), ("learned_fp32", 0, True, 0.5)):
-        eng = make_engine(args.walkers, prec, len(thins) * (fs + 4), nblk, tw, target)
+    table = (("plain_fp64", 1, False, 0.5), ("plain_fp32", 0, False, 0.5), ("plain_fp32_wide", 0, False, 0.3),
+             ("plain_bf16x3_wide", 2, False, 0.3), ("learned_fp32", 0, True, 0.5))
+    for name, prec, learned, target in table:
+        eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
-        r = {"step": step, "target_acceptance": target}
+        r = {"precision": ["fp32", "fp64", "bf16x3"][prec], "step": step, "target_acceptance": target}
         if learned:
             t0 = time.perf_counter()
             rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)
@@ -240,29 +243,29 @@ def forms(args, thins):
         for thin in thins:
             eng.set_thin(thin)
             eng.sample(4)
-            wall, d = timed_sampling(eng, fs, nblk)
+            wall, d = timed_sampling(eng, fs, 1)
             taus = taus_of(eng, fs, tw)
             tmax = max(taus.values())
             ess = args.walkers * fs / (2.0 * tmax) / wall
-            spec, sig = spectra_blocks(eng, eng.dims)
             runs.append({"thin": thin, "ms_per_kept_sample": 1e3 * wall / fs,
                          "ms_sweep_per_sweep": d["ms_sweep"] / (fs * thin),
                          "tau_max": tmax, "tau_max_obs": max(taus, key=taus.get),
-                         "ess_per_s": ess, "acceptance": eng.stats()["acceptance"],
-                         "_spec": spec, "_sig": sig})
+                         "tau_p1": max(taus["Re p1"], taus["Im p1"]), "tau_V": taus["V"],
+                         "ess_per_s": ess, "acceptance": eng.stats()["acceptance"]})
             log(f"form {name} thin {thin}: {1e3 * wall / fs:.3f} ms/sample tau_max {tmax:.2f} ESS/s {ess:.3e}")
-        eng.close()
         best = max(runs, key=lambda x: x["ess_per_s"])
         r["best_thin"] = best["thin"]
         r["ess_per_s"] = best["ess_per_s"]
-        r["_spec"], r["_sig"] = best["_spec"], best["_sig"]
-        for x in runs:
-            x.pop("_spec")
-            x.pop("_sig")
         r["runs"] = runs
+        # spectra with batch errors from nblk blocks at the best thinning
+        eng.set_thin(best["thin"])
+        eng.sample(4)
+        timed_sampling(eng, fs, nblk)
+        r["_spec"], r["_sig"] = spectra_blocks(eng, eng.dims)
+        eng.close()
         res[name] = r
     ref = res["plain_fp64"]
-    for name in ("plain_fp32", "plain_fp32_wide", "learned_fp32"):
+    for name, *_ in table[1:]:
         r = res[name]
         r["ess_ratio_vs_plain_fp64"] = r["ess_per_s"] / ref["ess_per_s"]
         dev = []
@@ -276,48 +279,52 @@ def forms(args, thins):
                                       "max_dev_in_combined_sigma": float(dev.max()) if dev.size else None,
                                       "frac_within_1_sigma": float(np.mean(dev <= 1)) if dev.size else None,
                                       "frac_within_2_sigma": float(np.mean(dev <= 2)) if dev.size else None}
-    for r in res.values():
-        r.pop("_spec")
-        r.pop("_sig")
+    for name, *_ in table:
+        res[name].pop("_spec")
+        res[name].pop("_sig")
     res["note"] = ("independent chains per form: spectra differences are statistical, so about 68% "
-                   "(95%) of eigenvalues are expected within 1 (2) combined sigma; the systematic "
-                   "FP32 error on identical samples is in 'precision'")
+                   "(95%) of eigenvalues are expected within 1 (2) combined sigma; batch sigma from 8 "
+                   "blocks of the best thinning; the systematic error of each accumulation on identical "
+                   "samples is in 'precision'")
     return res
 
 
 def precision_check(eng, sig, n_fixed=8192):
-    """FP32 GPU moments vs the FP64 CPU oracle on identical configurations."""
+    """FP32 and BF16-split GPU moments vs the FP64 CPU oracle on identical configurations
+    (the production walkers' current positions)."""
     from oracle.oracle import Oracle, geneig
+    from laughlinedge_amd import Engine
     pos = eng.positions()[: n_fixed]
     o = Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
     t0 = time.perf_counter()
     So, Ho, _ = o.accumulate_fixed(pos)
     t_cpu = time.perf_counter() - t0
-    from laughlinedge_amd import Engine
-    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=1024, samples=1,
-                precision=0) as e32:
-        e32.accumulate_positions(pos)
-        mom = e32.moments()
-        spec32 = e32.spectra()
     n = pos.shape[0]
-    rs, rh, eig_rel, eig_sig = [], [], [], []
-    for s, (Sg, Hg) in enumerate(mom):
-        S64, H64 = So[s] / n, Ho[s] / n
-        rs.append(float(np.linalg.norm(Sg - S64) / np.linalg.norm(S64)))
-        rh.append(float(np.linalg.norm(Hg - H64) / np.linalg.norm(H64)))
-        try:
-            e64 = geneig(S64, H64)
-        except np.linalg.LinAlgError:
-            continue
-        if spec32[s] is None:
-            continue
-        eig_rel.append(float(np.max(np.abs(spec32[s] - e64) / np.abs(e64))))
-        if sig[s] is not None:
-            eig_sig.append(float(np.max(np.abs(spec32[s] - e64) / np.maximum(sig[s], 1e-300))))
-    return {"fixed_samples": int(n), "rel_fro_S_max": max(rs), "rel_fro_H_max": max(rh),
-            "eig_rel_err_max": max(eig_rel) if eig_rel else None,
-            "eig_shift_in_mc_sigma_max": max(eig_sig) if eig_sig else None,
-            "oracle_seconds": t_cpu}
+    out = {"fixed_samples": int(n), "oracle_seconds": t_cpu}
+    for label, prec in (("fp32", 0), ("bf16x3", 2)):
+        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=1024, samples=1,
+                    precision=prec) as eg:
+            eg.accumulate_positions(pos)
+            mom = eg.moments()
+            spec = eg.spectra()
+        rs, rh, eig_rel, eig_sig = [], [], [], []
+        for s, (Sg, Hg) in enumerate(mom):
+            S64, H64 = So[s] / n, Ho[s] / n
+            rs.append(float(np.linalg.norm(Sg - S64) / np.linalg.norm(S64)))
+            rh.append(float(np.linalg.norm(Hg - H64) / np.linalg.norm(H64)))
+            try:
+                e64 = geneig(S64, H64)
+            except np.linalg.LinAlgError:
+                continue
+            if spec[s] is None:
+                continue
+            eig_rel.append(float(np.max(np.abs(spec[s] - e64) / np.abs(e64))))
+            if sig[s] is not None:
+                eig_sig.append(float(np.max(np.abs(spec[s] - e64) / np.maximum(sig[s], 1e-300))))
+        out[label] = {"rel_fro_S_max": max(rs), "rel_fro_H_max": max(rh),
+                      "eig_rel_err_max": max(eig_rel) if eig_rel else None,
+                      "eig_shift_in_mc_sigma_max": max(eig_sig) if eig_sig else None}
+    return out
 
 
 def cpu_baseline():
@@ -325,7 +332,7 @@ def cpu_baseline():
     from oracle.oracle import Oracle
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
     o = Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
-    walkers, burn, samples = 4 * threads, 20, 200
+    walkers, burn, samples = 32 * threads, 20, 600
     t0 = time.perf_counter()
     o.plain_run(walkers, burn, samples, 0.5, SEED, threads=threads)
     dt = time.perf_counter() - t0

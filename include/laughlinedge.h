@@ -49,6 +49,8 @@ extern "C" {
 
 #define LE_PREC_FP32 0      /* FP32 sampler, FP32 MFMA tiles flushed to FP64 */
 #define LE_PREC_FP64 1      /* FP64 sampler, FP64 MFMA accumulation (the plain FP64 form) */
+#define LE_PREC_BF16X3 2    /* FP32 sampler; S/H tiles from BF16 hi/lo split products
+                               (hi.hi + hi.lo + lo.hi) on MFMA, FP32 tiles flushed to FP64 */
 
 /* sys_params.h, at run time.  V(r) = 0.5 w r^2 + eps r^4 + strength (r / RCl)^exponent,
  * RCl = sqrt(2 m N) (system_parameters.cpp:25-32 plus the w, eps form). */
@@ -227,7 +229,7 @@ typedef struct le_train_params {
     int snapshot_sweeps;       /* plain sweeps between snapshots of the walkers */
     uint64_t seed;             /* weights and training noise (99) */
     double sigma0;             /* width scale; <= 0: the engine's current step */
-    int table_size;            /* grid points per feature (24) */
+    int table_size;            /* grid points per feature (12) */
 } le_train_params;
 typedef struct le_train_report {
     double loss_first, loss_last;      /* -<min(1, alpha) |dz|^2> */

@@ -75,7 +75,8 @@ struct le_engine {
     le_system_params sys{};
     le_mc_params mc{};
     le::Basis basis;
-    int prec = LE_PREC_FP32;
+    int prec = LE_PREC_FP32;        // data precision of positions, samples and tiles
+    int gram_prec = le::kFP32;      // accumulation kernel (kBF16X3: split products)
     size_t rsz = 4;
     int G = 8;
     int spl = 1;                 // kept samples per launch round
@@ -484,7 +485,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
     if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
         return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
-    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64)
+    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64 && mc->precision != LE_PREC_BF16X3)
         return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
@@ -494,7 +495,8 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     auto e = new le_engine();
     e->sys = *sys;
     e->mc = *mc;
-    e->prec = mc->precision;
+    e->prec = mc->precision == LE_PREC_FP64 ? LE_PREC_FP64 : LE_PREC_FP32;
+    e->gram_prec = mc->precision == LE_PREC_FP64 ? le::kFP64 : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3 : le::kFP32;
     e->basis.build(sys->particles_number, dls);
     // capacity checks (reference: em_spectrum.cu:235-252 exits); here they are codes
     if (e->basis.K > 24) {
@@ -521,7 +523,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         g_err = msg;
         return rc;
     }
-    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, mc->precision);
+    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     *out = e;
     return LE_OK;
 }
@@ -822,7 +824,7 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         g.C0 = j.C[0];
         g.C1 = j.C[1];
         g.ldc = ldx;
-        HIPCHK(le::launch_gram(g, prec, (int)splits, s));
+        HIPCHK(le::launch_gram(g, gram_prec, (int)splits, s));
     }
     return LE_OK;
 }
@@ -1484,14 +1486,16 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
     if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
         return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
-    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64) return fail(LE_ERR_ARG, "unknown precision");
+    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64 && mc->precision != LE_PREC_BF16X3)
+        return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;
     e->mc = *mc;
-    e->prec = mc->precision;
+    e->prec = mc->precision == LE_PREC_FP64 ? LE_PREC_FP64 : LE_PREC_FP32;
+    e->gram_prec = mc->precision == LE_PREC_FP64 ? le::kFP64 : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3 : le::kFP32;
     e->two_sided = true;
     e->lcut = lcut;
     e->n_user = 2 * lcut;
@@ -1507,7 +1511,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         g_err = msg;
         return rc;
     }
-    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, mc->precision);
+    e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     *out = e;
     return LE_OK;
 }
@@ -1545,7 +1549,7 @@ extern "C" void le_train_params_default(le_train_params* p)
     p->snapshot_sweeps = 4;
     p->seed = 99;
     p->sigma0 = 0.0;
-    p->table_size = 24;
+    p->table_size = 12;
 }
 
 static int build_table(le_engine* e, int T)

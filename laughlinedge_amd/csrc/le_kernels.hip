@@ -24,58 +24,13 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "le_device.hpp"
 #include "le_kernels.hpp"
 
 namespace le {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-// ---------------------------------------------------------------------------
-// Philox-4x32-10 (Salmon et al., SC'11).  Stream layout:
-//   key = {seed_lo, seed_hi}; ctr = {particle, sweep_lo, sweep_hi16 | tag << 16, walker}
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t out[4])
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0;
-        uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-
-template <typename Real> struct V2T;
-template <> struct V2T<float> { using T = float2; };
-template <> struct V2T<double> { using T = double2; };
-
-// uniform in (0, 1]: (x + 1) 2^-32
-__device__ __forceinline__ double u01d(uint32_t x) { return ((double)x + 1.0) * 2.3283064365386963e-10; }
-__device__ __forceinline__ float u01f(uint32_t x) { return (float)(((double)x + 1.0) * 2.3283064365386963e-10); }
-
-__device__ __forceinline__ void gauss2(uint32_t a, uint32_t b, float& n1, float& n2)
-{
-    // sqrt(-2 ln u) with ln u = log2(u) ln 2; angle in revolutions for v_sin/v_cos
-    float r = __fsqrt_rn(-1.3862943611198906f * __builtin_amdgcn_logf(u01f(a)));
-    float t = u01f(b);
-    n1 = r * __builtin_amdgcn_cosf(t);
-    n2 = r * __builtin_amdgcn_sinf(t);
-}
-
-__device__ __forceinline__ void gauss2(uint32_t a, uint32_t b, double& n1, double& n2)
-{
-    double r = sqrt(-2.0 * log(u01d(a)));
-    double s, c;
-    sincos(6.283185307179586 * u01d(b), &s, &c);
-    n1 = r * c;
-    n2 = r * s;
-}
 
 // ---------------------------------------------------------------------------
 // initial positions
@@ -114,42 +69,6 @@ template <> struct LogOps<double> {
     __device__ static double lg(double x) { return log(fmax(x, 1e-300)); }
     static constexpr double to_ln = 1.0;
 };
-
-// Sum over the G lanes of a group, identical in every lane.  Within a 16-lane DPP row the
-// pairing steps are quad xor 1, quad xor 2, half-row mirror and row mirror (each pairs
-// lanes that hold equal partial sums of disjoint lane sets, so the result is the same in
-// every lane); wider groups add ds_swizzle xor steps across rows.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v)
-{
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-template <int G, typename Real>
-__device__ __forceinline__ Real group_sum(Real v)
-{
-    if constexpr (sizeof(Real) == 4) {
-        if constexpr (G >= 2) v += dpp_f<0xB1>(v);
-        if constexpr (G >= 4) v += dpp_f<0x4E>(v);
-        if constexpr (G >= 8) v += dpp_f<0x141>(v);
-        if constexpr (G >= 16) v += dpp_f<0x140>(v);
-    } else {
-        if constexpr (G >= 2) v += dpp_d<0xB1>(v);
-        if constexpr (G >= 4) v += dpp_d<0x4E>(v);
-        if constexpr (G >= 8) v += dpp_d<0x141>(v);
-        if constexpr (G >= 16) v += dpp_d<0x140>(v);
-    }
-    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
-    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
-    return v;
-}
 
 template <int G, typename Real>
 __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
@@ -607,6 +526,148 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 }
 
 // ---------------------------------------------------------------------------
+// BF16 hi/lo split Gram tiles.  Each staged FP32 value x (times the weight on the row side)
+// is split into hi = bf16(x) and lo = bf16(x - hi) (round to nearest even), and a tile
+// accumulates hi.hi' + hi.lo' + lo.hi' on v_mfma_f32_16x16x32_bf16.  Each product then
+// carries a relative error of order 2^-16 (the residual x - hi - lo and the dropped lo.lo');
+// on heavy-tailed monomials it averages down slowly: N = 200, dL 1..16, 8192 samples give
+// 5e-6 relative (Frobenius) against FP64, 3.8e-6 with lo.lo' added, so it is left out.  One MFMA covers the 32 staged
+// samples.  Lane l supplies row (or column) l % 16 with the 8 samples 8 (l / 16) .. +7 of
+// both operands; as both use the same lane-to-sample map, the sum over samples is the same
+// whatever order the hardware takes them in.  The result layout is that of the FP32
+// 16x16x4 MFMA (register r of lane l = row 4 (l / 16) + r, column l % 16).
+// LDS keeps [column][sample] with a row of 40 bf16 (80 bytes), so the 16-byte fragment
+// reads and the staging writes of 16 lanes hit distinct banks.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kLdsB = 40;
+
+__device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)x[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(x[e] - (float)h);
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
+{
+    __shared__ __align__(16) __bf16 ai[NW][2][64][kLdsB];   // row side: weight q, hi/lo
+    __shared__ __align__(16) __bf16 bj[2][64][kLdsB];       // column side: hi/lo
+
+    const Tile tile = g.tiles[blockIdx.x];
+    const long r_begin = (long)blockIdx.y * g.rows_per_split;
+    long r_end = r_begin + g.rows_per_split;
+    if (r_end > g.rows) r_end = g.rows;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int l16 = lane & 15, k4 = lane >> 4;
+
+    floatx4 acc[NW][2][2];
+    double sh[NW][2][2][4];
+#pragma unroll
+    for (int q = 0; q < NW; ++q)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                acc[q][a][b] = floatx4{0, 0, 0, 0};
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sh[q][a][b][r] = 0.0;
+            }
+
+    const float* X = reinterpret_cast<const float*>(g.X);
+    const float* wp[2] = {reinterpret_cast<const float*>(g.w0), reinterpret_cast<const float*>(g.w1)};
+    // staging: thread = (column lane, 8 consecutive samples 8 wave .. +7)
+    const int sc = lane, sk = wave * 8;
+    int steps = 0;
+    for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
+        float xi[8], xj[8], wv[NW][8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const long row = r0 + sk + e;
+            const bool in = row < r_end;
+            const float* src = X + (size_t)(in ? row : r_begin) * g.ldx;
+            xi[e] = in ? src[tile.i0 + sc] : 0.f;
+            xj[e] = in ? src[tile.j0 + sc] : 0.f;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
+        }
+        bf16x8 h, l;
+        __syncthreads();   // the previous step's fragments have been read
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            float y[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) y[e] = xi[e] * wv[q][e];
+            split8(y, h, l);
+            *reinterpret_cast<bf16x8*>(&ai[q][0][sc][sk]) = h;
+            *reinterpret_cast<bf16x8*>(&ai[q][1][sc][sk]) = l;
+        }
+        split8(xj, h, l);
+        *reinterpret_cast<bf16x8*>(&bj[0][sc][sk]) = h;
+        *reinterpret_cast<bf16x8*>(&bj[1][sc][sk]) = l;
+        __syncthreads();
+
+        bf16x8 bh[2], bl[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int c = wj * 32 + b * 16 + l16;
+            bh[b] = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
+            bl[b] = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
+        }
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int rr = wi * 32 + a * 16 + l16;
+                const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ai[q][0][rr][8 * k4]);
+                const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ai[q][1][rr][8 * k4]);
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    floatx4 c = acc[q][a][b];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], c, 0, 0, 0);
+                    acc[q][a][b] = c;
+                }
+            }
+        if (++steps == kFlushSteps) {
+            steps = 0;
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) sh[q][a][b][r] += (double)acc[q][a][b][r];
+                        acc[q][a][b] = floatx4{0, 0, 0, 0};
+                    }
+        }
+    }
+    double* Cs[2] = {g.C0, g.C1};
+#pragma unroll
+    for (int q = 0; q < NW; ++q)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double v = sh[q][a][b][r] + (double)acc[q][a][b][r];
+                    const long row = tile.i0 + wi * 32 + a * 16 + 4 * k4 + r;
+                    const long col = tile.j0 + wj * 32 + b * 16 + l16;
+                    atomicAdd(&Cs[q][row * g.ldc + col], v);
+                }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 tiles -> complex sector blocks: S_ab = (C[a][b] + C[d+a][d+b]) + i (C[a][d+b] - C[d+a][b])
 // (only upper 64-tiles were accumulated; the lower ones are read by symmetry)
 // ---------------------------------------------------------------------------
@@ -656,7 +717,7 @@ __device__ __forceinline__ cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; 
 __device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
 __device__ __forceinline__ cd cscale(cd a, double s) { return {a.r * s, a.i * s}; }
 
-constexpr int kEigThreads = 256;
+constexpr int kEigThreads = 1024;
 constexpr int kEigMaxPairs = 320;     // d <= 640
 
 __device__ double block_sum(double v, double* red)
@@ -946,7 +1007,11 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
 {
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
     dim3 grid((unsigned)a.ntiles, (unsigned)splits);
-    if (prec == kFP64) {
+    if (prec == kBF16X3) {
+        if (a.rows_per_split % kKC) return hipErrorInvalidValue;
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_gram_bf16<1>), grid, dim3(256), 0, s, a);
+    } else if (prec == kFP64) {
         if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_gram<double, 1>), grid, dim3(256), 0, s, a);
     } else {

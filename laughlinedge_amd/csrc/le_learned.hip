@@ -5,7 +5,7 @@
 //
 // Definitions (shared with the FP64 oracle):
 //   f1 = |z| / R;  m_c = min(8, min_{j != i, j = c mod 8} |z - z_j|);  f2 = min_c m_c;  f3 = mean_c m_c
-//   (s_r, s_t) = sigma0 exp(1.5 tanh(MLP(f)));  z' = z + e^{i arg z} (s_r n1 + i s_t n2)
+//   (s_r, s_t) = sigma0 exp(ln(2.5) tanh(MLP(f))), i.e. within [0.4, 2.5] sigma0;  z' = z + e^{i arg z} (s_r n1 + i s_t n2)
 //   log alpha = m sum_j log(|z'-z_j|^2/|z-z_j|^2) - (|z'|^2-|z|^2)/2 + log q(z|x') - log q(z'|x)
 //   loss = -mean min(1, e^{log alpha}) |z' - z|^2
 // Parameter layout: W1 (64x3), b1 (64), W2 (64x64), b2 (64), W3 (2x64), b3 (2).
@@ -13,6 +13,7 @@
 
 #include <cstdint>
 
+#include "le_device.hpp"
 #include "le_learned.hpp"
 
 namespace le {
@@ -21,28 +22,14 @@ constexpr int kH = 64;
 constexpr int kP = kH * 3 + kH + kH * kH + kH + 2 * kH + 2;   // 4546
 constexpr int oW1 = 0, oB1 = kH * 3, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + 2 * kH;
 constexpr float kDcap = 8.0f;
-constexpr float kWScale = 1.5f;
+// Width range: sigma0 exp(+-kWScale) = [0.4, 2.5] sigma0.  A range of [0.22, 4.5] sigma0
+// let the ESJD objective pick rare jumps of ~5 lengths at 8% acceptance (N = 200, nu = 1/3):
+// highest ESJD, but no shorter tau_int for p_1..p_4 and slowly mixing high-dL entries.
+constexpr float kWScale = 0.91629073187415511f;
 // per (sample, evaluation) record: f(3) h1(64) h2(64) do(2) dz2(64) dz1(64) pad(3)
 constexpr int kRec = 264;
 constexpr int rF = 0, rH1 = 3, rH2 = 67, rDO = 131, rDZ2 = 133, rDZ1 = 197;
 
-__device__ __forceinline__ void philox_l(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
-                                         uint32_t out[4])
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0;
-        uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-
-__device__ __forceinline__ float u01l(uint32_t x) { return (float)(((double)x + 1.0) * 2.3283064365386963e-10); }
 
 // features of particle i at (x, y) against configuration cfg (N particles) and d/d(x, y)
 __device__ void features_at(const float2* cfg, int N, int i, float x, float y, float invR, float f[3], float df[3][2])
@@ -149,12 +136,12 @@ __global__ void k_train_samples(int B, int ncfg, int N, unsigned long long seed,
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     uint32_t r[4];
-    philox_l((uint32_t)b, (uint32_t)step, 2u << 16, 0xfffffffeu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    philox((uint32_t)b, (uint32_t)step, 2u << 16, 0xfffffffeu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
     cfg[b] = (int)(r[0] % (uint32_t)ncfg);
     part[b] = (int)(r[1] % (uint32_t)N);
-    const float rr = sqrtf(-2.f * logf(u01l(r[2])));
+    const float rr = sqrtf(-2.f * logf(u01f(r[2])));
     float s, c;
-    sincosf(6.28318530717958648f * u01l(r[3]), &s, &c);
+    sincosf(6.28318530717958648f * u01f(r[3]), &s, &c);
     noise[2 * b] = rr * c;
     noise[2 * b + 1] = rr * s;
 }
@@ -300,8 +287,8 @@ __global__ void k_mlp_init(float* P, unsigned long long seed)
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= kP) return;
     uint32_t r[4];
-    philox_l((uint32_t)q, 0u, 3u << 16, 0xfffffffdu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
-    const float u = 2.f * u01l(r[0]) - 1.f;
+    philox((uint32_t)q, 0u, 3u << 16, 0xfffffffdu, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    const float u = 2.f * u01f(r[0]) - 1.f;
     float v = 0.f;
     if (q < oB1) v = u * sqrtf(6.f / (3 + kH));
     else if (q >= oW2 && q < oB2) v = u * sqrtf(6.f / (2 * kH));
@@ -373,21 +360,28 @@ __device__ __forceinline__ void widths_lookup(const Table3& tb, const float f[3]
 template <int G>
 __device__ __forceinline__ void class_features(float cmin2, float invR, float r, float f[3])
 {
-    // cmin2: this lane's minimum squared distance over its slots
-#pragma unroll
-    for (int off = 8; off < G; off <<= 1) cmin2 = fminf(cmin2, __shfl_xor(cmin2, off, G));
-    float d = fminf(sqrtf(cmin2), kDcap);
+    // cmin2: this lane's minimum squared distance over its slots.  Lanes l, l ^ 8, l ^ 16, ...
+    // hold particles of the same residue class j mod 8 (G a multiple of 8).
+    if constexpr (G >= 16) cmin2 = fminf(cmin2, dpp_f<0x128>(cmin2));   // row rotate 8 = xor 8
+    if constexpr (G >= 32) cmin2 = fminf(cmin2, __shfl_xor(cmin2, 16, 64));
+    if constexpr (G >= 64) cmin2 = fminf(cmin2, __shfl_xor(cmin2, 32, 64));
+    const float d = fminf(sqrtf(cmin2), kDcap);
     float mn = d, sm = d;
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
-        mn = fminf(mn, __shfl_xor(mn, off, G));
-        sm += __shfl_xor(sm, off, G);
-    }
+    mn = fminf(mn, dpp_f<0xB1>(mn));
+    sm += dpp_f<0xB1>(sm);
+    mn = fminf(mn, dpp_f<0x4E>(mn));
+    sm += dpp_f<0x4E>(sm);
+    mn = fminf(mn, dpp_f<0x141>(mn));
+    sm += dpp_f<0x141>(sm);
     f[0] = r * invR;
     f[1] = mn;
     f[2] = 0.125f * sm;
 }
 
+// Learned-proposal sweeps: same lane mapping, midpoint trick and random streams as k_sweep.
+// Per move: pass 1 takes the class minima at z (slot i and the pads hold a far point),
+// the table gives the forward widths, pass 2 takes the log-ratio and the class minima at z'
+// (slot i and the pads skipped), the table gives the reverse widths.
 template <int G>
 __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, float invR)
 {
@@ -402,6 +396,7 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
     const int walker = blockIdx.x * WPB + grp;
     const bool live = walker < a.walkers;
     float2* P = lds + grp * NP;
+    float4* rnd = reinterpret_cast<float4*>(lds + WPB * NP) + grp * G;   // (n1, n2, u bits, 0)
     float2* gpos = reinterpret_cast<float2*>(a.pos) + (size_t)(live ? walker : 0) * N;
     if (live)
         for (int j = lane; j < N; j += G) P[j] = gpos[j];
@@ -411,6 +406,7 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const bool has_pad = (lane + G * (S - 1)) >= N;
     const int pad_j = lane + G * (S - 1);
+    const int skip_p = has_pad ? (S - 1) : -1;
     const float2 far = make_float2(1e4f, 1e4f);
     unsigned accepted = 0;
     for (int sw = 0; sw < a.nsweeps; ++sw) {
@@ -419,25 +415,41 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int base = 0; base < N; base += G) {
             uint32_t r[4];
-            philox_l((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
-            const float rr = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01l(r[0])));
-            const float tt = u01l(r[1]);
-            const float g1 = rr * __builtin_amdgcn_cosf(tt), g2 = rr * __builtin_amdgcn_sinf(tt);
-            const uint32_t ubits = r[2];
+            philox((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
+            float g1, g2;
+            gauss2(r[0], r[1], g1, g2);
+            rnd[lane] = make_float4(g1, g2, __builtin_bit_cast(float, r[2]), 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int cnt = min(G, N - base);
+            float2 zo = P[base];
+            float4 rn = rnd[0];
             for (int t = 0; t < cnt; ++t) {
                 const int i = base + t;
-                const float n1 = __shfl(g1, t, G), n2 = __shfl(g2, t, G);
-                const uint32_t ub = __shfl(ubits, t, G);
-                const float2 zo = P[i];
+                const float n1 = rn.x, n2 = rn.y;
+                const uint32_t ub = __builtin_bit_cast(uint32_t, rn.z);
                 // pass 1: class minima at z
                 P[i] = far;
                 if (has_pad) P[pad_j] = far;
                 float c2min = 1e30f;
-                for (int s = 0; s < S; ++s) {
-                    const float2 q = P[lane + G * s];
-                    const float dx = zo.x - q.x, dy = zo.y - q.y;
-                    c2min = fminf(c2min, dx * dx + dy * dy);
+                {
+                    int s = 0;
+                    for (; s + 4 <= S; s += 4) {
+                        float2 q[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) q[u] = P[lane + G * (s + u)];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float dx = zo.x - q[u].x, dy = zo.y - q[u].y;
+                            c2min = fminf(c2min, dx * dx + dy * dy);
+                        }
+                    }
+                    for (; s < S; ++s) {
+                        const float2 q = P[lane + G * s];
+                        const float dx = zo.x - q.x, dy = zo.y - q.y;
+                        c2min = fminf(c2min, dx * dx + dy * dy);
+                    }
                 }
                 const float r0 = sqrtf(zo.x * zo.x + zo.y * zo.y);
                 float f[3];
@@ -454,10 +466,45 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                 mid.y = 0.5f * (zo.y + yn);
                 P[i] = mid;
                 if (has_pad) P[pad_j] = mid;
+                float2 zo_next = zo;
+                float4 rn_next = rn;
+                if (t + 1 < cnt) {   // slot i + 1 is not written by this move
+                    zo_next = P[i + 1];
+                    rn_next = rnd[t + 1];
+                }
                 const int skip_i = ((i % G) == lane) ? (i / G) : -1;
-                const int skip_p = has_pad ? (S - 1) : -1;
                 float lsum = 0.f, pn = 1.f, pd = 1.f, a2min = 1e30f;
-                for (int s = 0; s < S; ++s) {
+                int s = 0;
+                for (; s + 8 <= S; s += 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float2 q = P[lane + G * (s + u)];
+                        const float ax = xn - q.x, ay = yn - q.y;
+                        const float bx = zo.x - q.x, by = zo.y - q.y;
+                        const float A2 = ax * ax + ay * ay;
+                        pn *= A2;
+                        pd *= bx * bx + by * by;
+                        a2min = (s + u == skip_i || s + u == skip_p) ? a2min : fminf(a2min, A2);
+                    }
+                    lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
+                    pn = 1.f;
+                    pd = 1.f;
+                }
+                for (; s + 4 <= S; s += 4) {
+                    float2 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = P[lane + G * (s + u)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float ax = xn - q[u].x, ay = yn - q[u].y;
+                        const float bx = zo.x - q[u].x, by = zo.y - q[u].y;
+                        const float A2 = ax * ax + ay * ay;
+                        pn *= A2;
+                        pd *= bx * bx + by * by;
+                        a2min = (s + u == skip_i || s + u == skip_p) ? a2min : fminf(a2min, A2);
+                    }
+                }
+                for (; s < S; ++s) {
                     const float2 q = P[lane + G * s];
                     const float ax = xn - q.x, ay = yn - q.y;
                     const float bx = zo.x - q.x, by = zo.y - q.y;
@@ -465,35 +512,32 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                     pn *= A2;
                     pd *= bx * bx + by * by;
                     a2min = (s == skip_i || s == skip_p) ? a2min : fminf(a2min, A2);
-                    if ((s & 7) == 7) {
-                        lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
-                        pn = 1.f;
-                        pd = 1.f;
-                    }
                 }
                 lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
-#pragma unroll
-                for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
-                const float rn = sqrtf(xn * xn + yn * yn);
+                lsum = group_sum<G, float>(lsum);
+                const float rn_ = sqrtf(xn * xn + yn * yn);
                 float fp[3];
-                class_features<G>(a2min, invR, rn, fp);
+                class_features<G>(a2min, invR, rn_, fp);
                 float lsrp, lstp;
                 widths_lookup(tb, fp, lsrp, lstp);
                 const float srp = __expf(lsrp), stp = __expf(lstp);
-                const float cp = rn > 0.f ? xn / rn : 1.f, sp = rn > 0.f ? yn / rn : 0.f;
+                const float cp = rn_ > 0.f ? xn / rn_ : 1.f, sp = rn_ > 0.f ? yn / rn_ : 0.f;
                 const float wx = zo.x - xn, wy = zo.y - yn;
                 const float Dr = wx * cp + wy * sp, Dt = -wx * sp + wy * cp;
                 const float lq = -(lsrp + lstp) - 0.5f * ((Dr / srp) * (Dr / srp) + (Dt / stp) * (Dt / stp)) +
                                  (lsr + lst) + 0.5f * (n1 * n1 + n2 * n2);
                 const float logA = mfac * lsum - 0.5f * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y)) + lq;
                 const bool acc =
-                    (logA >= 0.f) || (__builtin_amdgcn_logf(u01l(ub)) * 0.69314718055994531f < logA);
+                    (logA >= 0.f) || (__builtin_amdgcn_logf(u01f(ub)) * 0.69314718055994531f < logA);
                 float2 nz;
                 nz.x = acc ? xn : zo.x;
                 nz.y = acc ? yn : zo.y;
                 P[i] = nz;
                 accepted += acc ? 1u : 0u;
+                zo = zo_next;
+                rn = rn_next;
             }
+            __builtin_amdgcn_wave_barrier();   // rnd is rewritten by the next batch
         }
     }
     for (int j = lane; j < N; j += G) gpos[j] = P[j];
@@ -542,7 +586,10 @@ hipError_t launch_mlp_table(const float* P, int T, const float* lo, const float*
     return hipGetLastError();
 }
 
-size_t learned_lds_bytes(int G, int N) { return (size_t)(64 / G) * (size_t)((N + G - 1) / G) * G * sizeof(float2); }
+size_t learned_lds_bytes(int G, int N)
+{
+    return (size_t)(64 / G) * (size_t)((N + G - 1) / G) * G * sizeof(float2) + 64 * sizeof(float4);
+}
 
 template <int G>
 static hipError_t sweep_learned_g(const SweepArgs& a, const LearnedTable& lt, hipStream_t s)

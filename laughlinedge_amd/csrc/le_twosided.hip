@@ -15,38 +15,17 @@
 
 #include <cstdint>
 
+#include "le_device.hpp"
 #include "le_kernels.hpp"
 
 namespace le {
 
-__device__ __forceinline__ void philox2(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
-                                        uint32_t out[4])
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0;
-        uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-
-template <typename Real> struct V2T2;
-template <> struct V2T2<float> { using T = float2; };
-template <> struct V2T2<double> { using T = double2; };
-
-__device__ __forceinline__ double u01d2(uint32_t x) { return ((double)x + 1.0) * 2.3283064365386963e-10; }
-
 template <typename Real>
 __device__ __forceinline__ void normals(uint32_t a, uint32_t b, Real& n1, Real& n2)
 {
-    double r = sqrt(-2.0 * log(u01d2(a)));
+    double r = sqrt(-2.0 * log(u01d(a)));
     double s, c;
-    sincos(6.283185307179586 * u01d2(b), &s, &c);
+    sincos(6.283185307179586 * u01d(b), &s, &c);
     n1 = (Real)(r * c);
     n2 = (Real)(r * s);
 }
@@ -63,7 +42,7 @@ __device__ __forceinline__ Real lg2(Real x)
 template <int G, typename Real>
 __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
 {
-    using V2 = typename V2T2<Real>::T;
+    using V2 = typename V2T<Real>::T;
     extern __shared__ __align__(16) unsigned char smem2[];
     V2* lds = reinterpret_cast<V2*>(smem2);
     constexpr int WPB = 64 / G;
@@ -96,7 +75,7 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int i = 0; i <= N; ++i) {
             uint32_t r[4];
-            philox2((uint32_t)i, c1, c2, (uint32_t)walker, k0, k1, r);   // same numbers in every lane
+            philox((uint32_t)i, c1, c2, (uint32_t)walker, k0, k1, r);   // same numbers in every lane
             Real g1, g2;
             normals<Real>(r[0], r[1], g1, g2);
             const bool bulk = (i > 0 && i < N);
@@ -138,7 +117,7 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
             } else {
                 logA = (Real)0.5 * mln2 * lsum - (Real)0.25 * dr2;
             }
-            const bool acc = (logA >= 0) || ((Real)log(u01d2(r[2])) < logA);
+            const bool acc = (logA >= 0) || ((Real)log(u01d(r[2])) < logA);
             V2 nz;
             nz.x = acc ? xn : zo.x;
             nz.y = acc ? yn : zo.y;

@@ -23,6 +23,7 @@ LE_ERR_STATE = 6
 LE_ERR_NOCONV = 7
 PREC_FP32 = 0
 PREC_FP64 = 1
+PREC_BF16X3 = 2
 
 EXPORTED = [
     "le_mc_params_default", "le_last_error", "le_engine_create", "le_engine_destroy",
@@ -270,7 +271,7 @@ class Engine:
         _check(lib().le_get_proposal_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
-    def set_proposal_params(self, p, sigma0, table_size=24):
+    def set_proposal_params(self, p, sigma0, table_size=12):
         p = np.ascontiguousarray(p, dtype=np.float32)
         _check(lib().le_set_proposal_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                             ctypes.c_double(sigma0), int(table_size)))

@@ -8,7 +8,7 @@ features of particle i at position z (others fixed), D_CAP = 8:
     f1 = |z| / R
     m_c = min(D_CAP, min_{j != i, j = c mod 8} |z - z_j|),  c = 0..7
     f2 = min_c m_c,   f3 = mean_c m_c
-widths: (sigma_r, sigma_t) = sigma0 * exp(1.5 tanh(MLP(f)))
+widths: (sigma_r, sigma_t) = sigma0 * exp(ln(2.5) tanh(MLP(f))), within [0.4, 2.5] sigma0
 move: z' = z + e^{i theta} (sigma_r n1 + i sigma_t n2),  theta = arg z
 log q(z'|x) = -log(2 pi sigma_r sigma_t) - |n|^2 / 2
 log q(z|x') = -log(2 pi s'_r s'_t) - ((D_r / s'_r)^2 + (D_t / s'_t)^2) / 2,
@@ -22,7 +22,7 @@ import numpy as np
 D_CAP = 8.0
 H = 64
 NPARAM = H * 3 + H + H * H + H + 2 * H + 2
-WSCALE = 1.5
+WSCALE = 0.91629073187415511   # ln 2.5
 
 
 def unpack(p):

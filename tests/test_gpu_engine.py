@@ -15,7 +15,7 @@ def _disc_positions(rng, n, N, R):
     return np.stack([r * np.cos(t), r * np.sin(t)], -1)
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-12)])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-12), (2, 2e-5)])
 def test_fixed_sample_moments(oracle_lib, prec, tol):
     """Same FP64 configurations through the GPU features+Gram path and the oracle."""
     from laughlinedge_amd import Engine
@@ -30,13 +30,16 @@ def test_fixed_sample_moments(oracle_lib, prec, tol):
     with Engine(N, m, sectors, w=w, eps=eps, walkers=64, precision=prec, samples=1) as e:
         e.accumulate_positions(pos)
         mom = e.moments()
-    for s, (Sg, Hg) in enumerate(mom):
-        assert _rel_fro(Sg, So[s] / n) < tol, (s, _rel_fro(Sg, So[s] / n))
-        assert _rel_fro(Hg, Ho[s] / n) < tol, (s, _rel_fro(Hg, Ho[s] / n))
+    errs = [(_rel_fro(Sg, So[s] / n), _rel_fro(Hg, Ho[s] / n)) for s, (Sg, Hg) in enumerate(mom)]
+    print(f"prec={prec} max rel Frobenius error S {max(e[0] for e in errs):.3e} H {max(e[1] for e in errs):.3e}")
+    for s, (es, eh) in enumerate(errs):
+        assert es < tol and eh < tol, (s, es, eh)
 
 
-def test_fixed_sample_large_tiles(oracle_lib):
-    """N=40, dL 10..12 spans several 64-column Gram tiles; FP32 path vs FP64 oracle."""
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5)])
+def test_fixed_sample_large_tiles(oracle_lib, prec, tol):
+    """N=40, dL 10..12 spans several 64-column Gram tiles, 3000 samples (not a multiple of
+    the 32-sample staging step); FP32 and BF16-split paths vs the FP64 oracle."""
     from laughlinedge_amd import Engine
     N, m, sectors = 40, 3, [10, 11, 12]
     print(f"case N={N} sectors={sectors}", flush=True)
@@ -44,13 +47,14 @@ def test_fixed_sample_large_tiles(oracle_lib):
     rng = np.random.default_rng(5)
     pos = _disc_positions(rng, 3000, N, o.R)
     So, Ho, _ = o.accumulate_fixed(pos)
-    with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=256, samples=1) as e:
+    with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=256, samples=1, precision=prec) as e:
         assert e.info.n_tiles > 3
         e.accumulate_positions(pos)
         mom = e.moments()
-    for s, (Sg, Hg) in enumerate(mom):
-        assert _rel_fro(Sg, So[s] / 3000) < 1e-6
-        assert _rel_fro(Hg, Ho[s] / 3000) < 1e-6
+    errs = [(_rel_fro(Sg, So[s] / 3000), _rel_fro(Hg, Ho[s] / 3000)) for s, (Sg, Hg) in enumerate(mom)]
+    print(f"prec={prec} max rel Frobenius error S {max(e[0] for e in errs):.3e} H {max(e[1] for e in errs):.3e}")
+    for s, (es, eh) in enumerate(errs):
+        assert es < tol and eh < tol, (s, es, eh)
 
 
 def test_fp64_chain_matches_oracle(oracle_lib):
@@ -355,7 +359,7 @@ def _occ_numpy(pos, N, m, states, lcut):
     return np.einsum("sa,sb,sl->abl", PL.conj(), PR, W) / pos.shape[0]
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 2e-5), (1, 1e-11)])
+@pytest.mark.parametrize("prec,tol", [(0, 2e-5), (1, 1e-11), (2, 1e-4)])
 def test_occupation_fixed_samples(prec, tol):
     """two-sided features + Gram on given configurations vs an independent numpy estimator"""
     from laughlinedge_amd.engine import OccupationEngine

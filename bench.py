@@ -156,6 +156,8 @@ def main():
     launches = max(1, d["gram_launches"])
     sweep_rate = 12.0 * N * (N - 1) * updates / (d["ms_sweep"] * 1e-3)
     stream_bytes = (8 * info.D + 4) * updates
+    rows_per_launch = info.samples_per_launch * args.walkers
+    traffic, traffic_note = gram_traffic(rows_per_launch)
     taus = taus_of(eng, steps, tw)
     tau_max = max(taus.values()) if taus else None
     mean_spec, sig = spectra_blocks(eng, dims)
@@ -179,8 +181,11 @@ def main():
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles},
         "roofline": {"bound": "mfma", "kernel": "k_gram<float>",
                      "achieved": gram_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": gram_rate / PEAK_FP32, "traffic": None,
+                     "frac": gram_rate / PEAK_FP32, "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "algorithmic_flop_per_launch": gram_flop / launches,
+                     "algorithmic_bytes_per_launch": rows_per_launch * (8 * info.D + 8),
+                     "rows_per_launch": rows_per_launch,
                      "avg_launch_ms": d["ms_gram"] / launches},
         "kernels": {
             "sampler": {"kernel": f"k_sweep<{eng.info.lanes_per_walker},float>", "bound": "valu",
@@ -213,6 +218,20 @@ def main():
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
+
+
+def gram_traffic(rows_per_launch):
+    """HBM bytes per Gram launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (copy-calibrated, per sample row), scaled to this launch's rows."""
+    path = os.path.join(ROOT, "profiles", "k_gram_traffic_N200_dL1-16.json")
+    if not os.path.exists(path):
+        return None, "no counter summary"
+    with open(path) as f:
+        t = json.load(f)
+    b = t["fetch_bytes_per_row"] * rows_per_launch + t["write_bytes_per_launch"]
+    return b, (f"rocprofv3 FETCH_SIZE x {t['calibration']['fetch_factor']} + WRITE_SIZE x "
+               f"{t['calibration']['write_factor']} (copy-calibrated) at {t['rows_per_launch_measured']} "
+               f"rows/launch, scaled to {rows_per_launch}; {t['algorithmic_bytes_per_row']} algorithmic B/row")
 
 
 def forms(args, thins):

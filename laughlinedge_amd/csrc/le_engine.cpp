@@ -819,6 +819,7 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         rps = ((rps + 31) / 32) * 32;
         splits = (rows + rps - 1) / rps;
         g.rows_per_split = rps;
+        g.splits = (int)splits;
         g.tiles = j.tiles;
         g.ntiles = j.ntiles;
         g.C0 = j.C[0];

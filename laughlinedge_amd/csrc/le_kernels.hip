@@ -405,6 +405,20 @@ __device__ __forceinline__ void load8(const double* p, double* o)
     }
 }
 
+// Workgroup -> (tile, row split): consecutive workgroups take consecutive tiles of one split,
+// so the dispatcher spreads a split's tiles over all 8 XCDs.  Their shared sample rows are
+// then fetched about 4.5 times per launch (FETCH_SIZE, N = 200, dL 1..16), mostly from the
+// Infinity Cache; placing all tiles of a split on one XCD cuts the fetches to ~1.1 times
+// the sample matrix but runs 8-14% slower (every workgroup of the XCD then hammers the same
+// L2 lines).
+__device__ __forceinline__ bool gram_block(const GramArgs& g, int& tile, int& split)
+{
+    const int id = blockIdx.x;
+    tile = id % g.ntiles;
+    split = id / g.ntiles;
+    return split < g.splits;
+}
+
 constexpr int kKC = 32;          // samples staged per step
 constexpr int kLdsStride = 80;   // padded LDS row (elements)
 constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
@@ -417,8 +431,10 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
     __shared__ __align__(16) Real xj[kKC][kLdsStride];
     __shared__ Real vs[NW][kKC];
 
-    const Tile tile = g.tiles[blockIdx.x];
-    const long r_begin = (long)blockIdx.y * g.rows_per_split;
+    int tile_idx, split;
+    if (!gram_block(g, tile_idx, split)) return;
+    const Tile tile = g.tiles[tile_idx];
+    const long r_begin = (long)split * g.rows_per_split;
     long r_end = r_begin + g.rows_per_split;
     if (r_end > g.rows) r_end = g.rows;
     const bool same = tile.i0 == tile.j0;
@@ -558,8 +574,10 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
     __shared__ __align__(16) __bf16 ai[NW][2][64][kLdsB];   // row side: weight q, hi/lo
     __shared__ __align__(16) __bf16 bj[2][64][kLdsB];       // column side: hi/lo
 
-    const Tile tile = g.tiles[blockIdx.x];
-    const long r_begin = (long)blockIdx.y * g.rows_per_split;
+    int tile_idx, split;
+    if (!gram_block(g, tile_idx, split)) return;
+    const Tile tile = g.tiles[tile_idx];
+    const long r_begin = (long)split * g.rows_per_split;
     long r_end = r_begin + g.rows_per_split;
     if (r_end > g.rows) r_end = g.rows;
     const int tid = threadIdx.x;
@@ -1006,7 +1024,8 @@ hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s)
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
 {
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
-    dim3 grid((unsigned)a.ntiles, (unsigned)splits);
+    if (a.splits != splits) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((long)a.ntiles * splits));
     if (prec == kBF16X3) {
         if (a.rows_per_split % kKC) return hipErrorInvalidValue;
         if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(256), 0, s, a);

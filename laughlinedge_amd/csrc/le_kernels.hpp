@@ -83,6 +83,7 @@ struct GramArgs {
     int nw;
     long rows;
     long rows_per_split;
+    int splits;              // ceil(rows / rows_per_split)
     const Tile* tiles;
     int ntiles;
     double* C0;              // [ldc][ldc] FP64 accumulators (upper tiles)

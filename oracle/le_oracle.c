@@ -23,6 +23,22 @@
  *
  * Random numbers: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the same
  * counter layout as the GPU sampler, Box-Muller normals in FP64.
+ *
+ * What pins each function (tests/test_oracle.py; the reference ships no tests or vectors):
+ *   or_philox4x32_10 ........ Random123 known-answer vectors
+ *   or_potential ............ numpy sum of V(r_i) in the fixed-accumulation test
+ *   or_accumulate_fixed ..... independent numpy evaluation; exact small-N expansion
+ *                             (oracle/exact_lll.py); H = w (L0 + dL + N) S at eps = 0
+ *   or_accumulate_weighted .. independent numpy evaluation
+ *   or_features ............. through or_accumulate_fixed (same routine)
+ *   or_plain_run ............ <sum r^2> = 2 (L0 + N); eigenvalues vs exact N = 3
+ *   or_plain_positions ...... the same chain code as or_plain_run
+ *   or_rowwise_sector ....... exact normalised metric and spectrum at N = 3
+ *   or_geneig, or_dsf_weights scipy.linalg.eigh; residual, S-orthonormality
+ *   or_evolve ............... closed form with F = 0; dense matrix exponential for a
+ *                             piecewise-constant drive
+ * Reference-produced numbers themselves are parity unpinned: the reference cannot be
+ * built here and publishes no outputs.
  */
 #include <complex.h>
 #include <math.h>

@@ -51,3 +51,20 @@ def test_mc_defaults_follow_reference():
     assert (p.burnin_sweeps, p.samples, p.tune_sweeps, p.walkers) == (1000, 50000, 100, 4096)
     assert abs(p.step - 0.8) < 1e-12 and abs(p.target_acceptance - 0.5) < 1e-12
     assert abs(p.acceptance_tol - 0.01) < 1e-12
+
+
+
+def test_basis_matches_golden_listing():
+    """C++ basis enumerator (no GPU) against the committed N = 4 listing"""
+    import os
+    import numpy as np
+    from laughlinedge_amd.engine import basis_describe
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_n4_m3.npz"))
+    for dl in range(5):
+        got = basis_describe(4, [dl])
+        want = g[f"pairs{dl}"]
+        assert len(got) == want.shape[0]
+        for (d, pairs), row in zip(got, want):
+            assert d == dl
+            flat = [x for pr in pairs for x in pr]
+            assert flat == [int(x) for x in row[:len(flat)]] and not any(row[len(flat):])

@@ -408,3 +408,49 @@ def test_occupation_vs_exact_small_n():
     assert 0.2 < st["acceptance"] < 0.95
     assert np.allclose(mean.sum(-1), N * Mexact)
     assert np.all(np.abs(mean - ref) < 5 * err + 2e-3), np.abs(mean - ref).max()
+
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+
+
+def test_geneig_golden():
+    """GPU eigensolver against the committed scipy.linalg.eigh vectors"""
+    from laughlinedge_amd import geneig
+    g = np.load(GOLDEN + "/geneig_scipy.npz")
+    print("case geneig golden", flush=True)
+    for k in range(3):
+        ev = geneig(g[f"S{k}"], g[f"H{k}"])
+        assert np.allclose(ev, g[f"E{k}"], rtol=1e-10, atol=1e-10), k
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_sampled_blocks_vs_exact_n4(prec):
+    """N = 4, m = 3, dL = 0..4: GPU chain + features + Gram against the exact matrices of
+    the committed fixture, within 5 batch sigma per entry, and the spectra within 5 sigma."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.stats import batch_sigma
+    g = np.load(GOLDEN + "/exact_n4_m3.npz")
+    N, m, w, eps = int(g["N"]), int(g["m"]), float(g["w"]), float(g["eps"])
+    print(f"case sampled vs exact N=4 prec={prec}", flush=True)
+    nb = 8
+    with Engine(N, m, [0, 1, 2, 3, 4], w=w, eps=eps, walkers=8192, samples=400, n_blocks=nb, seed=2024,
+                tune_sweeps=40, step=0.5, precision=prec) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(300)
+        for b in range(nb):
+            e.sample(50)
+        blocks = [e.moments(b) for b in range(nb)]
+        specs = [e.spectra(b) for b in range(nb)]
+    for dl in range(5):
+        for q, name in ((0, "S"), (1, "H")):
+            arr = np.array([bl[dl][q] for bl in blocks])
+            mean, sig = arr.mean(0), batch_sigma(arr.real) + 1j * batch_sigma(arr.imag)
+            ex = g[f"{name}{dl}"]
+            assert np.all(np.abs(mean.real - ex) <= 5 * sig.real + 1e-12 * np.abs(ex).max()), (dl, name)
+            assert np.all(np.abs(mean.imag) <= 5 * sig.imag + 1e-12 * np.abs(ex).max()), (dl, name)
+        from scipy.linalg import eigh
+        ex_e = eigh(g[f"H{dl}"], g[f"S{dl}"], eigvals_only=True)
+        sp = np.array([s[dl] for s in specs])
+        assert np.all(np.abs(sp.mean(0) - ex_e) <= 5 * batch_sigma(sp) + 1e-12), (dl, sp.mean(0), ex_e)

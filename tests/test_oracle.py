@@ -7,6 +7,7 @@ textbook library routines (scipy.linalg.eigh).
 """
 import itertools
 import math
+import os
 
 import numpy as np
 import pytest
@@ -130,6 +131,22 @@ def test_fixed_accumulation_vs_numpy(oracle_lib):
         assert np.allclose(So[s], S, rtol=1e-12, atol=1e-12 * np.abs(S).max())
         assert np.allclose(Ho[s], H, rtol=1e-12, atol=1e-12 * np.abs(H).max())
         off += d
+
+
+def test_weighted_accumulation_vs_numpy(oracle_lib):
+    """sum_s u_s conj(p_a) p_b between two sectors (harmonic and user-weight blocks)."""
+    N, m, sectors = 7, 3, [0, 2, 3]
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05, eps=1e-3)
+    rng = np.random.default_rng(4)
+    pos = rng.normal(size=(400, N, 2)) * 3
+    u = rng.normal(size=400)
+    P = _numpy_features(pos, N, m, o.basis.states)
+    offs = np.concatenate([[0], np.cumsum(o.dims)])
+    for a, b in ((0, 1), (1, 2), (2, 1), (2, 2)):
+        Pa, Pb = P[:, offs[a]:offs[a + 1]], P[:, offs[b]:offs[b + 1]]
+        ref = Pa.conj().T @ (u[:, None] * Pb)
+        got = o.accumulate_weighted(pos, u, a, b)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), (a, b)
 
 
 def test_geneig_vs_scipy(oracle_lib):
@@ -308,3 +325,31 @@ def test_learned_oracle_gradient_by_finite_differences():
         Lm, _ = learned.batch_loss_grad(p - e, configs, idx, noise, m, R, 0.7)
         fd = (Lp - Lm) / 2e-6
         assert abs(fd - G[k]) < 1e-5 + 1e-4 * abs(fd), (k, fd, G[k])
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_golden_geneig(oracle_lib):
+    """oracle eigensolver against the committed scipy.linalg.eigh vectors"""
+    g = np.load(os.path.join(GOLDEN, "geneig_scipy.npz"))
+    for k in range(3):
+        ev = oracle_lib.geneig(g[f"S{k}"], g[f"H{k}"])
+        assert np.allclose(ev, g[f"E{k}"], rtol=1e-10, atol=1e-10)
+
+
+def test_golden_exact_blocks():
+    """the exact N = 4 fixture is what oracle/exact_lll.py computes, and it obeys
+    H = w (L0 + dL + N) S at eps = 0 up to the eps r^4 part"""
+    from oracle.exact_lll import exact_blocks
+    from oracle.partitions import Basis
+    g = np.load(os.path.join(GOLDEN, "exact_n4_m3.npz"))
+    N, m, w, eps = int(g["N"]), int(g["m"]), float(g["w"]), float(g["eps"])
+    for dl in range(5):
+        b = Basis(N, [dl])
+        S, H = exact_blocks(N, m, b.states, w, eps)
+        assert np.allclose(S, g[f"S{dl}"], rtol=1e-13)
+        assert np.allclose(H, g[f"H{dl}"], rtol=1e-13)
+        S0, H0 = exact_blocks(N, m, b.states, w, 0.0)
+        L0 = m * N * (N - 1) // 2
+        assert np.allclose(H0, w * (L0 + dl + N) * np.array(S0), rtol=1e-12)

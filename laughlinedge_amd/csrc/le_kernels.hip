@@ -750,6 +750,192 @@ __device__ double block_sum(double v, double* red)
     return t;
 }
 
+// Cholesky S = L L^H (lower), right-looking: column j is scaled, then the trailing lower
+// triangle takes the rank-1 update with column j staged in LDS (coalesced row reads).
+// A holds S on entry (full); on return its lower triangle holds L.  false if S is not
+// positive definite (uniform across the block).
+__device__ bool chol_lower(cd* A, int d, cd* col, double* sh)
+{
+    const int tid = threadIdx.x;
+    for (int j = 0; j < d; ++j) {
+        __syncthreads();
+        if (tid == 0) {
+            const double a = A[(size_t)j * d + j].r;
+            *sh = a > 0.0 ? sqrt(a) : -1.0;
+        }
+        __syncthreads();
+        const double ljj = *sh;
+        if (!(ljj > 0.0)) return false;
+        const int n = d - j - 1;
+        const double inv = 1.0 / ljj;
+        for (int i = tid; i < n; i += kEigThreads) {
+            const cd v = cscale(A[(size_t)(j + 1 + i) * d + j], inv);
+            A[(size_t)(j + 1 + i) * d + j] = v;
+            col[i] = v;
+        }
+        if (tid == 0) A[(size_t)j * d + j] = {ljj, 0.0};
+        __syncthreads();
+        if (n == 0) break;
+        const int rows = kEigThreads / n;          // rows per pass (n < kEigThreads)
+        const int r0 = tid / n, k = tid % n;
+        if (r0 < rows)
+            for (int i = r0; i < n; i += rows)
+                if (k <= i) {
+                    cd* a = &A[(size_t)(j + 1 + i) * d + (j + 1 + k)];
+                    *a = csub(*a, cmulc(col[i], col[k]));
+                }
+    }
+    __syncthreads();
+    return true;
+}
+
+// C = L^-1 H L^-H from S and H: Cholesky into L, T = L^-1 H (one thread per column),
+// W = L^-1 T^H, C = (W^H + W) / 2 made exactly Hermitian.
+__device__ bool reduce_to_standard(const cd* S, const cd* H, cd* L, cd* C, cd* W, cd* T, int d, cd* col, double* sh)
+{
+    const int tid = threadIdx.x;
+    for (long x = tid; x < (long)d * d; x += kEigThreads) L[x] = S[x];
+    if (!chol_lower(L, d, col, sh)) return false;
+    for (int c = tid; c < d; c += kEigThreads)
+        for (int i = 0; i < d; ++i) {
+            cd t = H[(size_t)i * d + c];
+            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], T[(size_t)k * d + c]));
+            T[(size_t)i * d + c] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+        }
+    __syncthreads();
+    for (int r = tid; r < d; r += kEigThreads)
+        for (int i = 0; i < d; ++i) {
+            cd t = cconj(T[(size_t)r * d + i]);
+            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], W[(size_t)k * d + r]));
+            W[(size_t)i * d + r] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+        }
+    __syncthreads();
+    for (long x = tid; x < (long)d * d; x += kEigThreads) {
+        const int i = (int)(x / d), j = (int)(x % d);
+        cd c = cscale(cadd(cconj(W[(size_t)j * d + i]), W[(size_t)i * d + j]), 0.5);
+        if (i == j) c.i = 0.0;
+        C[x] = c;
+    }
+    __syncthreads();
+    return true;
+}
+
+// Eigenvalues only: reduce to C = L^-1 H L^-H, then Householder tridiagonalization of the
+// Hermitian C (H_k = I - tau v v^H zeroes column k below the subdiagonal; the trailing block
+// takes B <- B - v w^H - w v^H with p = tau B v, w = p - (tau/2)(v^H p) v, its columns read
+// coalesced through B[i][j] = conj(B[j][i])), then the real symmetric tridiagonal with
+// diagonal Re C_ii and off-diagonal |beta_k| (a diagonal unitary similarity removes the
+// phases) is solved by Sturm-count bisection, one thread per eigenvalue, ascending.
+__global__ __launch_bounds__(kEigThreads) void k_geneig_vals(const double* Sd, const double* Hd, double* workd,
+                                                            const GenEigProblem* probs, double* evals, int* status)
+{
+    __shared__ double red[kEigThreads / 64];
+    __shared__ cd vv[2 * kEigMaxPairs], ww[2 * kEigMaxPairs];
+    __shared__ double da[2 * kEigMaxPairs], db[2 * kEigMaxPairs];
+    __shared__ double sh;
+
+    const GenEigProblem pr = probs[blockIdx.x];
+    const int d = pr.d;
+    const int tid = threadIdx.x;
+    const cd* S = reinterpret_cast<const cd*>(Sd) + pr.in_off;
+    const cd* H = reinterpret_cast<const cd*>(Hd) + pr.in_off;
+    cd* L = reinterpret_cast<cd*>(workd) + pr.work_off;
+    cd* C = L + (size_t)d * d;
+    cd* W = C + (size_t)d * d;
+    cd* T = W + (size_t)d * d;
+    double* ev = evals + pr.eval_off;
+
+    if (!reduce_to_standard(S, H, L, C, W, T, d, vv, &sh)) {
+        if (tid == 0) status[blockIdx.x] = 1;
+        return;
+    }
+    for (int k = 0; k + 2 < d; ++k) {
+        const int n = d - k - 1;
+        double part = 0.0;
+        for (int i = tid; i < n; i += kEigThreads) {
+            const cd x = C[(size_t)(k + 1 + i) * d + k];
+            vv[i] = x;
+            part += x.r * x.r + x.i * x.i;
+        }
+        const double alpha2 = block_sum(part, red);   // synchronises
+        const double alpha = sqrt(alpha2);
+        const cd x0 = vv[0];
+        const double ax0 = sqrt(x0.r * x0.r + x0.i * x0.i);
+        const cd ph = ax0 > 0.0 ? cd{x0.r / ax0, x0.i / ax0} : cd{1.0, 0.0};
+        if (tid == 0) db[k] = alpha;                 // |beta_k|, beta_k = -ph alpha
+        __syncthreads();                              // every thread has read vv[0]
+        if (!(alpha > 0.0)) continue;                 // column already reduced (uniform)
+        const double tau = 1.0 / (alpha * (alpha + ax0));
+        if (tid == 0) vv[0] = cadd(x0, cscale(ph, alpha));
+        __syncthreads();
+        // p = tau B v (thread i: column i of B, coalesced across threads)
+        for (int i = tid; i < n; i += kEigThreads) {
+            cd acc = {0.0, 0.0};
+            const cd* colp = &C[(size_t)(k + 1) * d + (k + 1 + i)];
+            for (int j = 0; j < n; ++j) {
+                const cd b = colp[(size_t)j * d];
+                const cd v = vv[j];
+                acc.r += b.r * v.r + b.i * v.i;      // conj(b) v
+                acc.i += b.r * v.i - b.i * v.r;
+            }
+            ww[i] = cscale(acc, tau);
+        }
+        __syncthreads();
+        double vp = 0.0;
+        for (int i = tid; i < n; i += kEigThreads) vp += vv[i].r * ww[i].r + vv[i].i * ww[i].i;   // Re(v^H p)
+        const double K = 0.5 * tau * block_sum(vp, red);
+        for (int i = tid; i < n; i += kEigThreads) ww[i] = csub(ww[i], cscale(vv[i], K));
+        __syncthreads();
+        const int rows = kEigThreads / n;
+        const int r0 = tid / n, j = tid % n;
+        if (r0 < rows)
+            for (int i = r0; i < n; i += rows) {
+                cd* a = &C[(size_t)(k + 1 + i) * d + (k + 1 + j)];
+                const cd u = cadd(cmulc(vv[i], ww[j]), cmulc(ww[i], vv[j]));
+                *a = csub(*a, u);
+            }
+        __syncthreads();
+    }
+    if (d >= 2) {
+        __syncthreads();
+        if (tid == 0) {
+            const cd e = C[(size_t)(d - 1) * d + (d - 2)];
+            db[d - 2] = sqrt(e.r * e.r + e.i * e.i);
+        }
+    }
+    for (int i = tid; i < d; i += kEigThreads) da[i] = C[(size_t)i * d + i].r;
+    __syncthreads();
+    // Gershgorin interval and pivot floor
+    double lo = 1e300, hi = -1e300, bmax = 0.0;
+    for (int i = 0; i < d; ++i) {
+        const double r = (i > 0 ? db[i - 1] : 0.0) + (i + 1 < d ? db[i] : 0.0);
+        lo = fmin(lo, da[i] - r);
+        hi = fmax(hi, da[i] + r);
+        if (i + 1 < d) bmax = fmax(bmax, db[i]);
+    }
+    const double span = fmax(fabs(lo), fabs(hi));
+    const double pivmin = 1e-290 * fmax(1.0, bmax * bmax);
+    for (int m = tid; m < d; m += kEigThreads) {
+        double l = lo - 1e-12 * span - 1e-300, h = hi + 1e-12 * span + 1e-300;
+        for (int it = 0; it < 200 && h - l > 2.2e-16 * fmax(fabs(l), fabs(h)) + 1e-300; ++it) {
+            const double x = 0.5 * (l + h);
+            int cnt = 0;
+            double q = da[0] - x;
+            if (fabs(q) < pivmin) q = -pivmin;
+            cnt += q < 0.0;
+            for (int i = 1; i < d; ++i) {
+                q = da[i] - x - db[i - 1] * db[i - 1] / q;
+                if (fabs(q) < pivmin) q = -pivmin;
+                cnt += q < 0.0;
+            }
+            if (cnt > m) h = x;
+            else l = x;
+        }
+        ev[m] = 0.5 * (l + h);
+    }
+    if (tid == 0) status[blockIdx.x] = 0;
+}
+
 __global__ __launch_bounds__(kEigThreads) void k_geneig(const double* Sd, const double* Hd, double* workd,
                                                        const GenEigProblem* probs, double* evals,
                                                        double* evecsd, int* status)
@@ -758,6 +944,8 @@ __global__ __launch_bounds__(kEigThreads) void k_geneig(const double* Sd, const 
     __shared__ double rc[kEigMaxPairs], rs[kEigMaxPairs];
     __shared__ cd rph[kEigMaxPairs];
     __shared__ int rp[kEigMaxPairs], rq[kEigMaxPairs];
+    __shared__ cd gcol[2 * kEigMaxPairs];
+    __shared__ double gsh;
 
     const GenEigProblem pr = probs[blockIdx.x];
     const int d = pr.d;
@@ -771,56 +959,12 @@ __global__ __launch_bounds__(kEigThreads) void k_geneig(const double* Sd, const 
     cd* V = reinterpret_cast<cd*>(evecsd) + pr.in_off;
     double* ev = evals + pr.eval_off;
 
-    for (long x = tid; x < (long)d * d; x += kEigThreads) L[x] = S[x];
-    __syncthreads();
-
-    // Cholesky S = L L^H (lower)
-    for (int j = 0; j < d; ++j) {
-        double part = 0.0;
-        for (int k = tid; k < j; k += kEigThreads) {
-            cd l = L[(size_t)j * d + k];
-            part += l.r * l.r + l.i * l.i;
-        }
-        double sdot = block_sum(part, red);
-        double sjj = L[(size_t)j * d + j].r - sdot;
-        if (!(sjj > 0.0)) {
-            if (tid == 0) status[blockIdx.x] = 1;
-            return;   // uniform across the block
-        }
-        const double ljj = sqrt(sjj);
-        __syncthreads();
-        for (int i = j + 1 + tid; i < d; i += kEigThreads) {
-            cd t = L[(size_t)i * d + j];
-            for (int k = 0; k < j; ++k) t = csub(t, cmulc(L[(size_t)i * d + k], L[(size_t)j * d + k]));
-            L[(size_t)i * d + j] = cscale(t, 1.0 / ljj);
-        }
-        __syncthreads();
-        if (tid == 0) L[(size_t)j * d + j] = {ljj, 0.0};
-        __syncthreads();
+    if (!reduce_to_standard(S, H, L, C, W, T, d, gcol, &gsh)) {
+        if (tid == 0) status[blockIdx.x] = 1;
+        return;   // uniform across the block
     }
-    // T = L^-1 H (one thread per column)
-    for (int c = tid; c < d; c += kEigThreads)
-        for (int i = 0; i < d; ++i) {
-            cd t = H[(size_t)i * d + c];
-            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], T[(size_t)k * d + c]));
-            T[(size_t)i * d + c] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
-        }
-    __syncthreads();
-    // W = L^-1 T^H ; C = W^H  (so C = L^-1 H L^-H)
-    for (int r = tid; r < d; r += kEigThreads)
-        for (int i = 0; i < d; ++i) {
-            cd t = cconj(T[(size_t)r * d + i]);
-            for (int k = 0; k < i; ++k) t = csub(t, cmul(L[(size_t)i * d + k], W[(size_t)k * d + r]));
-            W[(size_t)i * d + r] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
-        }
-    __syncthreads();
     for (long x = tid; x < (long)d * d; x += kEigThreads) {
         const int i = (int)(x / d), j = (int)(x % d);
-        cd a = cconj(W[(size_t)j * d + i]);
-        cd b = W[(size_t)i * d + j];
-        cd c = cscale(cadd(a, b), 0.5);
-        if (i == j) c.i = 0.0;
-        C[x] = c;
         T[x] = (i == j) ? cd{1.0, 0.0} : cd{0.0, 0.0};   // eigenvector accumulator
     }
     __syncthreads();
@@ -1051,8 +1195,12 @@ hipError_t launch_geneig(const double* S, const double* H, double* work, const G
                          int nprob, double* evals, double* evecs, int* status, hipStream_t s)
 {
     if (nprob <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geneig, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
-                       evecs, status);
+    if (evecs)
+        hipLaunchKernelGGL(k_geneig, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
+                           evecs, status);
+    else
+        hipLaunchKernelGGL(k_geneig_vals, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
+                           status);
     return hipGetLastError();
 }
 

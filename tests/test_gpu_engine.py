@@ -107,6 +107,32 @@ def test_geneig_gpu_vs_oracle(oracle_lib):
         assert np.allclose(g, np.eye(d), atol=1e-8)
 
 
+def test_geneig_values_only_path():
+    """eigenvalue-only solves (Householder tridiagonalization + bisection) vs scipy.linalg.eigh
+    up to the 640-state limit, a fully degenerate pencil H = c S, and a block-diagonal pencil
+    whose Householder columns vanish"""
+    from scipy.linalg import eigh
+    from laughlinedge_amd import geneig
+    rng = np.random.default_rng(8)
+    for d in (1, 2, 3, 17, 131, 300, 640):
+        print(f"case geneig values d={d}", flush=True)
+        A = rng.standard_normal((d, 2 * d)) + 1j * rng.standard_normal((d, 2 * d))
+        S = A @ A.conj().T / (2 * d) + 0.1 * np.eye(d)
+        B = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        H = (B + B.conj().T) / 2
+        ev = geneig(S, H)
+        ref = eigh(H, S, eigvals_only=True)
+        assert np.allclose(ev, ref, rtol=0, atol=1e-9 * np.abs(ref).max()), d
+    d = 40
+    A = rng.standard_normal((d, 2 * d)) + 1j * rng.standard_normal((d, 2 * d))
+    S = A @ A.conj().T / (2 * d) + 0.1 * np.eye(d)
+    assert np.allclose(geneig(S, 3.25 * S), 3.25, rtol=1e-12)
+    Hb = np.diag(rng.standard_normal(d)).astype(complex)
+    Hb[:10, :10] += (B := rng.standard_normal((10, 10)) + 1j * rng.standard_normal((10, 10))) + B.conj().T
+    ev = geneig(np.eye(d, dtype=complex), Hb)
+    assert np.allclose(ev, np.linalg.eigvalsh(Hb), atol=1e-11)
+
+
 def test_sampler_r2_sum_rule():
     """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler."""
     from laughlinedge_amd import Engine

@@ -81,11 +81,11 @@ def spectra_blocks(eng, dims):
     return mean, sig
 
 
-def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5):
+def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1):
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
-                  trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01)
+                  trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs)
 
 
 def timed_sampling(eng, steps, nblk):
@@ -98,7 +98,7 @@ def timed_sampling(eng, steps, nblk):
     wall = time.perf_counter() - t0
     st1 = eng.stats()
     return wall, {k: st1[k] - st0[k] for k in ("ms_sweep", "ms_features", "ms_gram", "gram_launches",
-                                               "sweep_launches")}
+                                               "sweep_launches", "ms_sampler_graph")}
 
 
 def main():
@@ -144,18 +144,30 @@ def main():
     log(f"step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} burn-in done")
     if warm > 0:
         eng.sample(warm)
-    # timed region: walkers resident in HBM, steps x (sweep + features + Gram)
+    # timed region: walkers resident in HBM, steps x (sweep + features + Gram), sampler
+    # rounds replayed from hipGraphs
     wall, d = timed_sampling(eng, steps, nblk)
     updates = args.walkers * steps
     value = updates / wall
     st = eng.stats()
     log(f"timed: {value:.4e} sample-updates/s, {1e3 * wall / steps:.3f} ms/step")
+    # per-kernel split of the sampler stream: the same workload without graphs
+    ksteps = min(steps, 80)
+    eg = make_engine(args.walkers, 0, ksteps + 8, 1, 0, graphs=0)
+    eg.init_walkers()
+    eg.tune_step()
+    eg.burnin(min(args.burnin, 100))
+    eg.sample(8)
+    wall_eager, de = timed_sampling(eg, ksteps, 1)
+    eg.close()
+    log(f"eager: {args.walkers * ksteps / wall_eager:.4e} sample-updates/s")
 
     gram_flop = 16.0 * sum_d2 * updates
     gram_rate = gram_flop / (d["ms_gram"] * 1e-3)
     launches = max(1, d["gram_launches"])
-    sweep_rate = 12.0 * N * (N - 1) * updates / (d["ms_sweep"] * 1e-3)
-    stream_bytes = (8 * info.D + 4) * updates
+    eager_updates = args.walkers * ksteps
+    sweep_rate = 12.0 * N * (N - 1) * eager_updates / (de["ms_sweep"] * 1e-3)
+    stream_bytes = (8 * info.D + 4) * eager_updates
     rows_per_launch = info.samples_per_launch * args.walkers
     traffic, traffic_note = gram_traffic(rows_per_launch)
     taus = taus_of(eng, steps, tw)
@@ -178,7 +190,8 @@ def main():
         "config": {"workload": "laughlin-edge N=200 nu=1/3 dL=1..16 grouped (D=914), V=0.025 r^2+5e-5 r^4",
                    "walkers": args.walkers, "seed": SEED, "burnin_sweeps": args.burnin, "thin": 1,
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
-                   "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles},
+                   "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,
+                   "sampler_rounds": "hipGraph replay"},
         "roofline": {"bound": "mfma", "kernel": "k_gram<float>",
                      "achieved": gram_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": gram_rate / PEAK_FP32, "traffic": traffic,
@@ -191,15 +204,17 @@ def main():
             "sampler": {"kernel": f"k_sweep<{eng.info.lanes_per_walker},float>", "bound": "valu",
                         "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
                         "frac": sweep_rate / PEAK_FP32, "flop_per_walker_sweep": 12 * N * (N - 1),
-                        "ms_per_sweep": d["ms_sweep"] / steps},
+                        "ms_per_sweep": de["ms_sweep"] / ksteps},
             "features": {"kernel": "k_features<float,16>", "bound": "hbm",
-                         "ms_per_step": d["ms_features"] / steps,
-                         "sample_matrix_gb_per_s": stream_bytes / (d["ms_features"] * 1e-3) / 1e9,
+                         "ms_per_step": de["ms_features"] / ksteps,
+                         "sample_matrix_gb_per_s": stream_bytes / (de["ms_features"] * 1e-3) / 1e9,
                          "bytes_per_sample": 8 * info.D + 4},
-            "gram": {"ms_per_step": d["ms_gram"] / steps, "launches": d["gram_launches"],
-                     "sample_matrix_read_gb_per_s": stream_bytes / (d["ms_gram"] * 1e-3) / 1e9},
-            "note": "sampler and features share one stream; Gram runs on a second stream on the other "
-                    "sample buffer, so ms_per_step < sum of the three",
+            "gram": {"ms_per_step": d["ms_gram"] / steps, "launches": d["gram_launches"]},
+            "sampler_graph_ms_per_step": d["ms_sampler_graph"] / steps,
+            "eager_sample_updates_per_s": eager_updates / wall_eager,
+            "note": "sampler and features share one stream (replayed from hipGraphs in the timed run; "
+                    "their split is from an eager run of the same workload); Gram runs on a second "
+                    "stream on the other sample buffer, so ms_per_step < sum of the parts",
         },
         "acceptance": st["acceptance"],
         "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max,
@@ -267,7 +282,7 @@ def forms(args, thins):
             tmax = max(taus.values())
             ess = args.walkers * fs / (2.0 * tmax) / wall
             runs.append({"thin": thin, "ms_per_kept_sample": 1e3 * wall / fs,
-                         "ms_sweep_per_sweep": d["ms_sweep"] / (fs * thin),
+                         "ms_sampler_per_kept_sample": (d["ms_sweep"] + d["ms_features"] + d["ms_sampler_graph"]) / fs,
                          "tau_max": tmax, "tau_max_obs": max(taus, key=taus.get),
                          "tau_p1": max(taus["Re p1"], taus["Im p1"]), "tau_V": taus["V"],
                          "ess_per_s": ess, "acceptance": eng.stats()["acceptance"]})

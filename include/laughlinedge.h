@@ -79,6 +79,10 @@ typedef struct le_mc_params {
     int samples_per_launch;    /* kept samples per sweep+feature+Gram round (0: auto) */
     int n_blocks;              /* equal time blocks for error bars (>= 1) */
     int trace_walkers;         /* walkers whose p_1..p_4, V series are recorded */
+    int graphs;                /* 1 (default): capture each full sampling round (sweeps + features)
+                                  in a hipGraph and replay it (rounds with user weights, the
+                                  occupation chain and partial rounds run eagerly); 0: eager, with
+                                  per-kernel times in le_run_stats */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -135,6 +139,7 @@ typedef struct le_run_stats {
     double ms_sample_total;    /* wall time of le_sample calls */
     long long gram_launches;
     long long sweep_launches;
+    double ms_sampler_graph;   /* device time of replayed sampling graphs (sweeps + features) */
 } le_run_stats;
 
 void le_mc_params_default(le_mc_params* p);

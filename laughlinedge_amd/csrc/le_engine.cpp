@@ -158,13 +158,28 @@ struct le_engine {
     long long acc_moves = 0;           // proposed moves during sampling sweeps
     unsigned long long acc_sampling = 0;
     std::vector<hipEvent_t> ev;
+    // sampler rounds captured as graphs (mc.graphs): one per (buffer, traced) and the
+    // parameters baked into their kernel arguments
+    struct SampleGraph {
+        hipGraphExec_t exec;
+        int buf;
+        bool traced;
+        int thin, proposal, G, n, table_T;
+        double step;
+        const float* table;
+    };
+    std::vector<SampleGraph> graphs;
+    unsigned long long* bases_d = nullptr;   // [0] sweep base, [1] trace row base
 
     ~le_engine();
     int setup(const le_observables* obs);
     const void* weight_ptr(int wid, int buf = 0) const;
     long item_offset(int wid, int s1, int s2) const;
     int autotune_lanes();
-    int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions);
+    int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
+                   const unsigned long long* sweep_base = nullptr);
+    int sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
+    void drop_graphs();
     int close_block(long long samples);
     int gram_rows(long rows, int buf = 0, hipStream_t s = nullptr);
     double measure_acceptance(double trial_step, int attempt, int* rc);
@@ -196,8 +211,16 @@ hipError_t le_engine::features2(const void* positions, int pos_is_double, int ns
     return le::launch_features2(f, prec, stream);
 }
 
+void le_engine::drop_graphs()
+{
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 le_engine::~le_engine()
 {
+    drop_graphs();
+    (void)hipFree(bases_d);
     (void)hipFree(pos);
     (void)hipFree(acc);
     (void)hipFree(Xb[0]);
@@ -251,6 +274,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->samples_per_launch = 0;
     p->n_blocks = 1;
     p->trace_walkers = 0;
+    p->graphs = 1;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -489,6 +513,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
+    if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -596,9 +621,11 @@ extern "C" int le_basis_state(const le_engine* e, int state, int* dL, int* pairs
     return LE_OK;
 }
 
-int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* positions)
+int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
+                          const unsigned long long* sweep_base)
 {
     le::SweepArgs a{};
+    a.sweep_base = sweep_base;
     a.pos = positions;
     a.accepted = acc;
     a.seed = mc.seed;
@@ -850,6 +877,53 @@ int le_engine::close_block(long long samples)
     return LE_OK;
 }
 
+// Capture (once per set of baked-in parameters) the sampler work of one full round: spl x
+// (thin sweeps + features) on the sampler stream.  The Philox sweep base and the trace row
+// base are read from bases_d, which a one-thread kernel sets before each replay.
+int le_engine::sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out)
+{
+    for (auto& g : graphs)
+        if (g.buf == buf && g.traced == traced && g.thin == mc.thin && g.step == step && g.proposal == proposal &&
+            g.G == G && g.table == table && g.table_T == table_T && g.n == spl) {
+            *out = g.exec;
+            return LE_OK;
+        }
+    for (size_t q = 0; q < graphs.size();)
+        if (graphs[q].buf == buf && graphs[q].traced == traced) {
+            (void)hipGraphExecDestroy(graphs[q].exec);
+            graphs.erase(graphs.begin() + (long)q);
+        } else {
+            ++q;
+        }
+    if (!bases_d) HIPCHK(dalloc(&bases_d, 2));
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = LE_OK;
+    hipError_t h = hipSuccess;
+    for (int q = 0; q < spl && rc == LE_OK && h == hipSuccess; ++q) {
+        rc = run_sweeps(mc.thin, (unsigned long long)q * (unsigned long long)mc.thin, pos, bases_d);
+        f.row0 = (long)q * mc.walkers;
+        f.trace = traced ? trace_d : nullptr;
+        f.trace_walkers = traced ? tw : 0;
+        f.trace_row0 = (long)q * tw;
+        f.trace_base = traced ? reinterpret_cast<const long long*>(bases_d + 1) : nullptr;
+        if (rc == LE_OK) h = le::launch_features(f, prec, stream);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t he = hipStreamEndCapture(stream, &graph);
+    if (rc != LE_OK || h != hipSuccess || he != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != LE_OK) return rc;
+        return fail(LE_ERR_HIP, hipGetErrorString(h != hipSuccess ? h : he));
+    }
+    hipGraphExec_t ex = nullptr;
+    h = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
+    graphs.push_back({ex, buf, traced, mc.thin, proposal, G, spl, table_T, step, table});
+    *out = ex;
+    return LE_OK;
+}
+
 extern "C" int le_sample(le_engine* e, int samples)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
@@ -900,7 +974,25 @@ extern "C" int le_sample(le_engine* e, int samples)
         HIPCHK(hipStreamWaitEvent(e->stream, e->drained[b], 0));
         f.X = e->Xb[b];
         f.W = e->Wbb[b];
-        for (int q = 0; q < n; ++q) {
+        const bool traced_all = e->trace_d && e->trace_rows + (long)n * tw <= e->trace_cap_rows;
+        const bool trace_full = !e->trace_d || e->trace_rows + tw > e->trace_cap_rows;
+        const bool use_graph = e->mc.graphs && !e->two_sided && !(e->wfn && e->n_user) && n == e->spl &&
+                               (traced_all || trace_full);
+        if (use_graph) {
+            hipGraphExec_t gx = nullptr;
+            int rc = e->sample_graph(f, b, traced_all, tw, &gx);
+            if (rc) return rc;
+            hipEvent_t e0 = ev(), e1 = ev();
+            if (!e0 || !e1) return fail(LE_ERR_HIP, "event creation failed");
+            HIPCHK(hipEventRecord(e0, e->stream));
+            HIPCHK(le::launch_set_bases(e->bases_d, e->sweep_counter, traced_all ? e->trace_rows : 0, e->stream));
+            HIPCHK(hipGraphLaunch(gx, e->stream));
+            HIPCHK(hipEventRecord(e1, e->stream));
+            spans.push_back({e0, e1, 3});
+            e->sweep_counter += (unsigned long long)n * (unsigned long long)e->mc.thin;
+            if (traced_all) e->trace_rows += (long)n * tw;
+        }
+        for (int q = 0; q < n && !use_graph; ++q) {
             hipEvent_t e0 = ev(), e1 = ev(), e2 = ev();
             if (!e0 || !e1 || !e2) return fail(LE_ERR_HIP, "event creation failed");
             HIPCHK(hipEventRecord(e0, e->stream));
@@ -957,7 +1049,8 @@ extern "C" int le_sample(le_engine* e, int samples)
     for (auto& sp : spans) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, sp.a, sp.b));
-        if (sp.kind == 0) e->st.ms_sweep += ms;
+        if (sp.kind == 3) e->st.ms_sampler_graph += ms;
+        else if (sp.kind == 0) e->st.ms_sweep += ms;
         else if (sp.kind == 1) e->st.ms_features += ms;
         else e->st.ms_gram += ms;
     }
@@ -1491,6 +1584,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
+    if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;

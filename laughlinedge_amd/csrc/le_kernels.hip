@@ -99,13 +99,14 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 
     const Real step = (Real)a.step;
     const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
+    const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const bool has_pad = (lane + G * (S - 1)) >= N;
     const int pad_j = lane + G * (S - 1);
     unsigned accepted = 0;
 
     for (int sw = 0; sw < a.nsweeps; ++sw) {
-        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const unsigned long long sidx = sweep_first + (unsigned long long)sw;
         const uint32_t c1 = (uint32_t)sidx;
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int base = 0; base < N; base += G) {
@@ -365,7 +366,8 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
             float v;
             if (lane < 8) v = (lane / 2 < a.K) ? (float)sq[wv][lane] : 0.f;
             else v = (float)sq[wv][QV];
-            a.trace[((size_t)a.trace_row0 + w) * 9 + lane] = v;
+            const long long tb = a.trace_base ? *a.trace_base : 0ll;
+            a.trace[((size_t)(a.trace_row0 + tb) + w) * 9 + lane] = v;
         }
         __builtin_amdgcn_wave_barrier();   // sq / ptab are reused by the next sample
     }
@@ -713,6 +715,14 @@ __global__ void k_extract(ExtractArgs e)
     const double im = cget(it.C, ldc, pa, qb) - cget(it.C, ldc, qa, pb);
     e.out[2 * g] = re * e.scale;
     e.out[2 * g + 1] = im * e.scale;
+}
+
+__global__ void k_set_bases(unsigned long long* b, unsigned long long sweep_base, long long trace_base)
+{
+    if (threadIdx.x == 0) {
+        b[0] = sweep_base;
+        b[1] = (unsigned long long)trace_base;
+    }
 }
 
 __global__ void k_zero(double* p, long n)
@@ -1201,6 +1211,13 @@ hipError_t launch_geneig(const double* S, const double* H, double* work, const G
     else
         hipLaunchKernelGGL(k_geneig_vals, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
                            status);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_bases(unsigned long long* bases, unsigned long long sweep_base, long long trace_base,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_bases, dim3(1), dim3(64), 0, s, bases, sweep_base, trace_base);
     return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@ struct SweepArgs {
     unsigned long long sweep0;   // global sweep index of the first sweep (Philox counter)
     int N, m, walkers, nsweeps;
     double step;
+    const unsigned long long* sweep_base;   // optional device value added to sweep0 (graph replays)
 };
 
 constexpr int kMaxHarmonics = 8;
@@ -47,6 +48,7 @@ struct FeatureArgs {
     float* trace;            // optional [trace_rows][9]: Re/Im p1..p4, V
     int trace_walkers;
     long trace_row0;
+    const long long* trace_base;   // optional device value added to trace_row0 (graph replays)
 };
 
 // Two-sided chain features (occupation numbers): positions [walkers][N+1]; bulk power
@@ -132,6 +134,9 @@ hipError_t launch_geneig(const double* S, const double* H, double* work,
                          const GenEigProblem* probs, int nprob, double* evals, double* evecs,
                          int* status, hipStream_t s);
 hipError_t launch_zero(double* p, long n, hipStream_t s);
+// bases[0] = sweep base, bases[1] = trace row base (read by replayed sample graphs)
+hipError_t launch_set_bases(unsigned long long* bases, unsigned long long sweep_base, long long trace_base,
+                            hipStream_t s);
 
 size_t sweep_lds_bytes(int prec, int lanes_per_walker, int N);
 size_t sweep2_lds_bytes(int prec, int lanes_per_walker, int N);

@@ -404,13 +404,14 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
     if (!live) return;
     const float mfac = (float)a.m * 0.69314718055994531f;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
     const bool has_pad = (lane + G * (S - 1)) >= N;
     const int pad_j = lane + G * (S - 1);
     const int skip_p = has_pad ? (S - 1) : -1;
     const float2 far = make_float2(1e4f, 1e4f);
     unsigned accepted = 0;
     for (int sw = 0; sw < a.nsweeps; ++sw) {
-        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const unsigned long long sidx = sweep_first + (unsigned long long)sw;
         const uint32_t c1 = (uint32_t)sidx;
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int base = 0; base < N; base += G) {

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
     unsigned accepted = 0;
 
     for (int sw = 0; sw < a.nsweeps; ++sw) {
-        const unsigned long long sidx = a.sweep0 + (unsigned long long)sw;
+        const unsigned long long sidx = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull) + (unsigned long long)sw;
         const uint32_t c1 = (uint32_t)sidx;
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int i = 0; i <= N; ++i) {

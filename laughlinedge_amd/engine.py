@@ -88,7 +88,7 @@ class MCParams(ctypes.Structure):
                 ("acceptance_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
-                ("trace_walkers", ctypes.c_int)]
+                ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -108,7 +108,8 @@ class RunStats(ctypes.Structure):
                 ("sweeps", ctypes.c_longlong), ("samples", ctypes.c_longlong),
                 ("ms_sweep", ctypes.c_double), ("ms_features", ctypes.c_double),
                 ("ms_gram", ctypes.c_double), ("ms_sample_total", ctypes.c_double),
-                ("gram_launches", ctypes.c_longlong), ("sweep_launches", ctypes.c_longlong)]
+                ("gram_launches", ctypes.c_longlong), ("sweep_launches", ctypes.c_longlong),
+                ("ms_sampler_graph", ctypes.c_double)]
 
 
 _lib = None

@@ -133,6 +133,38 @@ def test_geneig_values_only_path():
     assert np.allclose(ev, np.linalg.eigvalsh(Hb), atol=1e-11)
 
 
+def test_sample_graphs_match_eager():
+    """Sampling rounds replayed from hipGraphs give exactly the eager rounds' moments, traces
+    and walkers: several replays per buffer, a partial (eager) round, then a thinning change
+    that forces a new capture, with Gram accumulation and block closing in between."""
+    from laughlinedge_amd import Engine
+    N, m, sectors, W = 24, 3, [1, 2, 3], 2048
+    print("case graphs vs eager N=24", flush=True)
+    out = []
+    for graphs in (0, 1):
+        with Engine(N, m, sectors, w=0.05, eps=1e-4, walkers=W, samples=400, seed=9, tune_sweeps=20, step=0.5,
+                    trace_walkers=32, n_blocks=3, graphs=graphs, harmonics=(1, 2)) as e:
+            e.init_walkers()
+            e.tune_step()
+            e.burnin(20)
+            spl = e.info.samples_per_launch
+            e.sample(2 * spl + 3)
+            e.set_thin(2)
+            e.sample(2 * spl)
+            e.sample(spl)
+            st = e.stats()
+            out.append(([e.moments(b) for b in range(e.n_blocks())], e.trace(), e.positions(),
+                        e.block(0x100 | 1, 0, 1), st))
+    (m0, t0, p0, h0, s0), (m1, t1, p1, h1, s1) = out
+    assert s1["ms_sampler_graph"] > 0 and s0["ms_sampler_graph"] == 0
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(h0, h1)
+    for b0, b1 in zip(m0, m1):
+        for (S0, H0), (S1, H1) in zip(b0, b1):
+            assert np.array_equal(S0, S1) and np.array_equal(H0, H1)
+
+
 def test_sampler_r2_sum_rule():
     """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler."""
     from laughlinedge_amd import Engine

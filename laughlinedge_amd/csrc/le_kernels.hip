@@ -461,22 +461,31 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
     const Real* wp[2] = {reinterpret_cast<const Real*>(g.w0), reinterpret_cast<const Real*>(g.w1)};
+    // staging: 32 rows x 64 columns per side; 256 threads x 8 elements.  The next step's
+    // rows are loaded into registers while the MFMAs run on the current LDS tile.
+    const int rr = tid >> 3;           // 0..31
+    const int cc = (tid & 7) * 8;      // 0..56
+    Real bufi[8], bufj[8], bufw = 0;
+    auto fetch = [&](long r0) {
+        const long row = r0 + rr;
+        if (row < r_end) {
+            const Real* src = X + (size_t)row * g.ldx;
+            load8(src + tile.i0 + cc, bufi);
+            if (!same) load8(src + tile.j0 + cc, bufj);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) { bufi[e] = 0; bufj[e] = 0; }
+        }
+        if (tid < NW * kKC) {
+            const int q = tid / kKC, k = tid % kKC;
+            const long row2 = r0 + k;
+            bufw = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
+        }
+    };
+    if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
-        // stage: 32 rows x 64 columns per side; 256 threads x 8 elements
         {
-            const int rr = tid >> 3;           // 0..31
-            const int cc = (tid & 7) * 8;      // 0..56
-            const long row = r0 + rr;
-            Real bufi[8], bufj[8];
-            if (row < r_end) {
-                const Real* src = X + (size_t)row * g.ldx;
-                load8(src + tile.i0 + cc, bufi);
-                if (!same) load8(src + tile.j0 + cc, bufj);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) { bufi[e] = 0; bufj[e] = 0; }
-            }
             __syncthreads();   // previous step's readers are done
 #pragma unroll
             for (int e = 0; e < 8; ++e) xi[rr][cc + e] = bufi[e];
@@ -484,12 +493,9 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 #pragma unroll
                 for (int e = 0; e < 8; ++e) xj[rr][cc + e] = bufj[e];
             }
-            if (tid < NW * kKC) {
-                const int q = tid / kKC, k = tid % kKC;
-                const long row2 = r0 + k;
-                vs[q][k] = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
-            }
+            if (tid < NW * kKC) vs[tid / kKC][tid % kKC] = bufw;
             __syncthreads();
+            if (r0 + kKC < r_end) fetch(r0 + kKC);
         }
         const Real(*yj)[kLdsStride] = same ? xi : xj;
 #pragma unroll
@@ -603,11 +609,11 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
 
     const float* X = reinterpret_cast<const float*>(g.X);
     const float* wp[2] = {reinterpret_cast<const float*>(g.w0), reinterpret_cast<const float*>(g.w1)};
-    // staging: thread = (column lane, 8 consecutive samples 8 wave .. +7)
+    // staging: thread = (column lane, 8 consecutive samples 8 wave .. +7); the next step's
+    // values are loaded into registers while the MFMAs run on the current LDS tile
     const int sc = lane, sk = wave * 8;
-    int steps = 0;
-    for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
-        float xi[8], xj[8], wv[NW][8];
+    float xi[8], xj[8], wv[NW][8];
+    auto fetch = [&](long r0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const long row = r0 + sk + e;
@@ -618,6 +624,10 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
 #pragma unroll
             for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
         }
+    };
+    if (r_begin < r_end) fetch(r_begin);
+    int steps = 0;
+    for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         bf16x8 h, l;
         __syncthreads();   // the previous step's fragments have been read
 #pragma unroll
@@ -633,6 +643,7 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
         *reinterpret_cast<bf16x8*>(&bj[0][sc][sk]) = h;
         *reinterpret_cast<bf16x8*>(&bj[1][sc][sk]) = l;
         __syncthreads();
+        if (r0 + kKC < r_end) fetch(r0 + kKC);
 
         bf16x8 bh[2], bl[2];
 #pragma unroll

@@ -39,3 +39,45 @@ extern "C" int le_test_weight_fn(void*, const void* pos, int pos_double, int wal
         hipLaunchKernelGGL((k_user<float2, float>), grid, block, 0, st, (const float2*)pos, walkers, N, (float*)w, ld, g_invR2);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
+
+// Edge kicks: U_k = sum_i cos(k theta_i) exp(-(r_i - R)^2 / 2), k = 1..K, the spatial profile
+// of the perturbations A cos(k theta) exp(-(r - R)^2 / 2) f(t).
+template <typename P, typename W>
+__global__ void k_kicks(const P* pos, int walkers, int N, W* w, long ld, int K, double R)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= walkers) return;
+    double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < N; ++i) {
+        const double x = pos[(size_t)s * N + i].x, y = pos[(size_t)s * N + i].y;
+        const double r = sqrt(x * x + y * y), th = atan2(y, x);
+        const double g = exp(-0.5 * (r - R) * (r - R));
+        for (int k = 0; k < K; ++k) u[k] += cos((k + 1) * th) * g;
+    }
+    for (int k = 0; k < K; ++k) w[(size_t)k * ld + s] = (W)u[k];
+}
+
+static int g_kicks = 1;
+static double g_kick_R = 1.0;
+
+extern "C" void le_test_set_kicks(int K, double R)
+{
+    g_kicks = K < 1 ? 1 : (K > 8 ? 8 : K);
+    g_kick_R = R;
+}
+
+extern "C" int le_test_kick_fn(void*, const void* pos, int pos_double, int walkers, int N, void* w, long ld,
+                               int w_double, void* stream)
+{
+    dim3 grid((walkers + 127) / 128), block(128);
+    hipStream_t st = (hipStream_t)stream;
+    if (pos_double && w_double)
+        hipLaunchKernelGGL((k_kicks<double2, double>), grid, block, 0, st, (const double2*)pos, walkers, N, (double*)w, ld, g_kicks, g_kick_R);
+    else if (pos_double)
+        hipLaunchKernelGGL((k_kicks<double2, float>), grid, block, 0, st, (const double2*)pos, walkers, N, (float*)w, ld, g_kicks, g_kick_R);
+    else if (w_double)
+        hipLaunchKernelGGL((k_kicks<float2, double>), grid, block, 0, st, (const float2*)pos, walkers, N, (double*)w, ld, g_kicks, g_kick_R);
+    else
+        hipLaunchKernelGGL((k_kicks<float2, float>), grid, block, 0, st, (const float2*)pos, walkers, N, (float*)w, ld, g_kicks, g_kick_R);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}

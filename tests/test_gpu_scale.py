@@ -1,0 +1,140 @@
+"""GPU parity at the largest configuration: nu = 1/5 (m = 5), N = 400, dL = 0..20 (D = 2714,
+sectors up to 627 states, K = 20), 65536 walkers, eight edge-kick weights
+U_k = sum_i cos(k theta_i) exp(-(r_i - R)^2 / 2), k = 1..8, accumulated between every sector pair.
+
+Checked at full size: fixed-sample S, H and kick blocks against the FP64 oracle; the GPU
+eigensolver on the sampled 627-state sector against scipy; the batched FP64 time evolution
+of the eight kicks (A = 1e-3, t0 = 20, tau = 5, 0 <= t <= 100, saved every 0.1) conserves
+C^H M C and responds linearly in A."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, M_INV, SECTORS = 400, 5, list(range(21))
+W_CONF, EPS_CONF = 0.05, 2e-5
+WALKERS = 65536
+R = float(np.sqrt(2.0 * M_INV * N))
+
+
+def _kicks():
+    helper = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "libuserweights.so"))
+    helper.le_test_set_kicks(8, ctypes.c_double(R))
+    return helper, ctypes.cast(helper.le_test_kick_fn, ctypes.c_void_p).value
+
+
+def _kick_numpy(pos):
+    z = pos[..., 0] + 1j * pos[..., 1]
+    g = np.exp(-0.5 * (np.abs(z) - R) ** 2)
+    th = np.angle(z)
+    return [np.sum(np.cos(k * th) * g, axis=1) for k in range(1, 9)]
+
+
+@pytest.fixture(scope="module")
+def sampled():
+    from laughlinedge_amd import Engine
+    print("case scale m=5 N=400 dL=0..20 65536 walkers: sampling", flush=True)
+    helper, fn = _kicks()
+    e = Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=8, seed=2024,
+               tune_sweeps=20, step=0.5, n_user_weights=8)
+    e.set_weight_callback(fn)
+    assert e.info.D == 2714 and max(e.dims) == 627
+    e.init_walkers()
+    e.tune_step()
+    e.burnin(150)
+    e.sample(8)
+    yield e, helper
+    e.close()
+
+
+def test_scale_fixed_samples_vs_oracle(sampled, oracle_lib):
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import W_USER
+    e, helper = sampled
+    pos = e.positions()[:256]
+    print("case scale fixed samples vs oracle (256 configurations)", flush=True)
+    o = oracle_lib.Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
+    So, Ho, _ = o.accumulate_fixed(pos)
+    n = pos.shape[0]
+    _, fn = _kicks()
+    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=256, samples=1, n_user_weights=8) as f:
+        f.set_weight_callback(fn)
+        f.accumulate_positions(pos)
+        mom = f.moments()
+        worst = 0.0
+        for s, (Sg, Hg) in enumerate(mom):
+            rs = np.linalg.norm(Sg - So[s] / n) / np.linalg.norm(So[s] / n)
+            rh = np.linalg.norm(Hg - Ho[s] / n) / np.linalg.norm(Ho[s] / n)
+            worst = max(worst, rs, rh)
+            assert rs < 1e-6 and rh < 1e-6, (s, rs, rh)
+        print(f"worst S/H relative Frobenius error {worst:.3e}")
+        us = _kick_numpy(pos)
+        for k in (0, 3, 7):
+            for a, b in ((20, 19), (3, 11), (0, 20), (17, 17)):
+                ref = o.accumulate_weighted(pos, us[k], a, b) / n
+                got = f.block(W_USER(k), a, b)
+                err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+                assert err < 1e-5, (k, a, b, err)
+
+
+def test_scale_spectra_vs_scipy(sampled):
+    from scipy.linalg import eigh
+    e, _ = sampled
+    print("case scale spectra (627-state sector) vs scipy", flush=True)
+    spec = e.spectra(-1)
+    assert all(sp is not None for sp in spec)
+    mom = e.moments()
+    for s in (12, 20):
+        S, H = mom[s]
+        S = (S + S.conj().T) / 2
+        H = (H + H.conj().T) / 2
+        ref = eigh(H, S, eigvals_only=True)
+        assert np.allclose(spec[s], ref, rtol=0, atol=1e-7 * np.abs(ref).max()), (s, np.abs(spec[s] - ref).max())
+    # the quadratic part alone gives w (L0 + dL + N) exactly; eps r^4 only raises it
+    L0 = M_INV * N * (N - 1) // 2
+    for s in range(len(SECTORS)):
+        assert spec[s][0] > W_CONF * (L0 + SECTORS[s] + N) * (1 - 1e-4), s
+
+
+def test_scale_evolution_of_kicks(sampled):
+    from laughlinedge_amd.engine import W_USER, evolve
+    e, _ = sampled
+    print("case scale FP64 evolution of 8 kicks, D=2714", flush=True)
+    dims, offs = e.dims, e.offsets
+    D = e.info.D
+    M = np.zeros((D, D), complex)
+    H = np.zeros((D, D), complex)
+    for s, (S, Hs) in enumerate(e.moments()):
+        o, d = offs[s], dims[s]
+        M[o:o + d, o:o + d] = (S + S.conj().T) / 2
+        H[o:o + d, o:o + d] = (Hs + Hs.conj().T) / 2
+    Us = []
+    for k in range(8):
+        U = np.zeros((D, D), complex)
+        for a in range(len(SECTORS)):
+            for b in range(a, len(SECTORS)):
+                blk = e.block(W_USER(k), a, b)
+                U[offs[a]:offs[a] + dims[a], offs[b]:offs[b] + dims[b]] = blk
+                U[offs[b]:offs[b] + dims[b], offs[a]:offs[a] + dims[a]] = blk.conj().T
+        Us.append((U + U.conj().T) / 2)
+    # observable: the k = 1 kick operator itself (density response at angular momentum 1)
+    rhos = [Us[0]]
+    prof = lambda A: (lambda t: A * np.exp(-(t - 20.0) ** 2 / (2 * 25.0)))
+    rho1, nrm1, steps1 = evolve(M, H, Us, rhos, prof(1e-3), 100.0, 0.1, dims)
+    rho2, nrm2, _ = evolve(M, H, Us, rhos, prof(2e-3), 100.0, 0.1, dims)
+    print(f"controller steps {steps1}")
+    assert rho1.shape == (8, 1000, 1) and np.all(np.isfinite(rho1))
+    # C^H M C is conserved (C(0) = e_0, M_00 = 1 for the constant dL = 0 state)
+    assert abs(M[0, 0] - 1.0) < 1e-12
+    assert np.max(np.abs(nrm1 - 1.0)) < 1e-7 and np.max(np.abs(nrm2 - 1.0)) < 1e-7
+    # linear response of the k = 1 kick (it couples dL = 0 to dL = 1, which rho_1 reads at
+    # first order): the change of <rho_1> doubles with A
+    d1 = rho1[0, :, 0] - rho1[0, 0, 0]
+    d2 = rho2[0, :, 0] - rho2[0, 0, 0]
+    assert np.abs(d2).max() > 1e-9
+    ratio = np.abs(d2 - 2 * d1).max() / np.abs(d2).max()
+    print(f"linear-response deviation {ratio:.2e}")
+    assert ratio < 1e-2, ratio

@@ -58,6 +58,8 @@ hipError_t dalloc(T** p, size_t count)
 // Lanes per walker without timing: the smallest power of two with at most 16 particle
 // slots per lane (FP32), 32 for FP64, whose sweeps are the slower ones and prefer longer
 // per-lane loops (N = 200 on MI355X: 0.52 ms FP32 at 16 lanes, 1.54 ms FP64 at 8 lanes).
+constexpr int kPsLd = 48;        // Re/Im p_1..p_24 per sample
+
 int default_lanes(int N, int prec)
 {
     const int per_lane = prec == LE_PREC_FP64 ? 32 : 16;
@@ -87,6 +89,7 @@ struct le_engine {
     // device buffers
     void* pos = nullptr;         // walkers x N x (float2|double2)
     unsigned* acc = nullptr;
+    double* ps_d = nullptr;      // [rows][kPsLd] per-sample power sums (features scratch)
     void* X = nullptr;           // rows x ldx: Xb[0] (fixed-sample and two-sided paths)
     void* Xb[2] = {nullptr, nullptr};     // double buffer: sampler fills one, Gram reads the other
     void* Wbb[2] = {nullptr, nullptr};
@@ -223,6 +226,7 @@ le_engine::~le_engine()
     (void)hipFree(bases_d);
     (void)hipFree(pos);
     (void)hipFree(acc);
+    (void)hipFree(ps_d);
     (void)hipFree(Xb[0]);
     (void)hipFree(Xb[1]);
     (void)hipFree(fac);
@@ -426,6 +430,7 @@ int le_engine::setup(const le_observables* obs)
         HIPCHK(hipMemcpy(side_d, side.data(), sizeof(int) * D, hipMemcpyHostToDevice));
     }
     HIPCHK(dalloc(&acc, (size_t)mc.walkers));
+    HIPCHK(dalloc(&ps_d, (size_t)rows * kPsLd));
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMalloc(&Xb[b], (size_t)rows * ldx * rsz));
         HIPCHK(hipMemsetAsync(Xb[b], 0, (size_t)rows * ldx * rsz, stream));
@@ -935,6 +940,8 @@ extern "C" int le_sample(le_engine* e, int samples)
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
     le::FeatureArgs f{};
+    f.ps = e->ps_d;
+    f.ps_ld = kPsLd;
     f.pos = e->pos;
     f.pos_is_double = 0;
     f.X = e->X;
@@ -1242,6 +1249,8 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
     double* stage = nullptr;
     HIPCHK(dalloc(&stage, (size_t)std::min(chunk, n) * N * 2));
     le::FeatureArgs f{};
+    f.ps = e->ps_d;
+    f.ps_ld = kPsLd;
     f.pos = stage;
     f.pos_is_double = 1;
     f.X = e->X;

@@ -4,15 +4,18 @@
 //  k_sweep<G, Real>  Metropolis sweeps on |Psi_L|^2; G lanes share one walker whose
 //                    positions live in LDS; the O(N) log-ratio of a single-particle move
 //                    is split over the G lanes and combined with a butterfly reduction.
-//  k_features<Real>  one wavefront per sample: FP64 power sums p_k of z/R, partition
-//                    monomials p_lambda, confinement V; written as a real-embedded row
-//                    [Re p(sector) | Im p(sector)] of the sample matrix X.
+//  k_psums, k_monomials
+//                    FP64 power sums p_k of z/R and confinement V (4 lanes per sample),
+//                    then partition monomials p_lambda (one wavefront per sample), written
+//                    as a real-embedded row [Re p(sector) | Im p(sector)] of the sample
+//                    matrix X.
 //  k_gram<Real>      weighted Gram tiles C = X^T X and X^T diag(V) X on MFMA
 //                    (16x16x4 f32 / f64), upper block-triangle tiles only, FP32 tiles
 //                    flushed into FP64 registers every 256 samples, FP64 atomics out.
 //  k_extract         FP64 real-embedded tiles -> complex Hermitian S, H per sector.
-//  k_geneig          FP64 Cholesky + parallel cyclic Jacobi generalized eigensolver,
-//                    one workgroup per sector.
+//  k_geneig          FP64 Cholesky + parallel cyclic Jacobi generalized eigensolver with
+//                    eigenvectors, one workgroup per sector.
+//  k_geneig_vals     eigenvalues only: Cholesky, Householder tridiagonalization, bisection.
 //
 // Reference behaviour these kernels reproduce (Nardin94/LaughlinEdge):
 //  Metropolis move and ratio ........ modules/edge_mc/em_markovchain.cu:282-328
@@ -220,97 +223,126 @@ __device__ __forceinline__ double potential_r2(const Potential& p, double r2)
     return v;
 }
 
-// One wavefront per sample (grid-stride over samples).  Power sums in FP64 are reduced with
-// a butterfly; a per-sample power table p_k^r (k r <= K) is built in LDS by all lanes, and
-// each monomial is a product of at most 8 table entries following a packed plan.
+// Features in two passes.  k_psums: 4 lanes per sample (particles j = sub + 4 s) accumulate the
+// FP64 power sums p_k = sum (z / R)^k, k <= KMAX, the potential and the harmonic sums, meet
+// in a 2-step DPP butterfly, and write p_k to a per-sample scratch row, the weights V and
+// cos/sin sums to the weight rows, and the trace.  k_monomials: one wavefront per sample
+// builds the table p_k^r (k r <= K) in LDS and writes every monomial (at most 8 table factors
+// following a packed plan) to the real-embedded sample row.  Splitting keeps each kernel's
+// live registers small, so both run at several waves per SIMD.
 constexpr int kMaxPow = 128;
 constexpr int kMaxStatesLds = 4096;
+constexpr int kLps = 4;          // lanes per sample in k_psums
 
 template <typename Real, int KMAX, bool HARM>
-__global__ __launch_bounds__(256) void k_features(FeatureArgs a)
+__global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
 {
     constexpr int NH = HARM ? kMaxHarmonics : 0;
     constexpr int NQ = 2 * KMAX + 1 + 2 * NH;   // Re/Im p_1..p_K, V, cos/sin sums
     constexpr int QV = 2 * KMAX;
     constexpr int QH = 2 * KMAX + 1;
-    __shared__ double sq[4][NQ];
-    __shared__ double2 ptab[4][kMaxPow];
-    __shared__ unsigned long long plan[kMaxStatesLds];
-    __shared__ uint32_t cols[kMaxStatesLds];     // col_re | col_dim << 16
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long w = gt / kLps;
+    const int sub = (int)(gt % kLps);
+    if (w >= a.walkers) return;                  // the 4 lanes of a sample leave together
+    const int nh = HARM ? a.nh : 0;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int j = sub; j < a.N; j += kLps) {
+        double x, y;
+        if (a.pos_is_double || sizeof(Real) == 8) {
+            const double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
+            x = q.x; y = q.y;
+        } else {
+            const float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
+            x = q.x; y = q.y;
+        }
+        const double r2 = x * x + y * y;
+        acc[QV] += potential_r2(a.pot, r2);
+        const double ur = x * a.invR, ui = y * a.invR;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            acc[2 * k] += qr;
+            acc[2 * k + 1] += qi;
+            const double t = qr * ur - qi * ui;
+            qi = qr * ui + qi * ur;
+            qr = t;
+        }
+        if constexpr (HARM) {
+            // e^{i L theta} = (z / |z|)^L by binary powering
+            const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
+            const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                if (h < nh) {
+                    unsigned e = (unsigned)a.harm[h];
+                    double br = er, bi = ei, pr = 1.0, pi = 0.0;
+                    while (e) {
+                        if (e & 1u) {
+                            const double t = pr * br - pi * bi;
+                            pi = pr * bi + pi * br;
+                            pr = t;
+                        }
+                        e >>= 1;
+                        const double t = br * br - bi * bi;
+                        bi = 2.0 * br * bi;
+                        br = t;
+                    }
+                    acc[QH + 2 * h] += pr;
+                    acc[QH + 2 * h + 1] += pi;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = group_sum<kLps, double>(acc[q]);
+    double* ps = a.ps + (size_t)w * a.ps_ld;
+#pragma unroll
+    for (int q = 0; q < 2 * KMAX; ++q)
+        if (q % kLps == sub) ps[q] = acc[q];
+    const long row = a.row0 + w;
+    Real* W = reinterpret_cast<Real*>(a.W);
+    if (sub == 0) W[row] = (Real)acc[QV];
+    if constexpr (HARM) {
+#pragma unroll
+        for (int q = 0; q < 2 * NH; ++q)
+            if (q % kLps == sub && q < 2 * nh) W[(size_t)(1 + q) * a.wld + row] = (Real)acc[QH + q];
+    }
+    if (a.trace && w < a.trace_walkers && sub == 0) {
+        const long long tb = a.trace_base ? *a.trace_base : 0ll;
+        float* tr = a.trace + ((size_t)(a.trace_row0 + tb) + (size_t)w) * 9;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tr[q] = (q / 2 < a.K) ? (float)acc[q] : 0.f;
+        tr[8] = (float)acc[QV];
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem_mono[];
+    double2* ptab = reinterpret_cast<double2*>(smem_mono);          // [4][kMaxPow]
+    double2* pk = ptab + 4 * kMaxPow;                                // [4][32]
+    unsigned long long* plan = reinterpret_cast<unsigned long long*>(pk + 4 * 32);
     const int D = a.D;
     const bool plan_lds = D <= kMaxStatesLds;
+    uint32_t* cols = reinterpret_cast<uint32_t*>(plan + (plan_lds ? D : 0));
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     if (plan_lds)
         for (int t = threadIdx.x; t < D; t += blockDim.x) {
             plan[t] = a.prod[t];
             cols[t] = (uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16);
         }
     __syncthreads();
-    const int nh = HARM ? a.nh : 0;
+    double2* pkw = pk + wv * 32;
+    double2* ptw = ptab + wv * kMaxPow;
     const int waves = gridDim.x * 4;
     for (int w = blockIdx.x * 4 + wv; w < a.walkers; w += waves) {
-        double acc[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-        for (int j = lane; j < a.N; j += 64) {
-            double x, y;
-            if (a.pos_is_double || sizeof(Real) == 8) {
-                double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
-                x = q.x; y = q.y;
-            } else {
-                float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
-                x = q.x; y = q.y;
-            }
-            const double r2 = x * x + y * y;
-            acc[QV] += potential_r2(a.pot, r2);
-            const double ur = x * a.invR, ui = y * a.invR;
-            double qr = ur, qi = ui;
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                acc[2 * k] += qr;
-                acc[2 * k + 1] += qi;
-                const double t = qr * ur - qi * ui;
-                qi = qr * ui + qi * ur;
-                qr = t;
-            }
-            if constexpr (HARM) {
-                // e^{i L theta} = (z / |z|)^L by binary powering
-                const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
-                const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    if (h < nh) {
-                        unsigned e = (unsigned)a.harm[h];
-                        double br = er, bi = ei, pr = 1.0, pi = 0.0;
-                        while (e) {
-                            if (e & 1u) {
-                                const double t = pr * br - pi * bi;
-                                pi = pr * bi + pi * br;
-                                pr = t;
-                            }
-                            e >>= 1;
-                            const double t = br * br - bi * bi;
-                            bi = 2.0 * br * bi;
-                            br = t;
-                        }
-                        acc[QH + 2 * h] += pr;
-                        acc[QH + 2 * h + 1] += pi;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double v = acc[q];
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-            acc[q] = v;
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) sq[wv][q] = acc[q];
-        }
+        const double* ps = a.ps + (size_t)w * a.ps_ld;
+        if (lane < a.K) pkw[lane] = make_double2(ps[2 * lane], ps[2 * lane + 1]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -318,7 +350,8 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
         for (int e = lane; e < a.npw; e += 64) {
             const int k = a.pw_k[e];
             unsigned r = (unsigned)a.pw_r[e];
-            double br = sq[wv][2 * (k - 1)], bi = sq[wv][2 * (k - 1) + 1], pr = 1.0, pi = 0.0;
+            const double2 b0 = pkw[k - 1];
+            double br = b0.x, bi = b0.y, pr = 1.0, pi = 0.0;
             while (r) {
                 if (r & 1u) {
                     const double t = pr * br - pi * bi;
@@ -332,12 +365,11 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
                     br = t;
                 }
             }
-            ptab[wv][e] = make_double2(pr, pi);
+            ptw[e] = make_double2(pr, pi);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
         const long row = a.row0 + w;
         Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
 #pragma unroll 2
@@ -349,7 +381,7 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 if (q < cnt) {
-                    const double2 f = ptab[wv][(pl >> (4 + 7 * q)) & 0x7full];
+                    const double2 f = ptw[(pl >> (4 + 7 * q)) & 0x7full];
                     const double tr = vr * f.x - vi * f.y;
                     vi = vr * f.y + vi * f.x;
                     vr = tr;
@@ -359,17 +391,7 @@ __global__ __launch_bounds__(256) void k_features(FeatureArgs a)
             X[c] = (Real)vr;
             X[c + (int)(cw >> 16)] = (Real)vi;
         }
-        Real* W = reinterpret_cast<Real*>(a.W);
-        if (lane == 0) W[row] = (Real)sq[wv][QV];
-        if (HARM && lane < 2 * nh) W[(size_t)(1 + lane) * a.wld + row] = (Real)sq[wv][QH + lane];
-        if (a.trace && w < a.trace_walkers && lane < 9) {
-            float v;
-            if (lane < 8) v = (lane / 2 < a.K) ? (float)sq[wv][lane] : 0.f;
-            else v = (float)sq[wv][QV];
-            const long long tb = a.trace_base ? *a.trace_base : 0ll;
-            a.trace[((size_t)(a.trace_row0 + tb) + w) * 9 + lane] = v;
-        }
-        __builtin_amdgcn_wave_barrier();   // sq / ptab are reused by the next sample
+        __builtin_amdgcn_wave_barrier();   // pk / ptab are reused by the next sample
     }
 }
 
@@ -1159,24 +1181,30 @@ hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
 template <typename Real, bool HARM>
 static hipError_t features_h(const FeatureArgs& a, hipStream_t s)
 {
-    // enough waves to fill the chip, each looping over samples
-    const long blocks = std::min<long>((a.walkers + 3) / 4, 2048);
-    dim3 grid((unsigned)blocks);
+    const long threads = (long)a.walkers * kLps;
+    dim3 grid_ps((unsigned)((threads + 255) / 256));
     if (a.K <= 8)
-        hipLaunchKernelGGL((k_features<Real, 8, HARM>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 8, HARM>), grid_ps, dim3(256), 0, s, a);
     else if (a.K <= 16)
-        hipLaunchKernelGGL((k_features<Real, 16, HARM>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 16, HARM>), grid_ps, dim3(256), 0, s, a);
     else if (a.K <= 24)
-        hipLaunchKernelGGL((k_features<Real, 24, HARM>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 24, HARM>), grid_ps, dim3(256), 0, s, a);
     else
         return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // enough waves to fill the chip, each looping over samples
+    const long blocks = std::min<long>((a.walkers + 3) / 4, 2048);
+    const size_t lds = (size_t)(4 * kMaxPow + 4 * 32) * sizeof(double2) +
+                       (a.D <= kMaxStatesLds ? (size_t)a.D * (sizeof(unsigned long long) + sizeof(uint32_t)) : 0);
+    hipLaunchKernelGGL((k_monomials<Real>), dim3((unsigned)blocks), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
 template <typename Real>
 static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
 {
-    if (a.npw > kMaxPow) return hipErrorInvalidValue;
+    if (a.npw > kMaxPow || !a.ps || a.ps_ld < 2 * a.K) return hipErrorInvalidValue;
     return a.nh > 0 ? features_h<Real, true>(a, s) : features_h<Real, false>(a, s);
 }
 

@@ -49,6 +49,8 @@ struct FeatureArgs {
     int trace_walkers;
     long trace_row0;
     const long long* trace_base;   // optional device value added to trace_row0 (graph replays)
+    double* ps;              // [walkers][ps_ld] scratch: Re/Im p_1..p_K per sample
+    int ps_ld;
 };
 
 // Two-sided chain features (occupation numbers): positions [walkers][N+1]; bulk power

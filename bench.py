@@ -205,7 +205,7 @@ def main():
                         "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
                         "frac": sweep_rate / PEAK_FP32, "flop_per_walker_sweep": 12 * N * (N - 1),
                         "ms_per_sweep": de["ms_sweep"] / ksteps},
-            "features": {"kernel": "k_features<float,16>", "bound": "hbm",
+            "features": {"kernel": "k_psums<float,16> + k_monomials<float>", "bound": "hbm",
                          "ms_per_step": de["ms_features"] / ksteps,
                          "sample_matrix_gb_per_s": stream_bytes / (de["ms_features"] * 1e-3) / 1e9,
                          "bytes_per_sample": 8 * info.D + 4},

@@ -1312,7 +1312,7 @@ static int geneig_batch(const std::vector<int>& dims, const double* S, const dou
         probs[p] = {dims[p], in_off, ev_off, w_off};
         in_off += (long)dims[p] * dims[p];
         ev_off += dims[p];
-        w_off += 4L * dims[p] * dims[p];
+        w_off += 8L * dims[p] * dims[p] + 4L * dims[p];
     }
     double *dS = nullptr, *dH = nullptr, *dW = nullptr, *dE = nullptr, *dV = nullptr;
     le::GenEigProblem* dP = nullptr;
@@ -1332,7 +1332,8 @@ static int geneig_batch(const std::vector<int>& dims, const double* S, const dou
         chk(hipMemcpy(dP, probs.data(), sizeof(le::GenEigProblem) * np, hipMemcpyHostToDevice));
         chk(hipMemset(dSt, 0xff, sizeof(int) * np));
     }
-    if (h == hipSuccess) chk(le::launch_geneig(dS, dH, dW, dP, np, dE, dV, dSt, nullptr));
+    const int dmax = dims.empty() ? 1 : *std::max_element(dims.begin(), dims.end());
+    if (h == hipSuccess) chk(le::launch_geneig(dS, dH, dW, dP, np, dmax, dE, dV, dSt, nullptr));
     std::vector<int> st(np, -1);
     if (h == hipSuccess) chk(hipDeviceSynchronize());
     if (h == hipSuccess) {

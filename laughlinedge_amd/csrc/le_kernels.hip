@@ -13,9 +13,11 @@
 //                    (16x16x4 f32 / f64), upper block-triangle tiles only, FP32 tiles
 //                    flushed into FP64 registers every 256 samples, FP64 atomics out.
 //  k_extract         FP64 real-embedded tiles -> complex Hermitian S, H per sector.
-//  k_geneig          FP64 Cholesky + parallel cyclic Jacobi generalized eigensolver with
-//                    eigenvectors, one workgroup per sector.
-//  k_geneig_vals     eigenvalues only: Cholesky, Householder tridiagonalization, bisection.
+//  k_geneig_vals     FP64 generalized eigensolver, one workgroup per sector: Cholesky,
+//                    Householder tridiagonalization, bisection;
+//  k_eig_invit, k_eig_back
+//                    eigenvectors: inverse iteration on the tridiagonal form, then the
+//                    reflectors and L^-H.
 //
 // Reference behaviour these kernels reproduce (Nardin94/LaughlinEdge):
 //  Metropolis move and ratio ........ modules/edge_mc/em_markovchain.cu:282-328
@@ -877,8 +879,17 @@ __device__ bool reduce_to_standard(const cd* S, const cd* H, cd* L, cd* C, cd* W
 // coalesced through B[i][j] = conj(B[j][i])), then the real symmetric tridiagonal with
 // diagonal Re C_ii and off-diagonal |beta_k| (a diagonal unitary similarity removes the
 // phases) is solved by Sturm-count bisection, one thread per eigenvalue, ascending.
+// Per-problem scratch beyond L, C, W, T (4 d^2 complex): E, 8 d^2 + 8 d doubles holding the
+// real tridiagonal (a, b), the reflector factors tau, the off-diagonal phases, the real
+// tridiagonal eigenvectors X [d][d] and the pivoted LU factors of T - lambda I (5 x [d][d]).
+__device__ __forceinline__ double* eig_extra(double* workd, const GenEigProblem& pr)
+{
+    return workd + 2 * (pr.work_off + 4L * pr.d * pr.d);
+}
+
 __global__ __launch_bounds__(kEigThreads) void k_geneig_vals(const double* Sd, const double* Hd, double* workd,
-                                                            const GenEigProblem* probs, double* evals, int* status)
+                                                            const GenEigProblem* probs, double* evals, int* status,
+                                                            int keep)
 {
     __shared__ double red[kEigThreads / 64];
     __shared__ cd vv[2 * kEigMaxPairs], ww[2 * kEigMaxPairs];
@@ -915,10 +926,27 @@ __global__ __launch_bounds__(kEigThreads) void k_geneig_vals(const double* Sd, c
         const cd ph = ax0 > 0.0 ? cd{x0.r / ax0, x0.i / ax0} : cd{1.0, 0.0};
         if (tid == 0) db[k] = alpha;                 // |beta_k|, beta_k = -ph alpha
         __syncthreads();                              // every thread has read vv[0]
-        if (!(alpha > 0.0)) continue;                 // column already reduced (uniform)
+        if (!(alpha > 0.0)) {                         // column already reduced (uniform)
+            if (keep && tid == 0) {
+                double* E = eig_extra(workd, pr);
+                E[2 * d + k] = 0.0;
+                E[3 * d + 2 * k] = 1.0;
+                E[3 * d + 2 * k + 1] = 0.0;
+            }
+            continue;
+        }
         const double tau = 1.0 / (alpha * (alpha + ax0));
         if (tid == 0) vv[0] = cadd(x0, cscale(ph, alpha));
         __syncthreads();
+        if (keep) {   // H_k = I - tau v v^H: v into column k below the diagonal, beta_k = -ph alpha
+            for (int i = tid; i < n; i += kEigThreads) C[(size_t)(k + 1 + i) * d + k] = vv[i];
+            if (tid == 0) {
+                double* E = eig_extra(workd, pr);
+                E[2 * d + k] = tau;
+                E[3 * d + 2 * k] = -ph.r;
+                E[3 * d + 2 * k + 1] = -ph.i;
+            }
+        }
         // p = tau B v (thread i: column i of B, coalesced across threads)
         for (int i = tid; i < n; i += kEigThreads) {
             cd acc = {0.0, 0.0};
@@ -951,11 +979,25 @@ __global__ __launch_bounds__(kEigThreads) void k_geneig_vals(const double* Sd, c
         __syncthreads();
         if (tid == 0) {
             const cd e = C[(size_t)(d - 1) * d + (d - 2)];
-            db[d - 2] = sqrt(e.r * e.r + e.i * e.i);
+            const double ae = sqrt(e.r * e.r + e.i * e.i);
+            db[d - 2] = ae;
+            if (keep) {
+                double* E = eig_extra(workd, pr);
+                E[2 * d + (d - 2)] = 0.0;
+                E[3 * d + 2 * (d - 2)] = ae > 0.0 ? e.r / ae : 1.0;
+                E[3 * d + 2 * (d - 2) + 1] = ae > 0.0 ? e.i / ae : 0.0;
+            }
         }
     }
     for (int i = tid; i < d; i += kEigThreads) da[i] = C[(size_t)i * d + i].r;
     __syncthreads();
+    if (keep) {
+        double* E = eig_extra(workd, pr);
+        for (int i = tid; i < d; i += kEigThreads) {
+            E[i] = da[i];
+            E[d + i] = db[i];
+        }
+    }
     // Gershgorin interval and pivot floor
     double lo = 1e300, hi = -1e300, bmax = 0.0;
     for (int i = 0; i < d; ++i) {
@@ -987,143 +1029,180 @@ __global__ __launch_bounds__(kEigThreads) void k_geneig_vals(const double* Sd, c
     if (tid == 0) status[blockIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(kEigThreads) void k_geneig(const double* Sd, const double* Hd, double* workd,
-                                                       const GenEigProblem* probs, double* evals,
-                                                       double* evecsd, int* status)
+// Eigenvectors of the real tridiagonal (a, b) by inverse iteration (one thread per cluster of
+// eigenvalues closer than 1e-7 |T|; members after the first are shifted apart by 10 eps |T| and
+// made orthogonal to the earlier members by modified Gram-Schmidt in every iteration, as in
+// LAPACK's stein).  (T - lambda I) is factored with partial pivoting (LAPACK gttrf layout:
+// multipliers dl, diagonal dd, super-diagonals du, du2, pivot flags); zero pivots become
+// eps |T|.  Each vector gets 3 solves from a fixed pseudo-random start and is normalised.
+__global__ __launch_bounds__(kEigThreads) void k_eig_invit(double* workd, const GenEigProblem* probs,
+                                                           const double* evals, const int* status)
 {
-    __shared__ double red[kEigThreads / 64];
-    __shared__ double rc[kEigMaxPairs], rs[kEigMaxPairs];
-    __shared__ cd rph[kEigMaxPairs];
-    __shared__ int rp[kEigMaxPairs], rq[kEigMaxPairs];
-    __shared__ cd gcol[2 * kEigMaxPairs];
-    __shared__ double gsh;
-
+    __shared__ double a[2 * kEigMaxPairs], b[2 * kEigMaxPairs];
+    __shared__ int cstart[2 * kEigMaxPairs + 1];
+    __shared__ int ncl;
+    __shared__ double tnorm;
     const GenEigProblem pr = probs[blockIdx.x];
+    if (status[blockIdx.x] != 0) return;
     const int d = pr.d;
     const int tid = threadIdx.x;
-    const cd* S = reinterpret_cast<const cd*>(Sd) + pr.in_off;
-    const cd* H = reinterpret_cast<const cd*>(Hd) + pr.in_off;
-    cd* L = reinterpret_cast<cd*>(workd) + pr.work_off;
-    cd* C = L + (size_t)d * d;
-    cd* W = C + (size_t)d * d;
-    cd* T = W + (size_t)d * d;
-    cd* V = reinterpret_cast<cd*>(evecsd) + pr.in_off;
-    double* ev = evals + pr.eval_off;
-
-    if (!reduce_to_standard(S, H, L, C, W, T, d, gcol, &gsh)) {
-        if (tid == 0) status[blockIdx.x] = 1;
-        return;   // uniform across the block
-    }
-    for (long x = tid; x < (long)d * d; x += kEigThreads) {
-        const int i = (int)(x / d), j = (int)(x % d);
-        T[x] = (i == j) ? cd{1.0, 0.0} : cd{0.0, 0.0};   // eigenvector accumulator
+    double* E = eig_extra(workd, pr);
+    const double* ev = evals + pr.eval_off;
+    for (int i = tid; i < d; i += kEigThreads) {
+        a[i] = E[i];
+        b[i] = E[d + i];
     }
     __syncthreads();
-
-    // parallel cyclic Jacobi, round-robin pairing over n = d rounded up to even
-    const int n = (d + 1) & ~1;
-    const int npair = n / 2;
-    for (int sweep = 0; sweep < 40; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (long x = tid; x < (long)d * d; x += kEigThreads) {
-            cd c = C[x];
-            double a2 = c.r * c.r + c.i * c.i;
-            tot += a2;
-            if ((x / d) != (x % d)) off += a2;
+    if (tid == 0) {
+        double an = 0.0, bn = 0.0;
+        for (int i = 0; i < d; ++i) {
+            an = fmax(an, fabs(a[i]));
+            if (i + 1 < d) bn = fmax(bn, b[i]);
         }
-        off = block_sum(off, red);
-        tot = block_sum(tot, red);
-        if (off <= 1e-30 * tot || off == 0.0) break;
-        for (int round = 0; round < n - 1; ++round) {
-            for (int k = tid; k < npair; k += kEigThreads) {
-                int p, q;
-                if (k == 0) {
-                    p = 0;
-                    q = 1 + (round + n - 2) % (n - 1);
-                } else {
-                    p = 1 + (round + k - 1) % (n - 1);
-                    q = 1 + (round + n - 2 - k) % (n - 1);
+        tnorm = fmax(an + 2.0 * bn, 1e-300);
+        int c = 0;
+        cstart[0] = 0;
+        for (int j = 1; j < d; ++j)
+            if (ev[j] - ev[j - 1] > 1e-7 * tnorm) cstart[++c] = j;
+        cstart[c + 1] = d;
+        ncl = c + 1;
+    }
+    __syncthreads();
+    const double eps = 2.220446049250313e-16;
+    const double tiny = eps * tnorm;
+    double* X = E + 5 * d;
+    double* dl = X + (size_t)d * d;
+    double* dd = dl + (size_t)d * d;
+    double* du = dd + (size_t)d * d;
+    double* du2 = du + (size_t)d * d;
+    double* pv = du2 + (size_t)d * d;
+    auto at = [d](double* base, int i, int j) -> double& { return base[(size_t)i * d + j]; };
+    for (int cl = tid; cl < ncl; cl += kEigThreads) {
+        double lam_prev = 0.0;
+        for (int j = cstart[cl]; j < cstart[cl + 1]; ++j) {
+            double lam = ev[j];
+            if (j > cstart[cl]) lam = fmax(lam, lam_prev + 10.0 * tiny);
+            lam_prev = lam;
+            // factor T - lam I
+            for (int i = 0; i < d; ++i) {
+                at(dd, i, j) = a[i] - lam;
+                if (i + 1 < d) {
+                    at(dl, i, j) = b[i];
+                    at(du, i, j) = b[i];
                 }
-                if (p > q) { int t = p; p = q; q = t; }
-                double cs = 1.0, sn = 0.0;
-                cd ph = {1.0, 0.0};
-                if (q < d) {
-                    cd apq = C[(size_t)p * d + q];
-                    double aabs = sqrt(apq.r * apq.r + apq.i * apq.i);
-                    if (aabs > 1e-300) {
-                        const double app = C[(size_t)p * d + p].r, aqq = C[(size_t)q * d + q].r;
-                        ph = {apq.r / aabs, apq.i / aabs};
-                        const double tau = (aqq - app) / (2.0 * aabs);
-                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        cs = 1.0 / sqrt(1.0 + t * t);
-                        sn = t * cs;
+            }
+            for (int i = 0; i + 1 < d; ++i) {
+                const double di = at(dd, i, j), li = at(dl, i, j);
+                if (fabs(di) >= fabs(li)) {
+                    const double piv = di != 0.0 ? di : tiny;
+                    at(dd, i, j) = piv;
+                    const double l = li / piv;
+                    at(dl, i, j) = l;
+                    at(dd, i + 1, j) -= l * at(du, i, j);
+                    if (i + 2 < d) at(du2, i, j) = 0.0;
+                    at(pv, i, j) = 0.0;
+                } else {
+                    const double l = di / li;
+                    at(dd, i, j) = li;
+                    at(dl, i, j) = l;
+                    const double t = at(du, i, j);
+                    at(du, i, j) = at(dd, i + 1, j);
+                    at(dd, i + 1, j) = t - l * at(dd, i + 1, j);
+                    if (i + 2 < d) {
+                        at(du2, i, j) = at(du, i + 1, j);
+                        at(du, i + 1, j) = -l * at(du, i + 1, j);
+                    }
+                    at(pv, i, j) = 1.0;
+                }
+            }
+            if (at(dd, d - 1, j) == 0.0) at(dd, d - 1, j) = tiny;
+            // start vector
+            for (int i = 0; i < d; ++i) {
+                uint32_t h = (uint32_t)(i * 2654435761u) ^ (uint32_t)((j + 1) * 40503u);
+                h ^= h >> 15;
+                h *= 2246822519u;
+                h ^= h >> 13;
+                at(X, i, j) = 0.5 + (double)(h & 0xffffu) / 65536.0;
+            }
+            for (int it = 0; it < 3; ++it) {
+                // solve L U x = x (pivoted forward, then backward)
+                for (int i = 0; i + 1 < d; ++i) {
+                    if (at(pv, i, j) == 0.0) {
+                        at(X, i + 1, j) -= at(dl, i, j) * at(X, i, j);
+                    } else {
+                        const double t = at(X, i, j);
+                        at(X, i, j) = at(X, i + 1, j);
+                        at(X, i + 1, j) = t - at(dl, i, j) * at(X, i, j);
                     }
                 }
-                rp[k] = p; rq[k] = q; rc[k] = cs; rs[k] = sn; rph[k] = ph;
-            }
-            __syncthreads();
-            // columns: C <- C G, T <- T G
-            for (long x = tid; x < (long)d * npair; x += kEigThreads) {
-                const int k = (int)(x % npair), row = (int)(x / npair);
-                const int p = rp[k], q = rq[k];
-                if (q >= d || rs[k] == 0.0) continue;
-                const double cs = rc[k], sn = rs[k];
-                const cd ph = rph[k];
-                cd ckp = C[(size_t)row * d + p], ckq = C[(size_t)row * d + q];
-                C[(size_t)row * d + p] = csub(cscale(ckp, cs), cscale(cmul(cconj(ph), ckq), sn));
-                C[(size_t)row * d + q] = cadd(cscale(cmul(ph, ckp), sn), cscale(ckq, cs));
-                cd tkp = T[(size_t)row * d + p], tkq = T[(size_t)row * d + q];
-                T[(size_t)row * d + p] = csub(cscale(tkp, cs), cscale(cmul(cconj(ph), tkq), sn));
-                T[(size_t)row * d + q] = cadd(cscale(cmul(ph, tkp), sn), cscale(tkq, cs));
-            }
-            __syncthreads();
-            // rows: C <- G^H C
-            for (long x = tid; x < (long)d * npair; x += kEigThreads) {
-                const int k = (int)(x % npair), col = (int)(x / npair);
-                const int p = rp[k], q = rq[k];
-                if (q >= d || rs[k] == 0.0) continue;
-                const double cs = rc[k], sn = rs[k];
-                const cd ph = rph[k];
-                cd cpk = C[(size_t)p * d + col], cqk = C[(size_t)q * d + col];
-                C[(size_t)p * d + col] = csub(cscale(cpk, cs), cscale(cmul(ph, cqk), sn));
-                C[(size_t)q * d + col] = cadd(cscale(cmul(cconj(ph), cpk), sn), cscale(cqk, cs));
-            }
-            __syncthreads();
-            for (int k = tid; k < npair; k += kEigThreads) {
-                const int p = rp[k], q = rq[k];
-                if (q < d && rs[k] != 0.0) {
-                    C[(size_t)p * d + q] = {0.0, 0.0};
-                    C[(size_t)q * d + p] = {0.0, 0.0};
+                at(X, d - 1, j) /= at(dd, d - 1, j);
+                if (d >= 2) at(X, d - 2, j) = (at(X, d - 2, j) - at(du, d - 2, j) * at(X, d - 1, j)) / at(dd, d - 2, j);
+                for (int i = d - 3; i >= 0; --i)
+                    at(X, i, j) = (at(X, i, j) - at(du, i, j) * at(X, i + 1, j) - at(du2, i, j) * at(X, i + 2, j)) /
+                                  at(dd, i, j);
+                // orthogonalise against the earlier members of the cluster, normalise
+                for (int k = cstart[cl]; k < j; ++k) {
+                    double dot = 0.0;
+                    for (int i = 0; i < d; ++i) dot += at(X, i, k) * at(X, i, j);
+                    for (int i = 0; i < d; ++i) at(X, i, j) -= dot * at(X, i, k);
                 }
-            }
-            __syncthreads();
-        }
-    }
-    // ascending order: rank of each diagonal entry
-    for (int i = tid; i < d; i += kEigThreads) {
-        const double ei = C[(size_t)i * d + i].r;
-        int rank = 0;
-        for (int j = 0; j < d; ++j) {
-            const double ej = C[(size_t)j * d + j].r;
-            rank += (ej < ei || (ej == ei && j < i)) ? 1 : 0;
-        }
-        ev[rank] = ei;
-        reinterpret_cast<int*>(W)[rank] = i;    // W is free now: permutation
-    }
-    __syncthreads();
-    // eigenvectors v = L^-H w (back substitution), S-normalised
-    if (evecsd) {
-        const int* perm = reinterpret_cast<const int*>(W);
-        for (int c = tid; c < d; c += kEigThreads) {
-            const int src = perm[c];
-            for (int i = d - 1; i >= 0; --i) {
-                cd t = T[(size_t)i * d + src];
-                for (int k = i + 1; k < d; ++k) t = csub(t, cmul(cconj(L[(size_t)k * d + i]), V[(size_t)k * d + c]));
-                V[(size_t)i * d + c] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+                double nrm = 0.0;
+                for (int i = 0; i < d; ++i) nrm += at(X, i, j) * at(X, i, j);
+                nrm = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+                for (int i = 0; i < d; ++i) at(X, i, j) *= nrm;
             }
         }
     }
-    if (tid == 0) status[blockIdx.x] = 0;
+}
+
+// Back to generalized eigenvectors, one thread per eigenvector j of problem blockIdx.y:
+// y = D x (D restores the off-diagonal phases), w = H_0 H_1 ... H_{d-3} y (reflectors applied
+// last to first), v = L^-H w, so that v^H S v = 1.
+__global__ __launch_bounds__(256) void k_eig_back(double* workd, const GenEigProblem* probs, double* evecsd,
+                                                  const int* status)
+{
+    const GenEigProblem pr = probs[blockIdx.y];
+    if (status[blockIdx.y] != 0) return;
+    const int d = pr.d;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    cd* L = reinterpret_cast<cd*>(workd) + pr.work_off;
+    cd* C = L + (size_t)d * d;
+    cd* Y = C + 2 * (size_t)d * d;    // the T area
+    const double* E = eig_extra(workd, pr);
+    const double* tau = E + 2 * d;
+    const double* ph = E + 3 * d;
+    const double* X = E + 5 * d;
+    cd* V = reinterpret_cast<cd*>(evecsd) + pr.in_off;
+    cd dk = {1.0, 0.0};
+    for (int i = 0; i < d; ++i) {
+        const double x = X[(size_t)i * d + j];
+        Y[(size_t)i * d + j] = cscale(dk, x);
+        if (i + 1 < d) dk = cmul(dk, cd{ph[2 * i], ph[2 * i + 1]});
+    }
+    for (int k = d - 3; k >= 0; --k) {
+        const double t = tau[k];
+        if (t == 0.0) continue;
+        const int n = d - k - 1;
+        cd s = {0.0, 0.0};
+        for (int i = 0; i < n; ++i) {
+            const cd v = C[(size_t)(k + 1 + i) * d + k];
+            const cd y = Y[(size_t)(k + 1 + i) * d + j];
+            s.r += v.r * y.r + v.i * y.i;     // conj(v) y
+            s.i += v.r * y.i - v.i * y.r;
+        }
+        s = cscale(s, t);
+        for (int i = 0; i < n; ++i) {
+            const cd v = C[(size_t)(k + 1 + i) * d + k];
+            cd* y = &Y[(size_t)(k + 1 + i) * d + j];
+            *y = csub(*y, cmul(v, s));
+        }
+    }
+    for (int i = d - 1; i >= 0; --i) {
+        cd t = Y[(size_t)i * d + j];
+        for (int k = i + 1; k < d; ++k) t = csub(t, cmul(cconj(L[(size_t)k * d + i]), V[(size_t)k * d + j]));
+        V[(size_t)i * d + j] = cscale(t, 1.0 / L[(size_t)i * d + i].r);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1249,15 +1328,15 @@ hipError_t launch_extract(const ExtractArgs& a, hipStream_t s)
 }
 
 hipError_t launch_geneig(const double* S, const double* H, double* work, const GenEigProblem* probs,
-                         int nprob, double* evals, double* evecs, int* status, hipStream_t s)
+                         int nprob, int dmax, double* evals, double* evecs, int* status, hipStream_t s)
 {
     if (nprob <= 0) return hipSuccess;
-    if (evecs)
-        hipLaunchKernelGGL(k_geneig, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
-                           evecs, status);
-    else
-        hipLaunchKernelGGL(k_geneig_vals, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
-                           status);
+    hipLaunchKernelGGL(k_geneig_vals, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, S, H, work, probs, evals,
+                       status, evecs ? 1 : 0);
+    if (!evecs) return hipGetLastError();
+    hipLaunchKernelGGL(k_eig_invit, dim3((unsigned)nprob), dim3(kEigThreads), 0, s, work, probs, evals, status);
+    hipLaunchKernelGGL(k_eig_back, dim3((unsigned)((dmax + 255) / 256), (unsigned)nprob), dim3(256), 0, s, work, probs,
+                       evecs, status);
     return hipGetLastError();
 }
 

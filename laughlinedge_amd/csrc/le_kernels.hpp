@@ -117,7 +117,7 @@ struct GenEigProblem {
     int d;
     long in_off;             // complex-element offset of S, H, evecs blocks
     long eval_off;
-    long work_off;           // complex-element offset into the work buffer (4 d^2)
+    long work_off;           // complex-element offset into the work buffer (8 d^2 + 4 d)
 };
 
 // precision selector for templated paths
@@ -132,9 +132,8 @@ hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t s);
-hipError_t launch_geneig(const double* S, const double* H, double* work,
-                         const GenEigProblem* probs, int nprob, double* evals, double* evecs,
-                         int* status, hipStream_t s);
+hipError_t launch_geneig(const double* S, const double* H, double* work, const GenEigProblem* probs,
+                         int nprob, int dmax, double* evals, double* evecs, int* status, hipStream_t s);
 hipError_t launch_zero(double* p, long n, hipStream_t s);
 // bases[0] = sweep base, bases[1] = trace row base (read by replayed sample graphs)
 hipError_t launch_set_bases(unsigned long long* bases, unsigned long long sweep_base, long long trace_base,

@@ -133,6 +133,39 @@ def test_geneig_values_only_path():
     assert np.allclose(ev, np.linalg.eigvalsh(Hb), atol=1e-11)
 
 
+def test_geneig_vectors_large_and_clustered():
+    """eigenvectors (inverse iteration on the tridiagonal form + reflectors + L^-H): residual
+    and S-orthonormality up to 640 states, for a fully degenerate pencil (one cluster of 60)
+    and for clusters split by 1e-10 relative"""
+    from laughlinedge_amd import geneig
+    rng = np.random.default_rng(21)
+
+    def check(S, H, tag):
+        ev, V = geneig(S, H, vectors=True)
+        scale = max(1.0, np.abs(H).max())
+        r = np.abs(H @ V - S @ V * ev[None, :]).max()
+        g = np.abs(V.conj().T @ S @ V - np.eye(len(ev))).max()
+        print(f"{tag}: residual {r / scale:.2e}, S-orthonormality {g:.2e}", flush=True)
+        assert r < 1e-8 * scale and g < 1e-8, tag
+        return ev
+
+    for d in (300, 640):
+        A = rng.standard_normal((d, 2 * d)) + 1j * rng.standard_normal((d, 2 * d))
+        S = A @ A.conj().T / (2 * d) + 0.1 * np.eye(d)
+        B = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        check(S, (B + B.conj().T) / 2, f"random d={d}")
+    d = 60
+    A = rng.standard_normal((d, 2 * d)) + 1j * rng.standard_normal((d, 2 * d))
+    S = A @ A.conj().T / (2 * d) + 0.1 * np.eye(d)
+    ev = check(S, 2.5 * S, "degenerate d=60")
+    assert np.allclose(ev, 2.5, rtol=1e-12)
+    Q, _ = np.linalg.qr(rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d)))
+    lam = np.repeat(np.linspace(1.0, 4.0, 12), 5) * (1 + 1e-10 * rng.standard_normal(d))
+    Hc = Q @ np.diag(lam) @ Q.conj().T
+    ev = check(np.eye(d, dtype=complex), (Hc + Hc.conj().T) / 2, "clusters d=60")
+    assert np.allclose(ev, np.sort(lam), atol=1e-12)
+
+
 def test_sample_graphs_match_eager():
     """Sampling rounds replayed from hipGraphs give exactly the eager rounds' moments, traces
     and walkers: several replays per buffer, a partial (eager) round, then a thinning change

@@ -172,6 +172,9 @@ def main():
     traffic, traffic_note = gram_traffic(rows_per_launch)
     taus = taus_of(eng, steps, tw)
     tau_max = max(taus.values()) if taus else None
+    t_eig = time.perf_counter()
+    eng.spectra(-1)
+    eig_ms = 1e3 * (time.perf_counter() - t_eig)
     mean_spec, sig = spectra_blocks(eng, dims)
 
     out = {
@@ -222,6 +225,9 @@ def main():
                 "traced_walkers": tw},
         "spectrum_lowest": {str(SECTORS[s]): (float(mean_spec[s][0]) if mean_spec[s] is not None else None)
                             for s in range(len(dims))},
+        "eigensolve_ms": {"sectors": len(dims), "largest": max(dims), "ms": eig_ms,
+                          "note": "FP64 Cholesky + tridiagonalization + bisection on the GPU, all sectors "
+                                  "in one launch, host copies included"},
     }
     if not args.no_precision:
         log("precision check (fixed samples vs FP64 oracle)")
@@ -232,6 +238,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
+        log("small configuration in full on CPU and GPU")
+        out["config0_full"] = config0_full()
     print(json.dumps(out), flush=True)
 
 
@@ -361,6 +369,17 @@ def precision_check(eng, sig, n_fixed=8192):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline():
     """FP64 oracle port (plain |Psi_L|^2 chain + S/H accumulation), OpenMP, host cores."""
     from oracle.oracle import Oracle
@@ -372,8 +391,35 @@ def cpu_baseline():
     dt = time.perf_counter() - t0
     rate = walkers * samples / dt
     return {"value": rate, "unit": "sample-updates/s", "cores": threads, "kind": "port",
+            "host_logical_cpus": os.cpu_count(), "cpu_model": cpu_model(),
             "sample": f"{walkers} walkers x ({burn} burn-in + {samples} kept sweeps) at N=200, "
                       f"dL=1..16, FP64 (burn-in time included), {dt:.1f} s"}
+
+
+def config0_full():
+    """The small plain configuration in full (N=10, m=3, dL 0..6, 64 walkers, seed 1234,
+    step 0.5, 2000 burn-in sweeps, 2e4 samples per walker, FP64) on the oracle port and on the
+    GPU: the two chains follow the same random streams, so S and H agree to rounding."""
+    from oracle.oracle import Oracle
+    from laughlinedge_amd import Engine
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    n10, secs = 10, list(range(7))
+    o = Oracle(n10, 3, secs, w=0.05, eps=1e-4)
+    t0 = time.perf_counter()
+    So, Ho, _, _, _ = o.plain_run(64, 2000, 20000, 0.5, 1234, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with Engine(n10, 3, secs, w=0.05, eps=1e-4, walkers=64, precision=1, samples=20000, seed=1234,
+                tune_sweeps=0, step=0.5) as e:
+        e.init_walkers()
+        e.burnin(2000)
+        e.sample(20000)
+        mom = e.moments()
+    gpu_s = time.perf_counter() - t0
+    rel = max(max(np.linalg.norm(Sg - So[s]) / np.linalg.norm(So[s]),
+                  np.linalg.norm(Hg - Ho[s]) / np.linalg.norm(Ho[s])) for s, (Sg, Hg) in enumerate(mom))
+    return {"cpu_seconds": cpu_s, "cpu_threads": threads, "gpu_seconds": gpu_s,
+            "max_rel_diff_S_H": float(rel), "samples": 64 * 20000}
 
 
 if __name__ == "__main__":

@@ -545,3 +545,31 @@ def test_sampled_blocks_vs_exact_n4(prec):
         ex_e = eigh(g[f"H{dl}"], g[f"S{dl}"], eigvals_only=True)
         sp = np.array([s[dl] for s in specs])
         assert np.all(np.abs(sp.mean(0) - ex_e) <= 5 * batch_sigma(sp) + 1e-12), (dl, sp.mean(0), ex_e)
+
+
+def test_config0_full_run_matches_oracle(oracle_lib):
+    """The small plain configuration in full: N = 10, m = 3, V = 0.025 r^2 + 1e-4 r^4,
+    dL = 0..6, 64 walkers, seed 1234, uniform start in the disc of radius R, step 0.5,
+    2000 burn-in sweeps, 2e4 samples per walker, FP64.  The GPU chain follows the oracle
+    chain move for move (same Philox streams), so S, H, the acceptance and the spectra agree
+    to rounding, not only statistically."""
+    from laughlinedge_amd import Engine
+    from oracle.oracle import geneig as or_geneig
+    N, m, sectors = 10, 3, list(range(7))
+    print("case config 0 full run (64 walkers x 22000 sweeps, FP64)", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05, eps=1e-4)
+    So, Ho, acc_o, _, _ = o.plain_run(64, 2000, 20000, 0.5, 1234, threads=8)
+    with Engine(N, m, sectors, w=0.05, eps=1e-4, walkers=64, precision=1, samples=20000, seed=1234,
+                tune_sweeps=0, step=0.5, lanes_per_walker=2) as e:
+        e.init_walkers()
+        e.burnin(2000)
+        e.sample(20000)
+        mom = e.moments()
+        acc_g = e.stats()["acceptance"]
+        spec = e.spectra(-1)
+    assert abs(acc_g - acc_o) < 1e-9, (acc_g, acc_o)
+    for s, (Sg, Hg) in enumerate(mom):
+        es = np.linalg.norm(Sg - So[s]) / np.linalg.norm(So[s])
+        eh = np.linalg.norm(Hg - Ho[s]) / np.linalg.norm(Ho[s])
+        assert es < 1e-10 and eh < 1e-10, (s, es, eh)
+        assert np.allclose(spec[s], or_geneig(So[s], Ho[s]), rtol=1e-9)

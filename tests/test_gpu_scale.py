@@ -138,3 +138,36 @@ def test_scale_evolution_of_kicks(sampled):
     ratio = np.abs(d2 - 2 * d1).max() / np.abs(d2).max()
     print(f"linear-response deviation {ratio:.2e}")
     assert ratio < 1e-2, ratio
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5)])
+def test_config1_fixed_samples(oracle_lib, prec, tol):
+    """nu = 1/3, N = 100, V = 0.025 r^2 + 5e-5 r^4, dL = 0..12 (D = 272), 16384 walkers, seed 7:
+    65536 configurations of the GPU chain through the FP32 (or BF16-split) accumulation and
+    through the FP64 oracle; every sector's S and H within tol relative Frobenius."""
+    from laughlinedge_amd import Engine
+    n100, secs = 100, list(range(13))
+    print(f"case config 1 fixed samples prec={prec}", flush=True)
+    with Engine(n100, 3, secs, w=0.05, eps=5e-5, walkers=16384, samples=4, seed=7, tune_sweeps=30,
+                step=0.5) as e:
+        assert e.info.D == 272
+        e.init_walkers()
+        e.tune_step()
+        pos = []
+        for _ in range(4):
+            e.burnin(50)
+            pos.append(e.positions())
+    pos = np.concatenate(pos)
+    o = oracle_lib.Oracle(n100, 3, secs, w=0.05, eps=5e-5)
+    So, Ho, _ = o.accumulate_fixed(pos)
+    n = pos.shape[0]
+    with Engine(n100, 3, secs, w=0.05, eps=5e-5, walkers=16384, samples=1, precision=prec) as f:
+        f.accumulate_positions(pos)
+        mom = f.moments()
+    worst = 0.0
+    for s, (Sg, Hg) in enumerate(mom):
+        es = np.linalg.norm(Sg - So[s] / n) / np.linalg.norm(So[s] / n)
+        eh = np.linalg.norm(Hg - Ho[s] / n) / np.linalg.norm(Ho[s] / n)
+        worst = max(worst, es, eh)
+        assert es < tol and eh < tol, (s, es, eh)
+    print(f"worst relative Frobenius error {worst:.3e}")

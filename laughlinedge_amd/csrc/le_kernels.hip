@@ -90,10 +90,12 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
     const int walker = blockIdx.x * WPB + grp;
     const bool live = walker < a.walkers;
     V2* P = lds + grp * NP;
-    // per-group random numbers of the current batch of G moves: (n1, n2, u bits)
+    // per-group random numbers of the current batch of G moves: (n1, n2, u bits, ln u); the
+    // FP32 path takes ln u = log2(u) ln 2 here, once per move, instead of in every lane
     struct Rnd {
         Real n1, n2;
-        uint32_t ub, pad;
+        uint32_t ub;
+        float lnu;
     };
     Rnd* rnd = reinterpret_cast<Rnd*>(lds + WPB * NP) + grp * G;
     V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)(live ? walker : 0) * N;
@@ -119,7 +121,9 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
             philox((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
             Real g1, g2;
             gauss2(r[0], r[1], g1, g2);
-            rnd[lane] = Rnd{g1, g2, r[2], 0u};
+            float lnu = 0.f;
+            if constexpr (sizeof(Real) == 4) lnu = __builtin_amdgcn_logf(u01f(r[2])) * 0.69314718055994531f;
+            rnd[lane] = Rnd{g1, g2, r[2], lnu};
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -132,6 +136,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 const int i = base + t;
                 const Real d1 = rn.n1, d2 = rn.n2;
                 const uint32_t ub = rn.ub;
+                const float lnu_move = rn.lnu;
                 const Real xn = zo.x + step * d1, yn = zo.y + step * d2;
                 V2 mid;
                 mid.x = (Real)0.5 * (zo.x + xn);
@@ -146,6 +151,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 }
                 Real lsum = 0;
                 Real pn = 1, pd = 1;
+                int ebal = 0;
                 // FP32: |z' - q|^2 = |b|^2 + |d|^2 + 2 d.b with b = z - q, d = z' - z (9 VALU per
                 // pair instead of 10); FP64 takes both distances directly
                 const Real dx = xn - zo.x, dy = yn - zo.y;
@@ -175,9 +181,17 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 #pragma unroll
                         for (int u = 0; u < CL; ++u) pair(q[u]);
                     }
-                    lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
-                    pn = 1;
-                    pd = 1;
+                    if constexpr (sizeof(Real) == 4) {
+                        // keep the products in range by moving their exponents to an integer
+                        int en, ed;
+                        pn = frexpf(pn, &en);
+                        pd = frexpf(pd, &ed);
+                        ebal += en - ed;
+                    } else {
+                        lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                        pn = 1;
+                        pd = 1;
+                    }
                 }
                 // remainder (< C slots): blocks of 4, then 2 and 1, whose loads issue together
                 for (; s + 4 <= S; s += 4) {
@@ -198,13 +212,16 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                         s += blk;
                     }
                 }
-                lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                if constexpr (sizeof(Real) == 4)
+                    lsum = __builtin_amdgcn_logf(pn) - __builtin_amdgcn_logf(pd) + (float)ebal;
+                else
+                    lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
                 lsum = group_sum<G, Real>(lsum);
                 const Real logA = mfac * lsum -
                                   (Real)0.5 * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y));
                 bool acc;
                 if constexpr (sizeof(Real) == 4) {
-                    acc = (logA >= 0.f) || (__builtin_amdgcn_logf(u01f(ub)) * 0.69314718055994531f < logA);
+                    acc = (logA >= 0.f) || (lnu_move < logA);
                 } else {
                     acc = (logA >= 0.0) || (log(u01d(ub)) < logA);
                 }

@@ -85,7 +85,8 @@ def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1)
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
-                  trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs)
+                  trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs,
+                  lanes_per_walker=-1)
 
 
 def timed_sampling(eng, steps, nblk):

@@ -166,8 +166,8 @@ class Engine:
         obs = Observables(len(self.harmonics), self._harr, int(n_user_weights))
         _check(lib().le_engine_create(ctypes.byref(self.sys), ctypes.byref(self.mc), arr,
                                       len(self.sectors), ctypes.byref(obs), ctypes.byref(self._h)))
-        self.info = EngineInfo()
-        _check(lib().le_engine_get_info(self._h, ctypes.byref(self.info)))
+        self._info = EngineInfo()
+        _check(lib().le_engine_get_info(self._h, ctypes.byref(self._info)))
         ns = self.info.n_sectors
         dims = (ctypes.c_int * ns)()
         offs = (ctypes.c_int * (ns + 1))()
@@ -176,6 +176,13 @@ class Engine:
         self.offsets = list(offs)
         self.N, self.m = int(N), int(m)
         self.R = float(np.sqrt(2.0 * m * N))
+
+    @property
+    def info(self):
+        """engine shape and the lane mapping in use (the autotuner may change it at init)"""
+        if self._h:
+            _check(lib().le_engine_get_info(self._h, ctypes.byref(self._info)))
+        return self._info
 
     def close(self):
         if self._h:
@@ -398,8 +405,8 @@ class OccupationEngine(Engine):
         self._h = ctypes.c_void_p()
         _check(lib().le_engine_create_occupation(ctypes.byref(self.sys), ctypes.byref(self.mc), int(dL), int(lcut),
                                                  ctypes.byref(self._h)))
-        self.info = EngineInfo()
-        _check(lib().le_engine_get_info(self._h, ctypes.byref(self.info)))
+        self._info = EngineInfo()
+        _check(lib().le_engine_get_info(self._h, ctypes.byref(self._info)))
         dims = (ctypes.c_int * 2)()
         offs = (ctypes.c_int * 3)()
         _check(lib().le_sector_dims(self._h, dims, offs))

@@ -12,9 +12,9 @@ FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
   kernels       sampler (VALU FLOP/s), features (sample-matrix bytes/s), Gram (MFMA FLOP/s)
   ess           tau_int (Madras-Sokal) and ESS/s over Re/Im p_1..p_4, V and rotation-
                 invariant S entries (|p_1|^2, |p_2|^2, Re conj(p_2) p_1^2, ...)
-  forms         plain FP64 / plain FP32 / wide-step FP32 and BF16-split / learned FP32 at
-                their best thinning: ESS/s, ratios to the plain FP64 form, spectra vs the
-                FP64 form in MC sigma
+  forms         plain FP64 / plain FP32 / wide-step FP32 and BF16-split / with exact
+                symmetry moves / learned FP32 at their best thinning: ESS/s, ratios to the
+                plain FP64 form, spectra vs the FP64 form in MC sigma
   precision     FP32 and BF16-split GPU moments vs the FP64 oracle on one fixed sample set
   cpu_baseline  the FP64 oracle port (OpenMP, host cores) on a bounded sample
 """
@@ -81,12 +81,12 @@ def spectra_blocks(eng, dims):
     return mean, sig
 
 
-def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1):
+def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1, sym=0):
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
                   trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs,
-                  lanes_per_walker=-1)
+                  lanes_per_walker=-1, symmetry_moves=sym)
 
 
 def timed_sampling(eng, steps, nblk):
@@ -263,20 +263,25 @@ def forms(args, thins):
 
     The plain forms tune the step to the reference's 50% acceptance; the *_wide forms tune it
     to 30%, which moves particles further per sweep (larger jump distance, shorter tau_int).
-    learned_fp32 trains the MLP proposal first (2000 Adam steps, lr 1e-3, 1e5 configurations,
-    seed 99) and samples with it; its training time is reported, not charged to ESS/s."""
+    The *_sym forms add, after each kept sample's sweeps, the exact centre-of-mass heat bath
+    and a uniform global rotation.  learned_* train the MLP proposal first (2000 Adam steps,
+    lr 1e-3, 1e5 configurations, seed 99) and sample with it; the training time is reported,
+    not charged to ESS/s."""
     res = {}
     tw = min(args.trace_walkers, args.walkers)
     fs = args.form_steps
     nblk = 8
-    table = (("plain_fp64", 1, False, 0.5), ("plain_fp32", 0, False, 0.5), ("plain_fp32_wide", 0, False, 0.3),
-             ("plain_bf16x3_wide", 2, False, 0.3), ("learned_fp32", 0, True, 0.5))
-    for name, prec, learned, target in table:
-        eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target)
+    table = (("plain_fp64", 1, False, 0.5, 0), ("plain_fp32", 0, False, 0.5, 0), ("plain_fp32_wide", 0, False, 0.3, 0),
+             ("plain_bf16x3_wide", 2, False, 0.3, 0), ("plain_fp32_wide_sym", 0, False, 0.3, 1),
+             ("plain_bf16x3_wide_sym", 2, False, 0.3, 1), ("learned_fp32", 0, True, 0.5, 0),
+             ("learned_fp32_sym", 0, True, 0.5, 1))
+    for name, prec, learned, target, sym in table:
+        eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target, sym=sym)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
-        r = {"precision": ["fp32", "fp64", "bf16x3"][prec], "step": step, "target_acceptance": target}
+        r = {"precision": ["fp32", "fp64", "bf16x3"][prec], "step": step, "target_acceptance": target,
+             "symmetry_moves": bool(sym)}
         if learned:
             t0 = time.perf_counter()
             rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)

@@ -182,6 +182,7 @@ struct le_engine {
     int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
                    const unsigned long long* sweep_base = nullptr);
     int sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
+    int symmetry(unsigned long long sweep0, const unsigned long long* sweep_base);
     void drop_graphs();
     int close_block(long long samples);
     int gram_rows(long rows, int buf = 0, hipStream_t s = nullptr);
@@ -279,6 +280,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->n_blocks = 1;
     p->trace_walkers = 0;
     p->graphs = 1;
+    p->symmetry_moves = 0;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -519,6 +521,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
+    if (mc->symmetry_moves != 0 && mc->symmetry_moves != 1) return fail(LE_ERR_ARG, "symmetry_moves must be 0 or 1");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -655,6 +658,22 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     } else {
         HIPCHK(le::launch_sweep(a, prec, G, stream));
     }
+    return LE_OK;
+}
+
+// Exact centre-of-mass heat bath + global rotation of every walker (edge engine only).
+int le_engine::symmetry(unsigned long long sweep0, const unsigned long long* sweep_base)
+{
+    if (!mc.symmetry_moves || two_sided) return LE_OK;
+    le::SweepArgs a{};
+    a.pos = pos;
+    a.seed = mc.seed;
+    a.sweep0 = sweep0;
+    a.sweep_base = sweep_base;
+    a.N = sys.particles_number;
+    a.m = sys.inverse_filling;
+    a.walkers = mc.walkers;
+    HIPCHK(le::launch_symmetry(a, prec, stream));
     return LE_OK;
 }
 
@@ -906,6 +925,7 @@ int le_engine::sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hip
     hipError_t h = hipSuccess;
     for (int q = 0; q < spl && rc == LE_OK && h == hipSuccess; ++q) {
         rc = run_sweeps(mc.thin, (unsigned long long)q * (unsigned long long)mc.thin, pos, bases_d);
+        if (rc == LE_OK) rc = symmetry((unsigned long long)(q + 1) * (unsigned long long)mc.thin, bases_d);
         f.row0 = (long)q * mc.walkers;
         f.trace = traced ? trace_d : nullptr;
         f.trace_walkers = traced ? tw : 0;
@@ -1006,6 +1026,8 @@ extern "C" int le_sample(le_engine* e, int samples)
             int rc = e->run_sweeps(e->mc.thin, e->sweep_counter, e->pos);
             if (rc) return rc;
             e->sweep_counter += (unsigned long long)e->mc.thin;
+            rc = e->symmetry(e->sweep_counter, nullptr);
+            if (rc) return rc;
             HIPCHK(hipEventRecord(e1, e->stream));
             f.row0 = (long)q * e->mc.walkers;
             if (e->trace_d && e->trace_rows + tw <= e->trace_cap_rows) {
@@ -1595,6 +1617,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
+    if (mc->symmetry_moves != 0 && mc->symmetry_moves != 1) return fail(LE_ERR_ARG, "symmetry_moves must be 0 or 1");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;

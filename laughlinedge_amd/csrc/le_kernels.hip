@@ -241,7 +241,62 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Features: one wavefront per sample.
+// Exact symmetry moves of |Psi_L|^2, one wavefront per walker.  With Z = sum z_i / N,
+// sum |z_i|^2 = N |Z|^2 + sum |z_i - Z|^2 and the Jastrow factor depends on differences only,
+// so Z is an independent Gaussian of variance 1/N per component: it is redrawn from that
+// Gaussian (heat bath).  |Psi_L|^2 is also invariant under z -> e^{i phi} z, so every
+// configuration is then rotated by a uniform phi.  Both moves are accepted with probability 1.
+// Random numbers: Philox block (walker, sweep_lo, sweep_hi16 | tag 4 << 16, 0xfffffff0).
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(64) void k_symmetry(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    const int walker = blockIdx.x;
+    if (walker >= a.walkers) return;
+    const int lane = threadIdx.x;
+    const int N = a.N;
+    V2* P = reinterpret_cast<V2*>(a.pos) + (size_t)walker * N;
+    double sx = 0.0, sy = 0.0;
+    for (int j = lane; j < N; j += 64) {
+        const V2 q = P[j];
+        sx += (double)q.x;
+        sy += (double)q.y;
+    }
+    sx = group_sum<64, double>(sx);
+    sy = group_sum<64, double>(sy);
+    const unsigned long long sidx = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    uint32_t r[4];
+    philox((uint32_t)walker, (uint32_t)sidx, (uint32_t)((sidx >> 32) & 0xffffu) | (4u << 16), 0xfffffff0u,
+           (uint32_t)a.seed, (uint32_t)(a.seed >> 32), r);
+    double g1, g2;
+    gauss2(r[0], r[1], g1, g2);
+    const double sd = 1.0 / sqrt((double)N);
+    const double dx = g1 * sd - sx / N, dy = g2 * sd - sy / N;     // Z -> Z'
+    double sphi, cphi;
+    sincos(6.283185307179586 * u01d(r[2]), &sphi, &cphi);
+    for (int j = lane; j < N; j += 64) {
+        const V2 q = P[j];
+        const double x = (double)q.x + dx, y = (double)q.y + dy;
+        V2 o;
+        o.x = (Real)(cphi * x - sphi * y);
+        o.y = (Real)(sphi * x + cphi * y);
+        P[j] = o;
+    }
+}
+
+hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    if (prec == kFP64)
+        hipLaunchKernelGGL(k_symmetry<double>, dim3((unsigned)a.walkers), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_symmetry<float>, dim3((unsigned)a.walkers), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Features
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double potential_r2(const Potential& p, double r2)
 {

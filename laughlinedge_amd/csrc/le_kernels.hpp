@@ -126,6 +126,9 @@ enum Prec { kFP32 = 0, kFP64 = 1, kBF16X3 = 2 };   // kBF16X3: Gram only (FP32 d
 hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
                                  unsigned long long seed, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+// exact centre-of-mass heat bath + global rotation of every walker (a.sweep0 + *a.sweep_base
+// selects the random numbers)
+hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s);
 // two-sided chain: a.N is the number of particles per state (positions hold N + 1)
 hipError_t launch_sweep2(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);

@@ -88,7 +88,7 @@ class MCParams(ctypes.Structure):
                 ("acceptance_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
-                ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int)]
+                ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):

@@ -166,17 +166,18 @@ def test_geneig_vectors_large_and_clustered():
     assert np.allclose(ev, np.sort(lam), atol=1e-12)
 
 
-def test_sample_graphs_match_eager():
+@pytest.mark.parametrize("sym", [0, 1])
+def test_sample_graphs_match_eager(sym):
     """Sampling rounds replayed from hipGraphs give exactly the eager rounds' moments, traces
     and walkers: several replays per buffer, a partial (eager) round, then a thinning change
     that forces a new capture, with Gram accumulation and block closing in between."""
     from laughlinedge_amd import Engine
     N, m, sectors, W = 24, 3, [1, 2, 3], 2048
-    print("case graphs vs eager N=24", flush=True)
+    print(f"case graphs vs eager N=24 symmetry={sym}", flush=True)
     out = []
     for graphs in (0, 1):
         with Engine(N, m, sectors, w=0.05, eps=1e-4, walkers=W, samples=400, seed=9, tune_sweeps=20, step=0.5,
-                    trace_walkers=32, n_blocks=3, graphs=graphs, harmonics=(1, 2)) as e:
+                    trace_walkers=32, n_blocks=3, graphs=graphs, harmonics=(1, 2), symmetry_moves=sym) as e:
             e.init_walkers()
             e.tune_step()
             e.burnin(20)
@@ -198,23 +199,29 @@ def test_sample_graphs_match_eager():
             assert np.array_equal(S0, S1) and np.array_equal(H0, H1)
 
 
-def test_sampler_r2_sum_rule():
-    """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler."""
+@pytest.mark.parametrize("sym", [0, 1])
+def test_sampler_r2_sum_rule(sym):
+    """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler; the
+    centre of mass Z is Gaussian with <N |Z|^2> = 2 (with or without its heat bath)."""
     from laughlinedge_amd import Engine
     N, m = 20, 3
     L0 = m * N * (N - 1) // 2
-    print("case r2 sum rule N=20", flush=True)
+    print(f"case r2 sum rule N=20 symmetry={sym}", flush=True)
     # H with w = 2, eps = 0 is sum_i r_i^2; sector 0 has S = 1
-    with Engine(N, m, [0], w=2.0, walkers=2048, precision=0, step=0.6, samples=200,
-                burnin_sweeps=300, tune_sweeps=0, n_blocks=4) as e:
+    with Engine(N, m, [0], w=2.0, walkers=8192, precision=0, step=0.6, samples=200,
+                burnin_sweeps=300, tune_sweeps=0, n_blocks=4, symmetry_moves=sym) as e:
         e.run()
         blocks = [e.moments(b)[0][1][0, 0].real for b in range(4)]
         st = e.stats()
+        pos = e.positions()
     mean = np.mean(blocks)
     err = np.std(blocks, ddof=1) / 2
     exact = 2 * (L0 + N)
     assert abs(mean - exact) < 4 * err + 1e-4 * exact, (mean, exact, err)
     assert 0.2 < st["acceptance"] < 0.9
+    z = pos[..., 0] + 1j * pos[..., 1]
+    nz2 = N * np.abs(z.mean(axis=1)) ** 2
+    assert abs(nz2.mean() - 2.0) < 5 * nz2.std() / np.sqrt(len(nz2)), nz2.mean()
 
 
 def test_harmonic_cross_blocks(oracle_lib):
@@ -515,23 +522,24 @@ def test_geneig_golden():
         assert np.allclose(ev, g[f"E{k}"], rtol=1e-10, atol=1e-10), k
 
 
-@pytest.mark.parametrize("prec", [0, 2])
-def test_sampled_blocks_vs_exact_n4(prec):
+@pytest.mark.parametrize("prec,sym", [(0, 0), (2, 0), (0, 1)])
+def test_sampled_blocks_vs_exact_n4(prec, sym):
     """N = 4, m = 3, dL = 0..4: GPU chain + features + Gram against the exact matrices of
-    the committed fixture, within 5 batch sigma per entry, and the spectra within 5 sigma."""
+    the committed fixture, within 5 batch sigma per entry, and the spectra within 5 sigma;
+    also with the centre-of-mass heat bath and random rotations after every kept sample."""
     from laughlinedge_amd import Engine
     from laughlinedge_amd.stats import batch_sigma
     g = np.load(GOLDEN + "/exact_n4_m3.npz")
     N, m, w, eps = int(g["N"]), int(g["m"]), float(g["w"]), float(g["eps"])
-    print(f"case sampled vs exact N=4 prec={prec}", flush=True)
-    nb = 8
+    print(f"case sampled vs exact N=4 prec={prec} symmetry={sym}", flush=True)
+    nb = 16     # batch errors of the heavy-tailed dL = 4 entries need more than 8 batches
     with Engine(N, m, [0, 1, 2, 3, 4], w=w, eps=eps, walkers=8192, samples=400, n_blocks=nb, seed=2024,
-                tune_sweeps=40, step=0.5, precision=prec) as e:
+                tune_sweeps=40, step=0.5, precision=prec, symmetry_moves=sym) as e:
         e.init_walkers()
         e.tune_step()
         e.burnin(300)
         for b in range(nb):
-            e.sample(50)
+            e.sample(25)
         blocks = [e.moments(b) for b in range(nb)]
         specs = [e.spectra(b) for b in range(nb)]
     for dl in range(5):

@@ -10,8 +10,11 @@ Prints ONE JSON line.  value = S,H sample-updates/s of the production form (FP32
 FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
   roofline      the Gram accumulation kernel (MFMA bound), live HIP-event timing
   kernels       sampler (VALU FLOP/s), features (sample-matrix bytes/s), Gram (MFMA FLOP/s)
-  ess           tau_int (Madras-Sokal) and ESS/s over Re/Im p_1..p_4, V and rotation-
-                invariant S entries (|p_1|^2, |p_2|^2, Re conj(p_2) p_1^2, ...)
+  ess           tau_int (Madras-Sokal) and ESS/s: the slowest of Re/Im p_1..p_4, V and traced
+                S and H entries (|p_1|^2, |p_2|^2, Re/Im conj(p_2) p_1^2, V |p_1|^2, ...), and
+                (*_sh) the slowest S or H entry alone: every S and H entry pairs states of
+                one dL, so all are invariant under z -> e^{i phi} z, while Re/Im p_k average
+                to zero and enter no S or H entry
   forms         plain FP64 / plain FP32 / wide-step FP32 and BF16-split / with exact
                 symmetry moves / learned FP32 at their best thinning: ESS/s, ratios to the
                 plain FP64 form, spectra vs the FP64 form in MC sigma
@@ -46,8 +49,9 @@ def log(msg):
 
 
 def observables(tr):
-    """(T, walkers, 9) trace -> dict name -> (T, walkers) series, including S entries of
-    sectors 1..4 that do not change under a global rotation z -> e^{i phi} z."""
+    """(T, walkers, 9) trace -> dict name -> (T, walkers) series: Re/Im p_1..p_4 and V, and
+    S and H entries of sectors 0..4 (conj(p_lambda) p_mu and conj(p_lambda) V p_mu with
+    |lambda| = |mu|, unchanged by a global rotation z -> e^{i phi} z)."""
     p = [tr[:, :, 2 * k].astype(np.float64) + 1j * tr[:, :, 2 * k + 1] for k in range(4)]
     obs = {nm: tr[:, :, q].astype(np.float64) for q, nm in enumerate(TRACE_NAMES)}
     obs["|p1|^2"] = np.abs(p[0]) ** 2
@@ -57,7 +61,23 @@ def observables(tr):
     obs["|p3|^2"] = np.abs(p[2]) ** 2
     obs["Re conj(p3) p1 p2"] = (np.conj(p[2]) * p[0] * p[1]).real
     obs["|p4|^2"] = np.abs(p[3]) ** 2
+    obs["Im conj(p2) p1^2"] = (np.conj(p[1]) * p[0] ** 2).imag
+    obs["V |p1|^2"] = obs["V"] * obs["|p1|^2"]
+    obs["V |p2|^2"] = obs["V"] * obs["|p2|^2"]
     return obs
+
+
+PK_NAMES = TRACE_NAMES[:8]
+
+
+def tau_sh(taus):
+    """largest tau_int over the traced S and H entries (the quantities the spectra use)"""
+    return max(v for k, v in taus.items() if k not in PK_NAMES)
+
+
+def tau_pk(taus):
+    """largest tau_int of Re/Im p_1..p_4 (diagnostic: no S or H entry)"""
+    return max(taus[k] for k in PK_NAMES)
 
 
 def taus_of(eng, n_samples, tw):
@@ -172,7 +192,8 @@ def main():
     rows_per_launch = info.samples_per_launch * args.walkers
     traffic, traffic_note = gram_traffic(rows_per_launch)
     taus = taus_of(eng, steps, tw)
-    tau_max = max(taus.values()) if taus else None
+    tau_max = tau_sh(taus) if taus else None
+    tau_max_all = max(taus.values()) if taus else None
     t_eig = time.perf_counter()
     eng.spectra(-1)
     eig_ms = 1e3 * (time.perf_counter() - t_eig)
@@ -221,8 +242,10 @@ def main():
                     "stream on the other sample buffer, so ms_per_step < sum of the parts",
         },
         "acceptance": st["acceptance"],
-        "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max,
-                "ess_per_s": updates / (2.0 * tau_max) / wall if tau_max else None,
+        "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max_all,
+                "ess_per_s": updates / (2.0 * tau_max_all) / wall if tau_max_all else None,
+                "tau_max_sh": tau_max,
+                "ess_per_s_sh": updates / (2.0 * tau_max) / wall if tau_max else None,
                 "traced_walkers": tw},
         "spectrum_lowest": {str(SECTORS[s]): (float(mean_spec[s][0]) if mean_spec[s] is not None else None)
                             for s in range(len(dims))},
@@ -263,25 +286,31 @@ def forms(args, thins):
 
     The plain forms tune the step to the reference's 50% acceptance; the *_wide forms tune it
     to 30%, which moves particles further per sweep (larger jump distance, shorter tau_int).
-    The *_sym forms add, after each kept sample's sweeps, the exact centre-of-mass heat bath
-    and a uniform global rotation.  learned_* train the MLP proposal first (2000 Adam steps,
-    lr 1e-3, 1e5 configurations, seed 99) and sample with it; the training time is reported,
-    not charged to ESS/s."""
+    After each kept sample's sweeps, the *_cmrot forms add the exact centre-of-mass heat bath
+    and a uniform global rotation, and the *_sym forms also the exact dilation heat bath of the
+    coordinates relative to the centre of mass.  learned_* train the MLP proposal first (2000
+    Adam steps, lr 1e-3, 1e5 configurations, seed 99) and sample with it; the training time is
+    reported, not charged to ESS/s.
+
+    Two ESS figures per run, each at its own best thinning: ess_per_s takes the slowest of all
+    traced observables (Re/Im p_1..p_4, V and the S and H entries); ess_per_s_sh the slowest S
+    or H entry only.  The rotation move decorrelates Re/Im p_k at once but leaves every S and H
+    entry unchanged, so only ess_per_s_sh measures what the spectra gain from it."""
     res = {}
     tw = min(args.trace_walkers, args.walkers)
     fs = args.form_steps
     nblk = 8
     table = (("plain_fp64", 1, False, 0.5, 0), ("plain_fp32", 0, False, 0.5, 0), ("plain_fp32_wide", 0, False, 0.3, 0),
-             ("plain_bf16x3_wide", 2, False, 0.3, 0), ("plain_fp32_wide_sym", 0, False, 0.3, 1),
-             ("plain_bf16x3_wide_sym", 2, False, 0.3, 1), ("learned_fp32", 0, True, 0.5, 0),
-             ("learned_fp32_sym", 0, True, 0.5, 1))
+             ("plain_bf16x3_wide", 2, False, 0.3, 0), ("plain_fp32_wide_cmrot", 0, False, 0.3, 1),
+             ("plain_fp32_wide_sym", 0, False, 0.3, 2), ("plain_bf16x3_wide_sym", 2, False, 0.3, 2),
+             ("learned_fp32", 0, True, 0.5, 0), ("learned_fp32_sym", 0, True, 0.5, 2))
     for name, prec, learned, target, sym in table:
         eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target, sym=sym)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
         r = {"precision": ["fp32", "fp64", "bf16x3"][prec], "step": step, "target_acceptance": target,
-             "symmetry_moves": bool(sym)}
+             "symmetry_moves": ["none", "centre of mass + rotation", "centre of mass + rotation + dilation"][sym]}
         if learned:
             t0 = time.perf_counter()
             rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)
@@ -293,20 +322,27 @@ def forms(args, thins):
             eng.sample(4)
             wall, d = timed_sampling(eng, fs, 1)
             taus = taus_of(eng, fs, tw)
-            tmax = max(taus.values())
+            sh = {k: v for k, v in taus.items() if k not in PK_NAMES}
+            tmax, tmax_sh = max(taus.values()), tau_sh(taus)
             ess = args.walkers * fs / (2.0 * tmax) / wall
+            ess_sh = args.walkers * fs / (2.0 * tmax_sh) / wall
             runs.append({"thin": thin, "ms_per_kept_sample": 1e3 * wall / fs,
                          "ms_sampler_per_kept_sample": (d["ms_sweep"] + d["ms_features"] + d["ms_sampler_graph"]) / fs,
                          "tau_max": tmax, "tau_max_obs": max(taus, key=taus.get),
-                         "tau_p1": max(taus["Re p1"], taus["Im p1"]), "tau_V": taus["V"],
-                         "ess_per_s": ess, "acceptance": eng.stats()["acceptance"]})
-            log(f"form {name} thin {thin}: {1e3 * wall / fs:.3f} ms/sample tau_max {tmax:.2f} ESS/s {ess:.3e}")
+                         "tau_max_sh": tmax_sh, "tau_max_sh_obs": max(sh, key=sh.get),
+                         "tau_re_im_pk": tau_pk(taus), "tau_V": taus["V"], "tau_p1sq": taus["|p1|^2"],
+                         "ess_per_s": ess, "ess_per_s_sh": ess_sh, "acceptance": eng.stats()["acceptance"]})
+            log(f"form {name} thin {thin}: {1e3 * wall / fs:.3f} ms/sample tau_max {tmax:.2f} (S/H {tmax_sh:.2f}) "
+                f"ESS/s {ess:.3e} (S/H {ess_sh:.3e})")
         best = max(runs, key=lambda x: x["ess_per_s"])
+        best_sh = max(runs, key=lambda x: x["ess_per_s_sh"])
         r["best_thin"] = best["thin"]
         r["ess_per_s"] = best["ess_per_s"]
+        r["best_thin_sh"] = best_sh["thin"]
+        r["ess_per_s_sh"] = best_sh["ess_per_s_sh"]
         r["runs"] = runs
-        # spectra with batch errors from nblk blocks at the best thinning
-        eng.set_thin(best["thin"])
+        # spectra with batch errors from nblk blocks at the best thinning for S and H
+        eng.set_thin(best_sh["thin"])
         eng.sample(4)
         timed_sampling(eng, fs, nblk)
         r["_spec"], r["_sig"] = spectra_blocks(eng, eng.dims)
@@ -316,6 +352,7 @@ def forms(args, thins):
     for name, *_ in table[1:]:
         r = res[name]
         r["ess_ratio_vs_plain_fp64"] = r["ess_per_s"] / ref["ess_per_s"]
+        r["ess_sh_ratio_vs_plain_fp64"] = r["ess_per_s_sh"] / ref["ess_per_s_sh"]
         dev = []
         for s in range(len(SECTORS)):
             a, b, sa, sb = r["_spec"][s], ref["_spec"][s], r["_sig"][s], ref["_sig"][s]

@@ -84,9 +84,10 @@ typedef struct le_mc_params {
                                   occupation chain and partial rounds run eagerly); 0: eager, with
                                   per-kernel times in le_run_stats */
     int symmetry_moves;        /* 1: after each kept sample's sweeps, redraw the centre of mass from
-                                  its exact Gaussian and rotate by a uniform angle (exact moves of
-                                  |Psi_L|^2, edge engine only); 0 (default): single-particle
-                                  moves only, as the reference */
+                                  its exact Gaussian and rotate by a uniform angle; 2: also redraw
+                                  the scale of the coordinates relative to the centre of mass from
+                                  its exact Gamma law (all exact moves of |Psi_L|^2, edge engine
+                                  only); 0 (default): single-particle moves only, as the reference */
 } le_mc_params;
 
 typedef struct le_engine le_engine;

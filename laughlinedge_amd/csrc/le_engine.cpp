@@ -521,7 +521,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
-    if (mc->symmetry_moves != 0 && mc->symmetry_moves != 1) return fail(LE_ERR_ARG, "symmetry_moves must be 0 or 1");
+    if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -661,7 +661,8 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     return LE_OK;
 }
 
-// Exact centre-of-mass heat bath + global rotation of every walker (edge engine only).
+// Exact centre-of-mass heat bath + global rotation (+ dilation of the relative coordinates at
+// symmetry_moves = 2) of every walker (edge engine only).
 int le_engine::symmetry(unsigned long long sweep0, const unsigned long long* sweep_base)
 {
     if (!mc.symmetry_moves || two_sided) return LE_OK;
@@ -673,6 +674,7 @@ int le_engine::symmetry(unsigned long long sweep0, const unsigned long long* swe
     a.N = sys.particles_number;
     a.m = sys.inverse_filling;
     a.walkers = mc.walkers;
+    a.dilate = mc.symmetry_moves >= 2;
     HIPCHK(le::launch_symmetry(a, prec, stream));
     return LE_OK;
 }
@@ -863,7 +865,10 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         g.w1 = j.nw > 1 ? weight_ptr(j.wid[1], buf) : nullptr;
         g.nw = j.nw;
         g.rows = rows;
-        const long want_wg = 2048;
+        // ~6144 workgroups (24 per CU): with 2-3 resident 512-thread workgroups per CU the last,
+        // partial round of workgroups is then a small part of the launch (at 2048 workgroups
+        // N = 200, dL 1..16 needed 4.02 rounds and lost ~20% to the fifth)
+        const long want_wg = 6144;
         long splits = std::max<long>(1, (want_wg + j.ntiles - 1) / std::max(1, j.ntiles));
         long rps = (rows + splits - 1) / splits;
         rps = std::max<long>(rps, 1024);
@@ -1617,7 +1622,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
-    if (mc->symmetry_moves != 0 && mc->symmetry_moves != 1) return fail(LE_ERR_ARG, "symmetry_moves must be 0 or 1");
+    if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;

@@ -241,12 +241,23 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Exact symmetry moves of |Psi_L|^2, one wavefront per walker.  With Z = sum z_i / N,
-// sum |z_i|^2 = N |Z|^2 + sum |z_i - Z|^2 and the Jastrow factor depends on differences only,
-// so Z is an independent Gaussian of variance 1/N per component: it is redrawn from that
-// Gaussian (heat bath).  |Psi_L|^2 is also invariant under z -> e^{i phi} z, so every
-// configuration is then rotated by a uniform phi.  Both moves are accepted with probability 1.
-// Random numbers: Philox block (walker, sweep_lo, sweep_hi16 | tag 4 << 16, 0xfffffff0).
+// Exact symmetry moves of |Psi_L|^2, one wavefront per walker.  With Z = sum z_i / N and
+// w_i = z_i - Z, sum |z_i|^2 = N |Z|^2 + sum |w_i|^2 and the Jastrow factor depends on the w_i
+// only, so |Psi_L|^2 = exp(-N |Z|^2 / 2) x f(w), and three group moves leave it invariant:
+//  - centre of mass: Z is an independent Gaussian of variance 1/N per component, redrawn from
+//    it (heat bath);
+//  - dilation of the relative coordinates, w -> lambda w: f(lambda w) = lambda^(2 L0) f(w)
+//    exp(-(lambda^2 - 1) W / 2) with W = sum |w_i|^2 and L0 = m N (N - 1) / 2.  The generalized
+//    Gibbs step for a group (Liu and Sabatti, JASA 95 (2000) 1224) draws lambda from
+//    pi(lambda w) lambda^(2N - 2) d lambda / lambda, i.e. t = lambda^2 W / 2 ~ Gamma(L0 + N - 1);
+//    the Gamma variate comes from Marsaglia and Tsang's rejection method (ACM TOMS 26 (2000)
+//    363), at most 32 attempts (each accepts with probability > 0.95 for shape >= 1; if all
+//    fail, lambda = 1);
+//  - rotation z -> e^{i phi} z with uniform phi.
+// Together: z_i -> e^{i phi} (Z' + lambda (z_i - Z)), accepted with probability 1.
+// Random numbers: Philox block (walker, sweep_lo, sweep_hi16 | tag 4 << 16, 0xfffffff0) for Z'
+// and phi, and (walker, sweep_lo, sweep_hi16 | tag 5 << 16, attempt) for the Gamma variate.
+// All lanes hold the same butterfly sums and draw the same numbers, so they agree on lambda.
 // ---------------------------------------------------------------------------
 template <typename Real>
 __global__ __launch_bounds__(64) void k_symmetry(SweepArgs a)
@@ -257,27 +268,50 @@ __global__ __launch_bounds__(64) void k_symmetry(SweepArgs a)
     const int lane = threadIdx.x;
     const int N = a.N;
     V2* P = reinterpret_cast<V2*>(a.pos) + (size_t)walker * N;
-    double sx = 0.0, sy = 0.0;
+    double sx = 0.0, sy = 0.0, sr = 0.0;
     for (int j = lane; j < N; j += 64) {
         const V2 q = P[j];
-        sx += (double)q.x;
-        sy += (double)q.y;
+        const double x = (double)q.x, y = (double)q.y;
+        sx += x;
+        sy += y;
+        sr += x * x + y * y;
     }
     sx = group_sum<64, double>(sx);
     sy = group_sum<64, double>(sy);
+    sr = group_sum<64, double>(sr);
+    const double cx = sx / N, cy = sy / N;
+    const double W = sr - (sx * cx + sy * cy);
     const unsigned long long sidx = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    const uint32_t c1 = (uint32_t)sidx, c2 = (uint32_t)((sidx >> 32) & 0xffffu);
     uint32_t r[4];
-    philox((uint32_t)walker, (uint32_t)sidx, (uint32_t)((sidx >> 32) & 0xffffu) | (4u << 16), 0xfffffff0u,
-           (uint32_t)a.seed, (uint32_t)(a.seed >> 32), r);
+    philox((uint32_t)walker, c1, c2 | (4u << 16), 0xfffffff0u, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), r);
     double g1, g2;
     gauss2(r[0], r[1], g1, g2);
     const double sd = 1.0 / sqrt((double)N);
-    const double dx = g1 * sd - sx / N, dy = g2 * sd - sy / N;     // Z -> Z'
+    const double zx = g1 * sd, zy = g2 * sd;                        // Z'
     double sphi, cphi;
     sincos(6.283185307179586 * u01d(r[2]), &sphi, &cphi);
+    double lam = 1.0;
+    if (a.dilate && N >= 2 && W > 0.0) {
+        const double shape = 0.5 * a.m * (double)N * (N - 1) + (N - 1);
+        const double d = shape - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+        for (uint32_t att = 0; att < 32; ++att) {
+            philox((uint32_t)walker, c1, c2 | (5u << 16), att, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), r);
+            double x, unused;
+            gauss2(r[0], r[1], x, unused);
+            double v = 1.0 + c * x;
+            if (v <= 0.0) continue;
+            v = v * v * v;
+            const double u = u01d(r[2]);
+            if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) {
+                lam = sqrt(2.0 * d * v / W);
+                break;
+            }
+        }
+    }
     for (int j = lane; j < N; j += 64) {
         const V2 q = P[j];
-        const double x = (double)q.x + dx, y = (double)q.y + dy;
+        const double x = zx + lam * ((double)q.x - cx), y = zy + lam * ((double)q.y - cy);
         V2 o;
         o.x = (Real)(cphi * x - sphi * y);
         o.y = (Real)(sphi * x + cphi * y);
@@ -529,10 +563,23 @@ constexpr int kKC = 32;          // samples staged per step
 constexpr int kLdsStride = 80;   // padded LDS row (elements)
 constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 
+// Gram tiles C_q += X_i^T diag(w_q) X_j of one 64x64 upper tile (i0, j0) over one row split,
+// for the NW weights at once (w_0 = 1 for S, w_1 = V for H); 8 waves per tile.  Wave w owns
+// rows 32 (w / 4) .. +31 and columns 16 (w % 4) .. +15 as two 16x16 MFMA blocks per weight
+// (v_mfma_f32_16x16x4_f32, or _f64 in the FP64 form), and both weights share the column
+// operand.  Samples are staged 32 at a time through LDS (row stride 80 elements): threads
+// 0..255 load the row side's 32 x 64 block, threads 256..511 the column side's (idle on
+// diagonal tiles), 8 elements each, one step ahead in registers while the MFMAs run.  FP32
+// accumulators are flushed into FP64 registers every 256 samples; the workgroup's FP64
+// result is added to the global accumulator with FP64 atomics.  Small per-wave blocks keep
+// the FP32 form at 80 VGPRs (6 waves per SIMD), which hides the staging barriers and the load
+// latency: N = 200, dL 1..16, 131072 rows run at 0.87 of the FP32 MFMA peak in isolation,
+// against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).
 template <typename Real, int NW>
-__global__ __launch_bounds__(256) void k_gram(GramArgs g)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_gram(GramArgs g)
 {
     using acc_t = typename Mfma<Real>::acc_t;
+    constexpr bool kF32 = sizeof(Real) == 4;
     __shared__ __align__(16) Real xi[kKC][kLdsStride];
     __shared__ __align__(16) Real xj[kKC][kLdsStride];
     __shared__ Real vs[NW][kKC];
@@ -547,38 +594,38 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wi = wave >> 1, wj = wave & 1;
+    const int wi = wave >> 2, wj = wave & 3;
     const int l16 = lane & 15, k4 = lane >> 4;
 
-    acc_t acc[NW][2][2];
-    double sh[NW][2][2][4];
+    acc_t acc[NW][2];
+    double sh[NW][2][4];
 #pragma unroll
     for (int q = 0; q < NW; ++q)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) {
+            acc[q][a] = acc_t{0, 0, 0, 0};
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc[q][a][b] = acc_t{0, 0, 0, 0};
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sh[q][a][b][r] = 0.0;
-            }
+            for (int r = 0; r < 4; ++r) sh[q][a][r] = 0.0;
+        }
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
     const Real* wp[2] = {reinterpret_cast<const Real*>(g.w0), reinterpret_cast<const Real*>(g.w1)};
-    // staging: 32 rows x 64 columns per side; 256 threads x 8 elements.  The next step's
-    // rows are loaded into registers while the MFMAs run on the current LDS tile.
-    const int rr = tid >> 3;           // 0..31
-    const int cc = (tid & 7) * 8;      // 0..56
-    Real bufi[8], bufj[8], bufw = 0;
+    const int side = tid >> 8;                 // 0: row-side block, 1: column-side block
+    const int rr = (tid & 255) >> 3;           // 0..31
+    const int cc = (tid & 7) * 8;              // 0..56
+    const bool loader = !(side && same);
+    const long col0 = side ? tile.j0 : tile.i0;
+    Real(*dst)[kLdsStride] = side ? xj : xi;
+    Real buf[8], bufw = 0;
     auto fetch = [&](long r0) {
         const long row = r0 + rr;
-        if (row < r_end) {
-            const Real* src = X + (size_t)row * g.ldx;
-            load8(src + tile.i0 + cc, bufi);
-            if (!same) load8(src + tile.j0 + cc, bufj);
-        } else {
+        if (loader) {
+            if (row < r_end) {
+                load8(X + (size_t)row * g.ldx + col0 + cc, buf);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) { bufi[e] = 0; bufj[e] = 0; }
+                for (int e = 0; e < 8; ++e) buf[e] = 0;
+            }
         }
         if (tid < NW * kKC) {
             const int q = tid / kKC, k = tid % kKC;
@@ -589,50 +636,39 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
     if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
-        {
-            __syncthreads();   // previous step's readers are done
+        __syncthreads();   // previous step's readers are done
+        if (loader) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) xi[rr][cc + e] = bufi[e];
-            if (!same) {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) xj[rr][cc + e] = bufj[e];
-            }
-            if (tid < NW * kKC) vs[tid / kKC][tid % kKC] = bufw;
-            __syncthreads();
-            if (r0 + kKC < r_end) fetch(r0 + kKC);
+            for (int e = 0; e < 8; ++e) dst[rr][cc + e] = buf[e];
         }
+        if (tid < NW * kKC) vs[tid / kKC][tid % kKC] = bufw;
+        __syncthreads();
+        if (r0 + kKC < r_end) fetch(r0 + kKC);
         const Real(*yj)[kLdsStride] = same ? xi : xj;
 #pragma unroll
         for (int kk = 0; kk < kKC; kk += 4) {
             const int kr = kk + k4;
-            Real av[2], bv[2];
+            Real av[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) av[a] = xi[kr][wi * 32 + a * 16 + l16];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = yj[kr][wj * 32 + b * 16 + l16];
+            const Real bv = yj[kr][wj * 16 + l16];
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 const Real v = vs[q][kr];
 #pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const Real aw = av[a] * v;
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) acc[q][a][b] = Mfma<Real>::op(aw, bv[b], acc[q][a][b]);
-                }
+                for (int a = 0; a < 2; ++a) acc[q][a] = Mfma<Real>::op(av[a] * v, bv, acc[q][a]);
             }
         }
-        if (sizeof(Real) == 4 && ++steps == kFlushSteps) {
+        if (kF32 && ++steps == kFlushSteps) {
             steps = 0;
 #pragma unroll
             for (int q = 0; q < NW; ++q)
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < 2; ++a) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) sh[q][a][b][r] += (double)acc[q][a][b][r];
-                        acc[q][a][b] = acc_t{0, 0, 0, 0};
-                    }
+                    for (int r = 0; r < 4; ++r) sh[q][a][r] += (double)acc[q][a][r];
+                    acc[q][a] = acc_t{0, 0, 0, 0};
+                }
         }
     }
     double* Cs[2] = {g.C0, g.C1};
@@ -641,16 +677,14 @@ __global__ __launch_bounds__(256) void k_gram(GramArgs g)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const double v = sh[q][a][b][r] + (double)acc[q][a][b][r];
-                    // f32 16x16x4: reg r of lane l is row 4 (l/16) + r; f64 16x16x4: row 4 r + l/16
-                    const int orow = (sizeof(Real) == 4) ? (4 * k4 + r) : (4 * r + k4);
-                    const long row = tile.i0 + wi * 32 + a * 16 + orow;
-                    const long col = tile.j0 + wj * 32 + b * 16 + l16;
-                    atomicAdd(&Cs[q][row * g.ldc + col], v);
-                }
+            for (int r = 0; r < 4; ++r) {
+                const double v = sh[q][a][r] + (double)acc[q][a][r];
+                // f32 16x16x4: reg r of lane l is row 4 (l / 16) + r; f64 16x16x4: row 4 r + l / 16
+                const int orow = kF32 ? (4 * k4 + r) : (4 * r + k4);
+                const long row = tile.i0 + wi * 32 + a * 16 + orow;
+                const long col = tile.j0 + wj * 16 + l16;
+                atomicAdd(&Cs[q][row * g.ldc + col], v);
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -681,7 +715,7 @@ __device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo)
 }
 
 template <int NW>
-__global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
+__global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
 {
     __shared__ __align__(16) __bf16 ai[NW][2][64][kLdsB];   // row side: weight q, hi/lo
     __shared__ __align__(16) __bf16 bj[2][64][kLdsB];       // column side: hi/lo
@@ -695,38 +729,39 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wi = wave >> 1, wj = wave & 1;
+    const int wi = wave >> 2, wj = wave & 3;     // rows 32 wi .. +31, columns 16 wj .. +15
     const int l16 = lane & 15, k4 = lane >> 4;
 
-    floatx4 acc[NW][2][2];
-    double sh[NW][2][2][4];
+    floatx4 acc[NW][2];
+    double sh[NW][2][4];
 #pragma unroll
     for (int q = 0; q < NW; ++q)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) {
+            acc[q][a] = floatx4{0, 0, 0, 0};
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc[q][a][b] = floatx4{0, 0, 0, 0};
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sh[q][a][b][r] = 0.0;
-            }
+            for (int r = 0; r < 4; ++r) sh[q][a][r] = 0.0;
+        }
 
     const float* X = reinterpret_cast<const float*>(g.X);
     const float* wp[2] = {reinterpret_cast<const float*>(g.w0), reinterpret_cast<const float*>(g.w1)};
-    // staging: thread = (column lane, 8 consecutive samples 8 wave .. +7); the next step's
-    // values are loaded into registers while the MFMAs run on the current LDS tile
-    const int sc = lane, sk = wave * 8;
-    float xi[8], xj[8], wv[NW][8];
+    // staging: waves 0..3 take the row side (times each weight), waves 4..7 the column side;
+    // thread = (column lane, 8 consecutive samples 8 (wave % 4) .. +7), loaded into registers
+    // one step ahead while the MFMAs run on the current LDS tile
+    const int side = wave >> 2;
+    const int sc = lane, sk = (wave & 3) * 8;
+    const long col = (side ? tile.j0 : tile.i0) + sc;
+    float x[8], wv[NW][8];
     auto fetch = [&](long r0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const long row = r0 + sk + e;
             const bool in = row < r_end;
-            const float* src = X + (size_t)(in ? row : r_begin) * g.ldx;
-            xi[e] = in ? src[tile.i0 + sc] : 0.f;
-            xj[e] = in ? src[tile.j0 + sc] : 0.f;
+            x[e] = in ? X[(size_t)row * g.ldx + col] : 0.f;
+            if (!side) {
 #pragma unroll
-            for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
+                for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
+            }
         }
     };
     if (r_begin < r_end) fetch(r_begin);
@@ -734,28 +769,27 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         bf16x8 h, l;
         __syncthreads();   // the previous step's fragments have been read
+        if (!side) {
 #pragma unroll
-        for (int q = 0; q < NW; ++q) {
-            float y[8];
+            for (int q = 0; q < NW; ++q) {
+                float y[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) y[e] = xi[e] * wv[q][e];
-            split8(y, h, l);
-            *reinterpret_cast<bf16x8*>(&ai[q][0][sc][sk]) = h;
-            *reinterpret_cast<bf16x8*>(&ai[q][1][sc][sk]) = l;
+                for (int e = 0; e < 8; ++e) y[e] = x[e] * wv[q][e];
+                split8(y, h, l);
+                *reinterpret_cast<bf16x8*>(&ai[q][0][sc][sk]) = h;
+                *reinterpret_cast<bf16x8*>(&ai[q][1][sc][sk]) = l;
+            }
+        } else {
+            split8(x, h, l);
+            *reinterpret_cast<bf16x8*>(&bj[0][sc][sk]) = h;
+            *reinterpret_cast<bf16x8*>(&bj[1][sc][sk]) = l;
         }
-        split8(xj, h, l);
-        *reinterpret_cast<bf16x8*>(&bj[0][sc][sk]) = h;
-        *reinterpret_cast<bf16x8*>(&bj[1][sc][sk]) = l;
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
 
-        bf16x8 bh[2], bl[2];
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int c = wj * 32 + b * 16 + l16;
-            bh[b] = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
-            bl[b] = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
-        }
+        const int c = wj * 16 + l16;
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
 #pragma unroll
         for (int q = 0; q < NW; ++q)
 #pragma unroll
@@ -763,27 +797,22 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
                 const int rr = wi * 32 + a * 16 + l16;
                 const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ai[q][0][rr][8 * k4]);
                 const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ai[q][1][rr][8 * k4]);
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    floatx4 c = acc[q][a][b];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], c, 0, 0, 0);
-                    acc[q][a][b] = c;
-                }
+                floatx4 t = acc[q][a];
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, t, 0, 0, 0);
+                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, t, 0, 0, 0);
+                acc[q][a] = t;
             }
         if (++steps == kFlushSteps) {
             steps = 0;
 #pragma unroll
             for (int q = 0; q < NW; ++q)
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < 2; ++a) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) sh[q][a][b][r] += (double)acc[q][a][b][r];
-                        acc[q][a][b] = floatx4{0, 0, 0, 0};
-                    }
+                    for (int r = 0; r < 4; ++r) sh[q][a][r] += (double)acc[q][a][r];
+                    acc[q][a] = floatx4{0, 0, 0, 0};
+                }
         }
     }
     double* Cs[2] = {g.C0, g.C1};
@@ -792,14 +821,12 @@ __global__ __launch_bounds__(256) void k_gram_bf16(GramArgs g)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const double v = sh[q][a][b][r] + (double)acc[q][a][b][r];
-                    const long row = tile.i0 + wi * 32 + a * 16 + 4 * k4 + r;
-                    const long col = tile.j0 + wj * 32 + b * 16 + l16;
-                    atomicAdd(&Cs[q][row * g.ldc + col], v);
-                }
+            for (int r = 0; r < 4; ++r) {
+                const double v = sh[q][a][r] + (double)acc[q][a][r];
+                const long row = tile.i0 + wi * 32 + a * 16 + 4 * k4 + r;
+                const long cl = tile.j0 + wj * 16 + l16;
+                atomicAdd(&Cs[q][row * g.ldc + cl], v);
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -1380,14 +1407,14 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
     dim3 grid((unsigned)((long)a.ntiles * splits));
     if (prec == kBF16X3) {
         if (a.rows_per_split % kKC) return hipErrorInvalidValue;
-        if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_gram_bf16<1>), grid, dim3(256), 0, s, a);
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_gram_bf16<1>), grid, dim3(512), 0, s, a);
     } else if (prec == kFP64) {
-        if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_gram<double, 1>), grid, dim3(256), 0, s, a);
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_gram<double, 1>), grid, dim3(512), 0, s, a);
     } else {
-        if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_gram<float, 1>), grid, dim3(256), 0, s, a);
+        if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2>), grid, dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_gram<float, 1>), grid, dim3(512), 0, s, a);
     }
     return hipGetLastError();
 }

@@ -21,6 +21,7 @@ struct SweepArgs {
     int N, m, walkers, nsweeps;
     double step;
     const unsigned long long* sweep_base;   // optional device value added to sweep0 (graph replays)
+    int dilate;              // k_symmetry: also redraw the scale of the relative coordinates
 };
 
 constexpr int kMaxHarmonics = 8;

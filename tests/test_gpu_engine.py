@@ -166,7 +166,7 @@ def test_geneig_vectors_large_and_clustered():
     assert np.allclose(ev, np.sort(lam), atol=1e-12)
 
 
-@pytest.mark.parametrize("sym", [0, 1])
+@pytest.mark.parametrize("sym", [0, 1, 2])
 def test_sample_graphs_match_eager(sym):
     """Sampling rounds replayed from hipGraphs give exactly the eager rounds' moments, traces
     and walkers: several replays per buffer, a partial (eager) round, then a thinning change
@@ -199,10 +199,13 @@ def test_sample_graphs_match_eager(sym):
             assert np.array_equal(S0, S1) and np.array_equal(H0, H1)
 
 
-@pytest.mark.parametrize("sym", [0, 1])
+@pytest.mark.parametrize("sym", [0, 1, 2])
 def test_sampler_r2_sum_rule(sym):
     """<sum_i r_i^2> = 2 (L0 + N) for |Psi_L|^2 (exact LLL identity), FP32 sampler; the
-    centre of mass Z is Gaussian with <N |Z|^2> = 2 (with or without its heat bath)."""
+    centre of mass Z is Gaussian with <N |Z|^2> = 2 (with or without its heat bath).  Since
+    the Jastrow factor is homogeneous of degree 2 L0, sum_i r_i^2 / 2 ~ Gamma(L0 + N) exactly:
+    the walkers' sum r^2 has mean 2 (L0 + N) and variance 4 (L0 + N) (a dilation move with the
+    wrong Gamma shape moves the mean by 2 per unit of shape, > 3 sigma here)."""
     from laughlinedge_amd import Engine
     N, m = 20, 3
     L0 = m * N * (N - 1) // 2
@@ -222,6 +225,11 @@ def test_sampler_r2_sum_rule(sym):
     z = pos[..., 0] + 1j * pos[..., 1]
     nz2 = N * np.abs(z.mean(axis=1)) ** 2
     assert abs(nz2.mean() - 2.0) < 5 * nz2.std() / np.sqrt(len(nz2)), nz2.mean()
+    r2 = np.sum(np.abs(z) ** 2, axis=1)
+    n = len(r2)
+    mu, var = 2.0 * (L0 + N), 4.0 * (L0 + N)
+    assert abs(r2.mean() - mu) < 5 * np.sqrt(var / n), (r2.mean(), mu)
+    assert abs(r2.var(ddof=1) / var - 1.0) < 5 * np.sqrt(2.0 / n), (r2.var(ddof=1), var)
 
 
 def test_harmonic_cross_blocks(oracle_lib):
@@ -522,11 +530,12 @@ def test_geneig_golden():
         assert np.allclose(ev, g[f"E{k}"], rtol=1e-10, atol=1e-10), k
 
 
-@pytest.mark.parametrize("prec,sym", [(0, 0), (2, 0), (0, 1)])
+@pytest.mark.parametrize("prec,sym", [(0, 0), (2, 0), (0, 1), (0, 2)])
 def test_sampled_blocks_vs_exact_n4(prec, sym):
     """N = 4, m = 3, dL = 0..4: GPU chain + features + Gram against the exact matrices of
     the committed fixture, within 5 batch sigma per entry, and the spectra within 5 sigma;
-    also with the centre-of-mass heat bath and random rotations after every kept sample."""
+    also with the centre-of-mass heat bath and random rotations (and the dilation heat bath)
+    after every kept sample."""
     from laughlinedge_amd import Engine
     from laughlinedge_amd.stats import batch_sigma
     g = np.load(GOLDEN + "/exact_n4_m3.npz")

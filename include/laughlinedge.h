@@ -145,6 +145,9 @@ typedef struct le_run_stats {
     long long gram_launches;
     long long sweep_launches;
     double ms_sampler_graph;   /* device time of replayed sampling graphs (sweeps + features) */
+    double ms_gram_launch_min; /* shortest full-size Gram launch (samples_per_launch x walkers rows)
+                                  of le_sample so far, 0 before the first; the last launch of each
+                                  call runs with no sampling beside it, so this is the kernel alone */
 } le_run_stats;
 
 void le_mc_params_default(le_mc_params* p);

@@ -569,7 +569,9 @@ constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 // (v_mfma_f32_16x16x4_f32, or _f64 in the FP64 form), and both weights share the column
 // operand.  Samples are staged 32 at a time through LDS (row stride 80 elements): threads
 // 0..255 load the row side's 32 x 64 block, threads 256..511 the column side's (idle on
-// diagonal tiles), 8 elements each, one step ahead in registers while the MFMAs run.  FP32
+// diagonal tiles), 8 elements each, one step ahead in registers while the MFMAs run.  In the
+// FP32 form the row side is also stored times each weight other than 1, so the MFMA loop issues
+// no multiplies (VALU slots it would take from a sampler sweep running beside it).  FP32
 // accumulators are flushed into FP64 registers every 256 samples; the workgroup's FP64
 // result is added to the global accumulator with FP64 atomics.  Small per-wave blocks keep
 // the FP32 form at 80 VGPRs (6 waves per SIMD), which hides the staging barriers and the load
@@ -580,9 +582,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
 {
     using acc_t = typename Mfma<Real>::acc_t;
     constexpr bool kF32 = sizeof(Real) == 4;
+    // FP32: the row side is also staged times each weight other than 1; FP64 multiplies in the
+    // loop instead (its weighted copies would double the LDS and cut the resident workgroups)
+    constexpr bool kPre = kF32;
     __shared__ __align__(16) Real xi[kKC][kLdsStride];
     __shared__ __align__(16) Real xj[kKC][kLdsStride];
-    __shared__ Real vs[NW][kKC];
+    __shared__ __align__(16) Real xw[kPre ? NW : 1][kPre ? kKC : 1][kPre ? kLdsStride : 1];
+    __shared__ Real vs[kPre ? 1 : NW][kKC];
 
     int tile_idx, split;
     if (!gram_block(g, tile_idx, split)) return;
@@ -610,27 +616,38 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
     const Real* wp[2] = {reinterpret_cast<const Real*>(g.w0), reinterpret_cast<const Real*>(g.w1)};
+    // FP32 operand A of weight q: the row side itself when the weight is 1, else its weighted copy
+    const Real(*aq[2])[kLdsStride];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+        aq[q] = (q < NW && wp[q]) ? reinterpret_cast<const Real(*)[kLdsStride]>(&xw[q < NW ? q : 0][0][0]) : xi;
     const int side = tid >> 8;                 // 0: row-side block, 1: column-side block
     const int rr = (tid & 255) >> 3;           // 0..31
     const int cc = (tid & 7) * 8;              // 0..56
     const bool loader = !(side && same);
     const long col0 = side ? tile.j0 : tile.i0;
     Real(*dst)[kLdsStride] = side ? xj : xi;
-    Real buf[8], bufw = 0;
+    Real buf[8], bw[NW];
     auto fetch = [&](long r0) {
         const long row = r0 + rr;
+        const bool in = row < r_end;
         if (loader) {
-            if (row < r_end) {
+            if (in) {
                 load8(X + (size_t)row * g.ldx + col0 + cc, buf);
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) buf[e] = 0;
             }
         }
-        if (tid < NW * kKC) {
+        if constexpr (kPre) {
+            if (!side) {
+#pragma unroll
+                for (int q = 0; q < NW; ++q) bw[q] = (in && wp[q]) ? wp[q][row] : (Real)0;
+            }
+        } else if (tid < NW * kKC) {
             const int q = tid / kKC, k = tid % kKC;
             const long row2 = r0 + k;
-            bufw = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
+            bw[0] = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
         }
     };
     if (r_begin < r_end) fetch(r_begin);
@@ -641,22 +658,37 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
 #pragma unroll
             for (int e = 0; e < 8; ++e) dst[rr][cc + e] = buf[e];
         }
-        if (tid < NW * kKC) vs[tid / kKC][tid % kKC] = bufw;
+        if constexpr (kPre) {
+            if (!side) {
+#pragma unroll
+                for (int q = 0; q < NW; ++q)
+                    if (wp[q]) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) xw[q][rr][cc + e] = buf[e] * bw[q];
+                    }
+            }
+        } else if (tid < NW * kKC) {
+            vs[tid / kKC][tid % kKC] = bw[0];
+        }
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
         const Real(*yj)[kLdsStride] = same ? xi : xj;
 #pragma unroll
         for (int kk = 0; kk < kKC; kk += 4) {
             const int kr = kk + k4;
-            Real av[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) av[a] = xi[kr][wi * 32 + a * 16 + l16];
             const Real bv = yj[kr][wj * 16 + l16];
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
-                const Real v = vs[q][kr];
+                if constexpr (kPre) {
 #pragma unroll
-                for (int a = 0; a < 2; ++a) acc[q][a] = Mfma<Real>::op(av[a] * v, bv, acc[q][a]);
+                    for (int a = 0; a < 2; ++a)
+                        acc[q][a] = Mfma<Real>::op(aq[q][kr][wi * 32 + a * 16 + l16], bv, acc[q][a]);
+                } else {
+                    const Real v = vs[q][kr];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+                        acc[q][a] = Mfma<Real>::op(xi[kr][wi * 32 + a * 16 + l16] * v, bv, acc[q][a]);
+                }
             }
         }
         if (kF32 && ++steps == kFlushSteps) {

@@ -109,7 +109,7 @@ class RunStats(ctypes.Structure):
                 ("ms_sweep", ctypes.c_double), ("ms_features", ctypes.c_double),
                 ("ms_gram", ctypes.c_double), ("ms_sample_total", ctypes.c_double),
                 ("gram_launches", ctypes.c_longlong), ("sweep_launches", ctypes.c_longlong),
-                ("ms_sampler_graph", ctypes.c_double)]
+                ("ms_sampler_graph", ctypes.c_double), ("ms_gram_launch_min", ctypes.c_double)]
 
 
 _lib = None

@@ -34,18 +34,22 @@ def test_mlp_gradient_vs_oracle():
     assert np.linalg.norm(g - G) < 2e-3 * np.linalg.norm(G), np.linalg.norm(g - G) / np.linalg.norm(G)
 
 
-def test_learned_chain_is_exact():
+@pytest.mark.parametrize("sym", [0, 2])
+def test_learned_chain_is_exact(sym):
     """MH with the trained, state-dependent widths keeps |Psi_L|^2: the sum rule
-    <sum r^2> = 2 (L0 + N) and the dL = 2..3 spectra agree with the plain chain."""
+    <sum r^2> = 2 (L0 + N) and the dL = 2..3 spectra agree with the plain chain; with the
+    symmetry moves (centre of mass, dilation, rotation) after every kept sample as well, and
+    then the walkers' sum r^2 also follows its exact law (sum r^2 / 2 ~ Gamma(L0 + N))."""
     from laughlinedge_amd import Engine
     from laughlinedge_amd.stats import batch_sigma
     N, m = 16, 3
     L0 = m * N * (N - 1) // 2
-    print("case learned exactness N=16", flush=True)
+    print(f"case learned exactness N=16 symmetry={sym}", flush=True)
     res = {}
     for kind in ("plain", "learned"):
         with Engine(N, m, [0, 2, 3], w=2.0, eps=1e-3, walkers=4096, samples=400, burnin_sweeps=200, seed=31,
-                    tune_sweeps=40, step=0.6, n_blocks=8, lanes_per_walker=8) as e:
+                    tune_sweeps=40, step=0.6, n_blocks=8, lanes_per_walker=8,
+                    symmetry_moves=sym if kind == "learned" else 0) as e:
             e.init_walkers()
             e.tune_step()
             e.burnin(200)
@@ -58,6 +62,12 @@ def test_learned_chain_is_exact():
             r2 = [e.moments(b)[0][1][0, 0].real / 2.0 for b in range(8)]   # H with w = 2 on dL=0: sum r^2 (+eps r^4)
             spec = np.array([np.concatenate(e.spectra(b)[1:]) for b in range(8)])
             res[kind] = (np.array(r2), spec, e.stats()["acceptance"])
+            if kind == "learned":
+                z = e.positions()
+                rw = np.sum(z[..., 0] ** 2 + z[..., 1] ** 2, axis=1)
+                mu, var, n = 2.0 * (L0 + N), 4.0 * (L0 + N), len(rw)
+                assert abs(rw.mean() - mu) < 5 * np.sqrt(var / n), (rw.mean(), mu)
+                assert abs(rw.var(ddof=1) / var - 1.0) < 5 * np.sqrt(2.0 / n), (rw.var(ddof=1), var)
     for kind in ("plain", "learned"):
         r2, spec, acc = res[kind]
         assert 0.1 < acc < 0.95

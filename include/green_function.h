@@ -11,6 +11,15 @@
 // OccupationNumbers_{normalized|not_normalized}/fileNumber.tsv.
 // What changes underneath: one chain on |Psi(z_0..z_{N-1})| |Psi(z_1..z_N)| serves all
 // d^2 pairs (the reference launches one chain per pair).
+//
+// spectral_function_compute has no reference implementation (README.md:14-15 "coming soon";
+// the commented-out spectral_function_sampler, modules/green_function.h:18,90): it writes the
+// particle-removal spectral function of the Laughlin state at the edge orbitals,
+// A(l, w) = n_l sum_j |<j, N-1| c_l |Psi_L^N>|^2 / n_l delta(w - (E_0^N - E_j^{N-1})),
+// from three runs: the occupations n_l (two-sided chain, dL = 0), E_0^N (sector 0 of N
+// particles) and the (N-1)-particle sectors dL' = 0..dLmax, orbital l = m (N - 1) - dL'
+// (le_removal_weights).  Rows "l\tj\tomega\tweight" under
+// ../output/N={N}_m={m}/SpectralFunction/fileNumber.tsv.
 #ifndef LE_GREEN_FUNCTION_H
 #define LE_GREEN_FUNCTION_H
 
@@ -94,18 +103,8 @@ public:
 
 namespace detail {
 
-inline void on_compute(monteCarloParameters& params, int dL, const cmatrix<double>* metric, int fileNumber)
+inline le_mc_params mc_params(const monteCarloParameters& params)
 {
-    if (dL > sys_params::ansatzMaxAngularMomentum) {
-        std::cout << "The defined parameter ansatzMaxAngularMomentum is smaller than dL. Modify and re-run "
-                     "system_parameters.cpp"
-                  << std::endl;
-        std::exit(1);
-    }
-    params.injectedAngularMomentum = dL;
-    le_system_params s{};
-    s.particles_number = sys_params::particlesNumber;
-    s.inverse_filling = sys_params::inverseFilling;
     le_mc_params m;
     le_mc_params_default(&m);
     m.walkers = (int)params.gridSize;
@@ -122,14 +121,32 @@ inline void on_compute(monteCarloParameters& params, int dL, const cmatrix<doubl
         m.seed = (unsigned long long)std::rand();
     }
     m.precision = params.precision;
+    return m;
+}
+
+inline void check(int rc)
+{
+    if (rc != LE_OK) {
+        std::cout << "\nlaughlinedge: " << le_last_error() << std::endl;
+        std::exit(1);
+    }
+}
+
+inline void on_compute(monteCarloParameters& params, int dL, const cmatrix<double>* metric, int fileNumber)
+{
+    if (dL > sys_params::ansatzMaxAngularMomentum) {
+        std::cout << "The defined parameter ansatzMaxAngularMomentum is smaller than dL. Modify and re-run "
+                     "system_parameters.cpp"
+                  << std::endl;
+        std::exit(1);
+    }
+    params.injectedAngularMomentum = dL;
+    le_system_params s{};
+    s.particles_number = sys_params::particlesNumber;
+    s.inverse_filling = sys_params::inverseFilling;
+    const le_mc_params m = mc_params(params);
     const int lcut = sys_params::angularMomentumCutoff;
     le_engine* e = nullptr;
-    auto check = [&](int rc) {
-        if (rc != LE_OK) {
-            std::cout << "\nlaughlinedge: " << le_last_error() << std::endl;
-            std::exit(1);
-        }
-    };
     check(le_engine_create_occupation(&s, &m, dL, lcut, &e));
     le_engine_info info;
     check(le_engine_get_info(e, &info));
@@ -193,6 +210,76 @@ inline void occupation_numbers_compute(monteCarloParameters& params, int dL, cma
 inline void occupation_numbers_compute(monteCarloParameters& params, int dL, int fileNumber = 0)
 {
     detail::on_compute(params, dL, nullptr, fileNumber);
+}
+
+// Particle-removal spectral function of the N-particle Laughlin state at the orbitals
+// l = m (N - 1) - dL', dL' = 0..dLmax (dLmax <= ansatzMaxAngularMomentum and <= N - 1).
+inline void spectral_function_compute(monteCarloParameters& params, int dLmax, int fileNumber = 0)
+{
+    const int N = sys_params::particlesNumber, m = sys_params::inverseFilling;
+    if (dLmax < 0 || dLmax > sys_params::ansatzMaxAngularMomentum || dLmax > N - 1) {
+        std::cout << "dLmax must be in [0, min(ansatzMaxAngularMomentum, N - 1)]. Modify and re-run" << std::endl;
+        std::exit(1);
+    }
+    using detail::check;
+    const le_mc_params mc = detail::mc_params(params);
+    // 1. occupations n_l of the Laughlin state (two-sided chain, dL = 0), normalised to sum N
+    const int lcut = m * (N - 1) + 1;
+    le_system_params so{};
+    so.particles_number = N;
+    so.inverse_filling = m;
+    le_engine* e = nullptr;
+    check(le_engine_create_occupation(&so, &mc, 0, lcut, &e));
+    check(le_run(e));
+    std::vector<double> raw(2 * (size_t)lcut);
+    check(le_get_occupation(e, -1, raw.data()));
+    le_run_stats st;
+    check(le_get_stats(e, &st));
+    params.setRandomStep((float)st.step);
+    le_engine_destroy(e);
+    double total = 0.0;
+    for (int l = 0; l < lcut; ++l) total += raw[2 * l];
+    std::vector<double> nl(lcut);
+    for (int l = 0; l < lcut; ++l) nl[l] = raw[2 * l] / total * N;
+    // 2. E_0 of the N-particle Laughlin state (sector 0: S = 1)
+    le_system_params sn{};
+    sn.particles_number = N;
+    sn.inverse_filling = m;
+    sn.conf_w = sys_params::confinementW;
+    sn.conf_eps = sys_params::confinementEps;
+    sn.conf_strength = sys_params::confinementStrength;
+    sn.conf_exponent = sys_params::confinementExponent;
+    const int zero = 0;
+    check(le_engine_create(&sn, &mc, &zero, 1, nullptr, &e));
+    check(le_run(e));
+    double E0 = 0.0;
+    check(le_engine_spectrum(e, 0, -1, &E0));
+    le_engine_destroy(e);
+    // 3. N - 1 particles in the same trap: strength (r / R_N)^e = strength' (r / R_{N-1})^e
+    le_system_params s1 = sn;
+    s1.particles_number = N - 1;
+    s1.conf_strength = sn.conf_strength * std::pow((double)(N - 1) / N, 0.5 * sn.conf_exponent);
+    std::vector<int> sectors(dLmax + 1);
+    for (int k = 0; k <= dLmax; ++k) sectors[k] = k;
+    check(le_engine_create(&s1, &mc, sectors.data(), (int)sectors.size(), nullptr, &e));
+    check(le_run(e));
+    const std::string dir = "../output/N=" + std::to_string(N) + "_m=" + std::to_string(m) + "/SpectralFunction";
+    std::filesystem::create_directories(dir);
+    std::ofstream out(dir + "/" + std::to_string(fileNumber) + ".tsv");
+    char buf[160];
+    std::vector<int> dims(dLmax + 1), offs(dLmax + 2);
+    check(le_sector_dims(e, dims.data(), offs.data()));
+    for (int k = 0; k <= dLmax; ++k) {
+        const int d = dims[k];
+        std::vector<double> E(d), w(d);
+        int l = 0;
+        check(le_removal_weights(e, k, -1, &l, E.data(), w.data()));
+        for (int j = 0; j < d; ++j) {
+            std::snprintf(buf, sizeof buf, "%d\t%d\t%.6f\t%.6e\n", l, j, E0 - E[j], nl[l] * w[j]);
+            out << buf;
+        }
+    }
+    le_engine_destroy(e);
 }
 
 }  // namespace greenFunctionMC

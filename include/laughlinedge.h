@@ -204,6 +204,18 @@ int le_geneig(int d, const double* S, const double* H, double* evals, double* ev
 /* per-sector eigenvalues of the engine's moments (block -1 = mean); evals has d_s entries */
 int le_engine_spectrum(le_engine* e, int sector, int block, double* evals);
 
+/* Particle-removal spectral weights (the spectral function, which the reference plans but does
+ * not implement: README.md:14-15 "coming soon", the commented-out spectral_function_sampler
+ * in modules/green_function.h:18,90).  The engine holds the edge space of N particles; the
+ * droplet of N + 1 particles loses orbital l = m N - dL, dL = sectors[sector] <= N, and
+ *   c_l Psi_L^{N+1}  is proportional to  e^(m)_dL(z_1..z_N) Psi_L^N,
+ * e^(m)_k being the elementary symmetric polynomial of degree k of the multiset {z_j, each m
+ * times}.  On return (block -1 = mean): energies = the sector's ascending generalized
+ * eigenvalues E_j, weights[j] = |<j| c_l Psi>|^2 / |c_l Psi|^2 (they sum to 1), *orbital = l.
+ * Times the occupation n_l of the (N+1)-particle Laughlin state (le_get_occupation) they give
+ * the removal spectral function A(l, w) = n_l sum_j weights[j] delta(w - (E_0^{N+1} - E_j)). */
+int le_removal_weights(le_engine* e, int sector, int block, int* orbital, double* energies, double* weights);
+
 /* Time evolution (timeEvolution::edgeDensityResponse_compute, modules/dynamics/dynamics.cpp
  * :149-309) of C(0) = e_0 under M dC/dt = -i (H + F(t) U_u) C for n_u perturbations at once,
  * FP64 on the GPU.  M and H are D x D complex (only the diagonal blocks of sizes

@@ -1430,6 +1430,84 @@ extern "C" int le_engine_spectra(le_engine* e, int block, double* evals, int* st
     return rc;
 }
 
+// Particle-removal spectral weights of the Laughlin state.  Psi_L^{N+1}(z_1..z_N, z_0) =
+// Psi_L^N(z_1..z_N) prod_j (z_j - z_0)^m, and the z_0^l coefficient of prod_j (z_j - z_0)^m is
+// (-1)^l e_k over the multiset {z_j, each m times}, k = m N - l.  So c_l Psi_L^{N+1} is
+// proportional to e^(m)_k Psi_L^N, a state of this engine's sector dL = k; by Newton's
+// identities e^(m)_k = sum_{lambda |- k} (-1)^(k - len) m^len / z_lambda p_lambda, with
+// z_lambda = prod_i i^(m_i) m_i!.  Its weight on the S-normalized eigenvector v_j is
+// |v_j^H S a|^2 / (a^H S a).
+extern "C" int le_removal_weights(le_engine* e, int sector, int block, int* orbital, double* energies,
+                                  double* weights)
+{
+    if (!e || !energies || !weights || sector < 0 || sector >= e->basis.nsec())
+        return fail(LE_ERR_ARG, "le_removal_weights: bad argument");
+    if (e->two_sided) return fail(LE_ERR_STATE, "le_removal_weights: needs an edge engine, not the occupation chain");
+    const int N = e->sys.particles_number, m = e->sys.inverse_filling;
+    const int k = e->basis.sectors[sector];
+    if (k > N) return fail(LE_ERR_ARG, "le_removal_weights: needs dL <= particles_number");
+    const int d = e->basis.dim(sector);
+    const long be2 = 2 * e->basis.block_elems();
+    std::vector<double> S(be2), H(be2);
+    int rc = le_get_moments(e, block, S.data(), H.data());
+    if (rc) return rc;
+    const long off = 2 * e->basis.block_offset(sector);
+    std::vector<double> Ss(2 * (size_t)d * d), Hs(2 * (size_t)d * d);
+    for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) {
+            const size_t ij = 2 * ((size_t)i * d + j), ji = 2 * ((size_t)j * d + i);
+            Ss[ij] = 0.5 * (S[off + ij] + S[off + ji]);
+            Ss[ij + 1] = 0.5 * (S[off + ij + 1] - S[off + ji + 1]);
+            Hs[ij] = 0.5 * (H[off + ij] + H[off + ji]);
+            Hs[ij + 1] = 0.5 * (H[off + ij + 1] - H[off + ji + 1]);
+        }
+    // Newton coefficients, scaled by their largest magnitude (the weights do not depend on it)
+    std::vector<double> lc(d), sg(d), a(d);
+    double lmax = -1e300;
+    for (int q = 0; q < d; ++q) {
+        const auto& st = e->basis.states[e->basis.sector_offset[sector] + q];
+        int len = 0;
+        double lz = 0.0;
+        for (const auto& pp : st.pairs) {
+            if (pp.number == 0) continue;   // the dL = 0 state is the constant 1
+            len += pp.repetitions;
+            lz += pp.repetitions * std::log((double)pp.number) + std::lgamma(pp.repetitions + 1.0);
+        }
+        lc[q] = len * std::log((double)m) - lz;
+        sg[q] = ((k - len) % 2) ? -1.0 : 1.0;
+        lmax = std::max(lmax, lc[q]);
+    }
+    for (int q = 0; q < d; ++q) a[q] = sg[q] * std::exp(lc[q] - lmax);
+    std::vector<double> vec(2 * (size_t)d * d);
+    rc = geneig_batch({d}, Ss.data(), Hs.data(), energies, vec.data(), nullptr);
+    if (rc) return rc;
+    // S a and a^T S a (a is real)
+    std::vector<double> sa(2 * (size_t)d, 0.0);
+    double asa = 0.0;
+    for (int i = 0; i < d; ++i) {
+        double re = 0, im = 0;
+        for (int j = 0; j < d; ++j) {
+            re += Ss[2 * ((size_t)i * d + j)] * a[j];
+            im += Ss[2 * ((size_t)i * d + j) + 1] * a[j];
+        }
+        sa[2 * i] = re;
+        sa[2 * i + 1] = im;
+        asa += a[i] * re;
+    }
+    if (!(asa > 0)) return fail(LE_ERR_NOT_PD, "le_removal_weights: a^T S a is not positive");
+    for (int j = 0; j < d; ++j) {
+        double re = 0, im = 0;   // v_j^H (S a)
+        for (int i = 0; i < d; ++i) {
+            const double vr = vec[2 * ((size_t)i * d + j)], vi = vec[2 * ((size_t)i * d + j) + 1];
+            re += vr * sa[2 * i] + vi * sa[2 * i + 1];
+            im += vr * sa[2 * i + 1] - vi * sa[2 * i];
+        }
+        weights[j] = (re * re + im * im) / asa;
+    }
+    if (orbital) *orbital = m * N - k;
+    return LE_OK;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 time evolution (interaction picture in the generalized eigenbasis)
 // ---------------------------------------------------------------------------

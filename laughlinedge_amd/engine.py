@@ -34,7 +34,7 @@ EXPORTED = [
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
     "le_evolve", "le_engine_create_occupation", "le_get_occupation", "le_train_params_default",
     "le_train_proposal", "le_set_proposal", "le_get_proposal_params", "le_set_proposal_params",
-    "le_proposal_loss_grad", "le_set_thin",
+    "le_proposal_loss_grad", "le_set_thin", "le_removal_weights",
 ]
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
@@ -335,6 +335,16 @@ class Engine:
             o = self.offsets[s]
             out.append(ev[o:o + d].copy() if st[s] == 0 else None)
         return out
+
+    def removal_weights(self, sector, block=-1):
+        """Particle-removal spectral weights of the (N+1)-particle Laughlin state into this
+        engine's sector: (orbital l, ascending energies E_j, weights summing to 1)."""
+        d = self.dims[sector]
+        ev = np.zeros(d)
+        w = np.zeros(d)
+        orb = ctypes.c_int()
+        _check(lib().le_removal_weights(self._h, int(sector), int(block), ctypes.byref(orb), _dp(ev), _dp(w)))
+        return orb.value, ev, w
 
 
 def basis_describe(N, sectors):

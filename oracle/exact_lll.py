@@ -153,3 +153,38 @@ def exact_occupation(n, m, states, lcut):
             for l in range(lcut):
                 O[i][j][l] = acc[l] / scale
     return O
+
+
+def exact_removal(n, m, l, states):
+    """Particle removal from the n-particle Laughlin state, exactly, for tiny n.
+
+    c_l Psi_L^n, as a polynomial of the remaining n - 1 coordinates, is the z_n^l coefficient
+    of Psi_L^n (the orbital z^l exp(-|z|^2/4) is orthogonal to every other power of z_n).
+    states: (dL, pairs) of ONE (n-1)-particle sector, dL = m (n - 1) - l.  Returns (G, Hq, b, qq)
+    with G = (B_a | B_b), b = (B_a | Q), qq = (Q | Q) for B_a = p_a Psi_L^{n-1} and Q = c_l Psi
+    (raw power sums, one common normalization; only ratios are meaningful), and Hq the matrix
+    of sum_i r_i^2 / 2 + r_i^4 (w = 1, eps = 1) in the same basis."""
+    nm = n - 1
+    psi = laughlin(n, m)
+    q = {}
+    for a, c in psi.items():
+        if a[n - 1] == l:
+            q[a[:nm]] = q.get(a[:nm], 0) + c
+    q = {a: c for a, c in q.items() if c != 0}
+    psi1 = laughlin(nm, m)
+    basis = [_mul(monomial_poly(nm, p), psi1) for _, p in states]
+
+    def dot(p1, p2, v=False):
+        small, big = (p1, p2) if len(p1) < len(p2) else (p2, p1)
+        acc = Fraction(0)
+        for a, c in small.items():
+            c2 = big.get(a)
+            if c2 is not None:
+                acc += c * c2 * _norm(a) * (Fraction(_radial(a, 1, 1)) if v else 1)
+        return acc
+
+    d = len(states)
+    G = [[float(dot(basis[i], basis[j])) for j in range(d)] for i in range(d)]
+    Hq = [[float(dot(basis[i], basis[j], True)) for j in range(d)] for i in range(d)]
+    b = [float(dot(basis[i], q)) for i in range(d)]
+    return G, Hq, b, float(dot(q, q))

@@ -50,6 +50,7 @@ int main()
     gp.seed = 99;
     greenFunctionMC::occupation_numbers_compute(gp, 2, M2, 5);
     greenFunctionMC::occupation_numbers_compute(gp, 2, 5);
+    greenFunctionMC::spectral_function_compute(gp, 3, 6);
     std::printf("DONE\n");
     return 0;
 }

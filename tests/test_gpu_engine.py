@@ -403,6 +403,24 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
     Vl = 0.05 * (l + 1) + 4e-4 * (l + 1) * (l + 2)
     assert np.allclose((O * Vl).sum(-1), H2, rtol=0.02, atol=0.02 * np.abs(H2).max())
     assert (occ_dir / "OccupationNumbers_not_normalized" / "5.tsv").exists()
+    # spectral function (removal, dL' = 0..3 of 9 particles): orbitals 27..24, p(dL') rows each,
+    # per-orbital weight sum = occupation n_l in (0, 1] (fermions), omega at dL' = 0 = E_0^10 - E_0^9
+    rows = [r.split("\t") for r in open(root / "SpectralFunction" / "6.tsv").read().strip().split("\n")]
+    assert all(len(r) == 4 for r in rows) and len(rows) == 1 + 1 + 2 + 3
+    orb = np.array([int(r[0]) for r in rows])
+    om = np.array([float(r[2]) for r in rows])
+    wt = np.array([float(r[3]) for r in rows])
+    assert list(orb) == [27, 26, 25, 25, 24, 24, 24]
+    for l in (27, 26, 25, 24):
+        assert 0 < wt[orb == l].sum() <= 1 + 1e-6, (l, wt[orb == l].sum())
+    assert np.all(wt >= 0) and np.all(np.isfinite(om))
+    e0 = []
+    for n_ in (10, 9):
+        with Engine(n_, 3, [0], w=0.05, eps=1e-4, walkers=2048, samples=400, burnin_sweeps=200, seed=5,
+                    tune_sweeps=100, step=0.8) as e:
+            e.run()
+            e0.append(e.spectra()[0][0])
+    assert abs(om[0] - (e0[0] - e0[1])) < 0.02 * abs(e0[0] - e0[1]), (om[0], e0)
     assert "DONE" in lines
 
 

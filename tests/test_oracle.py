@@ -353,3 +353,32 @@ def test_golden_exact_blocks():
         S0, H0 = exact_blocks(N, m, b.states, w, 0.0)
         L0 = m * N * (N - 1) // 2
         assert np.allclose(H0, w * (L0 + dl + N) * np.array(S0), rtol=1e-12)
+
+
+def test_removal_newton_vs_coefficient_extraction():
+    """Removing orbital l from the 4-particle Laughlin state (m = 3) leaves exactly
+    e_k(M) Psi_L^3, k = 9 - l, M = {z_j, each 3 times}: the z_0^l coefficient of the expanded
+    polynomial equals the Newton-identity combination of power-sum states (b = +-G a and
+    |Q|^2 = a^T G a), so the Newton-route removal weights are the exact ones and sum to 1.
+    A wrong z_lambda, sign or power of m in the coefficients breaks b = +-G a."""
+    from scipy.linalg import eigh
+    from oracle.exact_lll import exact_removal
+    from oracle.partitions import sector_basis
+    from oracle.spectral import newton_coefficients, removal_weights
+    n, m = 4, 3
+    for k in range(0, 4):
+        l = m * (n - 1) - k
+        states = [(k, p) for p in sector_basis(n - 1, k)]
+        G, Hq, b, qq = exact_removal(n, m, l, states)
+        G, Hq, b = np.array(G), np.array(Hq), np.array(b)
+        a = newton_coefficients(k, m, states)
+        ga = G @ a
+        sgn = np.sign(b @ ga)
+        assert np.allclose(b, sgn * ga, rtol=1e-12, atol=1e-12 * np.abs(ga).max()), (k, b, ga)
+        assert abs(qq - a @ ga) <= 1e-12 * qq, k
+        ev, w = removal_weights(G, Hq, a)
+        assert abs(w.sum() - 1) < 1e-12
+        e2, V = eigh(Hq, G)
+        assert np.allclose(w, (V.T @ b) ** 2 / qq, atol=1e-12)
+        if k == 0:
+            assert len(w) == 1 and abs(w[0] - 1) < 1e-15

@@ -28,6 +28,16 @@
  *   le_accumulate_positions, le_get_positions, le_set_positions
  *       no reference counterpart: fixed-sample accumulation and walker state I/O, so the
  *       same configurations can be fed to the FP64 CPU oracle.
+ *   le_engine_create_occupation, le_get_occupation
+ *       greenFunctionMC::occupation_numbers_compute (modules/green_function/
+ *       gf_occupation_numbers.cu:309-562), all pairs from one two-sided chain.
+ *   le_removal_weights
+ *       the spectral function the reference plans but does not implement (README.md:14-15,
+ *       modules/green_function.h:18,90).
+ *   le_evolve
+ *       timeEvolution::edgeDensityResponse_compute (modules/dynamics/dynamics.cpp:149-309).
+ * Each declaration below repeats its reference interface; entry points with no reference
+ * counterpart say so.
  */
 #ifndef LAUGHLINEDGE_H
 #define LAUGHLINEDGE_H

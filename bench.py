@@ -162,7 +162,7 @@ def main():
     eng.init_walkers()
     step = eng.tune_step()
     eng.burnin(args.burnin)
-    log(f"step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} burn-in done")
+    log(f"step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} registers={eng.info.sweep_in_registers} burn-in done")
     if warm > 0:
         eng.sample(warm)
     # timed region: walkers resident in HBM, steps x (sweep + features + Gram), sampler
@@ -215,6 +215,7 @@ def main():
         "config": {"workload": "laughlin-edge N=200 nu=1/3 dL=1..16 grouped (D=914), V=0.025 r^2+5e-5 r^4",
                    "walkers": args.walkers, "seed": SEED, "burnin_sweeps": args.burnin, "thin": 1,
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
+                   "sweep_in_registers": bool(eng.info.sweep_in_registers),
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,
                    "sampler_rounds": "hipGraph replay"},
         "roofline": {"bound": "mfma", "kernel": "k_gram<float>",
@@ -226,7 +227,8 @@ def main():
                      "rows_per_launch": rows_per_launch,
                      "avg_launch_ms": d["ms_gram"] / launches},
         "kernels": {
-            "sampler": {"kernel": f"k_sweep<{eng.info.lanes_per_walker},float>", "bound": "valu",
+            "sampler": {"kernel": f"k_sweep{'_reg' if eng.info.sweep_in_registers else ''}<{eng.info.lanes_per_walker},float>",
+                        "bound": "valu",
                         "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
                         "frac": sweep_rate / PEAK_FP32, "flop_per_walker_sweep": 12 * N * (N - 1),
                         "ms_per_sweep": de["ms_sweep"] / ksteps},

@@ -98,6 +98,10 @@ typedef struct le_mc_params {
                                   the scale of the coordinates relative to the centre of mass from
                                   its exact Gamma law (all exact moves of |Psi_L|^2, edge engine
                                   only); 0 (default): single-particle moves only, as the reference */
+    int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (k_sweep_reg;
+                                  needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25, plain
+                                  proposal, edge engine); 0 (default): LDS.  lanes_per_walker = -1
+                                  times both kinds and keeps the faster */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -141,6 +145,7 @@ typedef struct le_engine_info {
     int samples_per_launch;
     int n_tiles;               /* 64x64 Gram tiles per weight */
     long x_ld;                 /* real columns per sample row */
+    int sweep_in_registers;    /* 1 when the plain sweep keeps particles in registers */
 } le_engine_info;
 
 typedef struct le_run_stats {

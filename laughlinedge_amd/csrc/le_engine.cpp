@@ -60,6 +60,12 @@ hipError_t dalloc(T** p, size_t count)
 // per-lane loops (N = 200 on MI355X: 0.52 ms FP32 at 16 lanes, 1.54 ms FP64 at 8 lanes).
 constexpr int kPsLd = 48;        // Re/Im p_1..p_24 per sample
 
+// Sweep mapping: lanes per walker in the low byte; kRegSweep marks the register-resident
+// kernel (k_sweep_reg) instead of the LDS one.
+constexpr int kRegSweep = 0x100;
+inline int map_lanes(int G) { return G & 0xff; }
+inline bool map_reg(int G) { return (G & kRegSweep) != 0; }
+
 int default_lanes(int N, int prec)
 {
     const int per_lane = prec == LE_PREC_FP64 ? 32 : 16;
@@ -281,6 +287,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->trace_walkers = 0;
     p->graphs = 1;
     p->symmetry_moves = 0;
+    p->sweep_registers = 0;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -522,6 +529,10 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
+    if (mc->sweep_registers != 0 && mc->sweep_registers != 1) return fail(LE_ERR_ARG, "sweep_registers must be 0 or 1");
+    if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
+        !le::sweep_reg_supported(mc->lanes_per_walker, sys->particles_number))
+        return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -557,6 +568,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return rc;
     }
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
+    if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     *out = e;
     return LE_OK;
 }
@@ -597,7 +609,8 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     info->n_sectors = e->basis.nsec();
     info->longest_partition = e->basis.longest;
     info->block_elems = e->basis.block_elems();
-    info->lanes_per_walker = e->G;
+    info->lanes_per_walker = map_lanes(e->G);
+    info->sweep_in_registers = map_reg(e->G) ? 1 : 0;
     info->samples_per_launch = e->spl;
     info->n_tiles = e->ntiles;
     info->x_ld = e->ldx;
@@ -654,7 +667,9 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
             lt.hi[q] = table_hi[q];
         }
         lt.invR = 1.0 / R;
-        HIPCHK(le::launch_sweep_learned(a, lt, std::max(G, 8), stream));
+        HIPCHK(le::launch_sweep_learned(a, lt, std::max(map_lanes(G), 8), stream));
+    } else if (map_reg(G)) {
+        HIPCHK(le::launch_sweep_reg(a, prec, map_lanes(G), stream));
     } else {
         HIPCHK(le::launch_sweep(a, prec, G, stream));
     }
@@ -704,10 +719,16 @@ int le_engine::autotune_lanes()
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    for (int cand : {4, 8, 16, 32, 64}) {
-        if ((N + cand - 1) / cand < 2 && cand > 4) continue;
-        if ((two_sided ? le::sweep2_lds_bytes(prec, cand, N) : le::sweep_lds_bytes(prec, cand, N)) > 160 * 1024)
-            continue;
+    std::vector<int> cands;
+    for (int c : {4, 8, 16, 32, 64}) {
+        if ((N + c - 1) / c < 2 && c > 4) continue;
+        if ((two_sided ? le::sweep2_lds_bytes(prec, c, N) : le::sweep_lds_bytes(prec, c, N)) > 160 * 1024) continue;
+        cands.push_back(c);
+    }
+    if (!two_sided)
+        for (int c : {8, 16})
+            if (le::sweep_reg_supported(c, N)) cands.push_back(c | kRegSweep);
+    for (int cand : cands) {
         G = cand;
         HIPCHK(hipMemcpyAsync(scratch, pos, bytes, hipMemcpyDeviceToDevice, stream));
         int rc = run_sweeps(1, 1ull << 46, scratch);   // warm-up, separate RNG range
@@ -1715,6 +1736,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
+    if (mc->sweep_registers != 0) return fail(LE_ERR_ARG, "sweep_registers is for the edge engine");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;

@@ -30,6 +30,8 @@
 #include <cstdint>
 
 #include "le_device.hpp"
+#include <utility>
+
 #include "le_kernels.hpp"
 
 namespace le {
@@ -63,6 +65,11 @@ __global__ void k_init_positions(typename V2T<Real>::T* pos, int N, int walkers,
 // to numerator and denominator; pad slots j >= N hold the same midpoint.  Products of
 // eight squared distances are taken to log2 at a time (FP32 range), FP64 takes one log.
 // ---------------------------------------------------------------------------
+// explicit fused multiply-add: both sweep kernels spell out every contraction, so their
+// arithmetic (and their chains) do not depend on the compiler's contraction choices
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename Real> struct LogOps;
 template <> struct LogOps<float> {
     static constexpr int chunk = 8;
@@ -137,7 +144,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 const Real d1 = rn.n1, d2 = rn.n2;
                 const uint32_t ub = rn.ub;
                 const float lnu_move = rn.lnu;
-                const Real xn = zo.x + step * d1, yn = zo.y + step * d2;
+                const Real xn = mad(step, d1, zo.x), yn = mad(step, d2, zo.y);
                 V2 mid;
                 mid.x = (Real)0.5 * (zo.x + xn);
                 mid.y = (Real)0.5 * (zo.y + yn);
@@ -155,16 +162,16 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 // FP32: |z' - q|^2 = |b|^2 + |d|^2 + 2 d.b with b = z - q, d = z' - z (9 VALU per
                 // pair instead of 10); FP64 takes both distances directly
                 const Real dx = xn - zo.x, dy = yn - zo.y;
-                const Real dd = dx * dx + dy * dy, tdx = dx + dx, tdy = dy + dy;
+                const Real dd = mad(dx, dx, dy * dy), tdx = dx + dx, tdy = dy + dy;
                 auto pair = [&](const V2 q) {
                     const Real bx = zo.x - q.x, by = zo.y - q.y;
-                    const Real B = bx * bx + by * by;
+                    const Real B = mad(bx, bx, by * by);
                     Real A;
                     if constexpr (sizeof(Real) == 4) {
-                        A = __builtin_fmaf(tdx, bx, __builtin_fmaf(tdy, by, B + dd));
+                        A = mad(tdx, bx, mad(tdy, by, B + dd));
                     } else {
                         const Real ax = xn - q.x, ay = yn - q.y;
-                        A = ax * ax + ay * ay;
+                        A = mad(ax, ax, ay * ay);
                     }
                     pn *= A;
                     pd *= B;
@@ -217,8 +224,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 else
                     lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
                 lsum = group_sum<G, Real>(lsum);
-                const Real logA = mfac * lsum -
-                                  (Real)0.5 * ((xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y));
+                const Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zo.x, zo.x, zo.y * zo.y), mfac * lsum);
                 bool acc;
                 if constexpr (sizeof(Real) == 4) {
                     acc = (logA >= 0.f) || (lnu_move < logA);
@@ -237,6 +243,161 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
         }
     }
     for (int j = lane; j < N; j += G) gpos[j] = P[j];
+    if (lane == 0) a.accepted[walker] += accepted;
+}
+
+// ---------------------------------------------------------------------------
+// The same sweeps with each lane's particles in registers instead of LDS: lane l of a group
+// holds particles j = l + G s, s < S, in px[s], py[s] (S = ceil(N / G) fixed at compile time).
+// Moves run in the order i = G s + t; with the loop over s unrolled, the slot of the moving
+// particle is a compile-time index in its owner lane t, which holds the midpoint there during
+// the pair loop.  The old position and the random numbers of move i come from lane t by
+// __shfl.  Products, chunking, pad slots and the acceptance follow k_sweep, so for the same G
+// the chain is bit-identical to it.  The register file holds more walkers per CU than the LDS,
+// so a launch needs fewer rounds of resident workgroups.
+// ---------------------------------------------------------------------------
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, Is...>, F&& f)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+
+// register budget per lane: positions (2 S values, doubled in FP64) plus ~40 for the move
+template <int G, int S, typename Real>
+constexpr int reg_sweep_waves()
+{
+    return (2 * S * (int)(sizeof(Real) / 4) + 40) <= 64 ? 8 : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 84 ? 6
+         : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 100 ? 5 : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 128 ? 4 : 3;
+}
+
+template <int G, int S, typename Real>
+__global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    constexpr int WPB = 64 / G;
+    constexpr int C = LogOps<Real>::chunk;
+    constexpr int FULL = (S / C) * C;   // slots covered by whole chunks
+    const int N = a.N;                  // (S - 1) G < N <= S G, checked by the launcher
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    if (walker >= a.walkers) return;    // whole groups leave; shuffles stay within a group
+    V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)walker * N;
+    Real px[S], py[S];
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        const V2 q = j < N ? gpos[j] : gpos[0];
+        px[s] = q.x;
+        py[s] = q.y;
+    });
+    const bool pad_last = lane + G * (S - 1) >= N;   // the last slot holds no particle
+
+    const Real step = (Real)a.step;
+    const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
+    const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    unsigned accepted = 0;
+
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = sweep_first + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const int base = G * s;
+            // opaque counter: keeps the compiler from hoisting the sweep-invariant part of
+            // Philox for every slot out of the sweep loop (S x 3 live registers)
+            uint32_t ctr = (uint32_t)(base + lane);
+            asm volatile("" : "+v"(ctr));
+            uint32_t r[4];
+            philox(ctr, c1, c2, (uint32_t)walker, k0, k1, r);
+            Real g1, g2;
+            gauss2(r[0], r[1], g1, g2);
+            float lnu = 0.f;
+            if constexpr (sizeof(Real) == 4) lnu = __builtin_amdgcn_logf(u01f(r[2])) * 0.69314718055994531f;
+            const uint32_t ub = r[2];
+            const int cnt = min(G, N - base);
+            for (int t = 0; t < cnt; ++t) {
+                const Real zox = __shfl(px[s], t, G), zoy = __shfl(py[s], t, G);
+                const Real d1 = __shfl(g1, t, G), d2 = __shfl(g2, t, G);
+                const Real xn = mad(step, d1, zox), yn = mad(step, d2, zoy);
+                V2 mid;
+                mid.x = (Real)0.5 * (zox + xn);
+                mid.y = (Real)0.5 * (zoy + yn);
+                const bool own = lane == t;
+                if (own) {
+                    px[s] = mid.x;
+                    py[s] = mid.y;
+                }
+                Real lsum = 0;
+                Real pn = 1, pd = 1;
+                int ebal = 0;
+                const Real dx = xn - zox, dy = yn - zoy;
+                const Real dd = mad(dx, dx, dy * dy), tdx = dx + dx, tdy = dy + dy;
+                static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    Real qx = px[u], qy = py[u];
+                    if constexpr (u == S - 1) {
+                        if (pad_last) {              // pads hold the midpoint
+                            qx = mid.x;
+                            qy = mid.y;
+                        }
+                    }
+                    const Real bx = zox - qx, by = zoy - qy;
+                    const Real B = mad(bx, bx, by * by);
+                    Real A;
+                    if constexpr (sizeof(Real) == 4) {
+                        A = mad(tdx, bx, mad(tdy, by, B + dd));
+                    } else {
+                        const Real ax = xn - qx, ay = yn - qy;
+                        A = mad(ax, ax, ay * ay);
+                    }
+                    pn *= A;
+                    pd *= B;
+                    if constexpr ((u + 1) % C == 0 && u + 1 <= FULL) {
+                        if constexpr (sizeof(Real) == 4) {
+                            int en, ed;
+                            pn = frexpf(pn, &en);
+                            pd = frexpf(pd, &ed);
+                            ebal += en - ed;
+                        } else {
+                            lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                            pn = 1;
+                            pd = 1;
+                        }
+                    }
+                });
+                if constexpr (sizeof(Real) == 4)
+                    lsum = __builtin_amdgcn_logf(pn) - __builtin_amdgcn_logf(pd) + (float)ebal;
+                else
+                    lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                lsum = group_sum<G, Real>(lsum);
+                const Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zox, zox, zoy * zoy), mfac * lsum);
+                bool acc;
+                if constexpr (sizeof(Real) == 4) {
+                    acc = (logA >= 0.f) || (__shfl(lnu, t, G) < logA);
+                } else {
+                    acc = (logA >= 0.0) || (log(u01d((uint32_t)__shfl((int)ub, t, G))) < logA);
+                }
+                if (own) {
+                    px[s] = acc ? xn : zox;
+                    py[s] = acc ? yn : zoy;
+                }
+                accepted += acc ? 1u : 0u;
+            }
+        });
+    }
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        if (j < N) {
+            V2 q;
+            q.x = px[s];
+            q.y = py[s];
+            gpos[j] = q;
+        }
+    });
     if (lane == 0) a.accepted[walker] += accepted;
 }
 
@@ -1388,6 +1549,41 @@ static hipError_t sweep_dispatch(const SweepArgs& a, int G, hipStream_t s)
     case 64: return sweep_g<64, Real>(a, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Register-resident sweeps: instantiated for G in {8, 16} and S = ceil(N / G) in {13, 25}
+// (N = 97..104, 193..200 at G = 8; N = 193..208, 385..400 at G = 16).
+template <int G, int S, typename Real>
+static hipError_t sweep_reg_gs(const SweepArgs& a, hipStream_t s)
+{
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    hipLaunchKernelGGL((k_sweep_reg<G, S, Real>), grid, dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+bool sweep_reg_supported(int G, int N)
+{
+    const int S = (N + G - 1) / G;
+    return (G == 8 || G == 16) && (S == 13 || S == 25);
+}
+
+template <typename Real>
+static hipError_t sweep_reg_dispatch(const SweepArgs& a, int G, hipStream_t s)
+{
+    const int S = (a.N + G - 1) / G;
+    if (G == 8 && S == 13) return sweep_reg_gs<8, 13, Real>(a, s);
+    if (G == 8 && S == 25) return sweep_reg_gs<8, 25, Real>(a, s);
+    if (G == 16 && S == 13) return sweep_reg_gs<16, 13, Real>(a, s);
+    if (G == 16 && S == 25) return sweep_reg_gs<16, 25, Real>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_reg(const SweepArgs& a, int prec, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    if (!sweep_reg_supported(G, a.N)) return hipErrorInvalidValue;
+    return prec == kFP64 ? sweep_reg_dispatch<double>(a, G, s) : sweep_reg_dispatch<float>(a, G, s);
 }
 
 hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)

@@ -144,6 +144,9 @@ hipError_t launch_set_bases(unsigned long long* bases, unsigned long long sweep_
                             hipStream_t s);
 
 size_t sweep_lds_bytes(int prec, int lanes_per_walker, int N);
+// register-resident sweeps (k_sweep_reg): the (G, ceil(N / G)) pairs that are instantiated
+bool sweep_reg_supported(int lanes_per_walker, int N);
+hipError_t launch_sweep_reg(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 size_t sweep2_lds_bytes(int prec, int lanes_per_walker, int N);
 bool sweep_supported(int lanes_per_walker);
 

@@ -88,7 +88,8 @@ class MCParams(ctypes.Structure):
                 ("acceptance_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
-                ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int)]
+                ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
+                ("sweep_registers", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -100,7 +101,7 @@ class EngineInfo(ctypes.Structure):
     _fields_ = [("D", ctypes.c_int), ("K", ctypes.c_int), ("n_sectors", ctypes.c_int),
                 ("longest_partition", ctypes.c_int), ("block_elems", ctypes.c_long),
                 ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
-                ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long)]
+                ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):

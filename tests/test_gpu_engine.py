@@ -608,3 +608,29 @@ def test_config0_full_run_matches_oracle(oracle_lib):
         eh = np.linalg.norm(Hg - Ho[s]) / np.linalg.norm(Ho[s])
         assert es < 1e-10 and eh < 1e-10, (s, es, eh)
         assert np.allclose(spec[s], or_geneig(So[s], Ho[s]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("prec,N,G", [(0, 200, 8), (0, 200, 16), (0, 100, 8), (1, 200, 16)])
+def test_register_sweep_matches_lds(prec, N, G):
+    """k_sweep_reg (each lane's particles in registers) reproduces k_sweep (particles in LDS)
+    at the same lanes per walker: bit-identical positions and acceptance after burn-in and
+    sampling, and the same moments (to 1e-12: the FP64 atomics of the Gram accumulation add
+    workgroup results in a run-dependent order); 1003 walkers leave the last workgroup partly
+    empty."""
+    from laughlinedge_amd import Engine
+    print(f"case register sweep vs LDS prec={prec} N={N} G={G}", flush=True)
+    out = []
+    for reg in (0, 1):
+        with Engine(N, 3, [1, 2, 3], w=0.05, eps=5e-5, walkers=1003, samples=12, seed=17, tune_sweeps=0,
+                    step=0.9, lanes_per_walker=G, sweep_registers=reg, precision=prec) as e:
+            assert e.info.sweep_in_registers == reg and e.info.lanes_per_walker == G
+            e.init_walkers()
+            e.burnin(20)
+            e.sample(12)
+            out.append((e.positions(), e.stats()["acceptance"], e.moments()))
+    (p0, a0, m0), (p1, a1, m1) = out
+    assert np.array_equal(p0, p1)
+    assert a0 == a1 and 0.2 < a0 < 0.9
+    for (S0, H0), (S1, H1) in zip(m0, m1):
+        assert np.allclose(S0, S1, rtol=1e-12, atol=1e-12 * np.abs(S0).max())
+        assert np.allclose(H0, H1, rtol=1e-12, atol=1e-12 * np.abs(H0).max())

@@ -634,3 +634,24 @@ def test_register_sweep_matches_lds(prec, N, G):
     for (S0, H0), (S1, H1) in zip(m0, m1):
         assert np.allclose(S0, S1, rtol=1e-12, atol=1e-12 * np.abs(S0).max())
         assert np.allclose(H0, H1, rtol=1e-12, atol=1e-12 * np.abs(H0).max())
+
+
+def test_autotuned_sweep_sum_rule():
+    """lanes_per_walker = -1 (the bench's setting): whichever sweep the autotuner keeps (LDS or
+    registers) samples |Psi_L|^2 -- the walkers' sum r^2 has the exact Gamma(L0 + N) law."""
+    from laughlinedge_amd import Engine
+    N, m = 200, 3
+    L0 = m * N * (N - 1) // 2
+    print("case autotuned sweep N=200 sum rule", flush=True)
+    with Engine(N, m, [1], w=0.05, eps=5e-5, walkers=8192, samples=4, seed=21, tune_sweeps=30, step=0.5,
+                lanes_per_walker=-1) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(200)
+        print(f"lanes {e.info.lanes_per_walker} registers {e.info.sweep_in_registers}")
+        assert e.info.lanes_per_walker in (4, 8, 16, 32, 64)
+        z = e.positions()
+    r2 = np.sum(z[..., 0] ** 2 + z[..., 1] ** 2, axis=1)
+    mu, var, n = 2.0 * (L0 + N), 4.0 * (L0 + N), len(r2)
+    assert abs(r2.mean() - mu) < 5 * np.sqrt(var / n), (r2.mean(), mu)
+    assert abs(r2.var(ddof=1) / var - 1.0) < 5 * np.sqrt(2.0 / n), (r2.var(ddof=1), var)

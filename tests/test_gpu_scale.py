@@ -171,3 +171,33 @@ def test_config1_fixed_samples(oracle_lib, prec, tol):
         worst = max(worst, es, eh)
         assert es < tol and eh < tol, (s, es, eh)
     print(f"worst relative Frobenius error {worst:.3e}")
+
+
+def test_dsf_harmonic_blocks_n200(oracle_lib):
+    """DSF blocks at the batched-sector scale: N = 200, m = 3, sectors dL = 0 and 16 (1 + 231
+    states), harmonics L = 8 and 16; 2048 configurations of the GPU chain.  The cos/sin(L theta)
+    weighted blocks between dL = 0 and dL = L match the FP64 oracle (1e-5 of the block scale),
+    and the reverse block is the conjugate transpose."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import W_COS, W_SIN
+    N, m, secs = 200, 3, [0, 8, 16]
+    print("case DSF harmonic blocks N=200 dL=0,8,16 L=8,16", flush=True)
+    with Engine(N, m, secs, w=0.05, eps=5e-5, walkers=2048, samples=1, seed=13, tune_sweeps=30, step=0.5) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(100)
+        pos = e.positions()
+    o = oracle_lib.Oracle(N, m, secs, w=0.05, eps=5e-5)
+    n = pos.shape[0]
+    with Engine(N, m, secs, w=0.05, eps=5e-5, walkers=n, samples=1, harmonics=[8, 16]) as f:
+        f.accumulate_positions(pos)
+        for L, (a, b) in ((8, (0, 1)), (16, (0, 2)), (8, (1, 2))):
+            c, s_ = oracle_lib.harmonic_sums(pos, L)
+            scale = np.sqrt(np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), a, a)) *
+                            np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), b, b))) / n * N
+            for wid, wv in ((W_COS(L), c), (W_SIN(L), s_)):
+                ref = o.accumulate_weighted(pos, wv, a, b) / n
+                got = f.block(wid, a, b)
+                err = np.linalg.norm(got - ref) / scale
+                assert err < 1e-5, (L, a, b, err)
+                assert np.allclose(f.block(wid, b, a), got.conj().T)

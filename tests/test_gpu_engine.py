@@ -655,3 +655,27 @@ def test_autotuned_sweep_sum_rule():
     mu, var, n = 2.0 * (L0 + N), 4.0 * (L0 + N), len(r2)
     assert abs(r2.mean() - mu) < 5 * np.sqrt(var / n), (r2.mean(), mu)
     assert abs(r2.var(ddof=1) / var - 1.0) < 5 * np.sqrt(2.0 / n), (r2.var(ddof=1), var)
+
+
+def test_register_sweep_graphs_match_eager():
+    """Sampling rounds with the register-resident sweep replayed from hipGraphs give exactly
+    the eager rounds' positions, traces and moments (N = 100, 8 lanes per walker, thin 1 and 2)."""
+    from laughlinedge_amd import Engine
+    print("case register sweep graphs vs eager N=100", flush=True)
+    out = []
+    for graphs in (0, 1):
+        with Engine(100, 3, [1, 2, 3, 4], w=0.05, eps=5e-5, walkers=4096, samples=40, seed=31, tune_sweeps=0,
+                    step=0.8, trace_walkers=32, graphs=graphs, lanes_per_walker=8, sweep_registers=1) as e:
+            assert e.info.sweep_in_registers == 1
+            e.init_walkers()
+            e.burnin(10)
+            spl = e.info.samples_per_launch
+            e.sample(2 * spl)
+            e.set_thin(2)
+            e.sample(spl)
+            out.append((e.positions(), e.trace(), e.moments(), e.stats()["ms_sampler_graph"]))
+    (p0, t0, m0, g0), (p1, t1, m1, g1) = out
+    assert g0 == 0 and g1 > 0
+    assert np.array_equal(p0, p1) and np.array_equal(t0, t1)
+    for (S0, H0), (S1, H1) in zip(m0, m1):
+        assert np.allclose(S0, S1, rtol=1e-12, atol=0) and np.allclose(H0, H1, rtol=1e-12, atol=0)

@@ -386,8 +386,41 @@ int le_engine::setup(const le_observables* obs)
                 for (int b = c2 / 64; b <= e2 / 64; ++b)
                     if (a <= b) tl.insert({a, b});
         }
+        // per tile, the 16x16 blocks that hold an entry the extraction reads (one with
+        // row <= column inside a requested block or its transpose), two to a wave within a row
+        // block so that they share the row operand; the Gram kernels compute only these
+        // (N = 200, dL 1..16: 0.76 of the products of whole tiles)
+        auto meets = [](int r0, int r1, int q0, int q1, int rb0, int rb1, int qb0, int qb1) {
+            const int ra = std::max(r0, rb0), rb = std::min(r1, rb1);
+            const int qa = std::max(q0, qb0), qb = std::min(q1, qb1);
+            return ra <= rb && qa <= qb && ra <= qb;
+        };
         std::vector<le::Tile> th;
-        for (auto& pr : tl) th.push_back({pr.first * 64, pr.second * 64});
+        for (auto& pr : tl) {
+            le::Tile t{};
+            t.i0 = pr.first * 64;
+            t.j0 = pr.second * 64;
+            std::vector<uint32_t> act;
+            for (int rb = 0; rb < 4; ++rb) {
+                const size_t first = act.size();
+                for (int cb = 0; cb < 4; ++cb) {
+                    const int r0 = t.i0 + 16 * rb, q0 = t.j0 + 16 * cb;
+                    for (auto& bp : j.blocks) {
+                        const int c1 = 2 * basis.sector_offset[bp.first], e1 = c1 + 2 * basis.dim(bp.first) - 1;
+                        const int c2 = 2 * basis.sector_offset[bp.second], e2 = c2 + 2 * basis.dim(bp.second) - 1;
+                        if (meets(r0, r0 + 15, q0, q0 + 15, c1, e1, c2, e2) ||
+                            meets(r0, r0 + 15, q0, q0 + 15, c2, e2, c1, e1)) {
+                            act.push_back((uint32_t)(rb << 2 | cb));
+                            break;
+                        }
+                    }
+                }
+                if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
+            }
+            t.nwave = (int)act.size() / 2;
+            for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
+            th.push_back(t);
+        }
         j.ntiles = (int)th.size();
         HIPCHK(dalloc(&j.tiles, (size_t)std::max(1, j.ntiles)));
         if (j.ntiles) HIPCHK(hipMemcpy(j.tiles, th.data(), j.ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));

@@ -739,7 +739,7 @@ constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 // latency: N = 200, dL 1..16, 131072 rows run at 0.87 of the FP32 MFMA peak in isolation,
 // against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).
 template <typename Real, int NW>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_gram(GramArgs g)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram(GramArgs g)
 {
     using acc_t = typename Mfma<Real>::acc_t;
     constexpr bool kF32 = sizeof(Real) == 4;
@@ -760,9 +760,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const bool same = tile.i0 == tile.j0;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wi = wave >> 2, wj = wave & 3;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l16 = lane & 15, k4 = lane >> 4;
+    // this wave's two 16x16 blocks: rows ro .. +15, columns co[a] .. +15 within the tile
+    const bool active = wave < tile.nwave;
+    const uint32_t wb = active ? g.tiles[tile_idx].wblk[wave] : 0u;
+    const int ro = (int)(wb >> 2 & 3) * 16;
+    const int co[2] = {(int)(wb & 3) * 16, (int)(wb >> 8 & 3) * 16};
+    const bool wr[2] = {true, !(wb >> 12 & 1)};
 
     acc_t acc[NW][2];
     double sh[NW][2][4];
@@ -834,21 +839,20 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
         const Real(*yj)[kLdsStride] = same ? xi : xj;
+        if (active) {
 #pragma unroll
-        for (int kk = 0; kk < kKC; kk += 4) {
-            const int kr = kk + k4;
-            const Real bv = yj[kr][wj * 16 + l16];
+            for (int kk = 0; kk < kKC; kk += 4) {
+                const int kr = kk + k4;
+                Real bv[2];
 #pragma unroll
-            for (int q = 0; q < NW; ++q) {
-                if constexpr (kPre) {
+                for (int a = 0; a < 2; ++a) bv[a] = yj[kr][co[a] + l16];
 #pragma unroll
-                    for (int a = 0; a < 2; ++a)
-                        acc[q][a] = Mfma<Real>::op(aq[q][kr][wi * 32 + a * 16 + l16], bv, acc[q][a]);
-                } else {
-                    const Real v = vs[q][kr];
+                for (int q = 0; q < NW; ++q) {
+                    Real av;
+                    if constexpr (kPre) av = aq[q][kr][ro + l16];
+                    else av = xi[kr][ro + l16] * vs[q][kr];
 #pragma unroll
-                    for (int a = 0; a < 2; ++a)
-                        acc[q][a] = Mfma<Real>::op(xi[kr][wi * 32 + a * 16 + l16] * v, bv, acc[q][a]);
+                    for (int a = 0; a < 2; ++a) acc[q][a] = Mfma<Real>::op(av, bv[a], acc[q][a]);
                 }
             }
         }
@@ -871,11 +875,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                if (!active || !wr[a]) continue;
                 const double v = sh[q][a][r] + (double)acc[q][a][r];
                 // f32 16x16x4: reg r of lane l is row 4 (l / 16) + r; f64 16x16x4: row 4 r + l / 16
                 const int orow = kF32 ? (4 * k4 + r) : (4 * r + k4);
-                const long row = tile.i0 + wi * 32 + a * 16 + orow;
-                const long col = tile.j0 + wj * 16 + l16;
+                const long row = tile.i0 + ro + orow;
+                const long col = tile.j0 + co[a] + l16;
                 atomicAdd(&Cs[q][row * g.ldc + col], v);
             }
 }
@@ -921,9 +926,14 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     if (r_end > g.rows) r_end = g.rows;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wi = wave >> 2, wj = wave & 3;     // rows 32 wi .. +31, columns 16 wj .. +15
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l16 = lane & 15, k4 = lane >> 4;
+    // this wave's two 16x16 blocks: rows ro .. +15, columns co[a] .. +15 within the tile
+    const bool active = wave < tile.nwave;
+    const uint32_t wb = active ? g.tiles[tile_idx].wblk[wave] : 0u;
+    const int ro = (int)(wb >> 2 & 3) * 16;
+    const int co[2] = {(int)(wb & 3) * 16, (int)(wb >> 8 & 3) * 16};
+    const bool wr[2] = {true, !(wb >> 12 & 1)};
 
     floatx4 acc[NW][2];
     double sh[NW][2][4];
@@ -980,22 +990,25 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
 
-        const int c = wj * 16 + l16;
-        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
+        if (active) {
 #pragma unroll
-        for (int q = 0; q < NW; ++q)
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int rr = wi * 32 + a * 16 + l16;
+            for (int q = 0; q < NW; ++q) {
+                const int rr = ro + l16;
                 const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ai[q][0][rr][8 * k4]);
                 const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ai[q][1][rr][8 * k4]);
-                floatx4 t = acc[q][a];
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, t, 0, 0, 0);
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, t, 0, 0, 0);
-                t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, t, 0, 0, 0);
-                acc[q][a] = t;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int c = co[a] + l16;
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
+                    floatx4 t = acc[q][a];
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, t, 0, 0, 0);
+                    acc[q][a] = t;
+                }
             }
+        }
         if (++steps == kFlushSteps) {
             steps = 0;
 #pragma unroll
@@ -1015,20 +1028,22 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                if (!active || !wr[a]) continue;
                 const double v = sh[q][a][r] + (double)acc[q][a][r];
-                const long row = tile.i0 + wi * 32 + a * 16 + 4 * k4 + r;
-                const long cl = tile.j0 + wj * 16 + l16;
+                const long row = tile.i0 + ro + 4 * k4 + r;
+                const long cl = tile.j0 + co[a] + l16;
                 atomicAdd(&Cs[q][row * g.ldc + cl], v);
             }
 }
 
 // ---------------------------------------------------------------------------
 // FP64 tiles -> complex sector blocks: S_ab = (C[a][b] + C[d+a][d+b]) + i (C[a][d+b] - C[d+a][b])
-// (only upper 64-tiles were accumulated; the lower ones are read by symmetry)
+// (an entry p > q is read as its mirror q < p, so only the upper triangle is ever read and
+// the extracted S and H are exactly Hermitian)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double cget(const double* C, long ldc, long p, long q)
 {
-    return ((p >> 6) <= (q >> 6)) ? C[p * ldc + q] : C[q * ldc + p];
+    return (p <= q) ? C[p * ldc + q] : C[q * ldc + p];
 }
 
 __global__ void k_extract(ExtractArgs e)

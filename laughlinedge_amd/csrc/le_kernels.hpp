@@ -75,8 +75,14 @@ struct Features2Args {
     double invR;
 };
 
+// A 64x64 upper Gram tile and the 16x16 blocks of it that hold requested entries, two per
+// wave and both in one row block: byte a of wblk[w] = (row block << 2) | column block, bit 4
+// set when the block only pads an odd count and is not written back.  Waves w >= nwave take
+// no blocks.
 struct Tile {
-    int i0, j0;              // column origins (multiples of 64) of an upper-triangle tile
+    int i0, j0;              // column origins (multiples of 64)
+    int nwave;
+    uint32_t wblk[8];
 };
 
 // One Gram job: up to two row weights (nullptr = the constant 1) sharing a tile list.

@@ -225,7 +225,15 @@ def main():
                      "algorithmic_flop_per_launch": gram_flop / launches,
                      "algorithmic_bytes_per_launch": rows_per_launch * (8 * info.D + 8),
                      "rows_per_launch": rows_per_launch,
-                     "avg_launch_ms": d["ms_gram"] / launches},
+                     "avg_launch_ms": d["ms_gram"] / launches,
+                     "issued_mfma_flop_per_launch": info.gram_mfma_flop_per_row * rows_per_launch,
+                     "issued_frac": info.gram_mfma_flop_per_row * rows_per_launch
+                     / (d["ms_gram"] / launches * 1e-3) / PEAK_FP32,
+                     "flop_note": "algorithmic = 16 sum d^2 per sample (complex GEMM count over whole "
+                                  "Hermitian blocks, S and H); the kernel computes only the 16x16 blocks "
+                                  "holding upper-triangle entries, issued_mfma_flop_per_launch, so "
+                                  "'achieved' can exceed the issued rate and even the peak; issued_frac "
+                                  "is the MFMA pipe's actual use"},
         "kernels": {
             "sampler": {"kernel": f"k_sweep{'_reg' if eng.info.sweep_in_registers else ''}<{eng.info.lanes_per_walker},float>",
                         "bound": "valu",
@@ -260,6 +268,7 @@ def main():
         iso_rate = 16.0 * sum_d2 * rows_per_launch / (iso_ms * 1e-3)
         out["roofline"]["alone"] = {
             "achieved": iso_rate / 1e12, "frac": iso_rate / PEAK_FP32, "launch_ms": iso_ms,
+            "issued_frac": info.gram_mfma_flop_per_row * rows_per_launch / (iso_ms * 1e-3) / PEAK_FP32,
             "note": "shortest full-size launch of the run (HIP events): the last launch of each sampling "
                     "call runs with no sweeps beside it; 'achieved' above averages the timed pipeline, "
                     "where the Gram stream shares the CUs with the sampler stream's sweeps"}

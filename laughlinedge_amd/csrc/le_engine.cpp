@@ -142,6 +142,7 @@ struct le_engine {
     };
     std::vector<Job> jobs;
     int ntiles = 0;                  // tiles of the S/H job
+    double gram_flop_row = 0;        // MFMA FLOPs issued per sample row over all Gram jobs
     struct Item {
         int wid, s1, s2;
         long off;                    // complex offset in a block record
@@ -418,6 +419,7 @@ int le_engine::setup(const le_observables* obs)
                 if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
             }
             t.nwave = (int)act.size() / 2;
+            gram_flop_row += 2.0 * 16 * 16 * (double)act.size() * j.nw;
             for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
             th.push_back(t);
         }
@@ -647,6 +649,7 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     info->samples_per_launch = e->spl;
     info->n_tiles = e->ntiles;
     info->x_ld = e->ldx;
+    info->gram_mfma_flop_per_row = e->gram_flop_row;
     return LE_OK;
 }
 

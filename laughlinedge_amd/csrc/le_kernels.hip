@@ -725,10 +725,13 @@ constexpr int kLdsStride = 80;   // padded LDS row (elements)
 constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 
 // Gram tiles C_q += X_i^T diag(w_q) X_j of one 64x64 upper tile (i0, j0) over one row split,
-// for the NW weights at once (w_0 = 1 for S, w_1 = V for H); 8 waves per tile.  Wave w owns
-// rows 32 (w / 4) .. +31 and columns 16 (w % 4) .. +15 as two 16x16 MFMA blocks per weight
-// (v_mfma_f32_16x16x4_f32, or _f64 in the FP64 form), and both weights share the column
-// operand.  Samples are staged 32 at a time through LDS (row stride 80 elements): threads
+// for the NW weights at once (w_0 = 1 for S, w_1 = V for H); 8 waves per tile.  Wave w takes
+// the two 16x16 blocks the tile lists for it (one row block, two column blocks; only blocks
+// holding requested entries are listed) with v_mfma_f32_16x16x4_f32, or _f64 in the FP64
+// form: one row operand per weight, one column operand per block.  Waves with no blocks only
+// stage; the MFMA loop is straight-line code behind one wave-uniform branch (a branch per
+// block made the compiler move every MFMA out of line and wait on each LDS read).
+// Samples are staged 32 at a time through LDS (row stride 80 elements): threads
 // 0..255 load the row side's 32 x 64 block, threads 256..511 the column side's (idle on
 // diagonal tiles), 8 elements each, one step ahead in registers while the MFMAs run.  In the
 // FP32 form the row side is also stored times each weight other than 1, so the MFMA loop issues
@@ -736,8 +739,9 @@ constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 // accumulators are flushed into FP64 registers every 256 samples; the workgroup's FP64
 // result is added to the global accumulator with FP64 atomics.  Small per-wave blocks keep
 // the FP32 form at 80 VGPRs (6 waves per SIMD), which hides the staging barriers and the load
-// latency: N = 200, dL 1..16, 131072 rows run at 0.87 of the FP32 MFMA peak in isolation,
-// against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).
+// latency: N = 200, dL 1..16, 131072 rows of whole tiles ran at 0.87 of the FP32 MFMA peak in
+// isolation, against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).  The FP64 form
+// targets 5 waves per SIMD: at 6 its two column operands spilled inside the loop.
 template <typename Real, int NW>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram(GramArgs g)
 {

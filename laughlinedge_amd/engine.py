@@ -101,7 +101,8 @@ class EngineInfo(ctypes.Structure):
     _fields_ = [("D", ctypes.c_int), ("K", ctypes.c_int), ("n_sectors", ctypes.c_int),
                 ("longest_partition", ctypes.c_int), ("block_elems", ctypes.c_long),
                 ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
-                ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int)]
+                ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int),
+                ("gram_mfma_flop_per_row", ctypes.c_double)]
 
 
 class RunStats(ctypes.Structure):

@@ -504,8 +504,11 @@ __device__ __forceinline__ double potential_r2(const Potential& p, double r2)
 // FP64 power sums p_k = sum (z / R)^k, k <= KMAX, the potential and the harmonic sums, meet
 // in a 2-step DPP butterfly, and write p_k to a per-sample scratch row, the weights V and
 // cos/sin sums to the weight rows, and the trace.  k_monomials: one wavefront per sample
-// builds the table p_k^r (k r <= K) in LDS and writes every monomial (at most 8 table factors
-// following a packed plan) to the real-embedded sample row.  Splitting keeps each kernel's
+// builds the table p_k^r (k r <= K) by FP64 binary powering, stores it in LDS at the output
+// precision and writes every monomial (at most 8 table factors following a packed plan) to the
+// real-embedded sample row.  The FP32 form multiplies the factors in FP32: its row is FP32
+// anyway, and FP64 products made the kernel FP64-VALU bound (N = 200, dL 1..16: 0.083 ms per
+// 32768 samples, ~700 FP64 instructions per sample and wave).  Splitting keeps each kernel's
 // live registers small, so both run at several waves per SIMD.
 constexpr int kMaxPow = 128;
 constexpr int kMaxStatesLds = 4096;
@@ -599,10 +602,11 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
 template <typename Real>
 __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
 {
+    using C2 = typename V2T<Real>::T;                                // float2 or double2
     extern __shared__ __align__(16) unsigned char smem_mono[];
-    double2* ptab = reinterpret_cast<double2*>(smem_mono);          // [4][kMaxPow]
-    double2* pk = ptab + 4 * kMaxPow;                                // [4][32]
-    unsigned long long* plan = reinterpret_cast<unsigned long long*>(pk + 4 * 32);
+    double2* pk = reinterpret_cast<double2*>(smem_mono);             // [4][32]
+    C2* ptab = reinterpret_cast<C2*>(pk + 4 * 32);                   // [4][kMaxPow]
+    unsigned long long* plan = reinterpret_cast<unsigned long long*>(ptab + 4 * kMaxPow);
     const int D = a.D;
     const bool plan_lds = D <= kMaxStatesLds;
     uint32_t* cols = reinterpret_cast<uint32_t*>(plan + (plan_lds ? D : 0));
@@ -615,7 +619,7 @@ __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
         }
     __syncthreads();
     double2* pkw = pk + wv * 32;
-    double2* ptw = ptab + wv * kMaxPow;
+    C2* ptw = ptab + wv * kMaxPow;
     const int waves = gridDim.x * 4;
     for (int w = blockIdx.x * 4 + wv; w < a.walkers; w += waves) {
         const double* ps = a.ps + (size_t)w * a.ps_ld;
@@ -642,7 +646,7 @@ __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
                     br = t;
                 }
             }
-            ptw[e] = make_double2(pr, pi);
+            ptw[e] = C2{(Real)pr, (Real)pi};
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -654,19 +658,19 @@ __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
             const unsigned long long pl = plan_lds ? plan[t] : a.prod[t];
             const uint32_t cw = plan_lds ? cols[t] : ((uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16));
             const int cnt = (int)(pl & 0xfull);
-            double vr = 1.0, vi = 0.0;
+            Real vr = 1, vi = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 if (q < cnt) {
-                    const double2 f = ptw[(pl >> (4 + 7 * q)) & 0x7full];
-                    const double tr = vr * f.x - vi * f.y;
+                    const C2 f = ptw[(pl >> (4 + 7 * q)) & 0x7full];
+                    const Real tr = vr * f.x - vi * f.y;
                     vi = vr * f.y + vi * f.x;
                     vr = tr;
                 }
             }
             const int c = (int)(cw & 0xffffu);
-            X[c] = (Real)vr;
-            X[c + (int)(cw >> 16)] = (Real)vi;
+            X[c] = vr;
+            X[c + (int)(cw >> 16)] = vi;
         }
         __builtin_amdgcn_wave_barrier();   // pk / ptab are reused by the next sample
     }
@@ -1628,7 +1632,7 @@ static hipError_t features_h(const FeatureArgs& a, hipStream_t s)
     if (e != hipSuccess) return e;
     // enough waves to fill the chip, each looping over samples
     const long blocks = std::min<long>((a.walkers + 3) / 4, 2048);
-    const size_t lds = (size_t)(4 * kMaxPow + 4 * 32) * sizeof(double2) +
+    const size_t lds = (size_t)4 * 32 * sizeof(double2) + (size_t)4 * kMaxPow * 2 * sizeof(Real) +
                        (a.D <= kMaxStatesLds ? (size_t)a.D * (sizeof(unsigned long long) + sizeof(uint32_t)) : 0);
     hipLaunchKernelGGL((k_monomials<Real>), dim3((unsigned)blocks), dim3(256), lds, s, a);
     return hipGetLastError();

@@ -226,13 +226,12 @@ def main():
                      "algorithmic_bytes_per_launch": rows_per_launch * (8 * info.D + 8),
                      "rows_per_launch": rows_per_launch,
                      "avg_launch_ms": d["ms_gram"] / launches,
-                     "issued_mfma_flop_per_launch": info.gram_mfma_flop_per_row * rows_per_launch,
-                     "issued_frac": info.gram_mfma_flop_per_row * rows_per_launch
-                     / (d["ms_gram"] / launches * 1e-3) / PEAK_FP32,
+                     "issued_mfma_flop_per_launch": info.gram_mfma_flop_per_row * updates / launches,
+                     "issued_frac": info.gram_mfma_flop_per_row * updates / (d["ms_gram"] * 1e-3) / PEAK_FP32,
                      "flop_note": "algorithmic = 16 sum d^2 per sample (complex GEMM count over whole "
                                   "Hermitian blocks, S and H); the kernel computes only the 16x16 blocks "
                                   "holding upper-triangle entries, issued_mfma_flop_per_launch, so "
-                                  "'achieved' can exceed the issued rate and even the peak; issued_frac "
+                                  "'achieved' is above the issued rate and can exceed the peak; issued_frac "
                                   "is the MFMA pipe's actual use"},
         "kernels": {
             "sampler": {"kernel": f"k_sweep{'_reg' if eng.info.sweep_in_registers else ''}<{eng.info.lanes_per_walker},float>",

@@ -146,8 +146,9 @@ typedef struct le_engine_info {
     int n_tiles;               /* 64x64 Gram tiles per weight */
     long x_ld;                 /* real columns per sample row */
     int sweep_in_registers;    /* 1 when the plain sweep keeps particles in registers */
-    double gram_mfma_flop_per_row; /* MFMA FLOPs the Gram launches issue per sample row (16x16
-                                      blocks that hold requested entries, all weights) */
+    double gram_mfma_flop_per_row; /* FP32-equivalent MFMA FLOPs the Gram launches issue per
+                                      sample row: 2 x 16 x 16 per computed 16x16 block (those
+                                      holding requested entries, an odd one padded) and weight */
 } le_engine_info;
 
 typedef struct le_run_stats {

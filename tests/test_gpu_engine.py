@@ -28,12 +28,20 @@ def test_fixed_sample_moments(oracle_lib, prec, tol):
     So, Ho, _ = o.accumulate_fixed(pos)
     n = pos.shape[0]
     with Engine(N, m, sectors, w=w, eps=eps, walkers=64, precision=prec, samples=1) as e:
+        # D = 30 states = 60 real columns, one 64-column tile.  The sector squares meet the
+        # 16x16 blocks (0,0) (0,1) | (1,1) (1,2) | (2,2) (2,3) | (3,3) above the diagonal; the odd
+        # block of row block 3 is computed twice: 8 blocks x 16^2 x 2 weights x 2 FLOP per row
+        assert e.info.gram_mfma_flop_per_row == 8 * 256 * 2 * 2
         e.accumulate_positions(pos)
         mom = e.moments()
     errs = [(_rel_fro(Sg, So[s] / n), _rel_fro(Hg, Ho[s] / n)) for s, (Sg, Hg) in enumerate(mom)]
     print(f"prec={prec} max rel Frobenius error S {max(e[0] for e in errs):.3e} H {max(e[1] for e in errs):.3e}")
     for s, (es, eh) in enumerate(errs):
         assert es < tol and eh < tol, (s, es, eh)
+    # entry (p, q), p > q, of the accumulated tiles is read as (q, p): the raw records
+    # (no host symmetrization) are exactly Hermitian
+    for Sg, Hg in mom:
+        assert np.array_equal(Sg, Sg.conj().T) and np.array_equal(Hg, Hg.conj().T)
 
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5)])

@@ -135,6 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip the timings of the other entry points")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy this many MiB device-to-device first (HBM counter calibration)")
     args = ap.parse_args()
@@ -277,6 +278,8 @@ def main():
     eng.close()
     if not args.no_forms:
         out["forms"] = forms(args, [int(x) for x in args.thins.split(",")])
+    if not args.no_rows:
+        out["rows"] = rows()
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
@@ -297,6 +300,110 @@ def gram_traffic(rows_per_launch):
     return b, (f"rocprofv3 FETCH_SIZE x {t['calibration']['fetch_factor']} + WRITE_SIZE x "
                f"{t['calibration']['write_factor']} (copy-calibrated) at {t['rows_per_launch_measured']} "
                f"rows/launch, scaled to {rows_per_launch}; {t['algorithmic_bytes_per_row']} algorithmic B/row")
+
+
+def _timed(eng, warm, steps):
+    """(wall s, Gram ms, kept samples) of `steps` sampling steps after `warm` untimed ones"""
+    eng.sample(warm)
+    eng.reset_moments()
+    g0 = eng.stats()["ms_gram"]
+    t0 = time.perf_counter()
+    eng.sample(steps)
+    return time.perf_counter() - t0, eng.stats()["ms_gram"] - g0
+
+
+def rows():
+    """Throughput of the other entry points at their working sizes, each a few seconds:
+    - excitation response (em_edgeResponse.tpp:129-164): m = 5, N = 400, dL 0..20 (D = 2714,
+      sectors up to 627 states), 65536 walkers; S, H and eight edge kicks
+      U_k = sum_i cos(k theta_i) exp(-(r_i - R)^2 / 2), k = 1..8, between every pair of
+      sectors.  The kick functor is the one the tests build (tests/cpp/user_weights.hip),
+      registered as a user's functor would be;
+    - the FP64 evolution of those eight kicks (dynamics.cpp:286-290): M dC/dt = -i (H + F(t) U_k) C,
+      F = 1e-3 exp(-(t - 20)^2 / 50), 0 <= t <= 100, saved every 0.1, one observable;
+    - the DSF at dL = 16 (em_dsf.cu:113-134): N = 200, sectors {0, 16}, harmonic 16, 32768 walkers;
+    - the occupation numbers of the dL = 0 state (gf_occupation_numbers.cu:16-239): N = 200,
+      two-sided chain of 201 coordinates, 598 orbitals, 32768 walkers.
+    Algorithmic FLOP per sample-update follow the Gram count of the headline (8 m n per
+    complex d_a x d_b block and weight; 16 sum d^2 for S and H)."""
+    import ctypes
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import OccupationEngine, W_USER, evolve
+    out = {}
+    # excitation response
+    n4, m4, sec4, w4 = 400, 5, list(range(21)), 65536
+    r4 = float(np.sqrt(2.0 * m4 * n4))
+    helper = ctypes.CDLL(os.path.join(ROOT, "tests", "cpp", "libuserweights.so"))
+    helper.le_test_set_kicks(8, ctypes.c_double(r4))
+    fn = ctypes.cast(helper.le_test_kick_fn, ctypes.c_void_p).value
+    log("rows: excitation response m=5 N=400 dL=0..20, 65536 walkers, 8 kicks")
+    steps = 4
+    with Engine(n4, m4, sec4, w=0.05, eps=2e-5, walkers=w4, samples=steps + 2, seed=2024, tune_sweeps=20,
+                step=0.5, n_user_weights=8) as e:
+        e.set_weight_callback(fn)
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(50)
+        wall, ms_gram = _timed(e, 2, steps)
+        d = e.dims
+        upd = w4 * steps
+        alg = 16.0 * sum(x * x for x in d) + 8 * 8.0 * sum(d[a] * d[b] for a in range(len(d)) for b in range(a, len(d)))
+        out["excitation_response_m5_N400_dL0-20"] = {
+            "sample_updates_per_s": upd / wall, "ms_per_step": 1e3 * wall / steps, "walkers": w4, "D": e.info.D,
+            "gram_algorithmic_flop_per_sample": alg,
+            "gram_tflops_algorithmic": alg * upd / (ms_gram * 1e-3) / 1e12,
+            "gram_issued_frac": e.info.gram_mfma_flop_per_row * upd / (ms_gram * 1e-3) / PEAK_FP32,
+            "gram_ms_per_step": ms_gram / steps, "step": e.stats()["step"]}
+        log("rows: FP64 evolution of the 8 kicks, D=2714")
+        offs, D = e.offsets, e.info.D
+        M = np.zeros((D, D), complex)
+        H = np.zeros((D, D), complex)
+        for s, (S, Hs) in enumerate(e.moments()):
+            o, n = offs[s], d[s]
+            M[o:o + n, o:o + n] = (S + S.conj().T) / 2
+            H[o:o + n, o:o + n] = (Hs + Hs.conj().T) / 2
+        Us = []
+        for k in range(8):
+            U = np.zeros((D, D), complex)
+            for a in range(len(d)):
+                for b in range(a, len(d)):
+                    blk = e.block(W_USER(k), a, b)
+                    U[offs[a]:offs[a] + d[a], offs[b]:offs[b] + d[b]] = blk
+                    U[offs[b]:offs[b] + d[b], offs[a]:offs[a] + d[a]] = blk.conj().T
+            Us.append((U + U.conj().T) / 2)
+    t0 = time.perf_counter()
+    rho, nrm, nsteps = evolve(M, H, Us, [Us[0]], lambda t: 1e-3 * np.exp(-(t - 20.0) ** 2 / 50.0), 100.0, 0.1, d)
+    wall = time.perf_counter() - t0
+    out["evolution_m5_N400_dL0-20_8kicks"] = {
+        "wall_s": wall, "controller_steps": int(nsteps), "saved_points": int(rho.shape[1]),
+        "max_norm_drift": float(np.max(np.abs(nrm - 1.0))),
+        "note": "per angular-momentum block eigenbasis on the GPU, then Cash-Karp 5(4) stages batched "
+                "over the 8 kicks; host-side matrix assembly excluded"}
+    del M, H, Us
+    # DSF at dL = 16
+    log("rows: DSF dL=16 N=200")
+    steps = 40
+    with Engine(N, M_INV, [0, 16], w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=steps + 4, seed=SEED,
+                tune_sweeps=50, step=0.5, harmonics=(16,), lanes_per_walker=-1) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(100)
+        wall, ms_gram = _timed(e, 4, steps)
+        out["dsf_N200_dL16"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                                "D": e.info.D, "gram_ms_per_step": ms_gram / steps}
+    # occupation numbers, dL = 0
+    log("rows: occupation numbers dL=0 N=200")
+    steps = 40
+    lcut = M_INV * (N - 1) + 1
+    with OccupationEngine(N, M_INV, 0, lcut, walkers=WALKERS, samples=steps + 4, seed=SEED, tune_sweeps=50,
+                          step=0.5) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(100)
+        wall, ms_gram = _timed(e, 4, steps)
+        out["occupation_N200_dL0"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                                      "orbitals": lcut, "gram_ms_per_step": ms_gram / steps}
+    return out
 
 
 def forms(args, thins):

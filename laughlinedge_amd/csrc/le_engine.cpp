@@ -141,6 +141,15 @@ struct le_engine {
         double* C[2] = {nullptr, nullptr};
     };
     std::vector<Job> jobs;
+    // consecutive jobs with the same sector blocks and weight count: one Gram launch each, over
+    // a device table of their weights and accumulators (one table per sample buffer)
+    struct Batch {
+        int first, count;
+    };
+    std::vector<Batch> batches;
+    le::GramJob* jobtab[2] = {nullptr, nullptr};
+    bool weights_gram = false;       // occupation weights on a block of <= 64 x 64 real columns:
+                                     // one k_gram_weights launch for all of them
     int ntiles = 0;                  // tiles of the S/H job
     double gram_flop_row = 0;        // MFMA FLOPs issued per sample row over all Gram jobs
     struct Item {
@@ -256,6 +265,8 @@ le_engine::~le_engine()
         (void)hipFree(j.C[0]);
         (void)hipFree(j.C[1]);
     }
+    (void)hipFree(jobtab[0]);
+    (void)hipFree(jobtab[1]);
     (void)hipFree(items_d);
     (void)hipFree(rec_d);
     (void)hipFree(lnfact_d);
@@ -485,6 +496,29 @@ int le_engine::setup(const le_observables* obs)
     }
     X = Xb[0];
     Wb = Wbb[0];
+    for (size_t a = 0; a < jobs.size();) {
+        size_t b = a + 1;
+        while (b < jobs.size() && b - a < 4096 && jobs[b].nw == jobs[a].nw && jobs[b].blocks == jobs[a].blocks) ++b;
+        batches.push_back({(int)a, (int)(b - a)});
+        a = b;
+    }
+    weights_gram = two_sided && gram_prec != le::kBF16X3 && 2 * basis.dim(0) <= 64 && n_user > 0;
+    if (weights_gram) {
+        const int P = 4 * basis.dim(0) * basis.dim(1), nwt = (n_user + 63) / 64;
+        gram_flop_row = 0;
+        for (int wt = 0; wt < nwt; ++wt)
+            for (int pt = 0; pt < (P + 63) / 64; ++pt)
+                for (int w = 0; w < 8; ++w)
+                    if (64 * wt + 32 * (w / 4) < n_user && 64 * pt + 16 * (w % 4) < P) gram_flop_row += 2.0 * 2 * 16 * 16;
+    }
+    for (int b = 0; b < 2; ++b) {
+        std::vector<le::GramJob> tab;
+        for (auto& j : jobs)
+            tab.push_back({weight_ptr(j.wid[0], b), j.nw > 1 ? weight_ptr(j.wid[1], b) : nullptr, j.C[0], j.C[1]});
+        HIPCHK(dalloc(&jobtab[b], std::max<size_t>(1, tab.size())));
+        if (!tab.empty())
+            HIPCHK(hipMemcpy(jobtab[b], tab.data(), tab.size() * sizeof(le::GramJob), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
     auto ft = basis.factor_table();
     HIPCHK(dalloc(&fac, std::max<size_t>(1, ft.size())));
@@ -914,19 +948,48 @@ hipEvent_t le_engine::next_event()
 int le_engine::gram_rows(long rows, int buf, hipStream_t s)
 {
     if (!s) s = stream;
-    for (auto& j : jobs) {
+    if (weights_gram) {
+        le::WeightGramArgs g{};
+        g.X = Xb[buf];
+        g.ldx = ldx;
+        g.W = weight_ptr(LE_W_USER(0), buf);
+        g.wld = wrows;
+        g.nweights = n_user;
+        g.jobs = jobtab[buf];
+        g.r0 = 2 * basis.sector_offset[0];
+        g.nr = 2 * basis.dim(0);
+        g.c0 = 2 * basis.sector_offset[1];
+        g.nc = 2 * basis.dim(1);
+        g.rows = rows;
+        g.ldc = ldx;
+        const long tiles = (long)((n_user + 63) / 64) * ((g.nr * g.nc + 63) / 64);
+        long splits = std::max<long>(1, (6144 + tiles - 1) / tiles);
+        long rps = std::max<long>((rows + splits - 1) / splits, 1024);
+        rps = ((rps + 31) / 32) * 32;
+        g.rows_per_split = rps;
+        g.splits = (int)((rows + rps - 1) / rps);
+        HIPCHK(le::launch_gram_weights(g, gram_prec, s));
+        return LE_OK;
+    }
+    for (auto& bt : batches) {
+        const Job& j = jobs[bt.first];
         le::GramArgs g{};
         g.X = Xb[buf];
         g.ldx = ldx;
         g.w0 = weight_ptr(j.wid[0], buf);
         g.w1 = j.nw > 1 ? weight_ptr(j.wid[1], buf) : nullptr;
+        if (bt.count > 1) {
+            g.jobs = jobtab[buf] + bt.first;
+            g.njobs = bt.count;
+        }
         g.nw = j.nw;
         g.rows = rows;
-        // ~6144 workgroups (24 per CU): with 2-3 resident 512-thread workgroups per CU the last,
-        // partial round of workgroups is then a small part of the launch (at 2048 workgroups
-        // N = 200, dL 1..16 needed 4.02 rounds and lost ~20% to the fifth)
+        // ~6144 workgroups (24 per CU) per launch: with 2-3 resident 512-thread workgroups per CU
+        // the last, partial round of workgroups is then a small part of the launch (at 2048
+        // workgroups N = 200, dL 1..16 needed 4.02 rounds and lost ~20% to the fifth)
         const long want_wg = 6144;
-        long splits = std::max<long>(1, (want_wg + j.ntiles - 1) / std::max(1, j.ntiles));
+        const long per_split = (long)std::max(1, j.ntiles) * bt.count;
+        long splits = std::max<long>(1, (want_wg + per_split - 1) / per_split);
         long rps = (rows + splits - 1) / splits;
         rps = std::max<long>(rps, 1024);
         rps = ((rps + 31) / 32) * 32;

@@ -789,7 +789,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
         }
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
-    const Real* wp[2] = {reinterpret_cast<const Real*>(g.w0), reinterpret_cast<const Real*>(g.w1)};
+    const GramJob job = g.jobs ? g.jobs[blockIdx.y] : GramJob{g.w0, g.w1, g.C0, g.C1};
+    const Real* wp[2] = {reinterpret_cast<const Real*>(job.w0), reinterpret_cast<const Real*>(job.w1)};
     // FP32 operand A of weight q: the row side itself when the weight is 1, else its weighted copy
     const Real(*aq[2])[kLdsStride];
 #pragma unroll
@@ -876,7 +877,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
                 }
         }
     }
-    double* Cs[2] = {g.C0, g.C1};
+    double* Cs[2] = {job.C0, job.C1};
 #pragma unroll
     for (int q = 0; q < NW; ++q)
 #pragma unroll
@@ -955,7 +956,8 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
         }
 
     const float* X = reinterpret_cast<const float*>(g.X);
-    const float* wp[2] = {reinterpret_cast<const float*>(g.w0), reinterpret_cast<const float*>(g.w1)};
+    const GramJob job = g.jobs ? g.jobs[blockIdx.y] : GramJob{g.w0, g.w1, g.C0, g.C1};
+    const float* wp[2] = {reinterpret_cast<const float*>(job.w0), reinterpret_cast<const float*>(job.w1)};
     // staging: waves 0..3 take the row side (times each weight), waves 4..7 the column side;
     // thread = (column lane, 8 consecutive samples 8 (wave % 4) .. +7), loaded into registers
     // one step ahead while the MFMAs run on the current LDS tile
@@ -1029,7 +1031,7 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
                 }
         }
     }
-    double* Cs[2] = {g.C0, g.C1};
+    double* Cs[2] = {job.C0, job.C1};
 #pragma unroll
     for (int q = 0; q < NW; ++q)
 #pragma unroll
@@ -1042,6 +1044,123 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
                 const long cl = tile.j0 + co[a] + l16;
                 atomicAdd(&Cs[q][row * g.ldc + cl], v);
             }
+}
+
+// ---------------------------------------------------------------------------
+// Many weights on one small block (the occupation numbers: one weight per orbital and real
+// or imaginary part, all on the block between the left and right copies of one sector).  As
+// a product over samples, C[q][(i, j)] = sum_s W[q][s] P[(i, j)][s] with the pair products
+// P = X[s][r0 + i] X[s][c0 + j]: weights on the MFMA rows, pairs on the columns.  One
+// 512-thread workgroup per 64-weight x 64-pair tile and row split; wave w owns weights
+// 32 (w / 4) .. +31 (two 16x16 blocks) and pairs 16 (w % 4) .. +15.  Per 32-sample step the
+// workgroup stages the weights ([sample][weight]) and the 64 + 64 columns of X it reads in
+// LDS; a lane forms its pair product when it reads the MFMA column operand.  Every weight
+// and X value is fetched once per pair tile, where one Gram launch per weight pair would
+// fetch a 64-column X tile per weight pair.  FP32 accumulators are flushed into FP64
+// registers every 256 samples, and the result is added to the weights' FP64 accumulators.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram_weights(WeightGramArgs g)
+{
+    using acc_t = typename Mfma<Real>::acc_t;
+    constexpr bool kF32 = sizeof(Real) == 4;
+    __shared__ __align__(16) Real wt[kKC][kLdsStride];   // [sample][weight]
+    __shared__ __align__(16) Real xr[kKC][kLdsStride];   // [sample][row column i]
+    __shared__ __align__(16) Real xc[kKC][kLdsStride];   // [sample][column column j]
+
+    const int P = g.nr * g.nc;
+    const int nwt = (g.nweights + 63) / 64;
+    const int tiles = nwt * ((P + 63) / 64);
+    const int tile = (int)(blockIdx.x % (unsigned)tiles);
+    const int split = (int)(blockIdx.x / (unsigned)tiles);
+    if (split >= g.splits) return;
+    const int q0 = (tile % nwt) * 64, p0 = (tile / nwt) * 64;
+    const long r_begin = (long)split * g.rows_per_split;
+    long r_end = r_begin + g.rows_per_split;
+    if (r_end > g.rows) r_end = g.rows;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wi = wave >> 2, wj = wave & 3;
+    const int l16 = lane & 15, k4 = lane >> 4;
+    const bool active = q0 + 32 * wi < g.nweights && p0 + 16 * wj < P;   // wave-uniform
+    const int pj = p0 + 16 * wj + l16;                                   // this lane's pair
+    const bool pv = pj < P;
+    const int pi_ = pv ? pj / g.nc : 0, pc = pv ? pj % g.nc : 0;
+
+    acc_t acc[2];
+    double sh[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        acc[a] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh[a][r] = 0.0;
+    }
+    const Real* X = reinterpret_cast<const Real*>(g.X);
+    const Real* W = reinterpret_cast<const Real*>(g.W);
+    // staging: weight q0 + (tid >> 3), samples 4 (tid & 7) .. +3; X sample tid >> 4, columns
+    // 8 (tid & 7) .. +7 of the row side (tid & 8 == 0) or of the column side
+    const int sq = tid >> 3, ss = (tid & 7) * 4;
+    const int xs = tid >> 4, xcol = (tid & 7) * 8;
+    const bool xside = (tid & 8) != 0;
+    const int xn = xside ? g.nc : g.nr;
+    const long xbase = xside ? g.c0 : g.r0;
+    const bool wq_ok = q0 + sq < g.nweights;
+    const Real* wrow = W + (size_t)(wq_ok ? q0 + sq : 0) * g.wld;
+    Real wb[4], xb[8];
+    auto fetch = [&](long r0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const long row = r0 + ss + e;
+            wb[e] = (wq_ok && row < r_end) ? wrow[row] : (Real)0;
+        }
+        const long row = r0 + xs;
+        const Real* xp = X + (size_t)(row < r_end ? row : 0) * g.ldx + xbase;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xb[e] = (row < r_end && xcol + e < xn) ? xp[xcol + e] : (Real)0;
+    };
+    if (r_begin < r_end) fetch(r_begin);
+    int steps = 0;
+    for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
+        __syncthreads();   // previous step's readers are done
+#pragma unroll
+        for (int e = 0; e < 4; ++e) wt[ss + e][sq] = wb[e];
+        Real(*xd)[kLdsStride] = xside ? xc : xr;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xd[xs][xcol + e] = xb[e];
+        __syncthreads();
+        if (r0 + kKC < r_end) fetch(r0 + kKC);
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < kKC; kk += 4) {
+                const int kr = kk + k4;
+                const Real bv = pv ? xr[kr][pi_] * xc[kr][pc] : (Real)0;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) acc[a] = Mfma<Real>::op(wt[kr][wi * 32 + a * 16 + l16], bv, acc[a]);
+            }
+        }
+        if (kF32 && ++steps == kFlushSteps) {
+            steps = 0;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sh[a][r] += (double)acc[a][r];
+                acc[a] = acc_t{0, 0, 0, 0};
+            }
+        }
+    }
+    if (!active || !pv) return;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            // f32 16x16x4: reg r of lane l is row 4 (l / 16) + r; f64 16x16x4: row 4 r + l / 16
+            const int q = q0 + wi * 32 + a * 16 + (kF32 ? (4 * k4 + r) : (4 * r + k4));
+            if (q >= g.nweights) continue;
+            const double v = sh[a][r] + (double)acc[a][r];
+            double* C = (q & 1) ? g.jobs[q >> 1].C1 : g.jobs[q >> 1].C0;
+            atomicAdd(&C[(long)(g.r0 + pi_) * g.ldc + g.c0 + pc], v);
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1655,7 +1774,8 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
 {
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
     if (a.splits != splits) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((long)a.ntiles * splits));
+    if (a.jobs ? (a.njobs < 1 || a.njobs > 65535) : a.njobs > 1) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((long)a.ntiles * splits), (unsigned)(a.jobs ? a.njobs : 1));
     if (prec == kBF16X3) {
         if (a.rows_per_split % kKC) return hipErrorInvalidValue;
         if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(512), 0, s, a);
@@ -1667,6 +1787,20 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
         if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2>), grid, dim3(512), 0, s, a);
         else hipLaunchKernelGGL((k_gram<float, 1>), grid, dim3(512), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_gram_weights(const WeightGramArgs& a, int prec, hipStream_t s)
+{
+    if (a.rows <= 0 || a.nweights <= 0 || a.nr <= 0 || a.nc <= 0) return hipSuccess;
+    if (a.nr > 64 || a.nc > 64 || a.rows_per_split <= 0 || a.rows_per_split % kKC || a.splits < 1 ||
+        (long)a.splits * a.rows_per_split < a.rows || !a.jobs)
+        return hipErrorInvalidValue;
+    const long tiles = (long)((a.nweights + 63) / 64) * ((a.nr * a.nc + 63) / 64);
+    dim3 grid((unsigned)(tiles * a.splits));
+    if (prec == kFP64) hipLaunchKernelGGL((k_gram_weights<double>), grid, dim3(512), 0, s, a);
+    else if (prec == kFP32) hipLaunchKernelGGL((k_gram_weights<float>), grid, dim3(512), 0, s, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -86,11 +86,21 @@ struct Tile {
 };
 
 // One Gram job: up to two row weights (nullptr = the constant 1) sharing a tile list.
+// Weights and accumulators of one job in a launch that covers several jobs with one tile list.
+struct GramJob {
+    const void* w0;
+    const void* w1;
+    double* C0;
+    double* C1;
+};
+
 struct GramArgs {
     const void* X;           // [rows][ldx]
     long ldx;
     const void* w0;          // [rows] or nullptr
     const void* w1;          // [rows] or nullptr (unused when nw == 1)
+    const GramJob* jobs;     // nullptr: w0, w1, C0, C1 here; else job blockIdx.y of njobs
+    int njobs;
     int nw;
     long rows;
     long rows_per_split;
@@ -99,6 +109,23 @@ struct GramArgs {
     int ntiles;
     double* C0;              // [ldc][ldc] FP64 accumulators (upper tiles)
     double* C1;
+    long ldc;
+};
+
+// Many weights on one small block: C_q[r0 + i][c0 + j] += sum_s w_q(s) X[s][r0 + i] X[s][c0 + j]
+// for i < nr, j < nc (both <= 64), q < nweights, with weight q's samples at W + q wld and its
+// accumulator jobs[q / 2].C0 (q even) or .C1 (q odd).
+struct WeightGramArgs {
+    const void* X;           // [rows][ldx]
+    long ldx;
+    const void* W;
+    long wld;
+    int nweights;
+    const GramJob* jobs;
+    int r0, nr, c0, nc;
+    long rows;
+    long rows_per_split;     // a multiple of 32
+    int splits;
     long ldc;
 };
 
@@ -141,6 +168,7 @@ hipError_t launch_sweep2(const SweepArgs& a, int prec, int lanes_per_walker, hip
 hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);
+hipError_t launch_gram_weights(const WeightGramArgs& a, int prec, hipStream_t s);
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t s);
 hipError_t launch_geneig(const double* S, const double* H, double* work, const GenEigProblem* probs,
                          int nprob, int dmax, double* evals, double* evecs, int* status, hipStream_t s);

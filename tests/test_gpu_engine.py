@@ -491,13 +491,18 @@ def _occ_numpy(pos, N, m, states, lcut):
     return np.einsum("sa,sb,sl->abl", PL.conj(), PR, W) / pos.shape[0]
 
 
+@pytest.mark.parametrize("N,dl,lcut", [(6, 3, 20), (6, 5, 70), (12, 10, 40)])
 @pytest.mark.parametrize("prec,tol", [(0, 2e-5), (1, 1e-11), (2, 1e-4)])
-def test_occupation_fixed_samples(prec, tol):
-    """two-sided features + Gram on given configurations vs an independent numpy estimator"""
+def test_occupation_fixed_samples(prec, tol, N, dl, lcut):
+    """two-sided features + Gram on given configurations vs an independent numpy estimator.
+    dL = 3 (6 real columns per side, 36 pairs) and dL = 5 (14 columns, 196 pairs in four
+    64-pair tiles, 140 weights in three 64-weight tiles) take the many-weights kernel in FP32
+    and FP64; dL = 10 at N = 12 (84 columns per side) and the BF16 split take one Gram launch
+    over all weight pairs."""
     from laughlinedge_amd.engine import OccupationEngine
     from oracle.partitions import Basis
-    N, m, dl, lcut = 6, 3, 3, 20
-    print(f"case occupation fixed prec={prec}", flush=True)
+    m = 3
+    print(f"case occupation fixed prec={prec} N={N} dl={dl} lcut={lcut}", flush=True)
     rng = np.random.default_rng(12)
     R = np.sqrt(2.0 * m * N)
     r = R * np.sqrt(rng.random((3000, N + 1)))

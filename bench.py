@@ -396,13 +396,14 @@ def rows():
     steps = 40
     lcut = M_INV * (N - 1) + 1
     with OccupationEngine(N, M_INV, 0, lcut, walkers=WALKERS, samples=steps + 4, seed=SEED, tune_sweeps=50,
-                          step=0.5) as e:
+                          step=0.5, lanes_per_walker=-1) as e:
         e.init_walkers()
         e.tune_step()
         e.burnin(100)
         wall, ms_gram = _timed(e, 4, steps)
         out["occupation_N200_dL0"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                                      "orbitals": lcut, "gram_ms_per_step": ms_gram / steps}
+                                      "orbitals": lcut, "gram_ms_per_step": ms_gram / steps,
+                                      "lanes_per_walker": e.info.lanes_per_walker, "acceptance": e.stats()["acceptance"]}
     return out
 
 

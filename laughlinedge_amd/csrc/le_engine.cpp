@@ -75,7 +75,7 @@ int default_lanes(int N, int prec)
 }
 
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec) -> G
+std::map<std::tuple<int, int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec, two-sided) -> G
 
 }  // namespace
 
@@ -768,7 +768,8 @@ int le_engine::symmetry(unsigned long long sweep0, const unsigned long long* swe
 int le_engine::autotune_lanes()
 {
     const int N = sys.particles_number;
-    auto key = std::make_tuple(N, mc.walkers, prec);
+    // the two-sided chain has its own kernel (and no register variant): its own entries
+    auto key = std::make_tuple(N, mc.walkers, prec, two_sided ? 1 : 0);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned_lanes.find(key);
@@ -1693,7 +1694,7 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     const long nt = (long)std::ceil(end_time / save_step);
     double *dV = nullptr, *dT = nullptr, *dX = nullptr, *dW = nullptr, *dR = nullptr, *dlam = nullptr, *db = nullptr,
-           *dbn = nullptr, *dks = nullptr, *dy = nullptr, *dcoef = nullptr, *dout = nullptr;
+           *dbn = nullptr, *dks = nullptr, *dy = nullptr, *dcoef = nullptr, *dout = nullptr, *dA = nullptr;
     unsigned long long* demax = nullptr;
     hipError_t h = hipSuccess;
     auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
@@ -1707,6 +1708,7 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
     chk(dalloc(&dbn, 2 * (size_t)D * n_u));
     chk(dalloc(&dks, 12 * (size_t)D * n_u));
     chk(dalloc(&dy, 2 * (size_t)D * n_u));
+    chk(dalloc(&dA, 2 * (size_t)D * n_u));
     chk(dalloc(&dcoef, 64));
     chk(dalloc(&dout, 2 * (size_t)nt * n_u * (n_rho + 1)));
     chk(dalloc(&demax, 1));
@@ -1795,14 +1797,14 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
             }
         }
         if (h == hipSuccess)
-            chk(le::launch_observe(db, dR, dlam, tk, D, n_u, n_rho, dout + 2 * (size_t)kt * n_u * (n_rho + 1), st));
+            chk(le::launch_observe(db, dR, dlam, tk, D, n_u, n_rho, dA, dout + 2 * (size_t)kt * n_u * (n_rho + 1), st));
     }
     std::vector<double> res(2 * (size_t)nt * n_u * (n_rho + 1));
     if (h == hipSuccess) {
         chk(hipMemcpyAsync(res.data(), dout, sizeof(double) * res.size(), hipMemcpyDeviceToHost, st));
         chk(hipStreamSynchronize(st));
     }
-    for (double* p : {dV, dT, dX, dW, dR, dlam, db, dbn, dks, dy, dcoef, dout}) (void)hipFree(p);
+    for (double* p : {dV, dT, dX, dW, dR, dlam, db, dbn, dks, dy, dcoef, dout, dA}) (void)hipFree(p);
     (void)hipFree(demax);
     (void)hipStreamDestroy(st);
     if (h != hipSuccess) return fail(LE_ERR_HIP, std::string("le_evolve: ") + hipGetErrorString(h));

@@ -134,46 +134,75 @@ __global__ void k_rk_combine(const double2* b, const double2* ks, const double* 
     if ((threadIdx.x & 63) == 0) atomicMax(emax, (unsigned long long)__double_as_longlong(r));  // r >= 0
 }
 
-// out[u][l] = a^H R_l a with a = e^{-i lam t} b_u; l = nrho is the norm a^H a
-__global__ __launch_bounds__(256) void k_observe(const double2* b, const double2* R, const double* lam, double t, int D,
-                                                 int nu, int nrho, double2* out)
+// Observation at time t, in two passes.  k_phase: a_u[m] = e^{-i lam_m t} b_u[m] for every
+// perturbation u (one sincos per m) and the norms out[u][nrho] = a_u^H a_u.  k_observe: one
+// wavefront per row j of R_l reads that row once and forms (R_l a_u)_j for eight u at a
+// time; out[u][l] += conj(a_u[j]) (R_l a_u)_j.
+__global__ __launch_bounds__(256) void k_phase(const double2* b, const double* lam, double t, int D, int nu,
+                                               int nrho, double2* a, double2* out)
 {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double s = 0, c = 1;
+    if (m < D) sincos(lam[m] * t, &s, &c);
+    for (int u = 0; u < nu; ++u) {
+        double n2 = 0;
+        if (m < D) {
+            const double2 bm = b[(size_t)u * D + m];
+            const double2 am = {bm.x * c + bm.y * s, bm.y * c - bm.x * s};
+            a[(size_t)u * D + m] = am;
+            n2 = am.x * am.x + am.y * am.y;
+        }
+        for (int off = 32; off > 0; off >>= 1) n2 += __shfl_xor(n2, off, 64);
+        if (lane == 0) atomicAdd(&out[(size_t)u * (nrho + 1) + nrho].x, n2);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_observe(const double2* a, const double2* R, int D, int nu, int nrho,
+                                                 double2* out)
+{
+    constexpr int U = 8;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wave >= D * nu * (nrho + 1)) return;
-    const int j = wave % D;
-    const int ul = wave / D;
-    const int l = ul % (nrho + 1), u = ul / (nrho + 1);
-    const double2* bu = b + (size_t)u * D;
-    double s, c;
-    sincos(lam[j] * t, &s, &c);
-    const z2 aj = {bu[j].x * c + bu[j].y * s, bu[j].y * c - bu[j].x * s};
-    double sr = 0, si = 0;
-    if (l == nrho) {
-        if (lane == 0) {
-            sr = aj.r * aj.r + aj.i * aj.i;
-        }
-    } else {
-        const double2* rr = R + ((size_t)l * D + j) * D;
+    if (wave >= D * nrho) return;
+    const int j = wave % D, l = wave / D;
+    const double2* rr = R + ((size_t)l * D + j) * D;
+    for (int u0 = 0; u0 < nu; u0 += U) {
+        double sr[U], si[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) sr[k] = si[k] = 0.0;
         for (int m = lane; m < D; m += 64) {
-            double s2, c2;
-            sincos(lam[m] * t, &s2, &c2);
-            const z2 am = {bu[m].x * c2 + bu[m].y * s2, bu[m].y * c2 - bu[m].x * s2};
-            const z2 p = zmul({rr[m].x, rr[m].y}, am);
-            sr += p.r;
-            si += p.i;
+            const double2 r = rr[m];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                if (u0 + k < nu) {
+                    const double2 am = a[(size_t)(u0 + k) * D + m];
+                    sr[k] += r.x * am.x - r.y * am.y;
+                    si[k] += r.x * am.y + r.y * am.x;
+                }
+            }
         }
-        const z2 q = zcmul(aj, {sr, si});
-        sr = q.r;
-        si = q.i;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        sr += __shfl_xor(sr, off, 64);
-        si += __shfl_xor(si, off, 64);
-    }
-    if (lane == 0) {
-        atomicAdd(&out[(size_t)u * (nrho + 1) + l].x, sr);
-        atomicAdd(&out[(size_t)u * (nrho + 1) + l].y, si);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            for (int off = 32; off > 0; off >>= 1) {
+                sr[k] += __shfl_xor(sr[k], off, 64);
+                si[k] += __shfl_xor(si[k], off, 64);
+            }
+        }
+        if (lane < U && u0 + lane < nu) {
+            double xr = 0, xi = 0;
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+                if (k == lane) {
+                    xr = sr[k];
+                    xi = si[k];
+                }
+            const int u = u0 + lane;
+            const double2 aj = a[(size_t)u * D + j];
+            const z2 q = zcmul({aj.x, aj.y}, {xr, xi});
+            atomicAdd(&out[(size_t)u * (nrho + 1) + l].x, q.r);
+            atomicAdd(&out[(size_t)u * (nrho + 1) + l].y, q.i);
+        }
     }
 }
 
@@ -213,11 +242,15 @@ hipError_t launch_rk_combine(const double* b, const double* ks, const double* b5
 }
 
 hipError_t launch_observe(const double* b, const double* R, const double* lam, double t, int D, int nu, int nrho,
-                          double* out, hipStream_t s)
+                          double* a, double* out, hipStream_t s)
 {
-    const long waves = (long)D * nu * (nrho + 1);
-    hipLaunchKernelGGL(k_observe, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (const double2*)b,
-                       (const double2*)R, lam, t, D, nu, nrho, (double2*)out);
+    hipLaunchKernelGGL(k_phase, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, s, (const double2*)b, lam, t, D, nu,
+                       nrho, (double2*)a, (double2*)out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nrho == 0) return e;
+    const long waves = (long)D * nrho;
+    hipLaunchKernelGGL(k_observe, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (const double2*)a,
+                       (const double2*)R, D, nu, nrho, (double2*)out);
     return hipGetLastError();
 }
 

@@ -13,7 +13,8 @@ hipError_t launch_stage_matvec(const double* W, const double* y, double* kout, d
 hipError_t launch_rk_combine(const double* b, const double* ks, const double* b5, const double* db, double h,
                              double eps_abs, double eps_rel, double* bnew, unsigned long long* emax, int n,
                              hipStream_t s);
+// a: scratch of 2 D nu doubles (the phased coefficients); out[u][l], l <= nrho, is added to
 hipError_t launch_observe(const double* b, const double* R, const double* lam, double t, int D, int nu, int nrho,
-                          double* out, hipStream_t s);
+                          double* a, double* out, hipStream_t s);
 
 }  // namespace le

@@ -692,3 +692,22 @@ def test_register_sweep_graphs_match_eager():
     assert np.array_equal(p0, p1) and np.array_equal(t0, t1)
     for (S0, H0), (S1, H1) in zip(m0, m1):
         assert np.allclose(S0, S1, rtol=1e-12, atol=0) and np.allclose(H0, H1, rtol=1e-12, atol=0)
+
+
+def test_autotuned_lanes_per_chain_kind():
+    """The lane autotuner caches its choice per (N, walkers, precision, chain kind): an edge
+    engine's register mapping is not reused by the two-sided chain, which has no register
+    sweep."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import OccupationEngine
+    N, m, W = 100, 3, 2048
+    print("case autotune edge then occupation N=100", flush=True)
+    with Engine(N, m, [1, 2], walkers=W, samples=2, lanes_per_walker=-1, tune_sweeps=10, step=0.5) as e:
+        e.init_walkers()
+        e.tune_step()
+    with OccupationEngine(N, m, 0, 20, walkers=W, samples=2, lanes_per_walker=-1, tune_sweeps=10, step=0.5) as o:
+        o.init_walkers()
+        o.tune_step()
+        o.burnin(2)
+        o.sample(1)
+        assert o.info.sweep_in_registers == 0 and o.info.lanes_per_walker in (4, 8, 16, 32, 64)

@@ -312,28 +312,15 @@ def _timed(eng, warm, steps):
     return time.perf_counter() - t0, eng.stats()["ms_gram"] - g0
 
 
-def rows():
-    """Throughput of the other entry points at their working sizes, each a few seconds:
-    - excitation response (em_edgeResponse.tpp:129-164): m = 5, N = 400, dL 0..20 (D = 2714,
-      sectors up to 627 states), 65536 walkers; S, H and eight edge kicks
-      U_k = sum_i cos(k theta_i) exp(-(r_i - R)^2 / 2), k = 1..8, between every pair of
-      sectors.  The kick functor is the one the tests build (tests/cpp/user_weights.hip),
-      registered as a user's functor would be;
-    - the FP64 evolution of those eight kicks (dynamics.cpp:286-290): M dC/dt = -i (H + F(t) U_k) C,
-      F = 1e-3 exp(-(t - 20)^2 / 50), 0 <= t <= 100, saved every 0.1, one observable;
-    - the DSF at dL = 16 (em_dsf.cu:113-134): N = 200, sectors {0, 16}, harmonic 16, 32768 walkers;
-    - the occupation numbers of the dL = 0 state (gf_occupation_numbers.cu:16-239): N = 200,
-      two-sided chain of 201 coordinates, 598 orbitals, 32768 walkers.
-    Algorithmic FLOP per sample-update follow the Gram count of the headline (8 m n per
-    complex d_a x d_b block and weight; 16 sum d^2 for S and H)."""
+def _rows_kicks(helper_path):
+    """excitation response with eight edge kicks and the FP64 evolution of the kicks (see rows)"""
     import ctypes
     from laughlinedge_amd import Engine
-    from laughlinedge_amd.engine import OccupationEngine, W_USER, evolve
+    from laughlinedge_amd.engine import W_USER, evolve
     out = {}
-    # excitation response
     n4, m4, sec4, w4 = 400, 5, list(range(21)), 65536
     r4 = float(np.sqrt(2.0 * m4 * n4))
-    helper = ctypes.CDLL(os.path.join(ROOT, "tests", "cpp", "libuserweights.so"))
+    helper = ctypes.CDLL(helper_path)
     helper.le_test_set_kicks(8, ctypes.c_double(r4))
     fn = ctypes.cast(helper.le_test_kick_fn, ctypes.c_void_p).value
     log("rows: excitation response m=5 N=400 dL=0..20, 65536 walkers, 8 kicks")
@@ -380,6 +367,31 @@ def rows():
         "note": "per angular-momentum block eigenbasis on the GPU, then Cash-Karp 5(4) stages batched "
                 "over the 8 kicks; host-side matrix assembly excluded"}
     del M, H, Us
+    return out
+
+
+def rows():
+    """Throughput of the other entry points at their working sizes, each a few seconds:
+    - excitation response (em_edgeResponse.tpp:129-164): m = 5, N = 400, dL 0..20 (D = 2714,
+      sectors up to 627 states), 65536 walkers; S, H and eight edge kicks
+      U_k = sum_i cos(k theta_i) exp(-(r_i - R)^2 / 2), k = 1..8, between every pair of
+      sectors.  The kick functor is the one the tests build (tests/cpp/user_weights.hip),
+      registered as a user's functor would be;
+    - the FP64 evolution of those eight kicks (dynamics.cpp:286-290): M dC/dt = -i (H + F(t) U_k) C,
+      F = 1e-3 exp(-(t - 20)^2 / 50), 0 <= t <= 100, saved every 0.1, one observable;
+    - the DSF at dL = 16 (em_dsf.cu:113-134): N = 200, sectors {0, 16}, harmonic 16, 32768 walkers;
+    - the occupation numbers of the dL = 0 state (gf_occupation_numbers.cu:16-239): N = 200,
+      two-sided chain of 201 coordinates, 598 orbitals, 32768 walkers.
+    Algorithmic FLOP per sample-update follow the Gram count of the headline (8 m n per
+    complex d_a x d_b block and weight; 16 sum d^2 for S and H)."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import OccupationEngine
+    out = {}
+    helper_path = os.path.join(ROOT, "tests", "cpp", "libuserweights.so")
+    if os.path.exists(helper_path):
+        out.update(_rows_kicks(helper_path))
+    else:
+        out["excitation_response_m5_N400_dL0-20"] = {"skipped": f"{helper_path} is not built"}
     # DSF at dL = 16
     log("rows: DSF dL=16 N=200")
     steps = 40

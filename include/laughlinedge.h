@@ -238,7 +238,8 @@ int le_removal_weights(le_engine* e, int sector, int block, int* orbital, double
  * :149-309) of C(0) = e_0 under M dC/dt = -i (H + F(t) U_u) C for n_u perturbations at once,
  * FP64 on the GPU.  M and H are D x D complex (only the diagonal blocks of sizes
  * block_dims[] are used: angular momentum is conserved by them), U and rho are n_u and
- * n_rho D x D complex matrices.  Observed at t_k = k save_step, k < ceil(end_time /
+ * n_rho D x D complex matrices (the Hermitian part of each U_u is used; the reference's U is
+ * written self-adjoint).  Observed at t_k = k save_step, k < ceil(end_time /
  * save_step): out_rho[u][k][l] = C^H rho_l C (complex), out_norm[u][k] = C^H M C.
  * Adaptive Cash-Karp 5(4) with the given tolerances (the reference: 1e-10, 1e-8). */
 typedef double (*le_time_fn)(void* ctx, double t);

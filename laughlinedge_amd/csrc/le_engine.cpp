@@ -1721,6 +1721,11 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
         chk(hipStreamSynchronize(st));
     };
     for (int u = 0; u < n_u && h == hipSuccess; ++u) sandwich(U + 2 * DD * u, dW + 2 * DD * u);
+    // the stage products read only the upper tiles of each W_u: use its Hermitian part
+    if (h == hipSuccess && n_u > 0) {
+        chk(le::launch_hermitian_part(dW, D, n_u, st));
+        chk(hipStreamSynchronize(st));
+    }
     for (int l = 0; l < n_rho && h == hipSuccess; ++l) sandwich(rho + 2 * DD * l, dR + 2 * DD * l);
     // 3. Cash-Karp 5(4) tableau
     static const double cc[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
@@ -1763,8 +1768,9 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
             for (int tries = 0;; ++tries) {
                 for (int sI = 0; sI < 6 && h == hipSuccess; ++sI) {
                     const double ts = t + cc[sI] * hs;
-                    chk(le::launch_stage_prep(db, dks, sI, dcoef + 8 * sI, hs, ts, dlam, dy, D, n_u, st));
-                    chk(le::launch_stage_matvec(dW, dy, dks + 2 * (size_t)sI * n, F(ctx, ts), ts, dlam, D, n_u, st));
+                    double* kz = dks + 2 * (size_t)sI * n;
+                    chk(le::launch_stage_prep(db, dks, sI, dcoef + 8 * sI, hs, ts, dlam, dy, kz, D, n_u, st));
+                    chk(le::launch_stage_hemv(dW, dy, kz, F(ctx, ts), ts, dlam, D, n_u, st));
                 }
                 chk(hipMemsetAsync(demax, 0, sizeof(unsigned long long), st));
                 chk(le::launch_rk_combine(db, dks, dcoef + 48, dcoef + 56, hs, eps_abs, eps_rel, dbn, demax, n, st));

@@ -63,9 +63,10 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2* A, const double2* 
     if (row < Mr && col < Nc) C[(size_t)row * ldc + col] = {ar, ai};
 }
 
-// y_u[m] = e^{-i lam_m t} (b_u[m] + h sum_r a_r k_r,u[m])
+// y_u[m] = e^{-i lam_m t} (b_u[m] + h sum_r a_r k_r,u[m]); kz (this stage's k) is cleared for
+// the tile products that add into it
 __global__ void k_stage_prep(const double2* b, const double2* ks, int nstage, const double* coef, double h, double t,
-                             const double* lam, double2* y, int D, int nu)
+                             const double* lam, double2* y, double2* kz, int D, int nu)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)D * nu) return;
@@ -79,33 +80,126 @@ __global__ void k_stage_prep(const double2* b, const double2* ks, int nstage, co
     double s, c;
     sincos(lam[m] * t, &s, &c);
     y[g] = {v.x * c + v.y * s, v.y * c - v.x * s};   // times e^{-i lam t}
+    kz[g] = {0.0, 0.0};
 }
 
-// k_u[j] = -i F e^{i lam_j t} sum_m W_u[j][m] y_u[m]; one wavefront per (u, j)
-__global__ __launch_bounds__(256) void k_stage_matvec(const double2* W, const double2* y, double2* kout, double F,
-                                                      double t, const double* lam, int D, int nu)
+// W <- (W + W^H) / 2 for each of nu D x D matrices
+__global__ void k_hermitian_part(double2* W, int D, int nu)
 {
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= D * nu) return;
-    const int u = wave / D, j = wave % D;
-    const double2* wr = W + ((size_t)u * D + j) * D;
-    const double2* yu = y + (size_t)u * D;
-    double sr = 0, si = 0;
-    for (int m = lane; m < D; m += 64) {
-        const double2 a = wr[m], x = yu[m];
-        sr += a.x * x.x - a.y * x.y;
-        si += a.x * x.y + a.y * x.x;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)D * D * nu) return;
+    const int u = (int)(g / ((long)D * D));
+    const long e = g % ((long)D * D);
+    const int i = (int)(e / D), j = (int)(e % D);
+    if (i > j) return;
+    double2* Wu = W + (size_t)u * D * D;
+    const double2 a = Wu[(size_t)i * D + j], b = Wu[(size_t)j * D + i];
+    const double2 w = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+    Wu[(size_t)i * D + j] = w;
+    Wu[(size_t)j * D + i] = {w.x, -w.y};
+}
+
+// k_u += -i F e^{i lam t} W_u y_u for Hermitian W_u, reading only its upper 64 x 64 tiles.
+// One workgroup per tile (I, J), I <= J, and perturbation u: it adds W_IJ y_J to the rows of
+// tile row I and, for I < J, W_IJ^H y_I to the rows of tile row J, so every off-diagonal
+// element is read once for two products (half the bytes of a dense product, which bounds
+// this kernel: nu D^2 complex values per stage).  The tile is staged in LDS with coalesced
+// row loads (16 in flight per thread); the row sums take lane = row (row stride 65 elements,
+// so 8 lanes' 16-byte reads hit distinct banks), the column sums lane = column, each over a
+// 16-element quarter per wave, and the quarters meet in LDS.  D = 2714, 8 perturbations:
+// 0.109 ms per stage against 0.172 ms for the dense product (one wavefront per row), and
+// 0.170 ms when each wave kept its 16 row sums in registers (206 VGPRs, 2 waves per SIMD,
+// load and reduction phases not overlapped).
+__global__ __launch_bounds__(256) void k_stage_hemv(const double2* W, const double2* y, double2* kout, double F,
+                                                    double t, const double* lam, int D, int nt)
+{
+    __shared__ double2 sW[64][65];
+    __shared__ double2 yI[64], yJ[64], phI[64], phJ[64];
+    __shared__ double2 part[4][64];
+    const int u = blockIdx.y;
+    int I = 0, rest = (int)blockIdx.x;
+    while (rest >= nt - I) {
+        rest -= nt - I;
+        ++I;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        sr += __shfl_xor(sr, off, 64);
-        si += __shfl_xor(si, off, 64);
+    const int J = I + rest;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double2* Wu = W + (size_t)u * D * D;
+    {
+        const int col = J * 64 + lane;
+        double2 v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = I * 64 + 4 * i + wv;
+            v[i] = (row < D && col < D) ? Wu[(size_t)row * D + col] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sW[4 * i + wv][lane] = v[i];
+        // y of the tile's row and column blocks, -i F e^{i lam t} of their rows
+        const int j = ((wv & 1) ? J : I) * 64 + lane;
+        const bool ok = j < D;
+        if (wv < 2) {
+            (wv ? yJ : yI)[lane] = ok ? y[(size_t)u * D + j] : double2{0.0, 0.0};
+        } else {
+            double s = 0.0, c = 0.0;
+            if (ok) sincos(lam[j] * t, &s, &c);
+            (wv == 3 ? phJ : phI)[lane] = {F * s, -F * c};
+        }
     }
-    if (lane == 0) {
-        double s, c;
-        sincos(lam[j] * t, &s, &c);
-        const double pr = sr * c - si * s, pi = sr * s + si * c;   // times e^{i lam t}
-        kout[(size_t)u * D + j] = {F * pi, -F * pr};              // times -i F
+    __syncthreads();
+    {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int c = 16 * wv + k;
+            const double2 w = sW[lane][c], x = yJ[c];
+            sr += w.x * x.x - w.y * x.y;
+            si += w.x * x.y + w.y * x.x;
+        }
+        part[wv][lane] = {sr, si};
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const int row = I * 64 + lane;
+        if (row < D) {
+            double xr = 0.0, xi = 0.0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                xr += part[w][lane].x;
+                xi += part[w][lane].y;
+            }
+            const double2 p = phI[lane];
+            atomicAdd(&kout[(size_t)u * D + row].x, p.x * xr - p.y * xi);
+            atomicAdd(&kout[(size_t)u * D + row].y, p.x * xi + p.y * xr);
+        }
+    }
+    if (I == J) return;   // workgroup-uniform
+    __syncthreads();      // part[] is reused
+    {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int r = 16 * wv + k;
+            const double2 w = sW[r][lane], x = yI[r];
+            sr += w.x * x.x + w.y * x.y;   // conj(w) y_r
+            si += w.x * x.y - w.y * x.x;
+        }
+        part[wv][lane] = {sr, si};
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const int col = J * 64 + lane;
+        if (col < D) {
+            double xr = 0.0, xi = 0.0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                xr += part[w][lane].x;
+                xi += part[w][lane].y;
+            }
+            const double2 p = phJ[lane];
+            atomicAdd(&kout[(size_t)u * D + col].x, p.x * xr - p.y * xi);
+            atomicAdd(&kout[(size_t)u * D + col].y, p.x * xi + p.y * xr);
+        }
     }
 }
 
@@ -216,20 +310,29 @@ hipError_t launch_zgemm(const double* A, const double* B, double* C, int M, int 
 }
 
 hipError_t launch_stage_prep(const double* b, const double* ks, int nstage, const double* coef, double h, double t,
-                             const double* lam, double* y, int D, int nu, hipStream_t s)
+                             const double* lam, double* y, double* kz, int D, int nu, hipStream_t s)
 {
     const long n = (long)D * nu;
     hipLaunchKernelGGL(k_stage_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double2*)b,
-                       (const double2*)ks, nstage, coef, h, t, lam, (double2*)y, D, nu);
+                       (const double2*)ks, nstage, coef, h, t, lam, (double2*)y, (double2*)kz, D, nu);
     return hipGetLastError();
 }
 
-hipError_t launch_stage_matvec(const double* W, const double* y, double* kout, double F, double t, const double* lam,
-                               int D, int nu, hipStream_t s)
+hipError_t launch_hermitian_part(double* W, int D, int nu, hipStream_t s)
 {
-    const long waves = (long)D * nu;
-    hipLaunchKernelGGL(k_stage_matvec, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (const double2*)W,
-                       (const double2*)y, (double2*)kout, F, t, lam, D, nu);
+    const long n = (long)D * D * nu;
+    hipLaunchKernelGGL(k_hermitian_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (double2*)W, D, nu);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_hemv(const double* W, const double* y, double* kout, double F, double t, const double* lam,
+                             int D, int nu, hipStream_t s)
+{
+    const int nt = (D + 63) / 64;
+    if (nu < 1 || nu > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((long)nt * (nt + 1) / 2), (unsigned)nu);
+    hipLaunchKernelGGL(k_stage_hemv, grid, dim3(256), 0, s, (const double2*)W, (const double2*)y, (double2*)kout, F,
+                       t, lam, D, nt);
     return hipGetLastError();
 }
 

@@ -693,17 +693,15 @@ template <> struct Mfma<double> {
     __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 };
 
-__device__ __forceinline__ void load8(const float* p, float* o)
+__device__ __forceinline__ void load4(const float* p, float* o)
 {
     const float4 a = reinterpret_cast<const float4*>(p)[0];
-    const float4 b = reinterpret_cast<const float4*>(p)[1];
     o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
 }
-__device__ __forceinline__ void load8(const double* p, double* o)
+__device__ __forceinline__ void load4(const double* p, double* o)
 {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < 2; ++e) {
         const double2 a = reinterpret_cast<const double2*>(p)[e];
         o[2 * e] = a.x;
         o[2 * e + 1] = a.y;
@@ -798,7 +796,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
         aq[q] = (q < NW && wp[q]) ? reinterpret_cast<const Real(*)[kLdsStride]>(&xw[q < NW ? q : 0][0][0]) : xi;
     const int side = tid >> 8;                 // 0: row-side block, 1: column-side block
     const int rr = (tid & 255) >> 3;           // 0..31
-    const int cc = (tid & 7) * 8;              // 0..56
+    // columns cc .. cc + 3 and cc + 32 .. cc + 35: each store instruction of 8 lanes then fills
+    // 128 contiguous bytes (all 32 banks once); 8 consecutive columns per lane hit every bank
+    // twice (7.3e7 conflict cycles per launch at N = 200, dL 1..16)
+    const int cc = (tid & 7) * 4;              // 0..28
     const bool loader = !(side && same);
     const long col0 = side ? tile.j0 : tile.i0;
     Real(*dst)[kLdsStride] = side ? xj : xi;
@@ -808,7 +809,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
         const bool in = row < r_end;
         if (loader) {
             if (in) {
-                load8(X + (size_t)row * g.ldx + col0 + cc, buf);
+                load4(X + (size_t)row * g.ldx + col0 + cc, buf);
+                load4(X + (size_t)row * g.ldx + col0 + cc + 32, buf + 4);
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) buf[e] = 0;
@@ -831,7 +833,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
         __syncthreads();   // previous step's readers are done
         if (loader) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) dst[rr][cc + e] = buf[e];
+            for (int e = 0; e < 4; ++e) {
+                dst[rr][cc + e] = buf[e];
+                dst[rr][cc + 32 + e] = buf[4 + e];
+            }
         }
         if constexpr (kPre) {
             if (!side) {
@@ -839,7 +844,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
                 for (int q = 0; q < NW; ++q)
                     if (wp[q]) {
 #pragma unroll
-                        for (int e = 0; e < 8; ++e) xw[q][rr][cc + e] = buf[e] * bw[q];
+                        for (int e = 0; e < 4; ++e) {
+                            xw[q][rr][cc + e] = buf[e] * bw[q];
+                            xw[q][rr][cc + 32 + e] = buf[4 + e] * bw[q];
+                        }
                     }
             }
         } else if (tid < NW * kKC) {

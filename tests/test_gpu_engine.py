@@ -432,14 +432,15 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
     assert "DONE" in lines
 
 
-def test_evolve_gpu_vs_oracle(oracle_lib):
+@pytest.mark.parametrize("dims", [[1, 2, 5], [1, 40, 100]])
+def test_evolve_gpu_vs_oracle(oracle_lib, dims):
     """FP64 GPU evolution (interaction picture, batched perturbations) vs the CPU RK-CK54
-    restatement of dynamics.cpp, driven and free."""
+    restatement of dynamics.cpp, driven and free.  D = 141 spans three 64-row tiles of the
+    stage products (the last partial), so both products of an off-diagonal tile are used."""
     from laughlinedge_amd.engine import evolve
     from tests.test_oracle import _block_system
-    print("case evolve gpu D=8", flush=True)
+    print(f"case evolve gpu dims={dims}", flush=True)
     rng = np.random.default_rng(8)
-    dims = [1, 2, 5]
     M, H, U, rhos = _block_system(rng, dims)
     B = rng.normal(size=U.shape) + 1j * rng.normal(size=U.shape)
     U2 = (B + B.conj().T) / 2

@@ -98,10 +98,11 @@ typedef struct le_mc_params {
                                   the scale of the coordinates relative to the centre of mass from
                                   its exact Gamma law (all exact moves of |Psi_L|^2, edge engine
                                   only); 0 (default): single-particle moves only, as the reference */
-    int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (k_sweep_reg;
-                                  needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25, plain
-                                  proposal, edge engine); 0 (default): LDS.  lanes_per_walker = -1
-                                  times both kinds and keeps the faster */
+    int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (edge
+                                  engine: lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25,
+                                  plain proposal; occupation engine: ceil((N - 1) / lanes) 13 or 25);
+                                  0 (default): LDS.  lanes_per_walker = -1 times both kinds and
+                                  keeps the faster */
 } le_mc_params;
 
 typedef struct le_engine le_engine;

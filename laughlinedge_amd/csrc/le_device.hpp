@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 namespace le {
 
@@ -89,6 +90,32 @@ __device__ __forceinline__ Real group_sum(Real v)
     if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
     if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
     return v;
+}
+
+// explicit fused multiply-add: the sweep kernels spell out every contraction, so their
+// arithmetic (and their chains) do not depend on the compiler's contraction choices
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// logarithms of pair-distance products: FP32 takes log2 of 8 products at a time (their
+// exponents kept in an integer), FP64 the natural log of 32
+template <typename Real> struct LogOps;
+template <> struct LogOps<float> {
+    static constexpr int chunk = 8;
+    __device__ static float lg(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-30f)); }
+    static constexpr float to_ln = 0.69314718055994531f;
+};
+template <> struct LogOps<double> {
+    static constexpr int chunk = 32;
+    __device__ static double lg(double x) { return log(fmax(x, 1e-300)); }
+    static constexpr double to_ln = 1.0;
+};
+
+// compile-time loop: f(integral_constant<int, I>) for each I of the sequence
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, Is...>, F&& f)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
 }
 
 }  // namespace le

@@ -727,7 +727,8 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     a.nsweeps = nsweeps;
     a.step = step;
     if (two_sided) {
-        HIPCHK(le::launch_sweep2(a, prec, G, stream));
+        if (map_reg(G)) HIPCHK(le::launch_sweep2_reg(a, prec, map_lanes(G), stream));
+        else HIPCHK(le::launch_sweep2(a, prec, G, stream));
     } else if (proposal == LE_PROPOSAL_LEARNED) {
         le::LearnedTable lt;
         lt.table = table;
@@ -796,9 +797,8 @@ int le_engine::autotune_lanes()
         if ((two_sided ? le::sweep2_lds_bytes(prec, c, N) : le::sweep_lds_bytes(prec, c, N)) > 160 * 1024) continue;
         cands.push_back(c);
     }
-    if (!two_sided)
-        for (int c : {8, 16})
-            if (le::sweep_reg_supported(c, N)) cands.push_back(c | kRegSweep);
+    for (int c : {8, 16})
+        if (two_sided ? le::sweep2_reg_supported(c, N) : le::sweep_reg_supported(c, N)) cands.push_back(c | kRegSweep);
     for (int cand : cands) {
         G = cand;
         HIPCHK(hipMemcpyAsync(scratch, pos, bytes, hipMemcpyDeviceToDevice, stream));
@@ -1843,7 +1843,10 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
-    if (mc->sweep_registers != 0) return fail(LE_ERR_ARG, "sweep_registers is for the edge engine");
+    if (mc->sweep_registers != 0 && mc->sweep_registers != 1) return fail(LE_ERR_ARG, "sweep_registers must be 0 or 1");
+    if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
+        !le::sweep2_reg_supported(mc->lanes_per_walker, sys->particles_number))
+        return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil((N - 1) / lanes) 13 or 25");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
     auto e = new le_engine();
     e->sys = *sys;
@@ -1866,6 +1869,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return rc;
     }
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
+    if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     *out = e;
     return LE_OK;
 }

@@ -65,22 +65,6 @@ __global__ void k_init_positions(typename V2T<Real>::T* pos, int N, int walkers,
 // to numerator and denominator; pad slots j >= N hold the same midpoint.  Products of
 // eight squared distances are taken to log2 at a time (FP32 range), FP64 takes one log.
 // ---------------------------------------------------------------------------
-// explicit fused multiply-add: both sweep kernels spell out every contraction, so their
-// arithmetic (and their chains) do not depend on the compiler's contraction choices
-__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
-template <typename Real> struct LogOps;
-template <> struct LogOps<float> {
-    static constexpr int chunk = 8;
-    __device__ static float lg(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-30f)); }
-    static constexpr float to_ln = 0.69314718055994531f;
-};
-template <> struct LogOps<double> {
-    static constexpr int chunk = 32;
-    __device__ static double lg(double x) { return log(fmax(x, 1e-300)); }
-    static constexpr double to_ln = 1.0;
-};
 
 template <int G, typename Real>
 __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
@@ -256,11 +240,6 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 // the chain is bit-identical to it.  The register file holds more walkers per CU than the LDS,
 // so a launch needs fewer rounds of resident workgroups.
 // ---------------------------------------------------------------------------
-template <typename F, int... Is>
-__device__ __forceinline__ void static_for(std::integer_sequence<int, Is...>, F&& f)
-{
-    (f(std::integral_constant<int, Is>{}), ...);
-}
 
 // register budget per lane: positions (2 S values, doubled in FP64) plus ~40 for the move
 template <int G, int S, typename Real>

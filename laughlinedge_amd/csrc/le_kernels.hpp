@@ -165,6 +165,10 @@ hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipS
 hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s);
 // two-sided chain: a.N is the number of particles per state (positions hold N + 1)
 hipError_t launch_sweep2(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+// two-sided chain with the N - 1 bulk particles in registers: 8 or 16 lanes, ceil((N - 1) / lanes)
+// = 13 or 25 slots; bit-identical to launch_sweep2 with the same lanes
+bool sweep2_reg_supported(int lanes_per_walker, int N);
+hipError_t launch_sweep2_reg(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);

@@ -27,12 +27,25 @@ __device__ __forceinline__ Real lg2(Real x)
     else return log2(fmax(x, 1e-300));
 }
 
+// log of |z' - q|^2 / |z - q|^2 (LogOps units) for the pairs of a bulk move with z_0 and z_N;
+// both sweep kernels below evaluate them the same way
+template <typename Real, typename V2>
+__device__ __forceinline__ Real pair_lr(Real xn, Real yn, V2 zo, V2 q)
+{
+    const Real ax = xn - q.x, ay = yn - q.y, bx = zo.x - q.x, by = zo.y - q.y;
+    return LogOps<Real>::lg(mad(ax, ax, ay * ay)) - LogOps<Real>::lg(mad(bx, bx, by * by));
+}
+
 // Bulk particles 1..N-1 sit in LDS slots 0..N-2 (lane l owns slots l + G s); z_0 and z_N
-// live in registers of every lane of the group.  Random numbers: one Philox block per move
-// (counter = (move, sweep, walker)), made G moves at a time, lane l for move i + l; lane
-// i mod G hands the move's normals and ln u to the group through __shfl.  The FP32 form takes
-// the hardware log2 / sin / cos, like the edge sweep; with one Philox block and one
-// Box-Muller pair per move in every lane the sweep took 2.9x the edge sweep's time at G = 16.
+// live in registers of every lane of the group.  Per move, the pair products follow the edge
+// sweep (k_sweep): the mover's slot and the pads hold the midpoint of old and proposed
+// positions, FP32 takes |z' - q|^2 = |b|^2 + |d|^2 + 2 d.b (b = z - q, d = z' - z) and keeps
+// the exponents of 8 products at a time in an integer, FP64 takes both distances and one log
+// per 32 products; the group sum is the DPP butterfly.  Random numbers: one Philox block per
+// move (counter = (move, sweep, walker)), made G moves at a time, lane l for move i + l; lane
+// i mod G hands the move's normals and ln u to the group through __shfl.  With one Philox
+// block and one FP64 Box-Muller pair per move in every lane the sweep took 2.9x the edge
+// sweep's time at G = 16.
 template <int G, typename Real>
 __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
 {
@@ -40,6 +53,7 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
     extern __shared__ __align__(16) unsigned char smem2[];
     V2* lds = reinterpret_cast<V2*>(smem2);
     constexpr int WPB = 64 / G;
+    constexpr int C = LogOps<Real>::chunk;
     const int N = a.N;
     const int NB = N - 1;
     const int S = (NB + G - 1) / G;
@@ -57,7 +71,7 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
     V2 z0 = gpos[0], zN = gpos[N];
 
     const Real step = (Real)a.step;
-    const Real mln2 = (Real)a.m * (Real)0.69314718055994531;
+    const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const bool has_pad = (lane + G * (S - 1)) >= NB;
     const int pad_j = lane + G * (S - 1);
@@ -67,7 +81,9 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
         const unsigned long long sidx = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull) + (unsigned long long)sw;
         const uint32_t c1 = (uint32_t)sidx;
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
-        Real rg1 = 0, rg2 = 0, rlnu = 0;
+        Real rg1 = 0, rg2 = 0;
+        float rlnu = 0.f;
+        uint32_t rub = 0u;
         for (int i = 0; i <= N; ++i) {
             const int t = i % G;
             if (t == 0 && i + lane <= N) {
@@ -75,49 +91,66 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
                 philox((uint32_t)(i + lane), c1, c2, (uint32_t)walker, k0, k1, r);
                 gauss2(r[0], r[1], rg1, rg2);
                 if constexpr (sizeof(Real) == 4) rlnu = __builtin_amdgcn_logf(u01f(r[2])) * 0.69314718055994531f;
-                else rlnu = log(u01d(r[2]));
+                rub = r[2];
             }
-            const Real g1 = __shfl(rg1, t, G), g2 = __shfl(rg2, t, G), lnu = __shfl(rlnu, t, G);
+            const Real g1 = __shfl(rg1, t, G), g2 = __shfl(rg2, t, G);
             const bool bulk = (i > 0 && i < N);
             const V2 zo = (i == 0) ? z0 : ((i == N) ? zN : P[i - 1]);
-            const Real xn = zo.x + step * g1, yn = zo.y + step * g2;
+            const Real xn = mad(step, g1, zo.x), yn = mad(step, g2, zo.y);
             V2 mid;
             mid.x = (Real)0.5 * (zo.x + xn);
             mid.y = (Real)0.5 * (zo.y + yn);
             if (bulk) P[i - 1] = mid;
             if (has_pad) P[pad_j] = mid;
+            const Real dx = xn - zo.x, dy = yn - zo.y;
+            const Real dd = mad(dx, dx, dy * dy), tdx = dx + dx, tdy = dy + dy;
             Real lsum = 0, pn = 1, pd = 1;
+            int ebal = 0;
             for (int s = 0; s < S; ++s) {
                 const V2 q = P[lane + G * s];
-                const Real ax = xn - q.x, ay = yn - q.y;
                 const Real bx = zo.x - q.x, by = zo.y - q.y;
-                pn *= ax * ax + ay * ay;
-                pd *= bx * bx + by * by;
-                if ((s & 7) == 7) {
-                    lsum += lg2<Real>(pn) - lg2<Real>(pd);
-                    pn = 1;
-                    pd = 1;
+                const Real B = mad(bx, bx, by * by);
+                Real A;
+                if constexpr (sizeof(Real) == 4) {
+                    A = mad(tdx, bx, mad(tdy, by, B + dd));
+                } else {
+                    const Real ax = xn - q.x, ay = yn - q.y;
+                    A = mad(ax, ax, ay * ay);
+                }
+                pn *= A;
+                pd *= B;
+                if ((s + 1) % C == 0) {
+                    if constexpr (sizeof(Real) == 4) {
+                        int en, ed;
+                        pn = frexpf(pn, &en);
+                        pd = frexpf(pd, &ed);
+                        ebal += en - ed;
+                    } else {
+                        lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                        pn = 1;
+                        pd = 1;
+                    }
                 }
             }
-            lsum += lg2<Real>(pn) - lg2<Real>(pd);
-#pragma unroll
-            for (int off = 1; off < G; off <<= 1) lsum += __shfl_xor(lsum, off, G);
-            const Real dr2 = (xn * xn + yn * yn) - (zo.x * zo.x + zo.y * zo.y);
+            if constexpr (sizeof(Real) == 4)
+                lsum = __builtin_amdgcn_logf(pn) - __builtin_amdgcn_logf(pd) + (float)ebal;
+            else
+                lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+            lsum = group_sum<G, Real>(lsum);
+            const Real dr2 = mad(xn, xn, yn * yn) - mad(zo.x, zo.x, zo.y * zo.y);
             Real logA;
             if (bulk) {
                 // bulk pairs count twice (both wavefunctions), the pairs with z_0 and z_N once
-                Real e = 0;
-                {
-                    Real ax = xn - z0.x, ay = yn - z0.y, bx = zo.x - z0.x, by = zo.y - z0.y;
-                    e += lg2<Real>(ax * ax + ay * ay) - lg2<Real>(bx * bx + by * by);
-                    ax = xn - zN.x; ay = yn - zN.y; bx = zo.x - zN.x; by = zo.y - zN.y;
-                    e += lg2<Real>(ax * ax + ay * ay) - lg2<Real>(bx * bx + by * by);
-                }
-                logA = mln2 * (lsum + (Real)0.5 * e) - (Real)0.5 * dr2;
+                const Real e = pair_lr(xn, yn, zo, z0) + pair_lr(xn, yn, zo, zN);
+                logA = mad((Real)-0.5, dr2, mfac * (lsum + (Real)0.5 * e));
             } else {
-                logA = (Real)0.5 * mln2 * lsum - (Real)0.25 * dr2;
+                logA = mad((Real)-0.25, dr2, (Real)0.5 * mfac * lsum);
             }
-            const bool acc = (logA >= 0) || (lnu < logA);
+            bool acc;
+            if constexpr (sizeof(Real) == 4)
+                acc = (logA >= 0.f) || (__shfl(rlnu, t, G) < logA);
+            else
+                acc = (logA >= 0.0) || (log(u01d((uint32_t)__shfl((int)rub, t, G))) < logA);
             V2 nz;
             nz.x = acc ? xn : zo.x;
             nz.y = acc ? yn : zo.y;
@@ -128,6 +161,181 @@ __global__ __launch_bounds__(64) void k_sweep2(SweepArgs a)
         }
     }
     for (int j = lane; j < NB; j += G) gpos[1 + j] = P[j];
+    if (lane == 0) {
+        gpos[0] = z0;
+        gpos[N] = zN;
+        a.accepted[walker] += accepted;
+    }
+}
+
+// The same chain with each lane's bulk particles in registers: lane l of a group holds bulk
+// particles j = l + G s (s < S, j < N - 1) in px[s], py[s]; the moves of z_0 and z_N pair
+// with every bulk particle, a bulk move i = 1 + G s + t takes its old position and random
+// numbers from lane t by __shfl.  Products, chunking, pads and acceptance are those of
+// k_sweep2, so for the same G the chain is bit-identical to it.
+template <int G, int S, typename Real>
+__global__ __launch_bounds__(64) void k_sweep2_reg(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    constexpr int WPB = 64 / G;
+    constexpr int C = LogOps<Real>::chunk;
+    constexpr int FULL = (S / C) * C;
+    const int N = a.N;
+    const int NB = N - 1;               // (S - 1) G < NB <= S G, checked by the launcher
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    if (walker >= a.walkers) return;    // whole groups leave; shuffles stay within a group
+    V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)walker * (N + 1);
+    Real px[S], py[S];
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        const V2 q = j < NB ? gpos[1 + j] : gpos[1];
+        px[s] = q.x;
+        py[s] = q.y;
+    });
+    V2 z0 = gpos[0], zN = gpos[N];
+    const bool pad_last = lane + G * (S - 1) >= NB;
+
+    const Real step = (Real)a.step;
+    const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
+    const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    unsigned accepted = 0;
+
+    // group sum of the bulk pair log-ratio of a move from zo to (xn, yn) with midpoint mid
+    auto bulk_lsum = [&](const V2 zo, const Real xn, const Real yn, const V2 mid) {
+        const Real dx = xn - zo.x, dy = yn - zo.y;
+        const Real dd = mad(dx, dx, dy * dy), tdx = dx + dx, tdy = dy + dy;
+        Real lsum = 0, pn = 1, pd = 1;
+        int ebal = 0;
+        static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            Real qx = px[u], qy = py[u];
+            if constexpr (u == S - 1) {
+                if (pad_last) {              // pads hold the midpoint
+                    qx = mid.x;
+                    qy = mid.y;
+                }
+            }
+            const Real bx = zo.x - qx, by = zo.y - qy;
+            const Real B = mad(bx, bx, by * by);
+            Real A;
+            if constexpr (sizeof(Real) == 4) {
+                A = mad(tdx, bx, mad(tdy, by, B + dd));
+            } else {
+                const Real ax = xn - qx, ay = yn - qy;
+                A = mad(ax, ax, ay * ay);
+            }
+            pn *= A;
+            pd *= B;
+            if constexpr ((u + 1) % C == 0 && u + 1 <= FULL) {
+                if constexpr (sizeof(Real) == 4) {
+                    int en, ed;
+                    pn = frexpf(pn, &en);
+                    pd = frexpf(pd, &ed);
+                    ebal += en - ed;
+                } else {
+                    lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+                    pn = 1;
+                    pd = 1;
+                }
+            }
+        });
+        if constexpr (sizeof(Real) == 4)
+            lsum = __builtin_amdgcn_logf(pn) - __builtin_amdgcn_logf(pd) + (float)ebal;
+        else
+            lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
+        return group_sum<G, Real>(lsum);
+    };
+    // the moves of z_0 (move 0) and z_N (move N): same numbers in every lane
+    auto end_move = [&](V2& z, uint32_t mv, uint32_t c1, uint32_t c2) {
+        uint32_t r[4];
+        philox(mv, c1, c2, (uint32_t)walker, k0, k1, r);
+        Real g1, g2;
+        gauss2(r[0], r[1], g1, g2);
+        const Real xn = mad(step, g1, z.x), yn = mad(step, g2, z.y);
+        V2 mid;
+        mid.x = (Real)0.5 * (z.x + xn);
+        mid.y = (Real)0.5 * (z.y + yn);
+        const Real lsum = bulk_lsum(z, xn, yn, mid);
+        const Real dr2 = mad(xn, xn, yn * yn) - mad(z.x, z.x, z.y * z.y);
+        const Real logA = mad((Real)-0.25, dr2, (Real)0.5 * mfac * lsum);
+        bool acc;
+        if constexpr (sizeof(Real) == 4)
+            acc = (logA >= 0.f) || (__builtin_amdgcn_logf(u01f(r[2])) * 0.69314718055994531f < logA);
+        else
+            acc = (logA >= 0.0) || (log(u01d(r[2])) < logA);
+        if (acc) {
+            z.x = xn;
+            z.y = yn;
+        }
+        accepted += acc ? 1u : 0u;
+    };
+
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = sweep_first + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        end_move(z0, 0u, c1, c2);
+        static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const int base = G * s;
+            // opaque counter: keeps the compiler from hoisting the sweep-invariant part of
+            // Philox for every slot out of the sweep loop
+            uint32_t ctr = (uint32_t)(1 + base + lane);
+            asm volatile("" : "+v"(ctr));
+            uint32_t r[4];
+            philox(ctr, c1, c2, (uint32_t)walker, k0, k1, r);
+            Real g1, g2;
+            gauss2(r[0], r[1], g1, g2);
+            float lnu = 0.f;
+            if constexpr (sizeof(Real) == 4) lnu = __builtin_amdgcn_logf(u01f(r[2])) * 0.69314718055994531f;
+            const uint32_t ub = r[2];
+            const int cnt = min(G, NB - base);
+            for (int t = 0; t < cnt; ++t) {
+                V2 zo;
+                zo.x = __shfl(px[s], t, G);
+                zo.y = __shfl(py[s], t, G);
+                const Real d1 = __shfl(g1, t, G), d2 = __shfl(g2, t, G);
+                const Real xn = mad(step, d1, zo.x), yn = mad(step, d2, zo.y);
+                V2 mid;
+                mid.x = (Real)0.5 * (zo.x + xn);
+                mid.y = (Real)0.5 * (zo.y + yn);
+                const bool own = lane == t;
+                if (own) {
+                    px[s] = mid.x;
+                    py[s] = mid.y;
+                }
+                const Real lsum = bulk_lsum(zo, xn, yn, mid);
+                const Real dr2 = mad(xn, xn, yn * yn) - mad(zo.x, zo.x, zo.y * zo.y);
+                const Real e = pair_lr(xn, yn, zo, z0) + pair_lr(xn, yn, zo, zN);
+                const Real logA = mad((Real)-0.5, dr2, mfac * (lsum + (Real)0.5 * e));
+                bool acc;
+                if constexpr (sizeof(Real) == 4)
+                    acc = (logA >= 0.f) || (__shfl(lnu, t, G) < logA);
+                else
+                    acc = (logA >= 0.0) || (log(u01d((uint32_t)__shfl((int)ub, t, G))) < logA);
+                if (own) {
+                    px[s] = acc ? xn : zo.x;
+                    py[s] = acc ? yn : zo.y;
+                }
+                accepted += acc ? 1u : 0u;
+            }
+        });
+        end_move(zN, (uint32_t)N, c1, c2);
+    }
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        if (j < NB) {
+            V2 q;
+            q.x = px[s];
+            q.y = py[s];
+            gpos[1 + j] = q;
+        }
+    });
     if (lane == 0) {
         gpos[0] = z0;
         gpos[N] = zN;
@@ -259,13 +467,27 @@ __global__ __launch_bounds__(256) void k_features2(Features2Args a)
     const double g0 = -0.25 * (x0 * x0 + y0 * y0 + xN * xN + yN * yN);
     const double er = wsh[wv][0], ei = wsh[wv][1];
     Real* W = reinterpret_cast<Real*>(a.W);
-    for (int l = lane; l < a.lcut; l += 64) {
-        const double mag = exp(g0 + l * lx - a.ln2pifact[l]);
-        double s, c;
-        sincos(l * ax, &s, &c);
-        const double wr = mag * (c * er - s * ei), wi = mag * (c * ei + s * er);
-        W[(size_t)(2 * l) * a.wld + row] = (Real)wr;
-        W[(size_t)(2 * l + 1) * a.wld + row] = (Real)wi;
+    if constexpr (sizeof(Real) == 4) {
+        // FP32 weights: log-magnitude and angle (in revolutions, reduced to [0, 1)) in FP64,
+        // then the hardware exp, sin and cos (relative error ~1e-6 per weight)
+        const double arev = ax * 0.15915494309189535;
+        const float fer = (float)er, fei = (float)ei;
+        for (int l = lane; l < a.lcut; l += 64) {
+            const float mag = __expf((float)(g0 + l * lx - a.ln2pifact[l]));
+            double tr = l * arev;
+            tr -= floor(tr);
+            const float s = __builtin_amdgcn_sinf((float)tr), c = __builtin_amdgcn_cosf((float)tr);
+            W[(size_t)(2 * l) * a.wld + row] = mag * (c * fer - s * fei);
+            W[(size_t)(2 * l + 1) * a.wld + row] = mag * (c * fei + s * fer);
+        }
+    } else {
+        for (int l = lane; l < a.lcut; l += 64) {
+            const double mag = exp(g0 + l * lx - a.ln2pifact[l]);
+            double s, c;
+            sincos(l * ax, &s, &c);
+            W[(size_t)(2 * l) * a.wld + row] = mag * (c * er - s * ei);
+            W[(size_t)(2 * l + 1) * a.wld + row] = mag * (c * ei + s * er);
+        }
     }
 }
 
@@ -310,6 +532,39 @@ hipError_t launch_sweep2(const SweepArgs& a, int prec, int G, hipStream_t s)
     if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
     if (a.N < 2) return hipErrorInvalidValue;
     return prec == kFP64 ? sweep2_dispatch<double>(a, G, s) : sweep2_dispatch<float>(a, G, s);
+}
+
+bool sweep2_reg_supported(int G, int N)
+{
+    const int S = (N - 1 + G - 1) / G;   // N - 1 bulk particles
+    return N >= 2 && (G == 8 || G == 16) && (S == 13 || S == 25);
+}
+
+template <int G, int S, typename Real>
+static hipError_t sweep2_reg_gs(const SweepArgs& a, hipStream_t s)
+{
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    hipLaunchKernelGGL((k_sweep2_reg<G, S, Real>), grid, dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t sweep2_reg_dispatch(const SweepArgs& a, int G, hipStream_t s)
+{
+    const int S = (a.N - 1 + G - 1) / G;
+    if (G == 8 && S == 13) return sweep2_reg_gs<8, 13, Real>(a, s);
+    if (G == 8 && S == 25) return sweep2_reg_gs<8, 25, Real>(a, s);
+    if (G == 16 && S == 13) return sweep2_reg_gs<16, 13, Real>(a, s);
+    if (G == 16 && S == 25) return sweep2_reg_gs<16, 25, Real>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep2_reg(const SweepArgs& a, int prec, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    if (!sweep2_reg_supported(G, a.N)) return hipErrorInvalidValue;
+    return prec == kFP64 ? sweep2_reg_dispatch<double>(a, G, s) : sweep2_reg_dispatch<float>(a, G, s);
 }
 
 template <typename Real>

@@ -697,8 +697,8 @@ def test_register_sweep_graphs_match_eager():
 
 def test_autotuned_lanes_per_chain_kind():
     """The lane autotuner caches its choice per (N, walkers, precision, chain kind): an edge
-    engine's register mapping is not reused by the two-sided chain, which has no register
-    sweep."""
+    engine's mapping (registers need ceil(N / lanes) = 13 or 25) is not reused by the two-sided
+    chain, whose register sweep needs ceil((N - 1) / lanes) = 13 or 25."""
     from laughlinedge_amd import Engine
     from laughlinedge_amd.engine import OccupationEngine
     N, m, W = 100, 3, 2048
@@ -711,4 +711,28 @@ def test_autotuned_lanes_per_chain_kind():
         o.tune_step()
         o.burnin(2)
         o.sample(1)
-        assert o.info.sweep_in_registers == 0 and o.info.lanes_per_walker in (4, 8, 16, 32, 64)
+        G = o.info.lanes_per_walker
+        assert G in (4, 8, 16, 32, 64)
+        assert not o.info.sweep_in_registers or (G in (8, 16) and -(-(N - 1) // G) in (13, 25))
+
+
+@pytest.mark.parametrize("G,prec", [(8, 0), (16, 0), (8, 1), (16, 1)])
+def test_twosided_register_sweep_matches_lds(G, prec):
+    """The two-sided chain with its bulk particles in registers (N = 200: 199 bulk particles in
+    25 slots of 8 lanes or 13 slots of 16) gives the LDS kernel's positions bit for bit."""
+    from laughlinedge_amd.engine import OccupationEngine
+    N, m = 200, 3
+    print(f"case two-sided register sweep G={G} prec={prec}", flush=True)
+    out = []
+    for reg in (0, 1):
+        with OccupationEngine(N, m, 0, 10, walkers=256, samples=1, precision=prec, lanes_per_walker=G,
+                              sweep_registers=reg, step=0.9, seed=11) as e:
+            assert e.info.sweep_in_registers == reg
+            e.init_walkers()
+            p0 = e.positions()
+            e.burnin(3)
+            out.append((p0, e.positions()))
+    assert np.array_equal(out[0][0], out[1][0])          # same start
+    assert np.array_equal(out[0][1], out[1][1])          # same chain
+    moved = np.mean(np.any(out[0][1] != out[0][0], axis=-1))
+    assert 0.5 < moved < 1.0, moved                      # three sweeps moved most particles

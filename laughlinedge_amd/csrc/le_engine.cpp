@@ -58,7 +58,7 @@ hipError_t dalloc(T** p, size_t count)
 // Lanes per walker without timing: the smallest power of two with at most 16 particle
 // slots per lane (FP32), 32 for FP64, whose sweeps are the slower ones and prefer longer
 // per-lane loops (N = 200 on MI355X: 0.52 ms FP32 at 16 lanes, 1.54 ms FP64 at 8 lanes).
-constexpr int kPsLd = 48;        // Re/Im p_1..p_24 per sample
+constexpr int kPsLd = 50;        // Re/Im p_1..p_24 per sample (+ the two-sided chain's phase)
 
 // Sweep mapping: lanes per walker in the low byte; kRegSweep marks the register-resident
 // kernel (k_sweep_reg) instead of the LDS one.
@@ -228,6 +228,8 @@ hipError_t le_engine::features2(const void* positions, int pos_is_double, int ns
     f.side = side_d;
     f.ln2pifact = lnfact_d;
     f.invR = 1.0 / R;
+    f.ps = ps_d;
+    f.ps_ld = kPsLd;
     return le::launch_features2(f, prec, stream);
 }
 

@@ -73,6 +73,8 @@ struct Features2Args {
     const int* side;         // [D] 0: left power sums, 1: right
     const double* ln2pifact; // [lcut] ln(2 pi l!)
     double invR;
+    double* ps;              // [walkers][ps_ld] scratch: bulk power sums (2 K) and the unit phase (2)
+    int ps_ld;               // >= 2 K + 2
 };
 
 // A 64x64 upper Gram tile and the 16x16 blocks of it that hold requested entries, two per

@@ -148,6 +148,8 @@ struct le_engine {
     };
     std::vector<Batch> batches;
     le::GramJob* jobtab[2] = {nullptr, nullptr};
+    double* cpool = nullptr;         // every job's FP64 accumulators, one allocation: one clear
+    size_t cpool_elems = 0;
     bool weights_gram = false;       // occupation weights on a block of <= 64 x 64 real columns:
                                      // one k_gram_weights launch for all of them
     int ntiles = 0;                  // tiles of the S/H job
@@ -262,13 +264,10 @@ le_engine::~le_engine()
         if (drained[b]) (void)hipEventDestroy(drained[b]);
     }
     if (gstream) (void)hipStreamDestroy(gstream);
-    for (auto& j : jobs) {
-        (void)hipFree(j.tiles);
-        (void)hipFree(j.C[0]);
-        (void)hipFree(j.C[1]);
-    }
+    for (auto& j : jobs) (void)hipFree(j.tiles);
     (void)hipFree(jobtab[0]);
     (void)hipFree(jobtab[1]);
+    (void)hipFree(cpool);
     (void)hipFree(items_d);
     (void)hipFree(rec_d);
     (void)hipFree(lnfact_d);
@@ -439,16 +438,20 @@ int le_engine::setup(const le_observables* obs)
         j.ntiles = (int)th.size();
         HIPCHK(dalloc(&j.tiles, (size_t)std::max(1, j.ntiles)));
         if (j.ntiles) HIPCHK(hipMemcpy(j.tiles, th.data(), j.ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
-        for (int q = 0; q < j.nw; ++q) {
-            HIPCHK(dalloc(&j.C[q], (size_t)ldx * ldx));
-            HIPCHK(le::launch_zero(j.C[q], ldx * ldx, stream));
-        }
+        cpool_elems += (size_t)j.nw * ldx * ldx;
         for (int q = 0; q < j.nw; ++q)
             for (auto& pr : j.blocks) {
                 items.push_back({j.wid[q], pr.first, pr.second, rec_elems});
                 rec_elems += (long)basis.dim(pr.first) * basis.dim(pr.second);
             }
     }
+    HIPCHK(dalloc(&cpool, std::max<size_t>(1, cpool_elems)));
+    {
+        size_t o = 0;
+        for (auto& j : jobs)
+            for (int q = 0; q < j.nw; ++q, o += (size_t)ldx * ldx) j.C[q] = cpool + o;
+    }
+    HIPCHK(le::launch_zero(cpool, (long)cpool_elems, stream));
     ntiles = jobs[0].ntiles;
     std::vector<le::ExtractItem> xi;
     for (auto& it : items) {
@@ -1021,8 +1024,7 @@ int le_engine::close_block(long long samples)
     HIPCHK(le::launch_extract(x, stream));
     std::vector<double> h(2 * rec_elems);
     HIPCHK(hipMemcpyAsync(h.data(), rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
-    for (auto& j : jobs)
-        for (int q = 0; q < j.nw; ++q) HIPCHK(le::launch_zero(j.C[q], ldx * ldx, stream));
+    HIPCHK(le::launch_zero(cpool, (long)cpool_elems, stream));
     HIPCHK(hipStreamSynchronize(stream));
     rec.push_back(std::move(h));
     blk_samples.push_back(samples);
@@ -1381,8 +1383,7 @@ extern "C" int le_reset_moments(le_engine* e)
     if (!e) return fail(LE_ERR_ARG, "null engine");
     e->rec.clear();
     e->blk_samples.clear();
-    for (auto& j : e->jobs)
-        for (int q = 0; q < j.nw; ++q) HIPCHK(le::launch_zero(j.C[q], e->ldx * e->ldx, e->stream));
+    HIPCHK(le::launch_zero(e->cpool, (long)e->cpool_elems, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return LE_OK;
 }

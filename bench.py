@@ -439,15 +439,15 @@ def forms(args, thins):
     fs = args.form_steps
     nblk = 32
     table = (("plain_fp64", 1, False, 0.5, 0), ("plain_fp32", 0, False, 0.5, 0), ("plain_fp32_wide", 0, False, 0.3, 0),
-             ("plain_bf16x3_wide", 2, False, 0.3, 0), ("plain_fp32_wide_cmrot", 0, False, 0.3, 1),
-             ("plain_fp32_wide_sym", 0, False, 0.3, 2), ("plain_bf16x3_wide_sym", 2, False, 0.3, 2),
+             ("plain_bf16x6_wide", 3, False, 0.3, 0), ("plain_fp32_wide_cmrot", 0, False, 0.3, 1),
+             ("plain_fp32_wide_sym", 0, False, 0.3, 2), ("plain_bf16x6_wide_sym", 3, False, 0.3, 2),
              ("learned_fp32", 0, True, 0.5, 0), ("learned_fp32_sym", 0, True, 0.5, 2))
     for name, prec, learned, target, sym in table:
         eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target, sym=sym)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
-        r = {"precision": ["fp32", "fp64", "bf16x3"][prec], "step": step, "target_acceptance": target,
+        r = {"precision": ["fp32", "fp64", "bf16x3", "bf16x6"][prec], "step": step, "target_acceptance": target,
              "symmetry_moves": ["none", "centre of mass + rotation", "centre of mass + rotation + dilation"][sym]}
         if learned:
             t0 = time.perf_counter()
@@ -534,7 +534,7 @@ def precision_check(eng, sig, n_fixed=8192):
     t_cpu = time.perf_counter() - t0
     n = pos.shape[0]
     out = {"fixed_samples": int(n), "oracle_seconds": t_cpu}
-    for label, prec in (("fp32", 0), ("bf16x3", 2)):
+    for label, prec in (("fp32", 0), ("bf16x3", 2), ("bf16x6", 3)):
         with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=1024, samples=1,
                     precision=prec) as eg:
             eg.accumulate_positions(pos)

@@ -61,6 +61,8 @@ extern "C" {
 #define LE_PREC_FP64 1      /* FP64 sampler, FP64 MFMA accumulation (the plain FP64 form) */
 #define LE_PREC_BF16X3 2    /* FP32 sampler; S/H tiles from BF16 hi/lo split products
                                (hi.hi + hi.lo + lo.hi) on MFMA, FP32 tiles flushed to FP64 */
+#define LE_PREC_BF16X6 3    /* as LE_PREC_BF16X3 with a hi/mid/lo split: the six products
+                               t_a.t_b with a + b < 3, dropping only terms of order 2^-24 */
 
 /* sys_params.h, at run time.  V(r) = 0.5 w r^2 + eps r^4 + strength (r / RCl)^exponent,
  * RCl = sqrt(2 m N) (system_parameters.cpp:25-32 plus the w, eps form). */

@@ -84,7 +84,7 @@ struct le_engine {
     le_mc_params mc{};
     le::Basis basis;
     int prec = LE_PREC_FP32;        // data precision of positions, samples and tiles
-    int gram_prec = le::kFP32;      // accumulation kernel (kBF16X3: split products)
+    int gram_prec = le::kFP32;      // accumulation kernel (kBF16X3, kBF16X6: split products)
     size_t rsz = 4;
     int G = 8;
     int spl = 1;                 // kept samples per launch round
@@ -507,7 +507,7 @@ int le_engine::setup(const le_observables* obs)
         batches.push_back({(int)a, (int)(b - a)});
         a = b;
     }
-    weights_gram = two_sided && gram_prec != le::kBF16X3 && 2 * basis.dim(0) <= 64 && n_user > 0;
+    weights_gram = two_sided && (gram_prec == le::kFP32 || gram_prec == le::kFP64) && 2 * basis.dim(0) <= 64 && n_user > 0;
     if (weights_gram) {
         const int P = 4 * basis.dim(0) * basis.dim(1), nwt = (n_user + 63) / 64;
         gram_flop_row = 0;
@@ -597,7 +597,7 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
     if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
         return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
-    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64 && mc->precision != LE_PREC_BF16X3)
+    if (mc->precision < LE_PREC_FP32 || mc->precision > LE_PREC_BF16X6)
         return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
@@ -614,7 +614,10 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     e->sys = *sys;
     e->mc = *mc;
     e->prec = mc->precision == LE_PREC_FP64 ? LE_PREC_FP64 : LE_PREC_FP32;
-    e->gram_prec = mc->precision == LE_PREC_FP64 ? le::kFP64 : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3 : le::kFP32;
+    e->gram_prec = mc->precision == LE_PREC_FP64     ? le::kFP64
+                  : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3
+                  : mc->precision == LE_PREC_BF16X6 ? le::kBF16X6
+                                                    : le::kFP32;
     e->basis.build(sys->particles_number, dls);
     // capacity checks (reference: em_spectrum.cu:235-252 exits); here they are codes
     if (e->basis.K > 24) {
@@ -1840,7 +1843,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return fail(LE_ERR_ARG, "need particles_number >= 2 and inverse_filling >= 1");
     if (mc->walkers < 1 || mc->thin < 1 || mc->n_blocks < 1 || mc->samples < 0 || mc->burnin_sweeps < 0)
         return fail(LE_ERR_ARG, "walkers, thin and n_blocks must be >= 1; samples, burn-in >= 0");
-    if (mc->precision != LE_PREC_FP32 && mc->precision != LE_PREC_FP64 && mc->precision != LE_PREC_BF16X3)
+    if (mc->precision < LE_PREC_FP32 || mc->precision > LE_PREC_BF16X6)
         return fail(LE_ERR_ARG, "unknown precision");
     if (mc->lanes_per_walker > 0 && !le::sweep_supported(mc->lanes_per_walker))
         return fail(LE_ERR_ARG, "lanes_per_walker must be -1, 0, 2, 4, 8, 16, 32 or 64");
@@ -1855,7 +1858,10 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
     e->sys = *sys;
     e->mc = *mc;
     e->prec = mc->precision == LE_PREC_FP64 ? LE_PREC_FP64 : LE_PREC_FP32;
-    e->gram_prec = mc->precision == LE_PREC_FP64 ? le::kFP64 : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3 : le::kFP32;
+    e->gram_prec = mc->precision == LE_PREC_FP64     ? le::kFP64
+                  : mc->precision == LE_PREC_BF16X3 ? le::kBF16X3
+                  : mc->precision == LE_PREC_BF16X6 ? le::kBF16X6
+                                                    : le::kFP32;
     e->two_sided = true;
     e->lcut = lcut;
     e->n_user = 2 * lcut;

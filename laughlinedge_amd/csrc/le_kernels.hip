@@ -882,12 +882,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
 }
 
 // ---------------------------------------------------------------------------
-// BF16 hi/lo split Gram tiles.  Each staged FP32 value x (times the weight on the row side)
-// is split into hi = bf16(x) and lo = bf16(x - hi) (round to nearest even), and a tile
-// accumulates hi.hi' + hi.lo' + lo.hi' on v_mfma_f32_16x16x32_bf16.  Each product then
-// carries a relative error of order 2^-16 (the residual x - hi - lo and the dropped lo.lo');
-// on heavy-tailed monomials it averages down slowly: N = 200, dL 1..16, 8192 samples give
-// 5e-6 relative (Frobenius) against FP64, 3.8e-6 with lo.lo' added, so it is left out.  One MFMA covers the 32 staged
+// BF16 split Gram tiles.  Each staged FP32 value x (times the weight on the row side) is
+// split into NS bf16 terms, t0 = bf16(x), t1 = bf16(x - t0), t2 = bf16(x - t0 - t1) (round to
+// nearest even; every residual is exact in FP32), and a tile accumulates the products
+// t_a.t_b' with a + b < NS on v_mfma_f32_16x16x32_bf16.
+// - NS = 2 (hi/lo, three products): each product carries a relative error of order 2^-16
+//   (the residual x - hi - lo and the dropped lo.lo'); on heavy-tailed monomials it averages
+//   down slowly: N = 200, dL 1..16, 8192 samples give 5e-6 relative (Frobenius) against FP64,
+//   3.8e-6 with lo.lo' added.
+// - NS = 3 (hi/mid/lo, six products): the dropped terms are of order 2^-24, that of the FP32
+//   operands themselves.
+// One MFMA covers the 32 staged
 // samples.  Lane l supplies row (or column) l % 16 with the 8 samples 8 (l / 16) .. +7 of
 // both operands; as both use the same lane-to-sample map, the sum over samples is the same
 // whatever order the hardware takes them in.  The result layout is that of the FP32
@@ -898,21 +903,26 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kLdsB = 40;
 
-__device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo)
+template <int NS>
+__device__ __forceinline__ void split8(const float* x, bf16x8 (&t)[NS])
 {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16)x[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(x[e] - (float)h);
+        float r = x[e];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            const __bf16 h = (__bf16)r;
+            t[a][e] = h;
+            r -= (float)h;
+        }
     }
 }
 
-template <int NW>
+template <int NW, int NS>
 __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
 {
-    __shared__ __align__(16) __bf16 ai[NW][2][64][kLdsB];   // row side: weight q, hi/lo
-    __shared__ __align__(16) __bf16 bj[2][64][kLdsB];       // column side: hi/lo
+    __shared__ __align__(16) __bf16 ai[NW][NS][64][kLdsB];   // row side: weight q, split term
+    __shared__ __align__(16) __bf16 bj[NS][64][kLdsB];       // column side: split term
 
     int tile_idx, split;
     if (!gram_block(g, tile_idx, split)) return;
@@ -967,7 +977,7 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
-        bf16x8 h, l;
+        bf16x8 t[NS];
         __syncthreads();   // the previous step's fragments have been read
         if (!side) {
 #pragma unroll
@@ -975,14 +985,14 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
                 float y[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) y[e] = x[e] * wv[q][e];
-                split8(y, h, l);
-                *reinterpret_cast<bf16x8*>(&ai[q][0][sc][sk]) = h;
-                *reinterpret_cast<bf16x8*>(&ai[q][1][sc][sk]) = l;
+                split8<NS>(y, t);
+#pragma unroll
+                for (int a = 0; a < NS; ++a) *reinterpret_cast<bf16x8*>(&ai[q][a][sc][sk]) = t[a];
             }
         } else {
-            split8(x, h, l);
-            *reinterpret_cast<bf16x8*>(&bj[0][sc][sk]) = h;
-            *reinterpret_cast<bf16x8*>(&bj[1][sc][sk]) = l;
+            split8<NS>(x, t);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) *reinterpret_cast<bf16x8*>(&bj[a][sc][sk]) = t[a];
         }
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
@@ -991,18 +1001,22 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 const int rr = ro + l16;
-                const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ai[q][0][rr][8 * k4]);
-                const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ai[q][1][rr][8 * k4]);
+                bf16x8 av[NS];
+#pragma unroll
+                for (int u = 0; u < NS; ++u) av[u] = *reinterpret_cast<const bf16x8*>(&ai[q][u][rr][8 * k4]);
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     const int c = co[a] + l16;
-                    const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&bj[0][c][8 * k4]);
-                    const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&bj[1][c][8 * k4]);
-                    floatx4 t = acc[q][a];
-                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, t, 0, 0, 0);
-                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, t, 0, 0, 0);
-                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, t, 0, 0, 0);
-                    acc[q][a] = t;
+                    floatx4 d = acc[q][a];
+                    // smallest products first
+#pragma unroll
+                    for (int v = NS - 1; v >= 0; --v) {
+                        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&bj[v][c][8 * k4]);
+#pragma unroll
+                        for (int u = NS - 1 - v; u >= 0; --u)
+                            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[u], bv, d, 0, 0, 0);
+                    }
+                    acc[q][a] = d;
                 }
             }
         }
@@ -1763,10 +1777,15 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
     if (a.splits != splits) return hipErrorInvalidValue;
     if (a.jobs ? (a.njobs < 1 || a.njobs > 65535) : a.njobs > 1) return hipErrorInvalidValue;
     dim3 grid((unsigned)((long)a.ntiles * splits), (unsigned)(a.jobs ? a.njobs : 1));
-    if (prec == kBF16X3) {
+    if (prec == kBF16X3 || prec == kBF16X6) {
         if (a.rows_per_split % kKC) return hipErrorInvalidValue;
-        if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2>), grid, dim3(512), 0, s, a);
-        else hipLaunchKernelGGL((k_gram_bf16<1>), grid, dim3(512), 0, s, a);
+        if (prec == kBF16X3) {
+            if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2, 2>), grid, dim3(512), 0, s, a);
+            else hipLaunchKernelGGL((k_gram_bf16<1, 2>), grid, dim3(512), 0, s, a);
+        } else {
+            if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2, 3>), grid, dim3(512), 0, s, a);
+            else hipLaunchKernelGGL((k_gram_bf16<1, 3>), grid, dim3(512), 0, s, a);
+        }
     } else if (prec == kFP64) {
         if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(512), 0, s, a);
         else hipLaunchKernelGGL((k_gram<double, 1>), grid, dim3(512), 0, s, a);

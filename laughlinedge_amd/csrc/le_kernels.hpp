@@ -157,7 +157,7 @@ struct GenEigProblem {
 };
 
 // precision selector for templated paths
-enum Prec { kFP32 = 0, kFP64 = 1, kBF16X3 = 2 };   // kBF16X3: Gram only (FP32 data)
+enum Prec { kFP32 = 0, kFP64 = 1, kBF16X3 = 2, kBF16X6 = 3 };   // kBF16X*: Gram only (FP32 data)
 
 hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
                                  unsigned long long seed, hipStream_t s);

@@ -24,6 +24,7 @@ LE_ERR_NOCONV = 7
 PREC_FP32 = 0
 PREC_FP64 = 1
 PREC_BF16X3 = 2
+PREC_BF16X6 = 3
 
 EXPORTED = [
     "le_mc_params_default", "le_last_error", "le_engine_create", "le_engine_destroy",

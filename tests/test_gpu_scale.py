@@ -140,10 +140,10 @@ def test_scale_evolution_of_kicks(sampled):
     assert ratio < 1e-2, ratio
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5)])
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5), (3, 1e-6)])
 def test_config1_fixed_samples(oracle_lib, prec, tol):
     """nu = 1/3, N = 100, V = 0.025 r^2 + 5e-5 r^4, dL = 0..12 (D = 272), 16384 walkers, seed 7:
-    65536 configurations of the GPU chain through the FP32 (or BF16-split) accumulation and
+    65536 configurations of the GPU chain through the FP32 (or a BF16-split) accumulation and
     through the FP64 oracle; every sector's S and H within tol relative Frobenius."""
     from laughlinedge_amd import Engine
     n100, secs = 100, list(range(13))

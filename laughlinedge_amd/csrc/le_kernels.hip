@@ -195,7 +195,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 #pragma unroll
                 for (int blk = 2; blk >= 1; blk >>= 1) {
                     if (S - s >= blk) {
-                        V2 q[blk];
+                        V2 q[2];
 #pragma unroll
                         for (int u = 0; u < blk; ++u) q[u] = P[lane + G * (s + u)];
 #pragma unroll

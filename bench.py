@@ -169,7 +169,7 @@ def main():
     ap.add_argument("--walkers", type=int, default=WALKERS)
     ap.add_argument("--trace-walkers", type=int, default=512)
     ap.add_argument("--form-steps", type=int, default=96, help="kept samples per (form, thinning)")
-    ap.add_argument("--thins", default="1,2,4")
+    ap.add_argument("--thins", default="1,2,4,8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")

@@ -1702,6 +1702,7 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
     double *dV = nullptr, *dT = nullptr, *dX = nullptr, *dW = nullptr, *dR = nullptr, *dlam = nullptr, *db = nullptr,
            *dbn = nullptr, *dks = nullptr, *dy = nullptr, *dcoef = nullptr, *dout = nullptr, *dA = nullptr;
     unsigned long long* demax = nullptr;
+    unsigned long long* hemax = nullptr;   // pinned: the per-step readback of the error maximum
     hipError_t h = hipSuccess;
     auto chk = [&](hipError_t x) { if (h == hipSuccess) h = x; };
     chk(dalloc(&dV, 2 * DD));
@@ -1718,6 +1719,7 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
     chk(dalloc(&dcoef, 64));
     chk(dalloc(&dout, 2 * (size_t)nt * n_u * (n_rho + 1)));
     chk(dalloc(&demax, 1));
+    chk(hipHostMalloc((void**)&hemax, sizeof(unsigned long long), hipHostMallocDefault));
     // 2. W_u = V^H U_u V, R_l = V^H rho_l V
     if (h == hipSuccess) chk(hipMemcpy(dV, V.data(), sizeof(double) * 2 * DD, hipMemcpyHostToDevice));
     auto sandwich = [&](const double* host, double* dst) {
@@ -1775,15 +1777,15 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
                 for (int sI = 0; sI < 6 && h == hipSuccess; ++sI) {
                     const double ts = t + cc[sI] * hs;
                     double* kz = dks + 2 * (size_t)sI * n;
-                    chk(le::launch_stage_prep(db, dks, sI, dcoef + 8 * sI, hs, ts, dlam, dy, kz, D, n_u, st));
+                    chk(le::launch_stage_prep(db, dks, sI, dcoef + 8 * sI, hs, ts, dlam, dy, kz, D, n_u,
+                                              sI == 0 ? demax : nullptr, st));
                     chk(le::launch_stage_hemv(dW, dy, kz, F(ctx, ts), ts, dlam, D, n_u, st));
                 }
-                chk(hipMemsetAsync(demax, 0, sizeof(unsigned long long), st));
                 chk(le::launch_rk_combine(db, dks, dcoef + 48, dcoef + 56, hs, eps_abs, eps_rel, dbn, demax, n, st));
-                unsigned long long bits = 0;
-                chk(hipMemcpyAsync(&bits, demax, sizeof(bits), hipMemcpyDeviceToHost, st));
+                chk(hipMemcpyAsync(hemax, demax, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
                 chk(hipStreamSynchronize(st));
                 if (h != hipSuccess) break;
+                const unsigned long long bits = *hemax;
                 double emax;
                 std::memcpy(&emax, &bits, sizeof(emax));
                 ++steps;
@@ -1818,6 +1820,7 @@ extern "C" int le_evolve(int D, int n_blocks, const int* block_dims, const doubl
     }
     for (double* p : {dV, dT, dX, dW, dR, dlam, db, dbn, dks, dy, dcoef, dout, dA}) (void)hipFree(p);
     (void)hipFree(demax);
+    if (hemax) (void)hipHostFree(hemax);
     (void)hipStreamDestroy(st);
     if (h != hipSuccess) return fail(LE_ERR_HIP, std::string("le_evolve: ") + hipGetErrorString(h));
     for (int u = 0; u < n_u; ++u)

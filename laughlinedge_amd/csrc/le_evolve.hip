@@ -66,9 +66,10 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2* A, const double2* 
 // y_u[m] = e^{-i lam_m t} (b_u[m] + h sum_r a_r k_r,u[m]); kz (this stage's k) is cleared for
 // the tile products that add into it
 __global__ void k_stage_prep(const double2* b, const double2* ks, int nstage, const double* coef, double h, double t,
-                             const double* lam, double2* y, double2* kz, int D, int nu)
+                             const double* lam, double2* y, double2* kz, int D, int nu, unsigned long long* emax)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (emax && g == 0) *emax = 0ull;   // the step's error maximum, raised by k_rk_combine
     if (g >= (long)D * nu) return;
     const int m = (int)(g % D);
     double2 v = b[g];
@@ -310,11 +311,12 @@ hipError_t launch_zgemm(const double* A, const double* B, double* C, int M, int 
 }
 
 hipError_t launch_stage_prep(const double* b, const double* ks, int nstage, const double* coef, double h, double t,
-                             const double* lam, double* y, double* kz, int D, int nu, hipStream_t s)
+                             const double* lam, double* y, double* kz, int D, int nu, unsigned long long* emax,
+                             hipStream_t s)
 {
     const long n = (long)D * nu;
     hipLaunchKernelGGL(k_stage_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double2*)b,
-                       (const double2*)ks, nstage, coef, h, t, lam, (double2*)y, (double2*)kz, D, nu);
+                       (const double2*)ks, nstage, coef, h, t, lam, (double2*)y, (double2*)kz, D, nu, emax);
     return hipGetLastError();
 }
 

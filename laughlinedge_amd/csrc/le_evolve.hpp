@@ -6,9 +6,10 @@ namespace le {
 
 hipError_t launch_zgemm(const double* A, const double* B, double* C, int M, int N, int K, int lda, int ldb, int ldc,
                         int conjA, hipStream_t s);
-// also clears kz, the stage's k that launch_stage_hemv adds into
+// also clears kz, the stage's k that launch_stage_hemv adds into, and *emax when emax is not null
 hipError_t launch_stage_prep(const double* b, const double* ks, int nstage, const double* coef, double h, double t,
-                             const double* lam, double* y, double* kz, int D, int nu, hipStream_t s);
+                             const double* lam, double* y, double* kz, int D, int nu, unsigned long long* emax,
+                             hipStream_t s);
 hipError_t launch_hermitian_part(double* W, int D, int nu, hipStream_t s);
 // kout += -i F e^{i lam t} W y for Hermitian W (its upper 64 x 64 tiles are read)
 hipError_t launch_stage_hemv(const double* W, const double* y, double* kout, double F, double t, const double* lam,

@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
 // registers every 256 samples, and the result is added to the weights' FP64 accumulators.
 // ---------------------------------------------------------------------------
 template <typename Real>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram_weights(WeightGramArgs g)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 4))) void k_gram_weights(WeightGramArgs g)
 {
     using acc_t = typename Mfma<Real>::acc_t;
     constexpr bool kF32 = sizeof(Real) == 4;

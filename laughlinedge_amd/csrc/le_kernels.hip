@@ -493,10 +493,10 @@ constexpr int kMaxPow = 128;
 constexpr int kMaxStatesLds = 4096;
 constexpr int kLps = 4;          // lanes per sample in k_psums
 
-template <typename Real, int KMAX, bool HARM>
+template <typename Real, int KMAX, int NH>   // NH: harmonic pairs held (0, or at least a.nh)
 __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
 {
-    constexpr int NH = HARM ? kMaxHarmonics : 0;
+    constexpr bool HARM = NH > 0;
     constexpr int NQ = 2 * KMAX + 1 + 2 * NH;   // Re/Im p_1..p_K, V, cos/sin sums
     constexpr int QV = 2 * KMAX;
     constexpr int QH = 2 * KMAX + 1;
@@ -1735,17 +1735,17 @@ hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
     return prec == kFP64 ? sweep_dispatch<double>(a, G, s) : sweep_dispatch<float>(a, G, s);
 }
 
-template <typename Real, bool HARM>
+template <typename Real, int NH>
 static hipError_t features_h(const FeatureArgs& a, hipStream_t s)
 {
     const long threads = (long)a.walkers * kLps;
     dim3 grid_ps((unsigned)((threads + 255) / 256));
     if (a.K <= 8)
-        hipLaunchKernelGGL((k_psums<Real, 8, HARM>), grid_ps, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 8, NH>), grid_ps, dim3(256), 0, s, a);
     else if (a.K <= 16)
-        hipLaunchKernelGGL((k_psums<Real, 16, HARM>), grid_ps, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 16, NH>), grid_ps, dim3(256), 0, s, a);
     else if (a.K <= 24)
-        hipLaunchKernelGGL((k_psums<Real, 24, HARM>), grid_ps, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_psums<Real, 24, NH>), grid_ps, dim3(256), 0, s, a);
     else
         return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
@@ -1762,7 +1762,11 @@ template <typename Real>
 static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
 {
     if (a.npw > kMaxPow || !a.ps || a.ps_ld < 2 * a.K) return hipErrorInvalidValue;
-    return a.nh > 0 ? features_h<Real, true>(a, s) : features_h<Real, false>(a, s);
+    // harmonic accumulators are FP64 registers: hold only as many pairs as the call has
+    if (a.nh > kMaxHarmonics) return hipErrorInvalidValue;
+    if (a.nh == 0) return features_h<Real, 0>(a, s);
+    if (a.nh <= 2) return features_h<Real, 2>(a, s);
+    return features_h<Real, kMaxHarmonics>(a, s);
 }
 
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s)

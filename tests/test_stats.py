@@ -61,3 +61,21 @@ def test_jackknife_shrinks_the_rayleigh_ritz_bias():
     assert bias_full < -3 * np.std(fulls) / np.sqrt(reps), bias_full   # well resolved
     assert abs(bias_corr) < 0.3 * abs(bias_full), (bias_full, bias_corr)
     assert 0.7 < np.mean(sigs) / np.std(fulls) < 1.4, (np.mean(sigs), np.std(fulls))
+
+
+def test_bench_spectra_comparison():
+    """bench.compare_spectra: |E - E_ref| / sqrt(s^2 + s_ref^2) over every eigenvalue, sectors
+    with a missing spectrum skipped, the lowest eigenvalue of each sector reported."""
+    import bench
+    n = len(bench.SECTORS)
+    spec = [np.array([1.0, 2.0]) for _ in range(n)]
+    ref = [np.array([1.3, 2.0]) for _ in range(n)]
+    sig = [np.array([0.3, 1.0]) for _ in range(n)]
+    ref_sig = [np.array([0.4, 1.0]) for _ in range(n)]
+    spec[1] = None
+    out = bench.compare_spectra(spec, sig, ref, ref_sig)
+    assert out["eigenvalues"] == 2 * (n - 1)
+    assert np.isclose(out["max_dev_in_combined_sigma"], 0.3 / 0.5)
+    assert out["frac_within_1_sigma"] == 1.0
+    assert str(bench.SECTORS[1]) not in out["lowest_per_sector_dev"]
+    assert out["lowest_per_sector_dev"][str(bench.SECTORS[0])] == 0.6

@@ -6,9 +6,12 @@ Workload (N=200 Laughlin droplet, nu=1/3, all sectors dL=1..16 at once, D=914):
 A step is one Metropolis sweep of every walker followed by one kept sample per walker,
 accumulated into every sector's S and H (one "sample-update" per walker and step).
 
-Prints ONE JSON line.  value = S,H sample-updates/s of the production form (FP32 sampler,
-FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
-  roofline      the Gram accumulation kernel (MFMA bound), live HIP-event timing
+The last stdout line is ONE compact JSON object (a few KB); the full result goes to
+--full-out (default gpurun_out/bench_full.json).  value = S,H sample-updates/s of the
+production form (FP32 sampler, FP32 MFMA accumulation, one sweep per kept sample), timed as
+one le_sample call of --steps kept samples per walker.  Alongside:
+  roofline      the Gram accumulation kernel (MFMA bound), average launch time from HIP
+                events on the Gram stream over the timed call
   kernels       sampler (VALU FLOP/s), features (sample-matrix bytes/s), Gram (MFMA FLOP/s)
   ess           tau_int (Madras-Sokal) and ESS/s: the slowest of Re/Im p_1..p_4, V and traced
                 S and H entries (|p_1|^2, |p_2|^2, Re/Im conj(p_2) p_1^2, V |p_1|^2, ...), and
@@ -19,7 +22,8 @@ FP32 MFMA accumulation, one sweep per kept sample).  Alongside:
                 symmetry moves / learned FP32 at their best thinning: ESS/s, ratios to the
                 plain FP64 form, spectra vs the FP64 form in MC sigma
   precision     FP32 and BF16-split GPU moments vs the FP64 oracle on one fixed sample set
-  cpu_baseline  the FP64 oracle port (OpenMP, host cores) on a bounded sample
+  cpu_baseline  the reference's row-wise estimator and the plain estimator restated in C
+                (FP64, OpenMP, host cores) on bounded samples
 """
 import argparse
 import json
@@ -160,6 +164,9 @@ def timed_sampling(eng, steps, nblk):
                                                "sweep_launches", "ms_sampler_graph")}
 
 
+ERR_BLOCKS, ERR_BLOCK_SAMPLES = 8, 8
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +181,13 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip the timings of the other entry points")
+    ap.add_argument("--no-eager", action="store_true", help="skip the eager per-kernel split run")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed headline run (implies every --no-* flag): the command whose "
+                         "Gram launches a kernel-trace profile averages")
+    ap.add_argument("--full-out", default=os.path.join(ROOT, "gpurun_out", "bench_full.json"),
+                    help="file for the full result (forms, rows, spectra); the last stdout line is a "
+                         "compact summary of it")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy this many MiB device-to-device first (HBM counter calibration)")
     args = ap.parse_args()
@@ -182,6 +196,8 @@ def main():
               "independent, so N GPUs would each run walkers/N chains and sum their FP64 moments "
               "in one all-reduce, which is not built here", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_cpu = args.no_precision = args.no_forms = args.no_rows = args.no_eager = True
     if args.copy_probe_mb > 0:
         import torch
         n = args.copy_probe_mb * (1 << 20) // 4
@@ -193,50 +209,46 @@ def main():
         del a, b
 
     steps, warm, tw = args.steps, args.warmup, min(args.trace_walkers, args.walkers)
-    nblk = 4 if steps >= 4 else 1
-    eng = make_engine(args.walkers, 0, steps + warm, nblk, tw)
+    eng = make_engine(args.walkers, 0, steps + warm + ERR_BLOCKS * ERR_BLOCK_SAMPLES, 1, tw)
     info, dims = eng.info, eng.dims
     sum_d2 = sum(d * d for d in dims)
-    log(f"N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} walkers={args.walkers}")
+    log(f"N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} walkers={args.walkers} "
+        f"samples_per_launch={info.samples_per_launch}")
     eng.init_walkers()
     step = eng.tune_step()
     eng.burnin(args.burnin)
     log(f"step={step:.4f} lanes_per_walker={eng.info.lanes_per_walker} registers={eng.info.sweep_in_registers} burn-in done")
     if warm > 0:
         eng.sample(warm)
-    # timed region: walkers resident in HBM, steps x (sweep + features + Gram), sampler
-    # rounds replayed from hipGraphs
-    wall, d = timed_sampling(eng, steps, nblk)
+    # Timed region: walkers resident in HBM; ONE le_sample call of `steps` kept samples per
+    # walker = ceil(steps / samples_per_launch) rounds of (sweeps + features, replayed from a
+    # hipGraph) on the sampler stream overlapped with the Gram of the previous round on the
+    # Gram stream, then the FP64 extraction of every S and H block and its copy to the host.
+    # le_sample returns after both streams are synchronised; the previous call ended the same way.
+    wall, d = timed_sampling(eng, steps, 1)
     updates = args.walkers * steps
     value = updates / wall
     st = eng.stats()
     log(f"timed: {value:.4e} sample-updates/s, {1e3 * wall / steps:.3f} ms/step")
-    # per-kernel split of the sampler stream: the same workload without graphs
-    ksteps = min(steps, 80)
-    eg = make_engine(args.walkers, 0, ksteps + 8, 1, 0, graphs=0)
-    eg.init_walkers()
-    eg.tune_step()
-    eg.burnin(min(args.burnin, 100))
-    eg.sample(8)
-    wall_eager, de = timed_sampling(eg, ksteps, 1)
-    eg.close()
-    log(f"eager: {args.walkers * ksteps / wall_eager:.4e} sample-updates/s")
-
-    gram_flop = 16.0 * sum_d2 * updates
-    gram_rate = gram_flop / (d["ms_gram"] * 1e-3)
-    launches = max(1, d["gram_launches"])
-    eager_updates = args.walkers * ksteps
-    sweep_rate = 12.0 * N * (N - 1) * eager_updates / (de["ms_sweep"] * 1e-3)
-    stream_bytes = (8 * info.D + 4) * eager_updates
-    rows_per_launch = info.samples_per_launch * args.walkers
-    traffic, traffic_note = gram_traffic(rows_per_launch)
     taus = taus_of(eng, steps, tw)
-    tau_max = tau_sh(taus) if taus else None
-    tau_max_all = max(taus.values()) if taus else None
+    # error bars: equal untimed blocks after the timed region
+    eng.reset_moments()
+    for _ in range(ERR_BLOCKS):
+        eng.sample(ERR_BLOCK_SAMPLES)
     t_eig = time.perf_counter()
     eng.spectra(-1)
     eig_ms = 1e3 * (time.perf_counter() - t_eig)
     mean_spec, sig = spectra_blocks(eng, dims)
+
+    gram_flop = 16.0 * sum_d2 * updates
+    herm_flop = 8.0 * sum(x * (x + 1) for x in dims) * updates
+    launches = max(1, d["gram_launches"])
+    avg_launch_ms = d["ms_gram"] / launches
+    gram_rate = gram_flop / (d["ms_gram"] * 1e-3)
+    rows_per_launch = info.samples_per_launch * args.walkers
+    traffic, traffic_note = gram_traffic(rows_per_launch)
+    tau_max = tau_sh(taus) if taus else None
+    tau_max_all = max(taus.values()) if taus else None
 
     out = {
         "metric": METRIC,
@@ -256,23 +268,67 @@ def main():
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
                    "sweep_in_registers": bool(eng.info.sweep_in_registers),
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,
+                   "gram_launches_timed": int(d["gram_launches"]),
                    "sampler_rounds": "hipGraph replay"},
-        "roofline": {"bound": "mfma", "kernel": "k_gram<float>",
+        "roofline": {"bound": "mfma", "kernel": "k_gram<float, 2>",
                      "achieved": gram_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": gram_rate / PEAK_FP32, "traffic": traffic,
                      "traffic_note": traffic_note,
+                     "avg_launch_ms": avg_launch_ms,
+                     "launch_time_source": "HIP events recorded on the Gram stream around every Gram launch "
+                                           "of the timed le_sample call (mean over its launches)",
                      "algorithmic_flop_per_launch": gram_flop / launches,
+                     "flop_per_sample": 16.0 * sum_d2,
                      "algorithmic_bytes_per_launch": rows_per_launch * (8 * info.D + 8),
                      "rows_per_launch": rows_per_launch,
-                     "avg_launch_ms": d["ms_gram"] / launches,
+                     "hermitian_flop_per_launch": herm_flop / launches,
+                     "hermitian_frac": herm_flop / (d["ms_gram"] * 1e-3) / PEAK_FP32,
                      "issued_mfma_flop_per_launch": info.gram_mfma_flop_per_row * updates / launches,
                      "issued_frac": info.gram_mfma_flop_per_row * updates / (d["ms_gram"] * 1e-3) / PEAK_FP32,
                      "flop_note": "algorithmic = 16 sum d^2 per sample (complex GEMM count over whole "
-                                  "Hermitian blocks, S and H); the kernel computes only the 16x16 blocks "
-                                  "holding upper-triangle entries, issued_mfma_flop_per_launch, so "
-                                  "'achieved' is above the issued rate and can exceed the peak; issued_frac "
-                                  "is the MFMA pipe's actual use"},
-        "kernels": {
+                                  "Hermitian blocks, S and H); hermitian = 8 sum d(d+1) (upper triangles "
+                                  "only); the kernel issues MFMAs only for the 16x16 blocks holding "
+                                  "upper-triangle entries (issued_mfma_flop_per_launch), so issued_frac is "
+                                  "the MFMA pipe's actual use"},
+        "acceptance": st["acceptance"],
+        "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max_all,
+                "ess_per_s": updates / (2.0 * tau_max_all) / wall if tau_max_all else None,
+                "tau_max_sh": tau_max,
+                "ess_per_s_sh": updates / (2.0 * tau_max) / wall if tau_max else None,
+                "traced_walkers": tw, "kept_samples": steps},
+        "spectrum_lowest": {str(SECTORS[s]): (float(mean_spec[s][0]) if mean_spec[s] is not None else None)
+                            for s in range(len(dims))},
+        "eigensolve_ms": {"sectors": len(dims), "largest": max(dims), "ms": eig_ms,
+                          "note": "FP64 Cholesky + tridiagonalization + bisection on the GPU, all sectors "
+                                  "in one launch, host copies included"},
+    }
+    iso_ms = eng.stats()["ms_gram_launch_min"]
+    if iso_ms > 0:
+        iso_rate = 16.0 * sum_d2 * rows_per_launch / (iso_ms * 1e-3)
+        out["roofline"]["alone"] = {
+            "achieved": iso_rate / 1e12, "frac": iso_rate / PEAK_FP32, "launch_ms": iso_ms,
+            "issued_frac": info.gram_mfma_flop_per_row * rows_per_launch / (iso_ms * 1e-3) / PEAK_FP32,
+            "note": "shortest full-size launch of the run (HIP events): the last launch of each sampling "
+                    "call runs with no sweeps beside it"}
+    if not args.no_precision:
+        log("precision check (fixed samples vs FP64 oracle)")
+        out["precision"] = precision_check(eng, sig)
+    eng.close()
+    if not args.no_eager:
+        # per-kernel split of the sampler stream: the same workload without graphs
+        ksteps = min(steps, 80)
+        eg = make_engine(args.walkers, 0, ksteps + 8, 1, 0, graphs=0)
+        eg.init_walkers()
+        eg.tune_step()
+        eg.burnin(min(args.burnin, 100))
+        eg.sample(8)
+        wall_eager, de = timed_sampling(eg, ksteps, 1)
+        eg.close()
+        eager_updates = args.walkers * ksteps
+        log(f"eager: {eager_updates / wall_eager:.4e} sample-updates/s")
+        sweep_rate = 12.0 * N * (N - 1) * eager_updates / (de["ms_sweep"] * 1e-3)
+        stream_bytes = (8 * info.D + 4) * eager_updates
+        out["kernels"] = {
             "sampler": {"kernel": f"k_sweep{'_reg' if eng.info.sweep_in_registers else ''}<{eng.info.lanes_per_walker},float>",
                         "bound": "valu",
                         "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
@@ -287,43 +343,91 @@ def main():
             "eager_sample_updates_per_s": eager_updates / wall_eager,
             "note": "sampler and features share one stream (replayed from hipGraphs in the timed run; "
                     "their split is from an eager run of the same workload); Gram runs on a second "
-                    "stream on the other sample buffer, so ms_per_step < sum of the parts",
-        },
-        "acceptance": st["acceptance"],
-        "ess": {"thin": 1, "tau_int": taus, "tau_max": tau_max_all,
-                "ess_per_s": updates / (2.0 * tau_max_all) / wall if tau_max_all else None,
-                "tau_max_sh": tau_max,
-                "ess_per_s_sh": updates / (2.0 * tau_max) / wall if tau_max else None,
-                "traced_walkers": tw},
-        "spectrum_lowest": {str(SECTORS[s]): (float(mean_spec[s][0]) if mean_spec[s] is not None else None)
-                            for s in range(len(dims))},
-        "eigensolve_ms": {"sectors": len(dims), "largest": max(dims), "ms": eig_ms,
-                          "note": "FP64 Cholesky + tridiagonalization + bisection on the GPU, all sectors "
-                                  "in one launch, host copies included"},
-    }
-    iso_ms = eng.stats()["ms_gram_launch_min"]
-    if iso_ms > 0:
-        iso_rate = 16.0 * sum_d2 * rows_per_launch / (iso_ms * 1e-3)
-        out["roofline"]["alone"] = {
-            "achieved": iso_rate / 1e12, "frac": iso_rate / PEAK_FP32, "launch_ms": iso_ms,
-            "issued_frac": info.gram_mfma_flop_per_row * rows_per_launch / (iso_ms * 1e-3) / PEAK_FP32,
-            "note": "shortest full-size launch of the run (HIP events): the last launch of each sampling "
-                    "call runs with no sweeps beside it; 'achieved' above averages the timed pipeline, "
-                    "where the Gram stream shares the CUs with the sampler stream's sweeps"}
-    if not args.no_precision:
-        log("precision check (fixed samples vs FP64 oracle)")
-        out["precision"] = precision_check(eng, sig)
-    eng.close()
+                    "stream on the other sample buffer",
+        }
     if not args.no_forms:
         out["forms"] = forms(args, [int(x) for x in args.thins.split(",")])
     if not args.no_rows:
         out["rows"] = rows()
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(value)
         log("small configuration in full on CPU and GPU")
         out["config0_full"] = config0_full()
-    print(json.dumps(out), flush=True)
+    if args.full_out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.full_out)), exist_ok=True)
+        with open(args.full_out, "w") as f:
+            json.dump(out, f, indent=1)
+        log(f"full result written to {args.full_out}")
+    print(json.dumps(compact(out, args.full_out)), flush=True)
+
+
+def _r(x, n=4):
+    """round for the compact line"""
+    if x is None:
+        return None
+    if isinstance(x, float):
+        return float(f"{x:.{n}g}")
+    return x
+
+
+def compact(out, full_path):
+    """The driver-facing summary line (a few KB): headline, roofline, CPU baseline, ESS ratios,
+    precision maxima and spectra agreement; the full result is in `full_path`."""
+    keep = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+            "scaling", "vs_baseline", "dtype", "data", "config", "acceptance")
+    c = {k: out[k] for k in keep}
+    c["value"] = _r(out["value"], 6)
+    c["ms_per_step"] = _r(out["ms_per_step"], 5)
+    rf = out["roofline"]
+    c["roofline"] = {k: _r(rf.get(k), 5) for k in ("bound", "kernel", "achieved", "peak", "unit", "frac", "traffic",
+                                                  "avg_launch_ms", "launch_time_source", "algorithmic_flop_per_launch",
+                                                  "flop_per_sample", "algorithmic_bytes_per_launch", "rows_per_launch",
+                                                  "hermitian_frac", "issued_frac")}
+    c["roofline"]["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated, committed profile"
+    if "alone" in rf:
+        c["roofline"]["alone"] = {k: _r(rf["alone"][k], 4) for k in ("achieved", "frac", "launch_ms", "issued_frac")}
+    e = out["ess"]
+    c["ess_headline"] = {"tau_max_sh": _r(e["tau_max_sh"]), "ess_per_s_sh": _r(e["ess_per_s_sh"]),
+                         "tau_max": _r(e["tau_max"]), "ess_per_s": _r(e["ess_per_s"]),
+                         "kept_samples": e["kept_samples"], "traced_walkers": e["traced_walkers"]}
+    if "kernels" in out:
+        k = out["kernels"]
+        c["sampler"] = {"kernel": k["sampler"]["kernel"], "ms_per_sweep": _r(k["sampler"]["ms_per_sweep"]),
+                        "frac_fp32_valu": _r(k["sampler"]["frac"])}
+        c["features_ms_per_step"] = _r(k["features"]["ms_per_step"])
+    if "precision" in out:
+        p = out["precision"]
+        c["precision"] = {"fixed_samples": p["fixed_samples"]}
+        for lab in ("fp32", "bf16x3", "bf16x6"):
+            c["precision"][lab] = {k: _r(v, 3) for k, v in p[lab].items()}
+    if "forms" in out:
+        f = out["forms"]
+        cf = {}
+        for name, r in f.items():
+            if not isinstance(r, dict):
+                continue
+            row = {"ess_sh": _r(r["ess_per_s_sh"]), "thin_sh": r["best_thin_sh"], "ess_all": _r(r["ess_per_s"])}
+            if "ess_sh_ratio_vs_plain_fp64" in r:
+                row["ratio_sh"] = _r(r["ess_sh_ratio_vs_plain_fp64"], 3)
+                row["ratio_all"] = _r(r["ess_ratio_vs_plain_fp64"], 3)
+                jk = r["spectra_vs_plain_fp64_jackknife"]
+                bt = r["spectra_vs_plain_fp64"]
+                row["lowest_max_sigma_jk"] = _r(max(jk["lowest_per_sector_dev"].values()) if jk["lowest_per_sector_dev"] else None, 3)
+                row["lowest_max_sigma_batch"] = _r(max(bt["lowest_per_sector_dev"].values()) if bt["lowest_per_sector_dev"] else None, 3)
+                row["within_1sigma_jk"] = _r(jk["frac_within_1_sigma"], 3)
+            cf[name] = row
+        c["forms"] = cf
+    if "rows" in out:
+        c["rows"] = {k: {kk: _r(vv) for kk, vv in v.items() if isinstance(vv, (int, float))} for k, v in out["rows"].items()}
+    if "cpu_baseline" in out:
+        cb = out["cpu_baseline"]
+        c["cpu_baseline"] = {k: _r(v, 5) for k, v in cb.items() if k != "legs"}
+        c["cpu_baseline"]["legs"] = {n: {k: _r(v, 5) for k, v in leg.items()} for n, leg in cb["legs"].items()}
+    if "config0_full" in out:
+        c["config0_full"] = {k: _r(v) for k, v in out["config0_full"].items()}
+    c["full_result"] = os.path.relpath(full_path, ROOT) if full_path else None
+    return c
 
 
 def gram_traffic(rows_per_launch):
@@ -599,20 +703,48 @@ def cpu_model():
     return None
 
 
-def cpu_baseline():
-    """FP64 oracle port (plain |Psi_L|^2 chain + S/H accumulation), OpenMP, host cores."""
+def cpu_baseline(gpu_value):
+    """CPU restatements of the reference's estimators (FP64, OpenMP, host cores), each on a
+    bounded sample of the headline workload (N=200, m=3, dL=1..16, D=914), burn-in included:
+
+    rowwise  the reference's own estimator (em_spectrum.cu:16-137, 318-390; oracle
+             or_rowwise_sector): one |p_f Psi_L|^2 chain per basis row f accumulating g/f,
+             V g/f and |g/f|^2 over the row's sector.  One kept sample of one chain fills one
+             row, so a full S,H sample-update (every row of every sector sampled once) costs D
+             chain samples: sample-updates/s = row-samples/s / D.  Timed on the rows of
+             sector dL=10 (42 rows); the cost per row-sample does not depend on the sector (the
+             sweep is O(N^2), the features O(N K + D)).
+    plain    the plain |Psi_L|^2 chain with the S and H accumulation (oracle or_plain_run),
+             the estimator the GPU path computes.
+    `value` is the rowwise leg, the reference's method."""
     from oracle.oracle import Oracle
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
     o = Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
-    walkers, burn, samples = 32 * threads, 20, 600
+    D = sum(o.dims)
+    legs = {}
+    sec = SECTORS.index(10)
+    d = o.dims[sec]
+    walkers, burn, samples = 4 * threads, 10, 60
+    t0 = time.perf_counter()
+    o.rowwise_sector(sec, walkers, burn, samples, 0.5, SEED, threads=threads)
+    dt = time.perf_counter() - t0
+    row_rate = d * walkers * samples / dt
+    legs["rowwise"] = {"value": row_rate / D, "unit": "sample-updates/s", "row_samples_per_s": row_rate,
+                       "seconds": dt,
+                       "sample": f"sector dL=10: {d} rows x {walkers} walkers x ({burn} burn-in + {samples} kept "
+                                 f"sweeps); sample-updates/s = row-samples/s / D (D={D})"}
+    walkers, burn, samples = 32 * threads, 20, 400
     t0 = time.perf_counter()
     o.plain_run(walkers, burn, samples, 0.5, SEED, threads=threads)
     dt = time.perf_counter() - t0
-    rate = walkers * samples / dt
-    return {"value": rate, "unit": "sample-updates/s", "cores": threads, "kind": "port",
-            "host_logical_cpus": os.cpu_count(), "cpu_model": cpu_model(),
-            "sample": f"{walkers} walkers x ({burn} burn-in + {samples} kept sweeps) at N=200, "
-                      f"dL=1..16, FP64 (burn-in time included), {dt:.1f} s"}
+    legs["plain"] = {"value": walkers * samples / dt, "unit": "sample-updates/s", "seconds": dt,
+                     "sample": f"{walkers} walkers x ({burn} burn-in + {samples} kept sweeps)"}
+    r = legs["rowwise"]
+    return {"value": r["value"], "unit": "sample-updates/s", "cores": threads, "kind": "port",
+            "estimator": "rowwise (the reference's)", "host_logical_cpus": os.cpu_count(),
+            "cpu_model": cpu_model(), "sample": r["sample"],
+            "gpu_over_cpu": gpu_value / r["value"],
+            "gpu_over_cpu_plain": gpu_value / legs["plain"]["value"], "legs": legs}
 
 
 def config0_full():

@@ -47,6 +47,9 @@ inline std::vector<std::complex<double>> read_tsv(const std::string& path, int c
     while (true) {
         if (!(in >> i >> j >> a >> b)) break;
         if (cols == 6 && !(in >> c >> d)) break;
+        // a negative index would write outside the matrix (the reference's Eigen indexing
+        // has no defined result for it): reject it as a parsing error
+        if (i < 0 || j < 0) throw std::runtime_error("Parsing error in " + path + ": negative index");
         std::complex<double> v(a, b);
         if (cols == 6) v = std::complex<double>(a, b) + std::complex<double>(0.0, 1.0) * std::complex<double>(c, d);
         entries.emplace_back(i, j, v);

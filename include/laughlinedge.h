@@ -15,7 +15,7 @@
  *   le_basis_state, le_basis_describe
  *       iparts::integer_partitions host partitions (modules/integer_partitions.h:29-70,
  *       modules/partitions/integer_partitions.cu:10-226).
- *   le_tune_step
+ *   le_tune_step, le_step_search
  *       edgeMC::optimal_step / mcmc_acceptanceBisection (modules/edge_mc/em_bisection.cu:7-153).
  *   le_burnin, le_sample, le_run
  *       the mcmc_hamiltonian launch and its host averaging
@@ -190,7 +190,22 @@ int le_basis_describe(int N, const int* sectors, int n_sectors, int capacity, in
                       int* dls, int* pairs, int* n_pairs);
 
 int le_init_walkers(le_engine* e);
+/* Tune the proposal width to target_acceptance +- acceptance_tol: le_step_search over the
+ * acceptance of tune_sweeps sweeps of freshly initialised walkers per trial width
+ * (edgeMC::optimal_step, modules/edge_mc/em_bisection.cu:49-153).  Leaves fresh walkers. */
 int le_tune_step(le_engine* e, double* step_out);
+
+/* The step search of edgeMC::optimal_step (em_bisection.cu:80-150) over a caller's acceptance
+ * measurement (no GPU call of its own): measure(ctx, step, &acceptance) returns LE_OK or an
+ * error code that the search returns unchanged.  From step0: keep it when |a - target| < tol;
+ * else double (a > target) or halve the step until the target is bracketed (at most 41
+ * measurements, else LE_ERR_BISECTION); then interpolate linearly between the bracket ends,
+ * replacing the end on the guess's side (LE_ERR_BISECTION when a guess falls outside the
+ * bracket, where the reference exits, or after 60 guesses).  *n_measured (optional) counts
+ * the measurements. */
+typedef int (*le_accept_fn)(void* ctx, double step, double* acceptance);
+int le_step_search(le_accept_fn measure, void* ctx, double step0, double target, double tol, double* step_out,
+                   int* n_measured);
 int le_burnin(le_engine* e, int sweeps);
 /* sample `samples` kept samples per walker into the current block; closes the block */
 int le_sample(le_engine* e, int samples);
@@ -260,8 +275,8 @@ int le_engine_create_occupation(const le_system_params* sys, const le_mc_params*
 int le_get_occupation(const le_engine* e, int block, double* out);
 
 /* Learned proposal (no reference counterpart): a 3 -> 64 -> 64 -> 2 tanh MLP maps the
- * features (|z|/R, nearest-neighbour distance, mean of the 8 residue-class nearest
- * distances) of the particle being moved to radial/tangential Gaussian widths; moves are
+ * features (|z|/R, the distance to the nearest particle and the mean distance to the 8 nearest,
+ * both capped at 8) of the particle being moved to radial/tangential Gaussian widths; moves are
  * accepted with the exact Metropolis-Hastings ratio (widths re-evaluated at the proposed
  * position), so |Psi_L|^2 stays the target.  Training maximises the expected squared jump
  * distance on configurations of the plain chain with Adam; the trained network is

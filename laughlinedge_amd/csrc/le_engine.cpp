@@ -79,6 +79,10 @@ std::map<std::tuple<int, int, int, int>, int> g_tuned_lanes;   // (N, walkers, p
 
 }  // namespace
 
+namespace le {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace le
+
 struct le_engine {
     le_system_params sys{};
     le_mc_params mc{};
@@ -200,6 +204,7 @@ struct le_engine {
     int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
                    const unsigned long long* sweep_base = nullptr);
     int sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
+    int capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
     int symmetry(unsigned long long sweep0, const unsigned long long* sweep_base);
     void drop_graphs();
     int close_block(long long samples);
@@ -876,55 +881,44 @@ double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
     return tot / ((double)mc.walkers * mc.tune_sweeps * npos());
 }
 
-// Secant/bracketing search of em_bisection.cu:80-150, restated; the reference loops
-// until |acc - target| < tol and exits the process on a failed bracket.
+// edgeMC::optimal_step (em_bisection.cu:49-153): le_step_search (le_tune.cpp) over
+// acceptances measured on fresh walkers, tune_sweeps sweeps each (mcmc_acceptanceBisection,
+// em_bisection.cu:7-47).
+namespace {
+struct TuneCtx {
+    le_engine* e;
+    int attempt;
+};
+int tune_measure(void* ctx, double step, double* acceptance)
+{
+    auto* t = static_cast<TuneCtx*>(ctx);
+    int rc = LE_OK;
+    *acceptance = t->e->measure_acceptance(step, t->attempt++, &rc);
+    return rc;
+}
+}  // namespace
+
 extern "C" int le_tune_step(le_engine* e, double* step_out)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
     if (e->mc.tune_sweeps <= 0) return fail(LE_ERR_ARG, "tune_sweeps must be > 0");
-    const double target = e->mc.target_acceptance, tol = e->mc.acceptance_tol;
-    int rc = LE_OK, attempt = 0;
-    auto meas = [&](double s) { return e->measure_acceptance(s, attempt++, &rc); };
-    auto done = [&](double s) {
-        e->step = s;
-        e->st.step = s;
-        if (step_out) *step_out = s;
-        // leave fresh walkers for the burn-in
-        return le_init_walkers(e);
-    };
-    double s2 = e->step, s1;
-    double a2 = meas(s2), a1;
-    if (rc) return rc;
-    if (std::fabs(a2 - target) < tol) return done(s2);
-    s1 = s2;
-    for (int it = 0;; ++it) {
-        s1 *= (a2 > target) ? 2.0 : 0.5;
-        a1 = meas(s1);
-        if (rc) return rc;
-        if ((a2 - target) * (a1 - target) <= 0) break;
-        if (it > 40) return fail(LE_ERR_BISECTION, "step search failed to bracket the target acceptance");
+    TuneCtx ctx{e, 0};
+    double s = e->step;
+    const double s_in = e->step;
+    int rc = le_step_search(&tune_measure, &ctx, s_in, e->mc.target_acceptance, e->mc.acceptance_tol, &s, nullptr);
+    if (rc != LE_OK) {
+        e->step = s_in;
+        // leave fresh walkers whatever the outcome
+        const std::string msg = g_err;
+        (void)le_init_walkers(e);
+        g_err = msg;
+        return rc;
     }
-    if (std::fabs(a1 - target) < tol) return done(s1);
-    if (s2 < s1) {
-        std::swap(s1, s2);
-        std::swap(a1, a2);
-    }
-    for (int it = 0; it < 60; ++it) {
-        const double g = ((target - a1) * s2 + (a2 - target) * s1) / (a2 - a1);
-        const double ag = meas(g);
-        if (rc) return rc;
-        if (std::fabs(ag - target) < tol) return done(g);
-        if ((ag - target) * (a1 - target) < 0) {
-            a2 = ag;
-            s2 = g;
-        } else if ((ag - target) * (a2 - target) < 0) {
-            a1 = ag;
-            s1 = g;
-        } else {
-            return fail(LE_ERR_BISECTION, "step bisection lost its bracket");
-        }
-    }
-    return fail(LE_ERR_BISECTION, "step bisection did not converge in 60 steps");
+    e->step = s;
+    e->st.step = s;
+    if (step_out) *step_out = s;
+    // leave fresh walkers for the burn-in
+    return le_init_walkers(e);
 }
 
 extern "C" int le_burnin(le_engine* e, int sweeps)
@@ -1039,12 +1033,31 @@ int le_engine::close_block(long long samples)
 // base are read from bases_d, which a one-thread kernel sets before each replay.
 int le_engine::sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out)
 {
-    for (auto& g : graphs)
-        if (g.buf == buf && g.traced == traced && g.thin == mc.thin && g.step == step && g.proposal == proposal &&
-            g.G == G && g.table == table && g.table_T == table_T && g.n == spl) {
-            *out = g.exec;
-            return LE_OK;
-        }
+    auto find = [&](int b) -> hipGraphExec_t {
+        for (auto& g : graphs)
+            if (g.buf == b && g.traced == traced && g.thin == mc.thin && g.step == step && g.proposal == proposal &&
+                g.G == G && g.table == table && g.table_T == table_T && g.n == spl)
+                return g.exec;
+        return nullptr;
+    };
+    if ((*out = find(buf))) return LE_OK;
+    // Capture the round for both sample buffers at once, so that a later call that starts on
+    // the other buffer replays instead of capturing inside its own time.
+    for (int b : {1 - buf, buf}) {
+        if (b != buf && find(b)) continue;
+        le::FeatureArgs fb = f;
+        fb.X = Xb[b];
+        fb.W = Wbb[b];
+        hipGraphExec_t ex = nullptr;
+        int rc = capture_round(fb, b, traced, tw, &ex);
+        if (rc) return rc;
+        if (b == buf) *out = ex;
+    }
+    return LE_OK;
+}
+
+int le_engine::capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out)
+{
     for (size_t q = 0; q < graphs.size();)
         if (graphs[q].buf == buf && graphs[q].traced == traced) {
             (void)hipGraphExecDestroy(graphs[q].exec);

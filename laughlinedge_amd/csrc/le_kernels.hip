@@ -558,9 +558,10 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = group_sum<kLps, double>(acc[q]);
     double* ps = a.ps + (size_t)w * a.ps_ld;
+    // only Re/Im p_1..p_K: a row holds ps_ld >= 2 K values (KMAX rounds K up to 8, 16 or 24)
 #pragma unroll
     for (int q = 0; q < 2 * KMAX; ++q)
-        if (q % kLps == sub) ps[q] = acc[q];
+        if (q % kLps == sub && q < 2 * a.K) ps[q] = acc[q];
     const long row = a.row0 + w;
     Real* W = reinterpret_cast<Real*>(a.W);
     if (sub == 0) W[row] = (Real)acc[QV];

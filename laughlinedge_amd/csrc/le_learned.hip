@@ -4,7 +4,8 @@
 // the sampler interpolates.  No reference counterpart.
 //
 // Definitions (shared with the FP64 oracle):
-//   f1 = |z| / R;  m_c = min(8, min_{j != i, j = c mod 8} |z - z_j|);  f2 = min_c m_c;  f3 = mean_c m_c
+//   f1 = |z| / R;  d_(1) <= ... <= d_(8) the distances from z to its 8 nearest other particles,
+//   each capped at 8 (missing neighbours count as 8);  f2 = d_(1);  f3 = mean_k d_(k)
 //   (s_r, s_t) = sigma0 exp(ln(2.5) tanh(MLP(f))), i.e. within [0.4, 2.5] sigma0;  z' = z + e^{i arg z} (s_r n1 + i s_t n2)
 //   log alpha = m sum_j log(|z'-z_j|^2/|z-z_j|^2) - (|z'|^2-|z|^2)/2 + log q(z|x') - log q(z'|x)
 //   loss = -mean min(1, e^{log alpha}) |z' - z|^2
@@ -31,45 +32,47 @@ constexpr int kRec = 264;
 constexpr int rF = 0, rH1 = 3, rH2 = 67, rDO = 131, rDZ2 = 133, rDZ1 = 197;
 
 
-// features of particle i at (x, y) against configuration cfg (N particles) and d/d(x, y)
+// features of particle i at (x, y) against configuration cfg (N particles) and d/d(x, y):
+// a sorted list of the 8 smallest squared distances (insertion), with the offsets they come
+// from for the gradients; distances at or beyond the cap have zero gradient
 __device__ void features_at(const float2* cfg, int N, int i, float x, float y, float invR, float f[3], float df[3][2])
 {
-    float mins[8], gx[8], gy[8];
+    constexpr float kCap2 = kDcap * kDcap;
+    float t[8], tx[8], ty[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { mins[c] = kDcap; gx[c] = 0.f; gy[c] = 0.f; }
+    for (int q = 0; q < 8; ++q) { t[q] = kCap2; tx[q] = 0.f; ty[q] = 0.f; }
     for (int j = 0; j < N; ++j) {
         if (j == i) continue;
-        const float dx = x - cfg[j].x, dy = y - cfg[j].y;
-        const float d = sqrtf(dx * dx + dy * dy);
-        const int c = j & 7;
+        float vx = x - cfg[j].x, vy = y - cfg[j].y;
+        float v = vx * vx + vy * vy;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == c && d < mins[q]) {
-                mins[q] = d;
-                gx[q] = d > 0.f ? dx / d : 0.f;
-                gy[q] = d > 0.f ? dy / d : 0.f;
-            }
+        for (int q = 0; q < 8; ++q) {
+            const bool sw = v < t[q];
+            const float a = sw ? v : t[q], ax = sw ? vx : tx[q], ay = sw ? vy : ty[q];
+            v = sw ? t[q] : v;
+            vx = sw ? tx[q] : vx;
+            vy = sw ? ty[q] : vy;
+            t[q] = a; tx[q] = ax; ty[q] = ay;
+        }
     }
     const float r = sqrtf(x * x + y * y);
     f[0] = r * invR;
     df[0][0] = r > 0.f ? x / r * invR : 0.f;
     df[0][1] = r > 0.f ? y / r * invR : 0.f;
-    int cm = 0;
-    float s = 0.f, sx = 0.f, sy = 0.f;
+    float s = 0.f, sx = 0.f, sy = 0.f, d0 = kDcap, g0x = 0.f, g0y = 0.f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        if (mins[q] < mins[cm]) cm = q;
-        s += mins[q];
-        sx += gx[q];
-        sy += gy[q];
+        const float d = sqrtf(t[q]);
+        const bool live = t[q] < kCap2 && d > 0.f;
+        const float gx = live ? tx[q] / d : 0.f, gy = live ? ty[q] / d : 0.f;
+        s += d;
+        sx += gx;
+        sy += gy;
+        if (q == 0) { d0 = d; g0x = gx; g0y = gy; }
     }
-    float mm = mins[0], mgx = gx[0], mgy = gy[0];
-#pragma unroll
-    for (int q = 1; q < 8; ++q)
-        if (q == cm) { mm = mins[q]; mgx = gx[q]; mgy = gy[q]; }
-    f[1] = mm;
-    df[1][0] = mgx;
-    df[1][1] = mgy;
+    f[1] = d0;
+    df[1][0] = g0x;
+    df[1][1] = g0y;
     f[2] = 0.125f * s;
     df[2][0] = 0.125f * sx;
     df[2][1] = 0.125f * sy;
@@ -322,9 +325,11 @@ __global__ void k_mlp_table(const float* Pg, int T, const float* lo, const float
 
 // ---------------------------------------------------------------------------
 // Metropolis-Hastings sweeps with the tabulated widths.  G lanes per walker (G in
-// {8,16,32,64}, lane l owns particles j = l + G s, so its class is l mod 8).  Pass 1 (slot
-// i and pads hold a far sentinel) gives the class minima at z; pass 2 (slot i and pads hold
-// the midpoint) gives the log-ratio and, skipping those slots, the class minima at z'.
+// {8,16,32,64}, lane l owns particles j = l + G s).  Pass 1 (slot i and pads hold a far
+// sentinel) gives the 8 nearest distances at z; pass 2 (slot i and pads hold the midpoint)
+// gives the log-ratio and, skipping those slots, the 8 nearest distances at z'.  Each lane
+// keeps its own sorted 8 smallest, and a butterfly merge over the walker's lanes gives the
+// walker's 8 nearest.
 // ---------------------------------------------------------------------------
 struct Table3 {
     const float* t;
@@ -357,31 +362,70 @@ __device__ __forceinline__ void widths_lookup(const Table3& tb, const float f[3]
     lst = b;
 }
 
-template <int G>
-__device__ __forceinline__ void class_features(float cmin2, float invR, float r, float f[3])
+// Sorted (ascending) list of a lane's 8 smallest squared distances, initialised to the cap.
+__device__ __forceinline__ void near8_init(float (&t)[8])
 {
-    // cmin2: this lane's minimum squared distance over its slots.  Lanes l, l ^ 8, l ^ 16, ...
-    // hold particles of the same residue class j mod 8 (G a multiple of 8).
-    if constexpr (G >= 16) cmin2 = fminf(cmin2, dpp_f<0x128>(cmin2));   // row rotate 8 = xor 8
-    if constexpr (G >= 32) cmin2 = fminf(cmin2, __shfl_xor(cmin2, 16, 64));
-    if constexpr (G >= 64) cmin2 = fminf(cmin2, __shfl_xor(cmin2, 32, 64));
-    const float d = fminf(sqrtf(cmin2), kDcap);
-    float mn = d, sm = d;
-    mn = fminf(mn, dpp_f<0xB1>(mn));
-    sm += dpp_f<0xB1>(sm);
-    mn = fminf(mn, dpp_f<0x4E>(mn));
-    sm += dpp_f<0x4E>(sm);
-    mn = fminf(mn, dpp_f<0x141>(mn));
-    sm += dpp_f<0x141>(sm);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t[q] = kDcap * kDcap;
+}
+__device__ __forceinline__ void near8_insert(float (&t)[8], float v)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float lo = fminf(t[q], v);
+        v = fmaxf(t[q], v);
+        t[q] = lo;
+    }
+}
+
+// One butterfly stage: with the partner lane's sorted list b, min(t_q, b_{7-q}) holds the 8
+// smallest of both lists as a bitonic sequence, which three half-cleaners sort.  Partner lanes
+// end with the same multiset, hence the same sorted list.
+template <typename Xchg>
+__device__ __forceinline__ void near8_stage(float (&t)[8], Xchg xchg)
+{
+    float b[8], c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) b[q] = xchg(t[q]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = fminf(t[q], b[7 - q]);
+#pragma unroll
+    for (int d = 4; d >= 1; d >>= 1)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (!(q & d)) {
+                const float lo = fminf(c[q], c[q + d]), hi = fmaxf(c[q], c[q + d]);
+                c[q] = lo;
+                c[q + d] = hi;
+            }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t[q] = c[q];
+}
+
+// Features from the lanes' lists: merge over the walker's G lanes (quad xor 1, quad xor 2,
+// half-row mirror, row mirror, then xor 16 / 32), then f = (r / R, d_(1), mean of d_(1..8)),
+// every lane of the walker with the same bits.
+template <int G>
+__device__ __forceinline__ void near_features(float (&t)[8], float invR, float r, float f[3])
+{
+    if constexpr (G >= 2) near8_stage(t, [](float v) { return dpp_f<0xB1>(v); });
+    if constexpr (G >= 4) near8_stage(t, [](float v) { return dpp_f<0x4E>(v); });
+    if constexpr (G >= 8) near8_stage(t, [](float v) { return dpp_f<0x141>(v); });
+    if constexpr (G >= 16) near8_stage(t, [](float v) { return dpp_f<0x140>(v); });
+    if constexpr (G >= 32) near8_stage(t, [](float v) { return __shfl_xor(v, 16, 64); });
+    if constexpr (G >= 64) near8_stage(t, [](float v) { return __shfl_xor(v, 32, 64); });
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += sqrtf(t[q]);
     f[0] = r * invR;
-    f[1] = mn;
-    f[2] = 0.125f * sm;
+    f[1] = sqrtf(t[0]);
+    f[2] = 0.125f * s;
 }
 
 // Learned-proposal sweeps: same lane mapping, midpoint trick and random streams as k_sweep.
-// Per move: pass 1 takes the class minima at z (slot i and the pads hold a far point),
-// the table gives the forward widths, pass 2 takes the log-ratio and the class minima at z'
-// (slot i and the pads skipped), the table gives the reverse widths.
+// Per move: pass 1 takes the nearest distances at z (slot i and the pads hold a far point),
+// the table gives the forward widths, pass 2 takes the log-ratio and the nearest distances at
+// z' (slot i and the pads skipped), the table gives the reverse widths.
 template <int G>
 __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, float invR)
 {
@@ -430,10 +474,11 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                 const int i = base + t;
                 const float n1 = rn.x, n2 = rn.y;
                 const uint32_t ub = __builtin_bit_cast(uint32_t, rn.z);
-                // pass 1: class minima at z
+                // pass 1: nearest distances at z (slot i and the pads hold a far point)
                 P[i] = far;
                 if (has_pad) P[pad_j] = far;
-                float c2min = 1e30f;
+                float near[8];
+                near8_init(near);
                 {
                     int s = 0;
                     for (; s + 4 <= S; s += 4) {
@@ -443,25 +488,25 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             const float dx = zo.x - q[u].x, dy = zo.y - q[u].y;
-                            c2min = fminf(c2min, dx * dx + dy * dy);
+                            near8_insert(near, dx * dx + dy * dy);
                         }
                     }
                     for (; s < S; ++s) {
                         const float2 q = P[lane + G * s];
                         const float dx = zo.x - q.x, dy = zo.y - q.y;
-                        c2min = fminf(c2min, dx * dx + dy * dy);
+                        near8_insert(near, dx * dx + dy * dy);
                     }
                 }
                 const float r0 = sqrtf(zo.x * zo.x + zo.y * zo.y);
                 float f[3];
-                class_features<G>(c2min, invR, r0, f);
+                near_features<G>(near, invR, r0, f);
                 float lsr, lst;
                 widths_lookup(tb, f, lsr, lst);
                 const float sr = __expf(lsr), st = __expf(lst);
                 const float cth = r0 > 0.f ? zo.x / r0 : 1.f, sth = r0 > 0.f ? zo.y / r0 : 0.f;
                 const float dr = sr * n1, dt = st * n2;
                 const float xn = zo.x + cth * dr - sth * dt, yn = zo.y + sth * dr + cth * dt;
-                // pass 2: ratio (midpoint in slot i and pads) and class minima at z'
+                // pass 2: ratio (midpoint in slot i and pads) and nearest distances at z'
                 float2 mid;
                 mid.x = 0.5f * (zo.x + xn);
                 mid.y = 0.5f * (zo.y + yn);
@@ -474,7 +519,8 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                     rn_next = rnd[t + 1];
                 }
                 const int skip_i = ((i % G) == lane) ? (i / G) : -1;
-                float lsum = 0.f, pn = 1.f, pd = 1.f, a2min = 1e30f;
+                float lsum = 0.f, pn = 1.f, pd = 1.f;
+                near8_init(near);
                 int s = 0;
                 for (; s + 8 <= S; s += 8) {
 #pragma unroll
@@ -485,7 +531,7 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                         const float A2 = ax * ax + ay * ay;
                         pn *= A2;
                         pd *= bx * bx + by * by;
-                        a2min = (s + u == skip_i || s + u == skip_p) ? a2min : fminf(a2min, A2);
+                        near8_insert(near, (s + u == skip_i || s + u == skip_p) ? kDcap * kDcap : A2);
                     }
                     lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
                     pn = 1.f;
@@ -502,7 +548,7 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                         const float A2 = ax * ax + ay * ay;
                         pn *= A2;
                         pd *= bx * bx + by * by;
-                        a2min = (s + u == skip_i || s + u == skip_p) ? a2min : fminf(a2min, A2);
+                        near8_insert(near, (s + u == skip_i || s + u == skip_p) ? kDcap * kDcap : A2);
                     }
                 }
                 for (; s < S; ++s) {
@@ -512,13 +558,13 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                     const float A2 = ax * ax + ay * ay;
                     pn *= A2;
                     pd *= bx * bx + by * by;
-                    a2min = (s == skip_i || s == skip_p) ? a2min : fminf(a2min, A2);
+                    near8_insert(near, (s == skip_i || s == skip_p) ? kDcap * kDcap : A2);
                 }
                 lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
                 lsum = group_sum<G, float>(lsum);
                 const float rn_ = sqrtf(xn * xn + yn * yn);
                 float fp[3];
-                class_features<G>(a2min, invR, rn_, fp);
+                near_features<G>(near, invR, rn_, fp);
                 float lsrp, lstp;
                 widths_lookup(tb, fp, lsrp, lstp);
                 const float srp = __expf(lsrp), stp = __expf(lstp);

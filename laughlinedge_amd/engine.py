@@ -35,7 +35,7 @@ EXPORTED = [
     "le_engine_spectra", "le_basis_describe", "le_get_block", "le_set_weight_callback",
     "le_evolve", "le_engine_create_occupation", "le_get_occupation", "le_train_params_default",
     "le_train_proposal", "le_set_proposal", "le_get_proposal_params", "le_set_proposal_params",
-    "le_proposal_loss_grad", "le_set_thin", "le_removal_weights",
+    "le_proposal_loss_grad", "le_set_thin", "le_removal_weights", "le_step_search",
 ]
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
@@ -138,6 +138,30 @@ def _check(rc):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+ACCEPT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double))
+
+
+def step_search(measure, step0, target, tol):
+    """The step search of edgeMC::optimal_step (le_step_search, no GPU call of its own) over a
+    Python acceptance function measure(step) -> acceptance, or -> (error code, acceptance).
+    Returns (step, number of measurements); raises LEError with the library's code."""
+    def cb(_ctx, s, out):
+        r = measure(s)
+        code, a = (r if isinstance(r, tuple) else (LE_OK, r))
+        out[0] = float(a)
+        return int(code)
+    fn = ACCEPT_FN(cb)
+    st = ctypes.c_double(0.0)
+    nm = ctypes.c_int(0)
+    rc = lib().le_step_search(fn, None, ctypes.c_double(step0), ctypes.c_double(target), ctypes.c_double(tol),
+                              ctypes.byref(st), ctypes.byref(nm))
+    if rc != LE_OK:
+        err = LEError(rc, lib().le_last_error().decode())
+        err.measurements = nm.value
+        raise err
+    return st.value, nm.value
 
 
 def default_mc():

@@ -783,7 +783,7 @@ int or_evolve(int D, const double *Min, const double *Hin, const double *Uin, or
     cplx *M = malloc(sizeof(cplx) * DD), *Mi = malloc(sizeof(cplx) * DD);
     cplx *Heff = malloc(sizeof(cplx) * DD), *Ueff = malloc(sizeof(cplx) * DD), *A = malloc(sizeof(cplx) * DD);
     for (size_t q = 0; q < DD; ++q) M[q] = Min[2 * q] + I * Min[2 * q + 1];
-    if (cinverse(D, M, Mi)) return -1;
+    if (cinverse(D, M, Mi)) { free(M); free(Mi); free(Heff); free(Ueff); free(A); return -1; }
     for (size_t q = 0; q < DD; ++q) A[q] = Hin[2 * q] + I * Hin[2 * q + 1];
     for (int i = 0; i < D; ++i)
         for (int j = 0; j < D; ++j) {

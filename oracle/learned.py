@@ -6,8 +6,9 @@ Metropolis-Hastings move.  This module restates the definitions the GPU kernels 
 
 features of particle i at position z (others fixed), D_CAP = 8:
     f1 = |z| / R
-    m_c = min(D_CAP, min_{j != i, j = c mod 8} |z - z_j|),  c = 0..7
-    f2 = min_c m_c,   f3 = mean_c m_c
+    d_(1) <= ... <= d_(8) = the distances |z - z_j| (j != i) to the 8 nearest particles,
+    each capped at D_CAP (fewer than 8 others: the missing ones count as D_CAP)
+    f2 = d_(1),   f3 = mean_k d_(k)
 widths: (sigma_r, sigma_t) = sigma0 * exp(ln(2.5) tanh(MLP(f))), within [0.4, 2.5] sigma0
 move: z' = z + e^{i theta} (sigma_r n1 + i sigma_t n2),  theta = arg z
 log q(z'|x) = -log(2 pi sigma_r sigma_t) - |n|^2 / 2
@@ -37,28 +38,30 @@ def unpack(p):
 
 
 def features(zs, i, z, R):
-    """features and their gradient w.r.t. (Re z, Im z): f (3,), df (3, 2)"""
-    N = len(zs)
+    """features and their gradient w.r.t. (Re z, Im z): f (3,), df (3, 2).
+    d_(1) <= ... <= d_(8): distances from z to the 8 nearest particles j != i, each capped at
+    D_CAP (missing neighbours count as D_CAP); f2 = d_(1), f3 = mean d_(k); a capped
+    distance has zero gradient."""
     f = np.zeros(3)
     df = np.zeros((3, 2))
     r = abs(z)
     f[0] = r / R
     if r > 0:
         df[0] = [z.real / (r * R), z.imag / (r * R)]
-    mins = np.full(8, D_CAP)
+    others = np.delete(np.asarray(zs), i)
+    diff = z - others
+    d = np.abs(diff)
+    order = np.argsort(d, kind="stable")[:8]
+    near = np.full(8, D_CAP)
     grads = np.zeros((8, 2))
-    for j in range(N):
-        if j == i:
-            continue
-        d = abs(z - zs[j])
-        c = j % 8
-        if d < mins[c]:
-            mins[c] = d
-            grads[c] = [(z - zs[j]).real / d, (z - zs[j]).imag / d] if d > 0 else [0, 0]
-    cmin = int(np.argmin(mins))
-    f[1] = mins[cmin]
-    df[1] = grads[cmin]
-    f[2] = mins.mean()
+    for q, j in enumerate(order):
+        if d[j] < D_CAP:
+            near[q] = d[j]
+            if d[j] > 0:
+                grads[q] = [diff[j].real / d[j], diff[j].imag / d[j]]
+    f[1] = near[0]
+    df[1] = grads[0]
+    f[2] = near.mean()
     df[2] = grads.mean(0)
     return f, df
 

@@ -30,6 +30,13 @@ def build():
                                       "-L", libdir, "-llaughlinedge", "-Wl,-rpath,$ORIGIN/../../laughlinedge_amd"],
                    check=True)
     os.replace(drv + ".tmp", drv)
+    # failure-path driver (no GPU call on any of its paths)
+    fdrv = os.path.join(HERE, "failure_driver")
+    subprocess.run([hipcc] + FLAGS + ["-x", "hip", "-I", HERE, "-I", os.path.join(ROOT, "include"),
+                                      os.path.join(HERE, "failure_driver.cpp"), "-o", fdrv + ".tmp",
+                                      "-L", libdir, "-llaughlinedge", "-Wl,-rpath,$ORIGIN/../../laughlinedge_amd"],
+                   check=True)
+    os.replace(fdrv + ".tmp", fdrv)
     return out
 
 

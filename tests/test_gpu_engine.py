@@ -265,6 +265,38 @@ def test_harmonic_cross_blocks(oracle_lib):
                     assert np.allclose(back, got.conj().T)
 
 
+def test_harmonic_blocks_five_harmonics(oracle_lib):
+    """The feature kernel's widest harmonic instantiation (more than two harmonic pairs in one
+    call, as excitationResponse_compute uses for three or more sectors): harmonics L = 1..5
+    between sectors 0..5, every |dL2 - dL1| = L block's cos and sin weights against the oracle."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import W_COS, W_SIN
+    N, m, sectors = 12, 3, [0, 1, 2, 3, 4, 5]
+    harm = [1, 2, 3, 4, 5]
+    print("case harmonics N=12 sectors=0..5 L=1..5", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05)
+    pos, _ = o.plain_positions(16, 50, 40, 0.6, 8)
+    pos = pos.reshape(-1, N, 2)
+    n = pos.shape[0]
+    with Engine(N, m, sectors, w=0.05, walkers=64, samples=1, harmonics=harm) as e:
+        e.accumulate_positions(pos)
+        checked = 0
+        for L in harm:
+            c, s_ = oracle_lib.harmonic_sums(pos, L)
+            for a in range(len(sectors)):
+                for b in range(a + 1, len(sectors)):
+                    if sectors[b] - sectors[a] != L:
+                        continue
+                    for wid, wv in ((W_COS(L), c), (W_SIN(L), s_)):
+                        ref = o.accumulate_weighted(pos, wv, a, b) / n
+                        got = e.block(wid, a, b)
+                        scale = np.sqrt(np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), a, a)) *
+                                        np.linalg.norm(o.accumulate_weighted(pos, np.ones(n), b, b))) / n * N
+                        assert np.linalg.norm(got - ref) < 1e-6 * scale, (L, a, b)
+                        checked += 1
+        assert checked == 2 * 15
+
+
 def test_user_weight_blocks(oracle_lib):
     """user weights via the C callback, accumulated between every sector pair"""
     import ctypes

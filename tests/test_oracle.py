@@ -327,6 +327,31 @@ def test_learned_oracle_gradient_by_finite_differences():
         assert abs(fd - G[k]) < 1e-5 + 1e-4 * abs(fd), (k, fd, G[k])
 
 
+def test_learned_features_are_nearest_neighbour_distances():
+    """f2, f3 = the nearest and the mean of the 8 nearest distances (capped at 8), against
+    scipy's k-d tree queries; the feature gradient against central differences"""
+    from scipy.spatial import cKDTree
+    from oracle import learned
+    rng = np.random.default_rng(11)
+    for N in (5, 9, 40, 200):
+        R = np.sqrt(6.0 * N)
+        z = R * np.sqrt(rng.random(N)) * np.exp(2j * np.pi * rng.random(N))
+        pts = np.stack([z.real, z.imag], 1)
+        tree = cKDTree(pts)
+        for i in rng.choice(N, min(N, 6), replace=False):
+            k = min(9, N)
+            d, _ = tree.query(pts[i], k=k)
+            d = np.concatenate([np.asarray(d)[1:], np.full(9 - k, np.inf)])   # drop the point itself
+            near = np.minimum(d, learned.D_CAP)
+            f, df = learned.features(z, int(i), z[i], R)
+            assert abs(f[0] - abs(z[i]) / R) < 1e-14
+            assert abs(f[1] - near[0]) < 1e-12 and abs(f[2] - near.mean()) < 1e-12, (N, i)
+            for c, dz in ((0, 1e-6), (1, 1e-6j)):
+                fp, _ = learned.features(z, int(i), z[i] + dz, R)
+                fm, _ = learned.features(z, int(i), z[i] - dz, R)
+                assert np.allclose((fp - fm) / 2e-6, df[:, c], atol=1e-6), (N, i, c)
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

@@ -166,6 +166,8 @@ struct le_engine {
     long rec_elems = 0;              // complex elements per block record
     le::ExtractItem* items_d = nullptr;
     double* rec_d = nullptr;
+    double* rec_h = nullptr;         // pinned staging of one block record (2 rec_elems doubles)
+    unsigned* acc_h = nullptr;       // pinned staging of the per-walker accepted-move counts
     float* trace_d = nullptr;
     long trace_cap_rows = 0;
     long trace_rows = 0;
@@ -275,6 +277,8 @@ le_engine::~le_engine()
     (void)hipFree(cpool);
     (void)hipFree(items_d);
     (void)hipFree(rec_d);
+    if (rec_h) (void)hipHostFree(rec_h);
+    if (acc_h) (void)hipHostFree(acc_h);
     (void)hipFree(lnfact_d);
     (void)hipFree(side_d);
     (void)hipFree(mlp);
@@ -470,6 +474,11 @@ int le_engine::setup(const le_observables* obs)
     HIPCHK(dalloc(&items_d, std::max<size_t>(1, xi.size())));
     if (!xi.empty()) HIPCHK(hipMemcpy(items_d, xi.data(), xi.size() * sizeof(le::ExtractItem), hipMemcpyHostToDevice));
     HIPCHK(dalloc(&rec_d, (size_t)std::max<long>(2, 2 * rec_elems)));
+    // pinned host staging: a pageable copy of the block record went through the runtime's
+    // bounce buffer at ~2 GB/s (2 ms for the 4 MB record of N = 200, dL 1..16)
+    HIPCHK(hipHostMalloc((void**)&rec_h, sizeof(double) * (size_t)std::max<long>(2, 2 * rec_elems),
+                         hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&acc_h, sizeof(unsigned) * (size_t)std::max(1, mc.walkers), hipHostMallocDefault));
 
     // kept samples per launch round: ~128k rows, X under 1 GiB
     const long row_bytes = ldx * (long)rsz;
@@ -869,15 +878,14 @@ double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
     }
     *rc = run_sweeps(mc.tune_sweeps, (1ull << 47) + ((unsigned long long)attempt << 20), pos);
     if (*rc) return 0;
-    std::vector<unsigned> h_acc(mc.walkers);
-    h = hipMemcpyAsync(h_acc.data(), acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream);
+    h = hipMemcpyAsync(acc_h, acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream);
     if (h == hipSuccess) h = hipStreamSynchronize(stream);
     if (h != hipSuccess) {
         *rc = fail(LE_ERR_HIP, hipGetErrorString(h));
         return 0;
     }
     double tot = 0;
-    for (unsigned v : h_acc) tot += v;
+    for (int w = 0; w < mc.walkers; ++w) tot += acc_h[w];
     return tot / ((double)mc.walkers * mc.tune_sweeps * npos());
 }
 
@@ -1019,11 +1027,10 @@ int le_engine::close_block(long long samples)
     x.scale = samples > 0 ? 1.0 / (double)samples : 0.0;
     x.out = rec_d;
     HIPCHK(le::launch_extract(x, stream));
-    std::vector<double> h(2 * rec_elems);
-    HIPCHK(hipMemcpyAsync(h.data(), rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(rec_h, rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
     HIPCHK(le::launch_zero(cpool, (long)cpool_elems, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    rec.push_back(std::move(h));
+    rec.emplace_back(rec_h, rec_h + 2 * rec_elems);
     blk_samples.push_back(samples);
     return LE_OK;
 }
@@ -1235,12 +1242,12 @@ extern "C" int le_sample(le_engine* e, int samples)
     }
     e->sweeps_done += (long long)samples * e->mc.thin;
     const long long nsamp = kept * e->mc.walkers;
+    // the accepted-move counts ride along with the block record (close_block synchronises)
+    HIPCHK(hipMemcpyAsync(e->acc_h, e->acc, sizeof(unsigned) * e->mc.walkers, hipMemcpyDeviceToHost, e->stream));
     int rc = e->close_block(nsamp);
     if (rc) return rc;
-    std::vector<unsigned> h(e->mc.walkers);
-    HIPCHK(hipMemcpy(h.data(), e->acc, sizeof(unsigned) * e->mc.walkers, hipMemcpyDeviceToHost));
     double tot = 0;
-    for (unsigned v : h) tot += v;
+    for (int w = 0; w < e->mc.walkers; ++w) tot += e->acc_h[w];
     e->acc_sampling += (unsigned long long)tot;
     e->acc_moves += (long long)samples * e->mc.thin * e->npos() * e->mc.walkers;
     e->st.acceptance = (double)e->acc_sampling / (double)std::max<long long>(1, e->acc_moves);

@@ -99,6 +99,70 @@ def test_scale_spectra_vs_scipy(sampled):
         assert spec[s][0] > W_CONF * (L0 + SECTORS[s] + N) * (1 - 1e-4), s
 
 
+def _matrices(e):
+    """dense M, H and the eight Hermitian kick matrices U_k of an engine's mean moments"""
+    from laughlinedge_amd.engine import W_USER
+    dims, offs = e.dims, e.offsets
+    D = e.info.D
+    M = np.zeros((D, D), complex)
+    H = np.zeros((D, D), complex)
+    for s, (S, Hs) in enumerate(e.moments()):
+        o, d = offs[s], dims[s]
+        M[o:o + d, o:o + d] = (S + S.conj().T) / 2
+        H[o:o + d, o:o + d] = (Hs + Hs.conj().T) / 2
+    Us = []
+    for k in range(8):
+        U = np.zeros((D, D), complex)
+        for a in range(len(SECTORS)):
+            for b in range(a, len(SECTORS)):
+                blk = e.block(W_USER(k), a, b)
+                U[offs[a]:offs[a] + dims[a], offs[b]:offs[b] + dims[b]] = blk
+                U[offs[b]:offs[b] + dims[b], offs[a]:offs[a] + dims[a]] = blk.conj().T
+        Us.append((U + U.conj().T) / 2)
+    return M, H, Us
+
+
+def test_scale_fp32_response_vs_fp64(sampled):
+    """The FP32-MFMA response matrices and the edge density response they drive, against the
+    plain FP64 form on the same 65536 configurations (dynamics.cpp:286-290 integrates whatever
+    matrices the sampling wrote): M, H and the eight kick matrices U_k within 1e-6 relative
+    (Frobenius); rho_1(t) = C^H U_1 C of every kick over 0 <= t <= 100 within a tenth of its
+    Monte-Carlo error, estimated from the FP64 response of the two halves of the configurations
+    (the pencils are ill-conditioned, so 1e-7 in the matrices moves rho by ~1e-2 of its response,
+    while the sampling error of 65536 configurations is far larger)."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import evolve
+    e, _ = sampled
+    pos = e.positions()
+    print(f"case scale FP32 vs FP64 response on {pos.shape[0]} configurations", flush=True)
+    _, fn = _kicks()
+    mats = {}
+    half = pos.shape[0] // 2
+    for key, prec, p in ((0, 0, pos), (1, 1, pos), ("a", 1, pos[:half]), ("b", 1, pos[half:])):
+        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=4096, samples=1, n_user_weights=8,
+                    precision=prec) as f:
+            f.set_weight_callback(fn)
+            f.accumulate_positions(p)
+            mats[key] = _matrices(f)
+            dims = f.dims
+    (M32, H32, U32), (M64, H64, U64) = mats[0], mats[1]
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    errs = [rel(M32, M64), rel(H32, H64)] + [rel(a, b) for a, b in zip(U32, U64)]
+    print(f"relative Frobenius FP32 vs FP64: M {errs[0]:.2e} H {errs[1]:.2e} U_k max {max(errs[2:]):.2e}")
+    assert max(errs) < 1e-6, errs
+    prof = lambda t: 1e-3 * np.exp(-(t - 20.0) ** 2 / (2 * 25.0))
+    r32, _, _ = evolve(M32, H32, U32, [U32[0]], prof, 100.0, 0.1, dims)
+    r64, _, _ = evolve(M64, H64, U64, [U64[0]], prof, 100.0, 0.1, dims)
+    ra, _, _ = evolve(*mats["a"], [mats["a"][2][0]], prof, 100.0, 0.1, dims)
+    rb, _, _ = evolve(*mats["b"], [mats["b"][2][0]], prof, 100.0, 0.1, dims)
+    resp = np.abs(r64[:, :, 0] - r64[:, :1, 0]).max(axis=1)
+    dev = np.abs(r32[:, :, 0] - r64[:, :, 0]).max(axis=1)
+    mc = 0.5 * np.abs(ra[:, :, 0] - rb[:, :, 0]).max(axis=1)
+    print(f"rho_1(t) response per kick {np.array2string(resp, precision=3)}; max |FP32 - FP64| "
+          f"{np.array2string(dev, precision=3)}; Monte-Carlo error (halves) {np.array2string(mc, precision=3)}")
+    assert np.all(dev <= 0.1 * mc), (dev, mc)
+
+
 def test_scale_evolution_of_kicks(sampled):
     from laughlinedge_amd.engine import W_USER, evolve
     e, _ = sampled

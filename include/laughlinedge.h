@@ -237,6 +237,19 @@ int le_set_thin(le_engine* e, int thin);
 /* FP64 generalized eigensolve on the GPU: d x d complex (interleaved) S, H.
  * evals ascending; evecs (optional, d x d complex, column j = eigenvector j, S-normalised). */
 int le_geneig(int d, const double* S, const double* H, double* evals, double* evecs);
+/* le_geneig without device allocations.  No reference counterpart: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  S_dev, H_dev (d x d complex, interleaved), evals_dev (d) and the optional
+ * evecs_dev (d x d complex) and status_dev (one int) are DEVICE pointers; the work is enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream) and the call returns without waiting.
+ * When the stream reaches it: evals ascending, evecs as le_geneig's, *status_dev = 0, or 1 when
+ * the metric is not positive definite (evals and evecs then undefined).  The workspace is a
+ * fixed device array of the library (one per device, 8.4 MB), so d <= LE_GENEIG_NOALLOC_MAX_DIM
+ * (else LE_ERR_ARG) and calls that overlap on one device (other streams or threads) race: the
+ * array keeps the last call's contents.  The call records no memset, so it may be captured. */
+#define LE_GENEIG_NOALLOC_MAX_DIM 256
+int le_geneig_noalloc(int d, const double* S_dev, const double* H_dev, double* evals_dev, double* evecs_dev,
+                      int* status_dev, void* stream);
 /* per-sector eigenvalues of the engine's moments (block -1 = mean); evals has d_s entries */
 int le_engine_spectrum(le_engine* e, int sector, int block, double* evals);
 

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblaughlinedge.so")
-SOURCES = ["le_kernels.hip", "le_twosided.hip", "le_evolve.hip", "le_learned.hip", "le_engine.cpp", "le_basis.cpp", "le_tune.cpp"]
+SOURCES = ["le_kernels.hip", "le_twosided.hip", "le_evolve.hip", "le_learned.hip", "le_noalloc.hip", "le_engine.cpp", "le_basis.cpp", "le_tune.cpp"]
 FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 

@@ -268,6 +268,7 @@ def main():
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
                    "sweep_in_registers": bool(eng.info.sweep_in_registers),
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,
+                   "gram_xcd_spread": eng.info.gram_xcd_spread,
                    "gram_launches_timed": int(d["gram_launches"]),
                    "sampler_rounds": "hipGraph replay"},
         "roofline": {"bound": "mfma", "kernel": "k_gram<float, 2>",

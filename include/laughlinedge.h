@@ -105,6 +105,12 @@ typedef struct le_mc_params {
                                   plain proposal; occupation engine: ceil((N - 1) / lanes) 13 or 25);
                                   0 (default): LDS.  lanes_per_walker = -1 times both kinds and
                                   keeps the faster */
+    int gram_xcd_spread;       /* XCDs that share one row split's S/H Gram tiles: 1, 2, 4 or 8
+                                  (8: consecutive workgroups take consecutive tiles, spreading every
+                                  split over all XCDs, whose L2s then each fetch most of every
+                                  sample row; fewer XCDs per split fetch each row fewer times but
+                                  load each L2 more); 0 (default): time 1, 2, 4 and 8 on the first
+                                  full-size launch and keep the fastest (cached per sector layout) */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -152,6 +158,8 @@ typedef struct le_engine_info {
     double gram_mfma_flop_per_row; /* FP32-equivalent MFMA FLOPs the Gram launches issue per
                                       sample row: 2 x 16 x 16 per computed 16x16 block (those
                                       holding requested entries, an odd one padded) and weight */
+    int gram_xcd_spread;       /* XCDs sharing one row split's Gram tiles in use (the autotuned
+                                  choice once the first full-size round has run) */
 } le_engine_info;
 
 typedef struct le_run_stats {

@@ -76,6 +76,8 @@ int default_lanes(int N, int prec)
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec, two-sided) -> G
+// Gram tile placement chosen per (rows per launch, Gram kind, sector layout)
+std::map<std::tuple<long, int, std::vector<int>, int>, int> g_tuned_spread;
 
 }  // namespace
 
@@ -157,6 +159,10 @@ struct le_engine {
     bool weights_gram = false;       // occupation weights on a block of <= 64 x 64 real columns:
                                      // one k_gram_weights launch for all of them
     int ntiles = 0;                  // tiles of the S/H job
+    int gram_spread = 8;             // XCDs sharing one row split's Gram tiles (1, 2, 4 or 8)
+    bool need_spread_tune = false;   // gram_xcd_spread = 0 and no choice cached for this layout yet
+    long long block_grams = 0;       // Gram launches into the open block (the tuning needs none yet)
+    int tune_gram_spread(int buf);
     double gram_flop_row = 0;        // MFMA FLOPs issued per sample row over all Gram jobs
     struct Item {
         int wid, s1, s2;
@@ -310,6 +316,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->graphs = 1;
     p->symmetry_moves = 0;
     p->sweep_registers = 0;
+    p->gram_xcd_spread = 0;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -618,6 +625,9 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
     if (mc->sweep_registers != 0 && mc->sweep_registers != 1) return fail(LE_ERR_ARG, "sweep_registers must be 0 or 1");
+    if (mc->gram_xcd_spread != 0 && mc->gram_xcd_spread != 1 && mc->gram_xcd_spread != 2 && mc->gram_xcd_spread != 4 &&
+        mc->gram_xcd_spread != 8)
+        return fail(LE_ERR_ARG, "gram_xcd_spread must be 0, 1, 2, 4 or 8");
     if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
         !le::sweep_reg_supported(mc->lanes_per_walker, sys->particles_number))
         return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25");
@@ -658,6 +668,8 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         g_err = msg;
         return rc;
     }
+    e->gram_spread = mc->gram_xcd_spread > 0 ? mc->gram_xcd_spread : 8;
+    e->need_spread_tune = mc->gram_xcd_spread == 0;
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     *out = e;
@@ -704,6 +716,7 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     info->sweep_in_registers = map_reg(e->G) ? 1 : 0;
     info->samples_per_launch = e->spl;
     info->n_tiles = e->ntiles;
+    info->gram_xcd_spread = e->gram_spread;
     info->x_ld = e->ldx;
     info->gram_mfma_flop_per_row = e->gram_flop_row;
     return LE_OK;
@@ -1001,10 +1014,15 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         const long want_wg = 6144;
         const long per_split = (long)std::max(1, j.ntiles) * bt.count;
         long splits = std::max<long>(1, (want_wg + per_split - 1) / per_split);
+        // with the tiles of a split on K XCDs, splits come in multiples of the 8 / K groups
+        const int K = gram_spread;
+        const long groups = (K == 1 || K == 2 || K == 4) ? 8 / K : 1;
+        splits = ((splits + groups - 1) / groups) * groups;
         long rps = (rows + splits - 1) / splits;
         rps = std::max<long>(rps, 1024);
         rps = ((rps + 31) / 32) * 32;
         splits = (rows + rps - 1) / rps;
+        g.xcd_spread = K;
         g.rows_per_split = rps;
         g.splits = (int)splits;
         g.tiles = j.tiles;
@@ -1031,6 +1049,7 @@ int le_engine::close_block(long long samples)
     HIPCHK(le::launch_zero(cpool, (long)cpool_elems, stream));
     HIPCHK(hipStreamSynchronize(stream));
     rec.emplace_back(rec_h, rec_h + 2 * rec_elems);
+    block_grams = 0;
     blk_samples.push_back(samples);
     return LE_OK;
 }
@@ -1099,6 +1118,53 @@ int le_engine::capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hi
     if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
     graphs.push_back({ex, buf, traced, mc.thin, proposal, G, spl, table_T, step, table});
     *out = ex;
+    return LE_OK;
+}
+
+// Time each placement of the Gram tiles over the XCDs (gram_xcd_spread 8, 4, 2, 1: two
+// launches each on the round in buffer `buf`, the shortest counts) on the Gram stream before
+// the round's own accumulation, keep the fastest for this layout, and clear the accumulators
+// the timing launches wrote.  Once per (rows per launch, Gram kind, sector layout).
+int le_engine::tune_gram_spread(int buf)
+{
+    need_spread_tune = false;
+    const long rows = (long)spl * mc.walkers;
+    std::vector<int> layout(basis.sectors);
+    layout.push_back(n_user);
+    layout.push_back(nh);
+    const auto key = std::make_tuple(rows, gram_prec, layout, (int)jobs.size());
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned_spread.find(key);
+        if (it != g_tuned_spread.end()) {
+            gram_spread = it->second;
+            return LE_OK;
+        }
+    }
+    hipEvent_t t0 = next_event(), t1 = next_event();
+    if (!t0 || !t1) return fail(LE_ERR_HIP, "event creation failed");
+    int best = 8;
+    float best_ms = 1e30f;
+    for (int K : {8, 4, 2, 1}) {
+        gram_spread = K;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIPCHK(hipEventRecord(t0, gstream));
+            int rc = gram_rows(rows, buf, gstream);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(t1, gstream));
+            HIPCHK(hipEventSynchronize(t1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = K;
+            }
+        }
+    }
+    gram_spread = best;
+    HIPCHK(le::launch_zero(cpool, (long)cpool_elems, gstream));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned_spread[key] = best;
     return LE_OK;
 }
 
@@ -1213,11 +1279,16 @@ extern "C" int le_sample(le_engine* e, int samples)
         }
         HIPCHK(hipEventRecord(e->filled[b], e->stream));
         HIPCHK(hipStreamWaitEvent(e->gstream, e->filled[b], 0));
+        if (e->need_spread_tune && n == e->spl && !e->weights_gram && e->block_grams == 0) {
+            int rc = e->tune_gram_spread(b);
+            if (rc) return rc;
+        }
         hipEvent_t g0 = ev(), g1 = ev();
         if (!g0 || !g1) return fail(LE_ERR_HIP, "event creation failed");
         HIPCHK(hipEventRecord(g0, e->gstream));
         int rc = e->gram_rows((long)n * e->mc.walkers, b, e->gstream);
         if (rc) return rc;
+        ++e->block_grams;
         HIPCHK(hipEventRecord(g1, e->gstream));
         HIPCHK(hipEventRecord(e->drained[b], e->gstream));
         spans.push_back({g0, g1, 2, n == e->spl});
@@ -1406,6 +1477,7 @@ extern "C" int le_reset_moments(le_engine* e)
     if (!e) return fail(LE_ERR_ARG, "null engine");
     e->rec.clear();
     e->blk_samples.clear();
+    e->block_grams = 0;
     HIPCHK(le::launch_zero(e->cpool, (long)e->cpool_elems, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return LE_OK;
@@ -1472,6 +1544,7 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
         h = hipEventRecord(g0, e->stream);
         if (h != hipSuccess) { rc = fail(LE_ERR_HIP, hipGetErrorString(h)); break; }
         rc = e->gram_rows(m);
+        ++e->block_grams;
         if (rc == LE_OK) {
             h = hipEventRecord(g1, e->stream);
             if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
@@ -1873,6 +1946,9 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
     if (mc->graphs != 0 && mc->graphs != 1) return fail(LE_ERR_ARG, "graphs must be 0 or 1");
     if (mc->symmetry_moves < 0 || mc->symmetry_moves > 2) return fail(LE_ERR_ARG, "symmetry_moves must be 0, 1 or 2");
     if (mc->sweep_registers != 0 && mc->sweep_registers != 1) return fail(LE_ERR_ARG, "sweep_registers must be 0 or 1");
+    if (mc->gram_xcd_spread != 0 && mc->gram_xcd_spread != 1 && mc->gram_xcd_spread != 2 && mc->gram_xcd_spread != 4 &&
+        mc->gram_xcd_spread != 8)
+        return fail(LE_ERR_ARG, "gram_xcd_spread must be 0, 1, 2, 4 or 8");
     if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
         !le::sweep2_reg_supported(mc->lanes_per_walker, sys->particles_number))
         return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil((N - 1) / lanes) 13 or 25");
@@ -1900,6 +1976,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         g_err = msg;
         return rc;
     }
+    e->gram_spread = mc->gram_xcd_spread > 0 ? mc->gram_xcd_spread : 8;
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     *out = e;

@@ -688,18 +688,38 @@ __device__ __forceinline__ void load4(const double* p, double* o)
     }
 }
 
-// Workgroup -> (tile, row split): consecutive workgroups take consecutive tiles of one split,
-// so the dispatcher spreads a split's tiles over all 8 XCDs.  Their shared sample rows are
-// then fetched about 4.5 times per launch (FETCH_SIZE, N = 200, dL 1..16), mostly from the
-// Infinity Cache; placing all tiles of a split on one XCD cuts the fetches to ~1.1 times
-// the sample matrix but runs 8-14% slower (every workgroup of the XCD then hammers the same
-// L2 lines).
+// Workgroup -> (tile, row split).  The dispatcher hands consecutive workgroups to the 8 XCDs in
+// turn (workgroup i to XCD i mod 8), and each XCD has its own L2.
+// - xcd_spread = 8 (or any value outside 1, 2, 4): consecutive workgroups take consecutive tiles
+//   of one split, so a split's tiles spread over all 8 XCDs and every XCD fetches most of each
+//   sample row (FETCH_SIZE: 5.9 times the sample matrix at N = 200, dL 1..16), mostly from the
+//   Infinity Cache.
+// - xcd_spread = K in {1, 2, 4}: the 8 XCDs form 8 / K groups of K; split s runs on group
+//   s mod (8 / K), its tile t on XCD t mod K of that group.  Each row is then fetched by K L2s,
+//   while the workgroups of one XCD share fewer, hotter L2 lines.
 __device__ __forceinline__ bool gram_block(const GramArgs& g, int& tile, int& split)
 {
     const int id = blockIdx.x;
-    tile = id % g.ntiles;
-    split = id / g.ntiles;
-    return split < g.splits;
+    const int K = g.xcd_spread;
+    if (K != 1 && K != 2 && K != 4) {
+        tile = id % g.ntiles;
+        split = id / g.ntiles;
+        return split < g.splits;
+    }
+    const int x = id & 7, j = id >> 3;
+    const int per = (g.ntiles + K - 1) / K;
+    const int sl = j / per, tq = j - sl * per;
+    tile = (x % K) + K * tq;
+    split = (x / K) + (8 / K) * sl;
+    return tile < g.ntiles && split < g.splits;
+}
+
+long gram_grid(int ntiles, int splits, int xcd_spread)
+{
+    const int K = xcd_spread;
+    if (K != 1 && K != 2 && K != 4) return (long)ntiles * splits;
+    const long groups = 8 / K;
+    return 8L * ((splits + groups - 1) / groups) * ((ntiles + K - 1) / K);
 }
 
 constexpr int kKC = 32;          // samples staged per step
@@ -1781,7 +1801,7 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
     if (a.splits != splits) return hipErrorInvalidValue;
     if (a.jobs ? (a.njobs < 1 || a.njobs > 65535) : a.njobs > 1) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((long)a.ntiles * splits), (unsigned)(a.jobs ? a.njobs : 1));
+    dim3 grid((unsigned)gram_grid(a.ntiles, splits, a.xcd_spread), (unsigned)(a.jobs ? a.njobs : 1));
     if (prec == kBF16X3 || prec == kBF16X6) {
         if (a.rows_per_split % kKC) return hipErrorInvalidValue;
         if (prec == kBF16X3) {

@@ -112,7 +112,12 @@ struct GramArgs {
     double* C0;              // [ldc][ldc] FP64 accumulators (upper tiles)
     double* C1;
     long ldc;
+    int xcd_spread;          // 1, 2 or 4: the tiles of one row split run on that many XCDs;
+                             // otherwise consecutive workgroups take consecutive tiles (all 8)
 };
+
+// Workgroups of a Gram launch: gram_grid(ntiles, splits, xcd_spread)
+long gram_grid(int ntiles, int splits, int xcd_spread);
 
 // Many weights on one small block: C_q[r0 + i][c0 + j] += sum_s w_q(s) X[s][r0 + i] X[s][c0 + j]
 // for i < nr, j < nc (both <= 64), q < nweights, with weight q's samples at W + q wld and its

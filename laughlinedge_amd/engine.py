@@ -90,7 +90,7 @@ class MCParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
-                ("sweep_registers", ctypes.c_int)]
+                ("sweep_registers", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -103,7 +103,7 @@ class EngineInfo(ctypes.Structure):
                 ("longest_partition", ctypes.c_int), ("block_elems", ctypes.c_long),
                 ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
                 ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int),
-                ("gram_mfma_flop_per_row", ctypes.c_double)]
+                ("gram_mfma_flop_per_row", ctypes.c_double), ("gram_xcd_spread", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):

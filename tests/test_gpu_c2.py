@@ -11,7 +11,10 @@ seed 42, V = 0.025 r^2 + 5e-5 r^4.
 - Sampler law at full size: sum_i r_i^2 / 2 ~ Gamma(L0 + N) exactly for |Psi_L|^2, so over the
   walkers the mean is 2 (L0 + N) and the variance 4 (L0 + N).
 - Sampling rounds replayed from hipGraphs leave the same walkers and traces, bit for bit, as
-  eager rounds, and the same moments up to the order of the FP64 atomic adds."""
+  eager rounds, and the same moments up to the order of the FP64 atomic adds.
+- Every placement of the Gram tiles over the XCDs (gram_xcd_spread 1, 2, 4, 8) accumulates the
+  same moments up to the order of the FP64 atomic adds, and the autotuned engine reports the
+  placement it chose."""
 import numpy as np
 import pytest
 
@@ -101,3 +104,26 @@ def test_c2_graph_replay_matches_eager(chain):
     for (S0, H0), (S1, H1) in zip(m0, m1):
         assert np.linalg.norm(S0 - S1) <= 1e-12 * np.linalg.norm(S0)
         assert np.linalg.norm(H0 - H1) <= 1e-12 * np.linalg.norm(H0)
+
+
+def test_c2_gram_placements_agree(chain):
+    from laughlinedge_amd import Engine
+    pos, step = chain
+    print("case c2 Gram tile placements over the XCDs", flush=True)
+    ref = None
+    for K in (8, 4, 2, 1):
+        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=1, gram_xcd_spread=K) as f:
+            f.accumulate_positions(pos)
+            mom = f.moments()
+        if ref is None:
+            ref = mom
+            continue
+        for (S0, H0), (S1, H1) in zip(ref, mom):
+            assert np.linalg.norm(S1 - S0) <= 1e-12 * np.linalg.norm(S0), K
+            assert np.linalg.norm(H1 - H0) <= 1e-12 * np.linalg.norm(H0), K
+    with _engine(tune_sweeps=0, step=step, lanes_per_walker=8, sweep_registers=1) as e:
+        e.init_walkers()
+        e.burnin(20)
+        e.sample(e.info.samples_per_launch)
+        assert e.info.gram_xcd_spread in (1, 2, 4, 8)
+        print(f"autotuned placement: tiles of a row split on {e.info.gram_xcd_spread} XCDs")

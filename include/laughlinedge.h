@@ -57,6 +57,9 @@ extern "C" {
 #define LE_ERR_STATE 6      /* call out of order */
 #define LE_ERR_NOCONV 7     /* eigensolver did not converge */
 
+#define LE_EST_PLAIN 0      /* estimator: sample |Psi_L|^2 once, S and H for all rows (Gram on MFMA) */
+#define LE_EST_ROWWISE 1    /* estimator: the reference's row-wise |p_f Psi_L|^2 chains */
+
 #define LE_PREC_FP32 0      /* FP32 sampler, FP32 MFMA tiles flushed to FP64 */
 #define LE_PREC_FP64 1      /* FP64 sampler, FP64 MFMA accumulation (the plain FP64 form) */
 #define LE_PREC_BF16X3 2    /* FP32 sampler; S/H tiles from BF16 hi/lo split products
@@ -105,6 +108,18 @@ typedef struct le_mc_params {
                                   plain proposal; occupation engine: ceil((N - 1) / lanes) 13 or 25);
                                   0 (default): LDS.  lanes_per_walker = -1 times both kinds and
                                   keeps the faster */
+    int estimator;             /* LE_EST_PLAIN (default): one population of walkers samples
+                                  |Psi_L|^2 and S, H are accumulated for all rows at once;
+                                  LE_EST_ROWWISE: the reference's estimator (em_spectrum.cu:16-137,
+                                  318-390): for every basis state f of every sector, `walkers`
+                                  chains sample |p_f Psi_L|^2 and accumulate g/f, V g/f and
+                                  |g/f|^2 over f's sector; le_get_moments then returns the
+                                  normalised, symmetrised M(f, g) = <g/f>_f / sqrt<|g/f|^2>_f and
+                                  H(f, g) = <V g/f>_f / sqrt<|g/f|^2>_f (unit diagonal) as S and H.
+                                  Row-wise engines run the LDS sweep (no registers, symmetry moves,
+                                  learned proposal, extra observables or fixed samples); all rows
+                                  share one step; chain c = f walkers + w uses Philox walker id
+                                  c + walkers */
     int gram_xcd_spread;       /* XCDs that share one row split's S/H Gram tiles: 1, 2, 4 or 8
                                   (8: consecutive workgroups take consecutive tiles, spreading every
                                   split over all XCDs, whose L2s then each fetch most of every
@@ -160,6 +175,8 @@ typedef struct le_engine_info {
                                       holding requested entries, an odd one padded) and weight */
     int gram_xcd_spread;       /* XCDs sharing one row split's Gram tiles in use (the autotuned
                                   choice once the first full-size round has run) */
+    long chains;               /* Markov chains held: walkers, or D x walkers for the row-wise
+                                  estimator (the leading dimension of le_get/set_positions) */
 } le_engine_info;
 
 typedef struct le_run_stats {

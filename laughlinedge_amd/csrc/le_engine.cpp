@@ -159,6 +159,16 @@ struct le_engine {
     bool weights_gram = false;       // occupation weights on a block of <= 64 x 64 real columns:
                                      // one k_gram_weights launch for all of them
     int ntiles = 0;                  // tiles of the S/H job
+    // row-wise estimator: mc.walkers = D x row_W chains, chain c on row state c / row_W
+    bool rowwise = false;
+    int row_W = 0;
+    double2* row_pk = nullptr;       // [chains][8] running p_k of each chain's row-state factors
+    double* row_acc = nullptr;       // [5][sum_s d_s^2] sums of Re, Im g/f, Re, Im V g/f, |g/f|^2
+    int* row_sector = nullptr;       // [D]
+    int* sec_off_d = nullptr;        // [nsec + 1]
+    long* sec_bo_d = nullptr;        // [nsec]
+    int sample_rowwise(int samples);
+    hipError_t init_positions();
     int gram_spread = 8;             // XCDs sharing one row split's Gram tiles (1, 2, 4 or 8)
     bool need_spread_tune = false;   // gram_xcd_spread = 0 and no choice cached for this layout yet
     long long block_grams = 0;       // Gram launches into the open block (the tuning needs none yet)
@@ -283,6 +293,11 @@ le_engine::~le_engine()
     (void)hipFree(cpool);
     (void)hipFree(items_d);
     (void)hipFree(rec_d);
+    (void)hipFree(row_pk);
+    (void)hipFree(row_acc);
+    (void)hipFree(row_sector);
+    (void)hipFree(sec_off_d);
+    (void)hipFree(sec_bo_d);
     if (rec_h) (void)hipHostFree(rec_h);
     if (acc_h) (void)hipHostFree(acc_h);
     (void)hipFree(lnfact_d);
@@ -316,6 +331,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->graphs = 1;
     p->symmetry_moves = 0;
     p->sweep_registers = 0;
+    p->estimator = LE_EST_PLAIN;
     p->gram_xcd_spread = 0;
 }
 
@@ -594,6 +610,26 @@ int le_engine::setup(const le_observables* obs)
         trace_cap_rows = (long)std::max(1, mc.samples) * tw;
         HIPCHK(dalloc(&trace_d, (size_t)trace_cap_rows * 9));
     }
+    if (rowwise) {
+        const long be = basis.block_elems();
+        std::vector<int> rs(std::max(1, D)), so(basis.sector_offset.begin(), basis.sector_offset.end());
+        std::vector<long> bo(std::max(1, ns));
+        for (int s = 0; s < ns; ++s) {
+            bo[s] = basis.block_offset(s);
+            for (int t = basis.sector_offset[s]; t < basis.sector_offset[s + 1]; ++t) rs[t] = s;
+        }
+        HIPCHK(dalloc(&row_pk, (size_t)mc.walkers * le::kMaxFactors));
+        HIPCHK(dalloc(&row_acc, (size_t)std::max<long>(1, 5 * be)));
+        HIPCHK(le::launch_zero(row_acc, 5 * be, stream));
+        HIPCHK(dalloc(&row_sector, rs.size()));
+        HIPCHK(hipMemcpy(row_sector, rs.data(), rs.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&sec_off_d, so.size()));
+        HIPCHK(hipMemcpy(sec_off_d, so.data(), so.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&sec_bo_d, bo.size()));
+        HIPCHK(hipMemcpy(sec_bo_d, bo.data(), bo.size() * sizeof(long), hipMemcpyHostToDevice));
+        if (rec_h) (void)hipHostFree(rec_h);
+        HIPCHK(hipHostMalloc((void**)&rec_h, sizeof(double) * (size_t)std::max<long>(2, 5 * be), hipHostMallocDefault));
+    }
     ev.resize(2 * spl + 4);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipStreamSynchronize(stream));
@@ -634,6 +670,11 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
+    if (mc->estimator != LE_EST_PLAIN && mc->estimator != LE_EST_ROWWISE)
+        return fail(LE_ERR_ARG, "estimator must be LE_EST_PLAIN or LE_EST_ROWWISE");
+    if (mc->estimator == LE_EST_ROWWISE &&
+        (mc->symmetry_moves || mc->sweep_registers || (obs && (obs->n_harmonics || obs->n_user_weights))))
+        return fail(LE_ERR_ARG, "the row-wise estimator takes no symmetry moves, register sweeps or extra observables");
     auto e = new le_engine();
     e->sys = *sys;
     e->mc = *mc;
@@ -661,6 +702,19 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
             delete e;
             return fail(LE_ERR_CAPACITY, "sector larger than 640 states");
         }
+    if (mc->estimator == LE_EST_ROWWISE) {
+        const long chains = (long)e->basis.D * mc->walkers;
+        if (chains > (1L << 26)) {
+            delete e;
+            return fail(LE_ERR_CAPACITY, "row-wise estimator: basis states x walkers above 2^26 chains");
+        }
+        e->rowwise = true;
+        e->row_W = mc->walkers;
+        e->mc.walkers = (int)chains;
+        e->mc.samples_per_launch = 1;
+        e->mc.graphs = 0;
+        if (e->mc.lanes_per_walker < 0) e->mc.lanes_per_walker = 0;   // the timed autotune would advance the chains
+    }
     int rc = e->setup(obs);
     if (rc != LE_OK) {
         std::string msg = g_err;
@@ -717,6 +771,7 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     info->samples_per_launch = e->spl;
     info->n_tiles = e->ntiles;
     info->gram_xcd_spread = e->gram_spread;
+    info->chains = e->mc.walkers;
     info->x_ld = e->ldx;
     info->gram_mfma_flop_per_row = e->gram_flop_row;
     return LE_OK;
@@ -761,6 +816,15 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
     a.walkers = mc.walkers;
     a.nsweeps = nsweeps;
     a.step = step;
+    if (rowwise) {
+        a.walker_id0 = row_W;
+        a.row_walkers = row_W;
+        a.row_fac = fac;
+        a.row_pk = row_pk;
+        a.R = R;
+        HIPCHK(le::launch_sweep_row(a, prec, map_lanes(G), stream));
+        return LE_OK;
+    }
     if (two_sided) {
         if (map_reg(G)) HIPCHK(le::launch_sweep2_reg(a, prec, map_lanes(G), stream));
         else HIPCHK(le::launch_sweep2(a, prec, G, stream));
@@ -862,10 +926,26 @@ int le_engine::autotune_lanes()
     return LE_OK;
 }
 
+// Fresh walkers (uniform in the disc); row-wise chains also take their row state's p_k.
+hipError_t le_engine::init_positions()
+{
+    hipError_t h = le::launch_init_positions(pos, prec, npos(), mc.walkers, R, mc.seed, stream, rowwise ? row_W : 0);
+    if (h != hipSuccess || !rowwise) return h;
+    le::SweepArgs a{};
+    a.pos = pos;
+    a.N = sys.particles_number;
+    a.walkers = mc.walkers;
+    a.row_walkers = row_W;
+    a.row_fac = fac;
+    a.row_pk = row_pk;
+    a.R = R;
+    return le::launch_row_init(a, prec, stream);
+}
+
 extern "C" int le_init_walkers(le_engine* e)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
-    HIPCHK(le::launch_init_positions(e->pos, e->prec, e->npos(), e->mc.walkers, e->R, e->mc.seed, e->stream));
+    HIPCHK(e->init_positions());
     HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->walkers_ready = true;
@@ -883,7 +963,7 @@ double le_engine::measure_acceptance(double trial_step, int attempt, int* rc)
     // fresh positions and tune_sweeps sweeps, as mcmc_acceptanceBisection (em_bisection.cu:7-47)
     *rc = LE_OK;
     step = trial_step;
-    hipError_t h = le::launch_init_positions(pos, prec, npos(), mc.walkers, R, mc.seed, stream);
+    hipError_t h = init_positions();
     if (h == hipSuccess) h = hipMemsetAsync(acc, 0, sizeof(unsigned) * mc.walkers, stream);
     if (h != hipSuccess) {
         *rc = fail(LE_ERR_HIP, hipGetErrorString(h));
@@ -1168,6 +1248,86 @@ int le_engine::tune_gram_spread(int buf)
     return LE_OK;
 }
 
+// Row-wise estimator (em_spectrum.cu:16-137): per kept sample, thin sweeps of every chain on
+// |p_f Psi_L|^2, the monomials of every chain, and the per-row sums of g/f, V g/f, |g/f|^2 over
+// f's sector; one block of sums per call.
+int le_engine::sample_rowwise(int samples)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemsetAsync(acc, 0, sizeof(unsigned) * mc.walkers, stream));
+    le::FeatureArgs f{};
+    f.ps = ps_d;
+    f.ps_ld = kPsLd;
+    f.pos = pos;
+    f.X = Xb[0];
+    f.ldx = ldx;
+    f.W = Wbb[0];
+    f.wld = wrows;
+    f.N = sys.particles_number;
+    f.walkers = mc.walkers;
+    f.D = basis.D;
+    f.K = basis.K;
+    f.fac = fac;
+    f.col_re = col_re;
+    f.col_dim = col_dim;
+    f.prod = prod_d;
+    f.pw_k = pwk_d;
+    f.pw_r = pwr_d;
+    f.npw = npw;
+    f.invR = 1.0 / R;
+    f.pot = {sys.conf_w, sys.conf_eps, sys.conf_strength, sys.conf_exponent, R};
+    le::RowAccumArgs ra{};
+    ra.X = Xb[0];
+    ra.ldx = ldx;
+    ra.V = Wbb[0];
+    ra.col_re = col_re;
+    ra.col_dim = col_dim;
+    ra.row_sector = row_sector;
+    ra.sec_off = sec_off_d;
+    ra.sec_bo = sec_bo_d;
+    ra.block_elems = basis.block_elems();
+    ra.D = basis.D;
+    ra.row_walkers = row_W;
+    ra.acc = row_acc;
+    const int tw = std::min(mc.trace_walkers, mc.walkers);
+    for (int q = 0; q < samples; ++q) {
+        int rc = run_sweeps(mc.thin, sweep_counter, pos);
+        if (rc) return rc;
+        sweep_counter += (unsigned long long)mc.thin;
+        if (trace_d && trace_rows + tw <= trace_cap_rows) {
+            f.trace = trace_d;
+            f.trace_walkers = tw;
+            f.trace_row0 = trace_rows;
+            trace_rows += tw;
+        } else {
+            f.trace = nullptr;
+            f.trace_walkers = 0;
+        }
+        HIPCHK(le::launch_features(f, prec, stream));
+        HIPCHK(le::launch_row_accum(ra, prec, stream));
+    }
+    sweeps_done += (long long)samples * mc.thin;
+    // close the block: raw sums to the host, accumulators cleared
+    const long n5 = 5 * basis.block_elems();
+    HIPCHK(hipMemcpyAsync(acc_h, acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(rec_h, row_acc, sizeof(double) * n5, hipMemcpyDeviceToHost, stream));
+    HIPCHK(le::launch_zero(row_acc, n5, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    rec.emplace_back(rec_h, rec_h + n5);
+    blk_samples.push_back((long long)samples * row_W);     // samples per row
+    double tot = 0;
+    for (int w = 0; w < mc.walkers; ++w) tot += acc_h[w];
+    acc_sampling += (unsigned long long)tot;
+    acc_moves += (long long)samples * mc.thin * npos() * mc.walkers;
+    st.acceptance = (double)acc_sampling / (double)std::max<long long>(1, acc_moves);
+    st.samples += (long long)samples * mc.walkers;
+    st.sweeps = sweeps_done;
+    st.step = step;
+    st.sweep_launches += samples;
+    st.ms_sample_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return LE_OK;
+}
+
 extern "C" int le_sample(le_engine* e, int samples)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
@@ -1176,6 +1336,7 @@ extern "C" int le_sample(le_engine* e, int samples)
         int rc = le_init_walkers(e);
         if (rc) return rc;
     }
+    if (e->rowwise) return e->sample_rowwise(samples);
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
     le::FeatureArgs f{};
@@ -1391,9 +1552,56 @@ static int record_slice(const le_engine* e, int block, long off, long n, double*
     return LE_OK;
 }
 
+// Row-wise estimator: normalised, symmetrised M and H (em_spectrum.cu:318-335, 382-390) from the
+// raw sums of one block or of all blocks (sums add; each row's sample count is blk_samples).
+static int rowwise_moments(const le_engine* e, int block, double* S, double* H)
+{
+    const int nb = (int)e->rec.size();
+    if (nb == 0) return fail(LE_ERR_STATE, "no closed sample blocks");
+    if (block >= nb || block < -1) return fail(LE_ERR_ARG, "block out of range");
+    const long be = e->basis.block_elems();
+    std::vector<double> sum(5 * be, 0.0);
+    double n = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (block >= 0 && b != block) continue;
+        for (long q = 0; q < 5 * be; ++q) sum[q] += e->rec[b][q];
+        n += (double)e->blk_samples[b];
+    }
+    for (int s = 0; s < e->basis.nsec(); ++s) {
+        const int d = e->basis.dim(s);
+        const long bo = e->basis.block_offset(s);
+        // X(f, g) = <x g/f>_f / sqrt(<|g/f|^2>_f), then (X + X^H) / 2
+        auto norm_entry = [&](int q0, int f, int g, double& re, double& im) {
+            const long ix = bo + (long)f * d + g;
+            const double nrm = std::sqrt(sum[4 * be + ix] / n);
+            re = sum[q0 * be + ix] / n / nrm;
+            im = sum[(q0 + 1) * be + ix] / n / nrm;
+        };
+        for (int a = 0; a < d; ++a)
+            for (int c = 0; c < d; ++c) {
+                double ar, ai, cr, ci;
+                const long out = 2 * (bo + (long)a * d + c);
+                if (S) {
+                    norm_entry(0, a, c, ar, ai);
+                    norm_entry(0, c, a, cr, ci);
+                    S[out] = 0.5 * (ar + cr);
+                    S[out + 1] = 0.5 * (ai - ci);
+                }
+                if (H) {
+                    norm_entry(2, a, c, ar, ai);
+                    norm_entry(2, c, a, cr, ci);
+                    H[out] = 0.5 * (ar + cr);
+                    H[out + 1] = 0.5 * (ai - ci);
+                }
+            }
+    }
+    return LE_OK;
+}
+
 extern "C" int le_get_moments(const le_engine* e, int block, double* S, double* H)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
+    if (e->rowwise) return rowwise_moments(e, block, S, H);
     long o = 0;
     for (int s = 0; s < e->basis.nsec(); ++s) {
         const long n = (long)e->basis.dim(s) * e->basis.dim(s);
@@ -1414,6 +1622,7 @@ extern "C" int le_get_moments(const le_engine* e, int block, double* S, double* 
 
 extern "C" int le_get_block(const le_engine* e, int block, int weight, int s1, int s2, double* out)
 {
+    if (e && e->rowwise) return fail(LE_ERR_ARG, "the row-wise estimator keeps only M and H (le_get_moments)");
     if (!e || !out) return fail(LE_ERR_ARG, "null argument");
     const int ns = e->basis.nsec();
     if (s1 < 0 || s2 < 0 || s1 >= ns || s2 >= ns) return fail(LE_ERR_ARG, "sector out of range");
@@ -1468,6 +1677,19 @@ extern "C" int le_set_positions(le_engine* e, const double* in)
         for (size_t q = 0; q < n; ++q) tmp[q] = (float)in[q];
         HIPCHK(hipMemcpy(e->pos, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
     }
+    if (e->rowwise) {
+        // the chains' running p_k of their row states follow the new positions
+        le::SweepArgs a{};
+        a.pos = e->pos;
+        a.N = e->sys.particles_number;
+        a.walkers = e->mc.walkers;
+        a.row_walkers = e->row_W;
+        a.row_fac = e->fac;
+        a.row_pk = e->row_pk;
+        a.R = e->R;
+        HIPCHK(le::launch_row_init(a, e->prec, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+    }
     e->walkers_ready = true;
     return LE_OK;
 }
@@ -1478,6 +1700,7 @@ extern "C" int le_reset_moments(le_engine* e)
     e->rec.clear();
     e->blk_samples.clear();
     e->block_grams = 0;
+    if (e->rowwise) HIPCHK(le::launch_zero(e->row_acc, 5 * e->basis.block_elems(), e->stream));
     HIPCHK(le::launch_zero(e->cpool, (long)e->cpool_elems, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return LE_OK;
@@ -1494,6 +1717,7 @@ extern "C" int le_set_thin(le_engine* e, int thin)
 extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
 {
     if (!e || !pos || n <= 0) return fail(LE_ERR_ARG, "null argument or n <= 0");
+    if (e->rowwise) return fail(LE_ERR_STATE, "fixed samples do not apply to the row-wise estimator's chains");
     const int N = e->npos();
     const long chunk = (long)e->spl * e->mc.walkers;
     double* stage = nullptr;
@@ -1662,6 +1886,7 @@ extern "C" int le_removal_weights(le_engine* e, int sector, int block, int* orbi
 {
     if (!e || !energies || !weights || sector < 0 || sector >= e->basis.nsec())
         return fail(LE_ERR_ARG, "le_removal_weights: bad argument");
+    if (e->rowwise) return fail(LE_ERR_ARG, "le_removal_weights needs the plain estimator's unnormalised S");
     if (e->two_sided) return fail(LE_ERR_STATE, "le_removal_weights: needs an edge engine, not the occupation chain");
     const int N = e->sys.particles_number, m = e->sys.inverse_filling;
     const int k = e->basis.sectors[sector];
@@ -2048,7 +2273,8 @@ extern "C" int le_set_proposal(le_engine* e, int kind)
     }
     if (kind != LE_PROPOSAL_LEARNED) return fail(LE_ERR_ARG, "unknown proposal");
     if (!e->table) return fail(LE_ERR_STATE, "no trained proposal (le_train_proposal or le_set_proposal_params)");
-    if (e->prec != LE_PREC_FP32 || e->two_sided) return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine");
+    if (e->prec != LE_PREC_FP32 || e->two_sided || e->rowwise)
+        return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine with the plain estimator");
     e->proposal = kind;
     return LE_OK;
 }
@@ -2117,7 +2343,8 @@ extern "C" int le_proposal_loss_grad(le_engine* e, const float* configs, int n_c
 extern "C" int le_train_proposal(le_engine* e, const le_train_params* tp, le_train_report* rep)
 {
     if (!e || !tp) return fail(LE_ERR_ARG, "null argument");
-    if (e->prec != LE_PREC_FP32 || e->two_sided) return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine");
+    if (e->prec != LE_PREC_FP32 || e->two_sided || e->rowwise)
+        return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine with the plain estimator");
     if (tp->steps < 0 || tp->batch <= 0 || tp->n_configs <= 0 || tp->snapshot_sweeps < 0)
         return fail(LE_ERR_ARG, "bad training parameters");
     const auto t0 = std::chrono::steady_clock::now();

@@ -32,6 +32,7 @@
 #include "le_device.hpp"
 #include <utility>
 
+#include "le_basis.hpp"
 #include "le_kernels.hpp"
 
 namespace le {
@@ -44,13 +45,13 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 template <typename Real>
 __global__ void k_init_positions(typename V2T<Real>::T* pos, int N, int walkers, double R,
-                                 unsigned long long seed)
+                                 unsigned long long seed, int walker_id0)
 {
     long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)N * walkers) return;
     int w = (int)(g / N), i = (int)(g % N);
     uint32_t r[4];
-    philox((uint32_t)i, 0u, 1u << 16, (uint32_t)w, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    philox((uint32_t)i, 0u, 1u << 16, (uint32_t)(w + walker_id0), (uint32_t)seed, (uint32_t)(seed >> 32), r);
     double rad = R * sqrt(u01d(r[0]));
     double s, c;
     sincos(6.283185307179586 * u01d(r[1]), &s, &c);
@@ -66,7 +67,60 @@ __global__ void k_init_positions(typename V2T<Real>::T* pos, int N, int walkers,
 // eight squared distances are taken to log2 at a time (FP32 range), FP64 takes one log.
 // ---------------------------------------------------------------------------
 
-template <int G, typename Real>
+// Importance factor of the row-wise estimator (em_markovchain.cu:185-220, em_spectrum.cu:49-96):
+// the chain samples |p_f Psi_L|^2, so a move adds log|p_f(z')|^2 - log|p_f(z)|^2 to log A.
+// p_f = prod_q p_{k_q}^{r_q} over the row state's factors; the running p_k are updated by
+// -u^k + u'^k with u = z / R (powers by repeated multiplication, k = 1..kmax), and every product
+// is rounded on its own (no contraction), as the FP64 CPU restatement computes them.
+struct RowState {
+    double2 pk[kMaxFactors];
+    int k[kMaxFactors], r[kMaxFactors];
+    int kmax;
+    double lm;               // log |p_f|^2 at the current positions
+};
+
+__device__ __forceinline__ double2 cmul_exact(double2 a, double2 b)
+{
+#pragma clang fp contract(off)
+    double2 c;
+    c.x = a.x * b.x - a.y * b.y;
+    c.y = a.x * b.y + a.y * b.x;
+    return c;
+}
+
+__device__ __forceinline__ double log_abs2_row(const RowState& st, const double2 (&pk)[kMaxFactors])
+{
+#pragma clang fp contract(off)
+    double2 v = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < kMaxFactors; ++q)
+        for (int t = 0; t < st.r[q]; ++t) v = cmul_exact(v, pk[q]);
+    return log(v.x * v.x + v.y * v.y);
+}
+
+// proposal z -> z' of a row chain: the updated p_k (prop) and log|p_f(z')|^2
+__device__ __forceinline__ double row_propose(const RowState& st, double xo, double yo, double xn, double yn, double R,
+                                              double2 (&prop)[kMaxFactors])
+{
+#pragma clang fp contract(off)
+    const double2 uo = make_double2(xo / R, yo / R), un = make_double2(xn / R, yn / R);
+    double2 qo = uo, qn = un;
+#pragma unroll
+    for (int q = 0; q < kMaxFactors; ++q) prop[q] = st.pk[q];
+    for (int k = 1; k <= st.kmax; ++k) {
+#pragma unroll
+        for (int q = 0; q < kMaxFactors; ++q)
+            if (st.k[q] == k) {
+                prop[q].x = (st.pk[q].x - qo.x) + qn.x;
+                prop[q].y = (st.pk[q].y - qo.y) + qn.y;
+            }
+        qo = cmul_exact(qo, uo);
+        qn = cmul_exact(qn, un);
+    }
+    return log_abs2_row(st, prop);
+}
+
+template <int G, typename Real, bool ROW = false>
 __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 {
     using V2 = typename V2T<Real>::T;
@@ -102,6 +156,22 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
     const bool has_pad = (lane + G * (S - 1)) >= N;
     const int pad_j = lane + G * (S - 1);
     unsigned accepted = 0;
+    const uint32_t wid = (uint32_t)(walker + a.walker_id0);
+    RowState row;
+    if constexpr (ROW) {
+        // every lane of the walker holds the same row state and takes the same decisions
+        const int f = walker / a.row_walkers;
+        row.kmax = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxFactors; ++q) {
+            const uint32_t w = a.row_fac[(size_t)f * kMaxFactors + q];
+            row.k[q] = (int)(w & 0xffu);
+            row.r[q] = row.k[q] ? (int)((w >> 8) & 0xffu) : 0;
+            row.kmax = max(row.kmax, row.k[q]);
+            row.pk[q] = a.row_pk[(size_t)walker * kMaxFactors + q];
+        }
+        row.lm = log_abs2_row(row, row.pk);
+    }
 
     for (int sw = 0; sw < a.nsweeps; ++sw) {
         const unsigned long long sidx = sweep_first + (unsigned long long)sw;
@@ -109,7 +179,7 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         for (int base = 0; base < N; base += G) {
             uint32_t r[4];
-            philox((uint32_t)(base + lane), c1, c2, (uint32_t)walker, k0, k1, r);
+            philox((uint32_t)(base + lane), c1, c2, wid, k0, k1, r);
             Real g1, g2;
             gauss2(r[0], r[1], g1, g2);
             float lnu = 0.f;
@@ -208,12 +278,25 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                 else
                     lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
                 lsum = group_sum<G, Real>(lsum);
-                const Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zo.x, zo.x, zo.y * zo.y), mfac * lsum);
+                Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zo.x, zo.x, zo.y * zo.y), mfac * lsum);
+                double2 prop[kMaxFactors];
+                double lm_new = 0.0;
+                if constexpr (ROW) {
+                    lm_new = row_propose(row, (double)zo.x, (double)zo.y, (double)xn, (double)yn, a.R, prop);
+                    logA = (Real)((double)logA + (lm_new - row.lm));
+                }
                 bool acc;
                 if constexpr (sizeof(Real) == 4) {
                     acc = (logA >= 0.f) || (lnu_move < logA);
                 } else {
                     acc = (logA >= 0.0) || (log(u01d(ub)) < logA);
+                }
+                if constexpr (ROW) {
+                    if (acc) {
+#pragma unroll
+                        for (int q = 0; q < kMaxFactors; ++q) row.pk[q] = prop[q];
+                        row.lm = lm_new;
+                    }
                 }
                 V2 nz;
                 nz.x = acc ? xn : zo.x;
@@ -228,6 +311,11 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
     }
     for (int j = lane; j < N; j += G) gpos[j] = P[j];
     if (lane == 0) a.accepted[walker] += accepted;
+    if constexpr (ROW) {
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < kMaxFactors; ++q) a.row_pk[(size_t)walker * kMaxFactors + q] = row.pk[q];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1675,42 +1763,44 @@ size_t sweep_lds_bytes(int prec, int G, int N)
 bool sweep_supported(int G) { return G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64; }
 
 hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
-                                 unsigned long long seed, hipStream_t s)
+                                 unsigned long long seed, hipStream_t s, int walker_id0)
 {
     long n = (long)N * walkers;
     dim3 grid((unsigned)((n + 255) / 256));
     if (prec == kFP64)
-        hipLaunchKernelGGL(k_init_positions<double>, grid, dim3(256), 0, s, (double2*)pos, N, walkers, R, seed);
+        hipLaunchKernelGGL(k_init_positions<double>, grid, dim3(256), 0, s, (double2*)pos, N, walkers, R, seed,
+                           walker_id0);
     else
-        hipLaunchKernelGGL(k_init_positions<float>, grid, dim3(256), 0, s, (float2*)pos, N, walkers, R, seed);
+        hipLaunchKernelGGL(k_init_positions<float>, grid, dim3(256), 0, s, (float2*)pos, N, walkers, R, seed,
+                           walker_id0);
     return hipGetLastError();
 }
 
-template <int G, typename Real>
+template <int G, typename Real, bool ROW = false>
 static hipError_t sweep_g(const SweepArgs& a, hipStream_t s)
 {
     const size_t lds = sweep_lds_bytes(sizeof(Real) == 8 ? kFP64 : kFP32, G, a.N);
     constexpr int WPB = 64 / G;
     dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sweep<G, Real>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep<G, Real, ROW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sweep<G, Real>), grid, dim3(64), lds, s, a);
+    hipLaunchKernelGGL((k_sweep<G, Real, ROW>), grid, dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-template <typename Real>
+template <typename Real, bool ROW = false>
 static hipError_t sweep_dispatch(const SweepArgs& a, int G, hipStream_t s)
 {
     switch (G) {
-    case 2: return sweep_g<2, Real>(a, s);
-    case 4: return sweep_g<4, Real>(a, s);
-    case 8: return sweep_g<8, Real>(a, s);
-    case 16: return sweep_g<16, Real>(a, s);
-    case 32: return sweep_g<32, Real>(a, s);
-    case 64: return sweep_g<64, Real>(a, s);
+    case 2: return sweep_g<2, Real, ROW>(a, s);
+    case 4: return sweep_g<4, Real, ROW>(a, s);
+    case 8: return sweep_g<8, Real, ROW>(a, s);
+    case 16: return sweep_g<16, Real, ROW>(a, s);
+    case 32: return sweep_g<32, Real, ROW>(a, s);
+    case 64: return sweep_g<64, Real, ROW>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1754,6 +1844,117 @@ hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
 {
     if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
     return prec == kFP64 ? sweep_dispatch<double>(a, G, s) : sweep_dispatch<float>(a, G, s);
+}
+
+// ---------------------------------------------------------------------------
+// Row-wise estimator (the reference's: em_spectrum.cu:16-137, em_markovchain.cu:185-220).
+// ---------------------------------------------------------------------------
+hipError_t launch_sweep_row(const SweepArgs& a, int prec, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    if (!a.row_fac || !a.row_pk || a.row_walkers < 1) return hipErrorInvalidValue;
+    return prec == kFP64 ? sweep_dispatch<double, true>(a, G, s) : sweep_dispatch<float, true>(a, G, s);
+}
+
+// p_k (k = 1..kmax of the chain's row state) of the chain's positions, summed over the
+// particles in order with u^k by repeated multiplication (sampling_ek_construct,
+// em_markovchain.cu:185-199); one thread per chain
+template <typename Real>
+__global__ void k_row_init(SweepArgs a)
+{
+#pragma clang fp contract(off)
+    using V2 = typename V2T<Real>::T;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.walkers) return;
+    const int f = c / a.row_walkers;
+    int k[kMaxFactors], kmax = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxFactors; ++q) {
+        k[q] = (int)(a.row_fac[(size_t)f * kMaxFactors + q] & 0xffu);
+        kmax = max(kmax, k[q]);
+    }
+    double2 pk[kMaxFactors];
+#pragma unroll
+    for (int q = 0; q < kMaxFactors; ++q) pk[q] = make_double2(0.0, 0.0);
+    const V2* P = reinterpret_cast<const V2*>(a.pos) + (size_t)c * a.N;
+    for (int j = 0; j < a.N; ++j) {
+        const double2 u = make_double2((double)P[j].x / a.R, (double)P[j].y / a.R);
+        double2 q = u;
+        for (int kk = 1; kk <= kmax; ++kk) {
+#pragma unroll
+            for (int t = 0; t < kMaxFactors; ++t)
+                if (k[t] == kk) {
+                    pk[t].x = pk[t].x + q.x;
+                    pk[t].y = pk[t].y + q.y;
+                }
+            q = cmul_exact(q, u);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxFactors; ++q) a.row_pk[(size_t)c * kMaxFactors + q] = pk[q];
+}
+
+hipError_t launch_row_init(const SweepArgs& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    if (!a.row_fac || !a.row_pk || a.row_walkers < 1) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((a.walkers + 127) / 128));
+    if (prec == kFP64) hipLaunchKernelGGL(k_row_init<double>, grid, dim3(128), 0, s, a);
+    else hipLaunchKernelGGL(k_row_init<float>, grid, dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+// One workgroup per row state f: thread g' sums, over the row's chains, g/f (1 at g = f),
+// V g/f and |g/f|^2 for g = o + g' of f's sector, and adds them to the FP64 accumulators (one
+// workgroup owns each row, so no atomics).  Complex division in FP64.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_row_accum(RowAccumArgs a)
+{
+    const int f = blockIdx.x;
+    if (f >= a.D) return;
+    const int s = a.row_sector[f];
+    const int o = a.sec_off[s], d = a.sec_off[s + 1] - o;
+    const int fa = f - o;
+    const Real* X = reinterpret_cast<const Real*>(a.X);
+    const Real* V = reinterpret_cast<const Real*>(a.V);
+    const int cf = a.col_re[f], df = a.col_dim[f];
+    const long c0 = (long)f * a.row_walkers;
+    for (int gp = threadIdx.x; gp < d; gp += blockDim.x) {
+        const int g = o + gp;
+        const int cg = a.col_re[g], dg = a.col_dim[g];
+        double sr = 0, si = 0, hr = 0, hi = 0, gg = 0;
+        for (int w = 0; w < a.row_walkers; ++w) {
+            const Real* x = X + (size_t)(c0 + w) * a.ldx;
+            double rr = 1.0, ri = 0.0;
+            if (g != f) {
+                const double fr = (double)x[cf], fi = (double)x[cf + df];
+                const double gr = (double)x[cg], gi = (double)x[cg + dg];
+                const double den = fr * fr + fi * fi;
+                rr = (gr * fr + gi * fi) / den;
+                ri = (gi * fr - gr * fi) / den;
+            }
+            const double v = (double)V[c0 + w];
+            sr += rr;
+            si += ri;
+            hr += v * rr;
+            hi += v * ri;
+            gg += rr * rr + ri * ri;
+        }
+        const size_t ix = (size_t)a.sec_bo[s] + (size_t)fa * d + gp;
+        a.acc[ix] += sr;
+        a.acc[a.block_elems + ix] += si;
+        a.acc[2 * a.block_elems + ix] += hr;
+        a.acc[3 * a.block_elems + ix] += hi;
+        a.acc[4 * a.block_elems + ix] += gg;
+    }
+}
+
+hipError_t launch_row_accum(const RowAccumArgs& a, int prec, hipStream_t s)
+{
+    if (a.D <= 0) return hipSuccess;
+    if (prec == kFP64) hipLaunchKernelGGL(k_row_accum<double>, dim3((unsigned)a.D), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_row_accum<float>, dim3((unsigned)a.D), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 template <typename Real, int NH>

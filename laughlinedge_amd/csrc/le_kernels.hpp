@@ -22,6 +22,31 @@ struct SweepArgs {
     double step;
     const unsigned long long* sweep_base;   // optional device value added to sweep0 (graph replays)
     int dilate;              // k_symmetry: also redraw the scale of the relative coordinates
+    // row-wise estimator (launch_sweep_row): chain c samples |p_f Psi_L|^2 for the row state
+    // f = c / row_walkers; its Philox walker id is c + walker_id0
+    int walker_id0;
+    int row_walkers;
+    const uint32_t* row_fac;         // [D][8] packed (k | r << 8) factors of every state
+    double2* row_pk;                 // [chains][8] running p_k of the row state's factors
+    double R;                        // power sums of z / R
+};
+
+// Row-wise accumulation (the reference's estimator): per row f (a basis state of sector s, d
+// states from basis offset o), over the row_walkers chains of f, sums of g/f = p_g / p_f (1 at
+// g = f), V g/f and |g/f|^2 for g in the sector, added to acc at [q][bo_s + a d + g'],
+// q = Re fg, Im fg, Re fHg, Im fHg, gg (a = f - o, g' = g - o).
+struct RowAccumArgs {
+    const void* X;           // [chains][ldx] real-embedded monomials of the current sample
+    long ldx;
+    const void* V;           // [chains] confinement of the current sample
+    const int* col_re;       // [D]
+    const int* col_dim;      // [D]
+    const int* row_sector;   // [D] sector of each state
+    const int* sec_off;      // [nsec + 1] basis offsets
+    const long* sec_bo;      // [nsec] block offsets sum_{q<s} d_q^2
+    long block_elems;        // sum_s d_s^2
+    int D, row_walkers;
+    double* acc;             // [5][block_elems]
 };
 
 constexpr int kMaxHarmonics = 8;
@@ -165,7 +190,12 @@ struct GenEigProblem {
 enum Prec { kFP32 = 0, kFP64 = 1, kBF16X3 = 2, kBF16X6 = 3 };   // kBF16X*: Gram only (FP32 data)
 
 hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double R,
-                                 unsigned long long seed, hipStream_t s);
+                                 unsigned long long seed, hipStream_t s, int walker_id0 = 0);
+// row-wise estimator: Metropolis on |p_f Psi_L|^2 (LDS sweep with the importance ratio), the
+// chains' starting p_k from their positions, and the per-sample row accumulation
+hipError_t launch_sweep_row(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+hipError_t launch_row_init(const SweepArgs& a, int prec, hipStream_t s);
+hipError_t launch_row_accum(const RowAccumArgs& a, int prec, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 // exact centre-of-mass heat bath + global rotation of every walker (a.sweep0 + *a.sweep_base
 // selects the random numbers)

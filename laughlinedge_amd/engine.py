@@ -39,6 +39,8 @@ EXPORTED = [
 ]
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
+EST_PLAIN = 0
+EST_ROWWISE = 1
 N_MLP_PARAMS = 4546
 
 
@@ -90,7 +92,7 @@ class MCParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
-                ("sweep_registers", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int)]
+                ("sweep_registers", ctypes.c_int), ("estimator", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -103,7 +105,8 @@ class EngineInfo(ctypes.Structure):
                 ("longest_partition", ctypes.c_int), ("block_elems", ctypes.c_long),
                 ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
                 ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int),
-                ("gram_mfma_flop_per_row", ctypes.c_double), ("gram_xcd_spread", ctypes.c_int)]
+                ("gram_mfma_flop_per_row", ctypes.c_double), ("gram_xcd_spread", ctypes.c_int),
+                ("chains", ctypes.c_long)]
 
 
 class RunStats(ctypes.Structure):
@@ -333,13 +336,13 @@ class Engine:
         return out[:n.value]
 
     def positions(self):
-        out = np.zeros((self.mc.walkers, self.N, 2))
+        out = np.zeros((self.info.chains, self.N, 2))
         _check(lib().le_get_positions(self._h, _dp(out)))
         return out
 
     def set_positions(self, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
-        assert pos.shape == (self.mc.walkers, self.N, 2)
+        assert pos.shape == (self.info.chains, self.N, 2)
         _check(lib().le_set_positions(self._h, _dp(pos)))
 
     def accumulate_positions(self, pos):

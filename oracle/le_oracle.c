@@ -451,11 +451,12 @@ int or_plain_positions(const or_system *sys, int walkers, int burnin, int sample
 /* |g/f|^2; then H(f,g) = <V g/f>/sqrt<|g/f|^2>, M(f,g) = <g/f>/sqrt<|g/f|^2>,*/
 /* then (X + X^H)/2.  Outputs d x d complex row-major (interleaved).          */
 /* Optional Vc/Vs rows (cos/sin(dLexc theta) sums, em_dsf.cu:113-134) when     */
-/* dl_exc > 0.                                                                */
+/* dl_exc > 0; optional final positions of every chain (final_pos).           */
 /* ------------------------------------------------------------------------ */
 int or_rowwise_sector(const or_system *sys, const or_basis *b, int sec, int walkers, int burnin,
                       int samples, double step, uint64_t seed, int threads, int dl_exc,
-                      double *Hout, double *Mout, double *VCout, double *VSout, double *acceptance)
+                      double *Hout, double *Mout, double *VCout, double *VSout, double *acceptance,
+                      double *final_pos)
 {
     int o = b->so[sec], d = b->so[sec + 1] - o;
     double *Hr = calloc(2 * (size_t)d * d, sizeof(double));
@@ -530,6 +531,11 @@ int or_rowwise_sector(const or_system *sys, const or_basis *b, int sec, int walk
                     lS[2 * g] += wS[2 * g] / samples; lS[2 * g + 1] += wS[2 * g + 1] / samples;
                 }
                 free(wH); free(wg); free(wgg); free(wC); free(wS);
+                if (final_pos)   /* chain (row f, walker wk): [f walkers + wk][N][2] */
+                    for (int i = 0; i < sys->N; ++i) {
+                        final_pos[(((size_t)f * walkers + wk) * sys->N + i) * 2] = c.x[i];
+                        final_pos[(((size_t)f * walkers + wk) * sys->N + i) * 2 + 1] = c.y[i];
+                    }
                 free(c.x); free(c.y);
             }
 #pragma omp critical

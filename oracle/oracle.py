@@ -125,17 +125,20 @@ class Oracle:
                                  ctypes.c_uint64(seed), _dp(out), ctypes.byref(acc))
         return out, acc.value
 
-    def rowwise_sector(self, sec, walkers, burnin, samples, step, seed, threads=0, dl_exc=0):
+    def rowwise_sector(self, sec, walkers, burnin, samples, step, seed, threads=0, dl_exc=0, want_pos=False):
         d = self.dims[sec]
         H = np.zeros(2 * d * d); M = np.zeros(2 * d * d)
         VC = np.zeros(2 * d * d); VS = np.zeros(2 * d * d)
         acc = ctypes.c_double()
+        fp = np.zeros((d * walkers, self.N, 2)) if want_pos else None
         lib().or_rowwise_sector(ctypes.byref(self.sys), ctypes.byref(self.cb), ctypes.c_int(sec),
                                 ctypes.c_int(walkers), ctypes.c_int(burnin), ctypes.c_int(samples),
                                 ctypes.c_double(step), ctypes.c_uint64(seed), ctypes.c_int(threads),
                                 ctypes.c_int(dl_exc), _dp(H), _dp(M), _dp(VC), _dp(VS),
-                                ctypes.byref(acc))
+                                ctypes.byref(acc), _dp(fp) if want_pos else None)
         c = lambda a: (a[0::2] + 1j * a[1::2]).reshape(d, d)
+        if want_pos:
+            return c(H), c(M), c(VC), c(VS), acc.value, fp
         return c(H), c(M), c(VC), c(VS), acc.value
 
 

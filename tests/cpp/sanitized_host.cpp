@@ -43,7 +43,7 @@ int or_plain_positions(const or_system* sys, int walkers, int burnin, int sample
                        uint64_t seed, double* out, double* acceptance);
 int or_rowwise_sector(const or_system* sys, const or_basis* b, int sec, int walkers, int burnin, int samples,
                       double step, uint64_t seed, int threads, int dl_exc, double* Hout, double* Mout, double* VCout,
-                      double* VSout, double* acceptance);
+                      double* VSout, double* acceptance, double* final_pos);
 int or_geneig(int d, const double* Sin, const double* Hin, double* evals, double* evecs);
 int or_dsf_weights(int dp1, const double* H, const double* M, const double* VC, const double* VS, double* energies,
                    double* weights);
@@ -128,7 +128,9 @@ int main()
     CHECK(dmax < 1e-9);
     // row-wise estimator of sector dL = 2 (2 rows) with its DSF sums
     std::vector<double> Hr(8), Mr(8), VC(8), VS(8);
-    CHECK(or_rowwise_sector(&sys, &b, 2, 2, 3, 4, 0.8, 11, 1, 2, Hr.data(), Mr.data(), VC.data(), VS.data(), &acc) == 0);
+    std::vector<double> rfin((size_t)2 * 2 * 6 * 2);
+    CHECK(or_rowwise_sector(&sys, &b, 2, 2, 3, 4, 0.8, 11, 1, 2, Hr.data(), Mr.data(), VC.data(), VS.data(), &acc,
+                            rfin.data()) == 0);
     CHECK(std::fabs(Mr[0] - 1.0) < 1e-12 && std::fabs(Mr[6] - 1.0) < 1e-12);
     // eigensolves: sector dL = 4 block, then an indefinite metric (error path)
     const long off4 = 2 * (1 + 1 + 4 + 9);

@@ -1,0 +1,122 @@
+"""The reference's own estimator on the GPU (LE_EST_ROWWISE): for every basis state f, chains on
+|p_f Psi_L|^2 (em_markovchain.cu:185-220: p_f updated by -u^k + u'^k per move) accumulate g/f,
+V g/f and |g/f|^2 over f's sector (em_spectrum.cu:98-115), normalised by sqrt<|g/f|^2> and
+symmetrised (em_spectrum.cu:318-335, 382-390).
+
+- FP64 chains follow the FP64 CPU restatement (or_rowwise_sector) move for move on the same
+  Philox streams (chain c = f W + w has walker id c + W in both): the same final positions for
+  at least 95% of the chains, and M, H within the spread a few diverging chains can cause.
+- FP32 and FP64 chains at N = 3 reproduce the exact normalised metric (unit diagonal,
+  Hermitian) and the exact spectrum of every sector, with several sectors in one engine, to
+  the bounds the CPU restatement meets (|g/f|^2 is heavy-tailed where p_f vanishes, so batch
+  sigma understates the error).
+- At N = 200 (dL = 4, 5: 5 + 7 rows) the row-wise spectra agree with the plain estimator's:
+  the lowest eigenvalue of each sector within 4.5 combined batch sigma, all within 1e-4."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rowwise_fp64_chains_match_oracle(oracle_lib):
+    from laughlinedge_amd import Engine
+    N, m, dl, W = 10, 3, 3, 64
+    print("case rowwise FP64 chains vs oracle N=10 dL=3", flush=True)
+    o = oracle_lib.Oracle(N, m, [dl], w=0.05, eps=1e-4)
+    Ho, Mo, _, _, acc_o, pos_o = o.rowwise_sector(0, W, 2, 4, 0.5, 1234, want_pos=True)
+    with Engine(N, m, [dl], w=0.05, eps=1e-4, walkers=W, precision=1, estimator=1, tune_sweeps=0, step=0.5,
+                seed=1234, samples=4) as e:
+        assert e.info.D == 3
+        e.init_walkers()
+        e.burnin(2)
+        e.sample(4)
+        (M, H), = e.moments()
+        pos = e.positions()
+        acc = e.stats()["acceptance"]
+    assert pos.shape == pos_o.shape == (3 * W, N, 2)
+    same = np.abs(pos - pos_o).max(axis=(1, 2)) < 1e-9
+    print(f"chains identical to the oracle's: {same.mean():.3f}; acceptance {acc:.4f} vs {acc_o:.4f}")
+    assert same.mean() >= 0.95
+    assert np.allclose(np.diag(M).real, 1.0, atol=1e-12)
+    assert np.allclose(M, M.conj().T, atol=1e-14) and np.allclose(H, H.conj().T, atol=1e-12)
+    tol = 1e-9 if same.all() else 0.05
+    assert np.abs(M - Mo).max() < tol, np.abs(M - Mo).max()
+    assert np.abs(H - Ho).max() < tol * np.abs(Ho).max(), np.abs(H - Ho).max()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_rowwise_exact_small_n(oracle_lib, prec):
+    import scipy.linalg
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.stats import batch_sigma
+    from oracle.exact_lll import exact_blocks
+    N, m, sectors = 3, 3, [1, 2, 3]
+    w, eps = 0.05, 2e-3
+    print(f"case rowwise exact N=3 precision={prec}", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=w, eps=eps)
+    with Engine(N, m, sectors, w=w, eps=eps, walkers=2048, precision=prec, estimator=1, tune_sweeps=40, step=0.9,
+                seed=31, samples=8 * 150, n_blocks=8) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(200)
+        for _ in range(8):
+            e.sample(150)
+        mean = e.moments()
+        blocks = [e.moments(b) for b in range(8)]
+        spec = [e.spectra(b) for b in range(8)]
+        spec_mean = e.spectra()
+    off = 0
+    for s, dl in enumerate(sectors):
+        d = o.dims[s]
+        states = o.basis.states[off:off + d]
+        off += d
+        Se, He = (np.array(x) for x in exact_blocks(N, m, states, w, eps))
+        dn = np.sqrt(np.diag(Se).real)
+        Mx = Se / np.outer(dn, dn)
+        Hx = He / np.outer(dn, dn)
+        M, H = mean[s]
+        assert np.allclose(np.diag(M).real, 1.0, atol=1e-12)
+        bM = np.array([b[s][0] for b in blocks])
+        bH = np.array([b[s][1] for b in blocks])
+        sM = np.hypot(batch_sigma(bM.real), batch_sigma(bM.imag)) + 1e-12
+        sH = np.hypot(batch_sigma(bH.real), batch_sigma(bH.imag)) + 1e-12
+        dM = np.abs(M - Mx) / np.abs(sM)
+        dH = np.abs(H - Hx) / np.abs(sH)
+        ex = scipy.linalg.eigh(He, Se, eigvals_only=True)
+        es = batch_sigma(np.array([sp[s] for sp in spec]))
+        de = np.abs(spec_mean[s] - ex) / es
+        print(f"dL={dl}: max |M - exact| {np.abs(M - Mx).max():.4f} ({dM.max():.2f} batch sigma), "
+              f"|H - exact| {np.abs(H - Hx).max():.5f} ({dH.max():.2f}), eigenvalues {de.max():.2f} sigma")
+        # |g/f|^2 is heavy-tailed where p_f vanishes, so batch sigma is optimistic: absolute
+        # bounds as for the CPU restatement of the same estimator
+        assert np.abs(M - Mx).max() < 0.03 and np.abs(H - Hx).max() < 0.03 * np.abs(Hx).max(), dl
+        assert np.allclose(spec_mean[s], ex, rtol=0.01), (dl, spec_mean[s], ex)
+
+
+def test_rowwise_vs_plain_n200():
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.stats import batch_sigma
+    N, m, sectors = 200, 3, [4, 5]
+    print("case rowwise vs plain N=200 dL=4,5", flush=True)
+    res = {}
+    for est, walkers in ((1, 2048), (0, 32768)):
+        with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=walkers, estimator=est, tune_sweeps=30, step=0.5,
+                    seed=42, samples=8 * 16, n_blocks=8) as e:
+            e.init_walkers()
+            e.tune_step()
+            e.burnin(200)
+            for _ in range(8):
+                e.sample(16)
+            res[est] = [e.spectra(b) for b in range(8)]
+            if est == 1:
+                assert e.info.D == 12 and e.stats()["samples"] == 8 * 16 * 12 * walkers
+    for s, dl in enumerate(sectors):
+        a = np.array([b[s] for b in res[1]])
+        c = np.array([b[s] for b in res[0]])
+        sig = np.sqrt(batch_sigma(a) ** 2 + batch_sigma(c) ** 2)
+        dev = np.abs(a.mean(0) - c.mean(0)) / sig
+        rel = np.abs(a.mean(0) - c.mean(0)) / c.mean(0)
+        print(f"dL={dl}: lowest row-wise {a.mean(0)[0]:.4f} plain {c.mean(0)[0]:.4f} ({dev[0]:.2f} sigma); "
+              f"all: max {dev.max():.2f} sigma, max relative difference {rel.max():.1e}")
+        assert dev[0] < 4.5, (dl, dev)
+        assert rel.max() < 1e-4, (dl, rel)

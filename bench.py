@@ -546,6 +546,27 @@ def rows():
         wall, ms_gram = _timed(e, 4, steps)
         out["dsf_N200_dL16"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
                                 "D": e.info.D, "gram_ms_per_step": ms_gram / steps}
+    # the reference's row-wise estimator on the same workload
+    log("rows: row-wise estimator N=200 dL=1..16")
+    steps, W = 16, 36
+    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=W, samples=steps + 4, seed=SEED, tune_sweeps=30,
+                step=0.5, estimator=1) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(100)
+        e.sample(4)
+        e.reset_moments()
+        t0 = time.perf_counter()
+        e.sample(steps)
+        wall = time.perf_counter() - t0
+        chains = e.info.chains
+        out["rowwise_N200_dL1-16"] = {
+            "chains": chains, "walkers_per_row": W, "ms_per_step": 1e3 * wall / steps,
+            "row_samples_per_s": chains * steps / wall,
+            "sample_updates_per_s": chains * steps / wall / e.info.D,
+            "acceptance": e.stats()["acceptance"],
+            "note": "one step = one sweep of every chain + its monomials + the per-row g/f sums; a full "
+                    "S,H sample-update needs one sample of every one of the D rows"}
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
     steps = 40

@@ -63,6 +63,9 @@ public:
     unsigned long long seed = 0;      // 0: seeded from the clock, like the reference
     int precision = LE_PREC_FP32;
     int statisticsBlocks = 1;
+    // LE_EST_PLAIN: one |Psi_L|^2 population for every row; LE_EST_ROWWISE: the reference's
+    // estimator, gridSize chains on |p_f Psi_L|^2 for every row f (em_spectrum.cu:16-137)
+    int estimator = LE_EST_PLAIN;
 
     monteCarloParameters()
     {
@@ -156,6 +159,7 @@ inline le_mc_params mc(const monteCarloParameters& p)
         m.seed = (unsigned long long)std::rand();
     }
     m.precision = p.precision;
+    m.estimator = p.estimator;
     m.n_blocks = p.statisticsBlocks > 0 ? p.statisticsBlocks : 1;
     return m;
 }

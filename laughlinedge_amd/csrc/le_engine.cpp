@@ -1622,7 +1622,18 @@ extern "C" int le_get_moments(const le_engine* e, int block, double* S, double* 
 
 extern "C" int le_get_block(const le_engine* e, int block, int weight, int s1, int s2, double* out)
 {
-    if (e && e->rowwise) return fail(LE_ERR_ARG, "the row-wise estimator keeps only M and H (le_get_moments)");
+    if (e && e->rowwise) {
+        // the row-wise estimator keeps the normalised M (weight 1) and H (weight V) of each sector
+        if (!out || s1 != s2 || s1 < 0 || s1 >= e->basis.nsec() || (weight != LE_W_ONE && weight != LE_W_V))
+            return fail(LE_ERR_ARG, "the row-wise estimator keeps only each sector's M and H blocks");
+        const long be = e->basis.block_elems();
+        std::vector<double> S(2 * be), H(2 * be);
+        const int rc = rowwise_moments(e, block, S.data(), H.data());
+        if (rc) return rc;
+        const long off = 2 * e->basis.block_offset(s1), n = 2L * e->basis.dim(s1) * e->basis.dim(s1);
+        std::memcpy(out, (weight == LE_W_ONE ? S.data() : H.data()) + off, sizeof(double) * n);
+        return LE_OK;
+    }
     if (!e || !out) return fail(LE_ERR_ARG, "null argument");
     const int ns = e->basis.nsec();
     if (s1 < 0 || s2 < 0 || s1 >= ns || s2 >= ns) return fail(LE_ERR_ARG, "sector out of range");

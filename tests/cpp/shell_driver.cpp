@@ -51,6 +51,17 @@ int main()
     greenFunctionMC::occupation_numbers_compute(gp, 2, M2, 5);
     greenFunctionMC::occupation_numbers_compute(gp, 2, 5);
     greenFunctionMC::spectral_function_compute(gp, 3, 6);
+    // the reference's own estimator: 64 chains on |p_f Psi_L|^2 for each of the 5 rows of dL = 4
+    edgeMC::monteCarloParameters pr;
+    pr.setGPU_threads_blocks(64, 1);
+    pr.setMCSamples(400);
+    pr.setBurnIn(200);
+    pr.seed = 4321;
+    pr.estimator = LE_EST_ROWWISE;
+    std::vector<double> evr = edgeMC::spectrum_compute(pr, 4, false, 0);
+    std::printf("ROWWISE");
+    for (double e : evr) std::printf(" %.12e", e);
+    std::printf("\nROWSTEP %.9e\n", pr.randomStep);
     std::printf("DONE\n");
     return 0;
 }

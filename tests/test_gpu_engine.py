@@ -374,6 +374,17 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
     assert np.allclose(oracle_lib.geneig(M, H), ev, rtol=1e-4)
     L0 = 3 * 10 * 9 // 2
     assert np.all(np.abs(ev / (0.05 * (L0 + 4 + 10)) - 1) < 0.5)
+    # the row-wise estimator through the shell: the Python path with the same parameters
+    evr = np.array([float(x) for x in lines["ROWWISE"]])
+    with Engine(10, 3, [4], w=0.05, eps=1e-4, walkers=64, samples=400, burnin_sweeps=200, seed=4321,
+                tune_sweeps=100, step=float(np.float32(0.8)), estimator=1) as e:
+        e.run()
+        refr = e.spectra()[0]
+        M_r, _ = e.moments()[0]
+        assert abs(e.stats()["step"] - float(lines["ROWSTEP"][0])) < 1e-6 * e.stats()["step"]
+    assert np.allclose(np.diag(M_r).real, 1.0, atol=1e-12)
+    assert np.allclose(evr, refr, rtol=1e-9)
+    assert np.allclose(evr, ev, rtol=0.02)     # same sector, other estimator and chains
     # DSF: the shell's weights vs the oracle's rotation of the same blocks
     dsf = np.array([float(x) for x in lines["DSF"]]).reshape(-1, 2)
     with Engine(10, 3, [0, 3], w=0.05, eps=1e-4, walkers=512, samples=400, burnin_sweeps=200, seed=1234,

@@ -421,6 +421,10 @@ def compact(out, full_path):
         c["forms"] = cf
     if "rows" in out:
         c["rows"] = {k: {kk: _r(vv) for kk, vv in v.items() if isinstance(vv, (int, float))} for k, v in out["rows"].items()}
+        rw = out["rows"].get("rowwise_N200_dL1-16", {}).get("lowest_eigenvalues")
+        if rw:
+            c["rows"]["rowwise_N200_dL1-16"]["lowest_dL16"] = rw["lowest"]["16"]
+            c["rows"]["rowwise_N200_dL1-16"]["max_seed_deviation_sigma"] = rw["max_seed_deviation_sigma"]
     if "cpu_baseline" in out:
         cb = out["cpu_baseline"]
         c["cpu_baseline"] = {k: _r(v, 5) for k, v in cb.items() if k != "legs"}
@@ -567,6 +571,7 @@ def rows():
             "acceptance": e.stats()["acceptance"],
             "note": "one step = one sweep of every chain + its monomials + the per-row g/f sums; a full "
                     "S,H sample-update needs one sample of every one of the D rows"}
+    out["rowwise_N200_dL1-16"]["lowest_eigenvalues"] = rowwise_spectra()
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
     steps = 40
@@ -581,6 +586,32 @@ def rows():
                                       "orbitals": lcut, "gram_ms_per_step": ms_gram / steps,
                                       "lanes_per_walker": e.info.lanes_per_walker, "acceptance": e.stats()["acceptance"]}
     return out
+
+
+def rowwise_spectra(seeds=(1, 2), steps=512):
+    """Lowest eigenvalue of every sector dL = 1..16 (N = 200) from the row-wise estimator, two
+    seeds, 36 chains per row, `steps` samples per chain in 8 blocks: the values, their batch
+    sigma and the seed-to-seed deviation in combined sigma."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.stats import batch_sigma
+    runs = []
+    for seed in seeds:
+        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=36, estimator=1, tune_sweeps=30, step=0.5,
+                    seed=seed, samples=steps, n_blocks=8, target_acceptance=0.3) as e:
+            e.init_walkers()
+            e.tune_step()
+            e.burnin(300)
+            for _ in range(8):
+                e.sample(steps // 8)
+            sp = [e.spectra(b) for b in range(8)]
+            mean = e.spectra()
+        runs.append({str(dl): (float(mean[s][0]), float(batch_sigma(np.array([b[s][0] for b in sp]))))
+                     for s, dl in enumerate(SECTORS)})
+    dev = {k: abs(runs[0][k][0] - runs[1][k][0]) / float(np.hypot(runs[0][k][1], runs[1][k][1])) for k in runs[0]}
+    return {"seeds": list(seeds), "samples_per_chain": steps, "chains_per_row": 36,
+            "lowest": {k: [round(runs[0][k][0], 4), round(runs[0][k][1], 4)] for k in runs[0]},
+            "seed_deviation_sigma": {k: round(v, 2) for k, v in dev.items()},
+            "max_seed_deviation_sigma": round(max(dev.values()), 2)}
 
 
 def forms(args, thins):

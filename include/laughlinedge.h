@@ -105,7 +105,8 @@ typedef struct le_mc_params {
                                   only); 0 (default): single-particle moves only, as the reference */
     int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (edge
                                   engine: lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25,
-                                  plain proposal; occupation engine: ceil((N - 1) / lanes) 13 or 25);
+                                  plain or learned proposal; occupation engine: ceil((N - 1) / lanes)
+                                  13 or 25);
                                   0 (default): LDS.  lanes_per_walker = -1 times both kinds and
                                   keeps the faster */
     int estimator;             /* LE_EST_PLAIN (default): one population of walkers samples

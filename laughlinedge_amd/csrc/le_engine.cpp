@@ -837,7 +837,10 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
             lt.hi[q] = table_hi[q];
         }
         lt.invR = 1.0 / R;
-        HIPCHK(le::launch_sweep_learned(a, lt, std::max(map_lanes(G), 8), stream));
+        if (map_reg(G) && le::learned_reg_supported(map_lanes(G), sys.particles_number))
+            HIPCHK(le::launch_sweep_learned_reg(a, lt, map_lanes(G), stream));
+        else
+            HIPCHK(le::launch_sweep_learned(a, lt, std::max(map_lanes(G), 8), stream));
     } else if (map_reg(G)) {
         HIPCHK(le::launch_sweep_reg(a, prec, map_lanes(G), stream));
     } else {

@@ -591,6 +591,142 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
     if (lane == 0) a.accepted[walker] += accepted;
 }
 
+// The learned sweep with each lane's particles in registers (as k_sweep_reg: lane l holds
+// j = l + G s in px[s], py[s], S = ceil(N / G) fixed at compile time, moves in the order
+// i = G s + t with the mover's old position and random numbers taken from lane t by __shfl).
+// Pass 1 inserts every slot but the mover's own and the pads (the LDS kernel parks a far point
+// there, which inserts nothing); pass 2 holds the midpoint in the mover's slot and the pads,
+// skips them in the nearest-distance lists, and takes the products and logs in the same order
+// as k_sweep_learned, so for the same G the chain is the same.
+template <int G, int S>
+__global__ __launch_bounds__(64) void k_sweep_learned_reg(SweepArgs a, Table3 tb, float invR)
+{
+    constexpr int WPB = 64 / G;
+    constexpr int FULL8 = (S / 8) * 8;
+    const int N = a.N;
+    const int lane = threadIdx.x % G;
+    const int grp = threadIdx.x / G;
+    const int walker = blockIdx.x * WPB + grp;
+    if (walker >= a.walkers) return;    // whole groups leave; shuffles stay within a group
+    float2* gpos = reinterpret_cast<float2*>(a.pos) + (size_t)walker * N;
+    float px[S], py[S];
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        const float2 q = j < N ? gpos[j] : gpos[0];
+        px[s] = q.x;
+        py[s] = q.y;
+    });
+    const bool pad_last = lane + G * (S - 1) >= N;
+    const float mfac = (float)a.m * 0.69314718055994531f;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    constexpr float kCap2 = kDcap * kDcap;
+    unsigned accepted = 0;
+    for (int sw = 0; sw < a.nsweeps; ++sw) {
+        const unsigned long long sidx = sweep_first + (unsigned long long)sw;
+        const uint32_t c1 = (uint32_t)sidx;
+        const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+        static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            const int base = G * s;
+            uint32_t ctr = (uint32_t)(base + lane);
+            asm volatile("" : "+v"(ctr));
+            uint32_t r[4];
+            philox(ctr, c1, c2, (uint32_t)walker, k0, k1, r);
+            float g1, g2;
+            gauss2(r[0], r[1], g1, g2);
+            const uint32_t ubl = r[2];
+            const int cnt = min(G, N - base);
+            for (int t = 0; t < cnt; ++t) {
+                const float zx = __shfl(px[s], t, G), zy = __shfl(py[s], t, G);
+                const float n1 = __shfl(g1, t, G), n2 = __shfl(g2, t, G);
+                const uint32_t ub = (uint32_t)__shfl((int)ubl, t, G);
+                const bool own = lane == t;
+                // pass 1: nearest distances at z
+                float near[8];
+                near8_init(near);
+                static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    const float dx = zx - px[u], dy = zy - py[u];
+                    bool skip = own && u == s;
+                    if constexpr (u == S - 1) skip = skip || pad_last;
+                    near8_insert(near, skip ? kCap2 : dx * dx + dy * dy);
+                });
+                const float r0 = sqrtf(zx * zx + zy * zy);
+                float f[3];
+                near_features<G>(near, invR, r0, f);
+                float lsr, lst;
+                widths_lookup(tb, f, lsr, lst);
+                const float sr = __expf(lsr), st = __expf(lst);
+                const float cth = r0 > 0.f ? zx / r0 : 1.f, sth = r0 > 0.f ? zy / r0 : 0.f;
+                const float dr = sr * n1, dt = st * n2;
+                const float xn = zx + cth * dr - sth * dt, yn = zy + sth * dr + cth * dt;
+                // pass 2: ratio (midpoint in the mover's slot and the pads) and nearest at z'
+                float2 mid;
+                mid.x = 0.5f * (zx + xn);
+                mid.y = 0.5f * (zy + yn);
+                if (own) {
+                    px[s] = mid.x;
+                    py[s] = mid.y;
+                }
+                float lsum = 0.f, pn = 1.f, pd = 1.f;
+                near8_init(near);
+                static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
+                    constexpr int u = decltype(uc)::value;
+                    float qx = px[u], qy = py[u];
+                    bool skip = own && u == s;
+                    if constexpr (u == S - 1) {
+                        if (pad_last) {
+                            qx = mid.x;
+                            qy = mid.y;
+                            skip = true;
+                        }
+                    }
+                    const float ax = xn - qx, ay = yn - qy;
+                    const float bx = zx - qx, by = zy - qy;
+                    const float A2 = ax * ax + ay * ay;
+                    pn *= A2;
+                    pd *= bx * bx + by * by;
+                    near8_insert(near, skip ? kCap2 : A2);
+                    if constexpr ((u + 1) % 8 == 0 && u + 1 <= FULL8) {
+                        lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
+                        pn = 1.f;
+                        pd = 1.f;
+                    }
+                });
+                lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
+                lsum = group_sum<G, float>(lsum);
+                const float rn_ = sqrtf(xn * xn + yn * yn);
+                float fp[3];
+                near_features<G>(near, invR, rn_, fp);
+                float lsrp, lstp;
+                widths_lookup(tb, fp, lsrp, lstp);
+                const float srp = __expf(lsrp), stp = __expf(lstp);
+                const float cp = rn_ > 0.f ? xn / rn_ : 1.f, sp = rn_ > 0.f ? yn / rn_ : 0.f;
+                const float wx = zx - xn, wy = zy - yn;
+                const float Dr = wx * cp + wy * sp, Dt = -wx * sp + wy * cp;
+                const float lq = -(lsrp + lstp) - 0.5f * ((Dr / srp) * (Dr / srp) + (Dt / stp) * (Dt / stp)) +
+                                 (lsr + lst) + 0.5f * (n1 * n1 + n2 * n2);
+                const float logA = mfac * lsum - 0.5f * ((xn * xn + yn * yn) - (zx * zx + zy * zy)) + lq;
+                const bool acc =
+                    (logA >= 0.f) || (__builtin_amdgcn_logf(u01f(ub)) * 0.69314718055994531f < logA);
+                if (own) {
+                    px[s] = acc ? xn : zx;
+                    py[s] = acc ? yn : zy;
+                }
+                accepted += acc ? 1u : 0u;
+            }
+        });
+    }
+    static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int j = lane + G * s;
+        if (j < N) gpos[j] = make_float2(px[s], py[s]);
+    });
+    if (lane == 0) a.accepted[walker] += accepted;
+}
+
 // ---------------------------------------------------------------------------
 hipError_t launch_mlp_init(float* P, unsigned long long seed, hipStream_t s)
 {
@@ -658,6 +794,40 @@ static hipError_t sweep_learned_g(const SweepArgs& a, const LearnedTable& lt, hi
     }
     hipLaunchKernelGGL((k_sweep_learned<G>), grid, dim3(64), lds, s, a, tb, (float)lt.invR);
     return hipGetLastError();
+}
+
+bool learned_reg_supported(int G, int N)
+{
+    const int S = (N + G - 1) / G;
+    return (G == 8 || G == 16) && (S == 13 || S == 25);
+}
+
+template <int G, int S>
+static hipError_t sweep_learned_reg_gs(const SweepArgs& a, const Table3& tb, float invR, hipStream_t s)
+{
+    constexpr int WPB = 64 / G;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    hipLaunchKernelGGL((k_sweep_learned_reg<G, S>), grid, dim3(64), 0, s, a, tb, invR);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_learned_reg(const SweepArgs& a, const LearnedTable& lt, int G, hipStream_t s)
+{
+    if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    Table3 tb;
+    tb.t = lt.table;
+    tb.T = lt.T;
+    for (int q = 0; q < 3; ++q) {
+        tb.lo[q] = lt.lo[q];
+        tb.inv_h[q] = (float)(lt.T - 1) / (lt.hi[q] - lt.lo[q]);
+    }
+    const int S = (a.N + G - 1) / G;
+    const float invR = (float)lt.invR;
+    if (G == 8 && S == 13) return sweep_learned_reg_gs<8, 13>(a, tb, invR, s);
+    if (G == 8 && S == 25) return sweep_learned_reg_gs<8, 25>(a, tb, invR, s);
+    if (G == 16 && S == 13) return sweep_learned_reg_gs<16, 13>(a, tb, invR, s);
+    if (G == 16 && S == 25) return sweep_learned_reg_gs<16, 25>(a, tb, invR, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_sweep_learned(const SweepArgs& a, const LearnedTable& lt, int G, hipStream_t s)

@@ -26,6 +26,9 @@ hipError_t launch_adam(float* P, float* M1, float* M2, const float* G, float lr,
 hipError_t launch_mlp_table(const float* P, int T, const float* lo, const float* hi, float sigma0, float* table,
                             hipStream_t s);
 hipError_t launch_sweep_learned(const SweepArgs& a, const LearnedTable& lt, int G, hipStream_t s);
+// the same sweeps with each lane's particles in registers: G in {8, 16}, ceil(N / G) in {13, 25}
+bool learned_reg_supported(int G, int N);
+hipError_t launch_sweep_learned_reg(const SweepArgs& a, const LearnedTable& lt, int G, hipStream_t s);
 size_t learned_lds_bytes(int G, int N);
 
 }  // namespace le

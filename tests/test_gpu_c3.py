@@ -8,7 +8,8 @@ mean distance to the 8 nearest neighbours) initialised from seed 99 and trained 
   batch sigma.
 - What the network buys: tau_int of Re/Im p_1..p_4 (Madras-Sokal, per kept sample at thin 1)
   is shorter than the plain chain's at the same 50% target the widths are scaled from; both
-  tau and ESS/s at equal wall time are printed."""
+  tau and ESS/s at equal wall time are printed.  Both chains run their register-resident
+  sweeps (8 lanes per walker)."""
 import time
 
 import numpy as np
@@ -26,7 +27,8 @@ def _run(kind):
     from laughlinedge_amd import Engine
     from laughlinedge_amd.stats import tau_int
     with Engine(N, M_INV, SECTORS, w=0.05, eps=5e-5, walkers=WALKERS, seed=SEED, samples=BLOCKS * PER_BLOCK + 8,
-                n_blocks=BLOCKS, tune_sweeps=50, step=0.5, trace_walkers=512, lanes_per_walker=8) as e:
+                n_blocks=BLOCKS, tune_sweeps=50, step=0.5, trace_walkers=512, lanes_per_walker=8,
+                sweep_registers=1) as e:
         e.init_walkers()
         e.tune_step()
         e.burnin(300)

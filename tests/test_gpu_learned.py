@@ -108,3 +108,34 @@ def test_training_gain_esjd():
             esjd[kind] = float(np.mean(jumps))
     print(f"ESJD per particle-sweep: plain {esjd['plain']:.3f} learned {esjd['learned']:.3f}")
     assert esjd["learned"] > 1.15 * esjd["plain"], esjd
+
+
+@pytest.mark.parametrize("N,G", [(200, 8), (100, 8), (200, 16)])
+def test_learned_register_sweep_matches_lds(N, G):
+    """The register-resident learned sweep follows the LDS one: the same trained widths table,
+    the same walkers and random streams give the same positions after several sweeps (the
+    products, logs and nearest-distance lists are taken in the same order)."""
+    from laughlinedge_amd import Engine
+    from laughlinedge_amd.engine import PROPOSAL_LEARNED
+    print(f"case learned register vs LDS sweep N={N} G={G}", flush=True)
+    with Engine(N, 3, [0], w=0.05, eps=5e-5, walkers=1024, samples=1, seed=11, tune_sweeps=20, step=0.5,
+                lanes_per_walker=G) as e:
+        e.init_walkers()
+        step = e.tune_step()
+        e.burnin(50)
+        e.train_proposal(steps=100, batch=2048, n_configs=8192, snapshot_sweeps=2, seed=99)
+        params = e.proposal_params()
+        start = e.positions()
+    out = []
+    for reg in (0, 1):
+        with Engine(N, 3, [0], w=0.05, eps=5e-5, walkers=1024, samples=1, seed=23, tune_sweeps=0, step=step,
+                    lanes_per_walker=G, sweep_registers=reg) as e:
+            assert e.info.sweep_in_registers == reg
+            e.set_proposal_params(params, step)
+            e.set_proposal(PROPOSAL_LEARNED)
+            e.set_positions(start)
+            e.burnin(4)
+            out.append(e.positions())
+    same = np.abs(out[0] - out[1]).max(axis=(1, 2)) == 0
+    print(f"walkers with identical positions after 4 sweeps: {same.mean():.4f}")
+    assert same.mean() >= 0.99, same.mean()

@@ -225,7 +225,7 @@ int le_tune_step(le_engine* e, double* step_out);
  * measurement (no GPU call of its own): measure(ctx, step, &acceptance) returns LE_OK or an
  * error code that the search returns unchanged.  From step0: keep it when |a - target| < tol;
  * else double (a > target) or halve the step until the target is bracketed (at most 41
- * measurements, else LE_ERR_BISECTION); then interpolate linearly between the bracket ends,
+ * doublings or halvings, else LE_ERR_BISECTION); then interpolate linearly between the bracket ends,
  * replacing the end on the guess's side (LE_ERR_BISECTION when a guess falls outside the
  * bracket, where the reference exits, or after 60 guesses).  *n_measured (optional) counts
  * the measurements. */

@@ -64,7 +64,9 @@ public:
     int precision = LE_PREC_FP32;
     int statisticsBlocks = 1;
     // LE_EST_PLAIN: one |Psi_L|^2 population for every row; LE_EST_ROWWISE: the reference's
-    // estimator, gridSize chains on |p_f Psi_L|^2 for every row f (em_spectrum.cu:16-137)
+    // estimator, gridSize chains on |p_f Psi_L|^2 for every row f (em_spectrum.cu:16-137);
+    // LE_EST_MIXTURE: one w |Psi_L|^2 population, S and H reweighted by 1 / w (bounded
+    // integrands in every sector; spectrum and metric only)
     int estimator = LE_EST_PLAIN;
 
     monteCarloParameters()

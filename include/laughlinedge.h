@@ -20,6 +20,8 @@
  *   le_burnin, le_sample, le_run
  *       the mcmc_hamiltonian launch and its host averaging
  *       (modules/edge_mc/em_spectrum.cu:139-208, 256-359), for all sectors at once.
+ *   le_set_mixture, le_get_mixture
+ *       no reference counterpart: the importance density of the mixture estimator.
  *   le_get_moments
  *       the D2H copy + block means of fHg/fg/gg (modules/edge_mc/em_spectrum.cu:297-335).
  *   le_geneig, le_engine_spectrum
@@ -59,6 +61,8 @@ extern "C" {
 
 #define LE_EST_PLAIN 0      /* estimator: sample |Psi_L|^2 once, S and H for all rows (Gram on MFMA) */
 #define LE_EST_ROWWISE 1    /* estimator: the reference's row-wise |p_f Psi_L|^2 chains */
+#define LE_EST_MIXTURE 2    /* estimator: one population samples w |Psi_L|^2 (le_set_mixture),
+                               S and H reweighted by 1 / w (Gram on MFMA) */
 
 #define LE_PREC_FP32 0      /* FP32 sampler, FP32 MFMA tiles flushed to FP64 */
 #define LE_PREC_FP64 1      /* FP64 sampler, FP64 MFMA accumulation (the plain FP64 form) */
@@ -120,13 +124,24 @@ typedef struct le_mc_params {
                                   Row-wise engines run the LDS sweep (no registers, symmetry moves,
                                   learned proposal, extra observables or fixed samples); all rows
                                   share one step; chain c = f walkers + w uses Philox walker id
-                                  c + walkers */
+                                  c + walkers;
+                                  LE_EST_MIXTURE: one population of walkers samples w |Psi_L|^2
+                                  with w = rho + sum_{k=1..L} c_k (|p_k|^2 / n_k)^(L/k), L the
+                                  largest sector (le_set_mixture); S and H are the sums of
+                                  conj(p_a) p_b / w and conj(p_a) V p_b / w over the sums of 1 / w.
+                                  By weighted AM-GM every |p_lambda|^2 of every sector is at most
+                                  a constant times w, so the reweighted integrands are bounded
+                                  (the plain estimator's |p_1|^(2 dL) tails are not).  S and H
+                                  only (no harmonics, user weights, symmetry moves or learned
+                                  proposal); register or LDS sweeps */
     int gram_xcd_spread;       /* XCDs that share one row split's S/H Gram tiles: 1, 2, 4 or 8
                                   (8: consecutive workgroups take consecutive tiles, spreading every
                                   split over all XCDs, whose L2s then each fetch most of every
                                   sample row; fewer XCDs per split fetch each row fewer times but
                                   load each L2 more); 0 (default): time 1, 2, 4 and 8 on the first
                                   full-size launch and keep the fastest (cached per sector layout) */
+    double mixture_rho;        /* LE_EST_MIXTURE: weight c0 of the constant term of w (0 < rho < 1,
+                                  default 0.5; see le_set_mixture) */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -232,7 +247,20 @@ int le_tune_step(le_engine* e, double* step_out);
 typedef int (*le_accept_fn)(void* ctx, double step, double* acceptance);
 int le_step_search(le_accept_fn measure, void* ctx, double step0, double target, double tol, double* step_out,
                    int* n_measured);
+/* Burn-in.  A mixture engine (LE_EST_MIXTURE) whose mixture is not set yet runs the first
+ * half of these sweeps on |Psi_L|^2, then sets it from the walkers (n_k = mean |p_k|^2 over
+ * the walkers, le_set_mixture with mixture_rho) and runs the second half on w |Psi_L|^2. */
 int le_burnin(le_engine* e, int sweeps);
+/* Mixture density of a LE_EST_MIXTURE engine (no reference counterpart):
+ *   w = rho + sum_{k=1..L} exp(a_k ln|p_k|^2 + b_k),  a_k = L / k,
+ *   b_k = ln((1 - rho) / L) - lnGamma(a_k + 1) - a_k ln n_k,
+ * with L the largest sector dL and p_k the power sums of z / sqrt(2 m N); each term has mean
+ * (1 - rho) / L when |p_k|^2 / n_k is a unit exponential.  nk[0..L-1] > 0, 0 < rho < 1.
+ * Sets the density for every later sweep (le_sample calls it from the walkers when it was not
+ * set); the walkers are kept. */
+int le_set_mixture(le_engine* e, const double* nk, int n_k, double rho);
+/* The density in use: L (0 when not set), c0 = rho, a[0..L-1], b[0..L-1] (buffers of >= L). */
+int le_get_mixture(const le_engine* e, int* L, double* c0, double* a, double* b);
 /* sample `samples` kept samples per walker into the current block; closes the block */
 int le_sample(le_engine* e, int samples);
 /* init + tune (if tune_sweeps > 0) + burn-in + n_blocks x (samples / n_blocks) */

@@ -59,6 +59,7 @@ hipError_t dalloc(T** p, size_t count)
 // slots per lane (FP32), 32 for FP64, whose sweeps are the slower ones and prefer longer
 // per-lane loops (N = 200 on MI355X: 0.52 ms FP32 at 16 lanes, 1.54 ms FP64 at 8 lanes).
 constexpr int kPsLd = 50;        // Re/Im p_1..p_24 per sample (+ the two-sided chain's phase)
+constexpr int kMixChunk = 8;          // mixture chains: sweeps per launch (FP32 running p_k drift)
 
 // Sweep mapping: lanes per walker in the low byte; kRegSweep marks the register-resident
 // kernel (k_sweep_reg) instead of the LDS one.
@@ -169,6 +170,17 @@ struct le_engine {
     long* sec_bo_d = nullptr;        // [nsec]
     int sample_rowwise(int samples);
     hipError_t init_positions();
+    // mixture estimator: walkers sample w |Psi_L|^2 (le::MixParams; kmix = 0 until set)
+    bool mixture = false;
+    le::MixParams mixp{};
+    double2* mix_pk = nullptr;       // [walkers][kMixMax] p_k at the start of each sweep launch
+    double* mix_acc = nullptr;       // [wrows] per-row sums of 1 / w in the open block
+    double* mix_acc_h = nullptr;     // pinned staging of mix_acc
+    long mix_row = 0;                // weight row of 1 / w (row 0 holds V / w)
+    int mix_L = 0;                   // largest sector dL: the terms k = 1..L
+    int set_mixture(const double* nk, int n, double rho);
+    int calibrate_mixture();
+    void mix_features(le::FeatureArgs& f) const;
     int gram_spread = 8;             // XCDs sharing one row split's Gram tiles (1, 2, 4 or 8)
     bool need_spread_tune = false;   // gram_xcd_spread = 0 and no choice cached for this layout yet
     long long block_grams = 0;       // Gram launches into the open block (the tuning needs none yet)
@@ -193,6 +205,7 @@ struct le_engine {
     // host state: one record (all items) per closed block
     std::vector<std::vector<double>> rec;
     std::vector<long long> blk_samples;
+    std::vector<double> blk_norm;    // per block: the samples, or the sum of 1 / w (mixture)
     bool walkers_ready = false;
     unsigned long long sweep_counter = 0;
     long long sweeps_done = 0;
@@ -306,6 +319,9 @@ le_engine::~le_engine()
     (void)hipFree(table);
     (void)hipFree(trace_d);
     (void)hipFree(eig_work);
+    (void)hipFree(mix_pk);
+    (void)hipFree(mix_acc);
+    if (mix_acc_h) (void)hipHostFree(mix_acc_h);
     for (auto e : ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -333,6 +349,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->sweep_registers = 0;
     p->estimator = LE_EST_PLAIN;
     p->gram_xcd_spread = 0;
+    p->mixture_rho = 0.5;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -342,7 +359,7 @@ const void* le_engine::weight_ptr(int wid, int buf) const
     const size_t rsz_ = rsz;
     const void* base = Wbb[buf];
     auto row = [&](long r) { return (const void*)((const char*)base + (size_t)r * wrows * rsz_); };
-    if (wid == LE_W_ONE) return nullptr;
+    if (wid == LE_W_ONE) return mixture ? row(mix_row) : nullptr;
     if (wid == LE_W_V) return row(0);
     if (wid >= LE_W_USER(0)) {
         const int k = wid - LE_W_USER(0);
@@ -528,11 +545,18 @@ int le_engine::setup(const le_observables* obs)
     }
     HIPCHK(dalloc(&acc, (size_t)mc.walkers));
     HIPCHK(dalloc(&ps_d, (size_t)rows * kPsLd));
+    if (mixture) {
+        mix_row = 1 + 2 * nh + n_user;
+        HIPCHK(dalloc(&mix_pk, (size_t)mc.walkers * le::kMixMax));
+        HIPCHK(dalloc(&mix_acc, (size_t)rows));
+        HIPCHK(le::launch_zero(mix_acc, rows, stream));
+        HIPCHK(hipHostMalloc((void**)&mix_acc_h, sizeof(double) * (size_t)rows, hipHostMallocDefault));
+    }
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMalloc(&Xb[b], (size_t)rows * ldx * rsz));
         HIPCHK(hipMemsetAsync(Xb[b], 0, (size_t)rows * ldx * rsz, stream));
-        HIPCHK(hipMalloc(&Wbb[b], (size_t)(1 + 2 * nh + n_user) * rows * rsz));
-        HIPCHK(hipMemsetAsync(Wbb[b], 0, (size_t)(1 + 2 * nh + n_user) * rows * rsz, stream));
+        HIPCHK(hipMalloc(&Wbb[b], (size_t)(1 + 2 * nh + n_user + (mixture ? 1 : 0)) * rows * rsz));
+        HIPCHK(hipMemsetAsync(Wbb[b], 0, (size_t)(1 + 2 * nh + n_user + (mixture ? 1 : 0)) * rows * rsz, stream));
         HIPCHK(hipEventCreateWithFlags(&filled[b], hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&drained[b], hipEventDisableTiming));
     }
@@ -670,8 +694,16 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
-    if (mc->estimator != LE_EST_PLAIN && mc->estimator != LE_EST_ROWWISE)
-        return fail(LE_ERR_ARG, "estimator must be LE_EST_PLAIN or LE_EST_ROWWISE");
+    if (mc->estimator != LE_EST_PLAIN && mc->estimator != LE_EST_ROWWISE && mc->estimator != LE_EST_MIXTURE)
+        return fail(LE_ERR_ARG, "estimator must be LE_EST_PLAIN, LE_EST_ROWWISE or LE_EST_MIXTURE");
+    if (mc->estimator == LE_EST_MIXTURE) {
+        if (mc->symmetry_moves || (obs && (obs->n_harmonics || obs->n_user_weights)))
+            return fail(LE_ERR_ARG, "the mixture estimator takes no symmetry moves or extra observables");
+        if (!(mc->mixture_rho > 0.0 && mc->mixture_rho < 1.0)) return fail(LE_ERR_ARG, "mixture_rho must be in (0, 1)");
+        int L = 0;
+        for (int i = 0; i < n_sectors; ++i) L = std::max(L, sectors[i]);
+        if (L < 1 || L > le::kMixMax) return fail(LE_ERR_ARG, "the mixture estimator needs a largest sector dL in [1, 24]");
+    }
     if (mc->estimator == LE_EST_ROWWISE &&
         (mc->symmetry_moves || mc->sweep_registers || (obs && (obs->n_harmonics || obs->n_user_weights))))
         return fail(LE_ERR_ARG, "the row-wise estimator takes no symmetry moves, register sweeps or extra observables");
@@ -684,6 +716,10 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
                   : mc->precision == LE_PREC_BF16X6 ? le::kBF16X6
                                                     : le::kFP32;
     e->basis.build(sys->particles_number, dls);
+    if (mc->estimator == LE_EST_MIXTURE) {
+        e->mixture = true;
+        for (int dl : dls) e->mix_L = std::max(e->mix_L, dl);
+    }
     // capacity checks (reference: em_spectrum.cu:235-252 exits); here they are codes
     if (e->basis.K > 24) {
         delete e;
@@ -777,6 +813,24 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     return LE_OK;
 }
 
+extern "C" int le_set_mixture(le_engine* e, const double* nk, int n_k, double rho)
+{
+    if (!e) return fail(LE_ERR_ARG, "null engine");
+    return e->set_mixture(nk, n_k, rho);
+}
+
+extern "C" int le_get_mixture(const le_engine* e, int* L, double* c0, double* a, double* b)
+{
+    if (!e || !L) return fail(LE_ERR_ARG, "null argument");
+    *L = e->mixp.kmix;
+    if (c0) *c0 = e->mixp.c0;
+    for (int k = 0; k < e->mixp.kmix; ++k) {
+        if (a) a[k] = e->mixp.a[k];
+        if (b) b[k] = e->mixp.b[k];
+    }
+    return LE_OK;
+}
+
 extern "C" int le_sector_dims(const le_engine* e, int* dims, int* offsets)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
@@ -841,12 +895,79 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
             HIPCHK(le::launch_sweep_learned_reg(a, lt, map_lanes(G), stream));
         else
             HIPCHK(le::launch_sweep_learned(a, lt, std::max(map_lanes(G), 8), stream));
+    } else if (mixture && mixp.kmix > 0) {
+        // launches of at most kMixChunk sweeps, each starting its running p_k from the positions
+        a.mix = mixp;
+        a.mix_pk = mix_pk;
+        a.R = R;
+        for (int done = 0; done < nsweeps; done += kMixChunk) {
+            a.sweep0 = sweep0 + (unsigned long long)done;
+            a.nsweeps = std::min(kMixChunk, nsweeps - done);
+            HIPCHK(le::launch_mix_init(a, prec, stream));
+            if (map_reg(G)) HIPCHK(le::launch_sweep_reg(a, prec, map_lanes(G), stream));
+            else HIPCHK(le::launch_sweep(a, prec, G, stream));
+        }
     } else if (map_reg(G)) {
         HIPCHK(le::launch_sweep_reg(a, prec, map_lanes(G), stream));
     } else {
         HIPCHK(le::launch_sweep(a, prec, G, stream));
     }
     return LE_OK;
+}
+
+// Mixture density from its normalisers n_k = <|p_k|^2> (k = 1..L): term k is
+// c_k (|p_k|^2 / n_k)^(L/k) with c_k = (1 - rho) / (L Gamma(L/k + 1)), stored as
+// exp(a_k ln|p_k|^2 + b_k).
+int le_engine::set_mixture(const double* nk, int n, double rho)
+{
+    if (!mixture) return fail(LE_ERR_STATE, "le_set_mixture needs an engine created with LE_EST_MIXTURE");
+    if (!nk || n != mix_L) return fail(LE_ERR_ARG, "le_set_mixture: n_k must hold L = " + std::to_string(mix_L) + " values");
+    if (!(rho > 0.0 && rho < 1.0)) return fail(LE_ERR_ARG, "mixture rho must be in (0, 1)");
+    for (int k = 0; k < n; ++k)
+        if (!(nk[k] > 0.0) || !std::isfinite(nk[k])) return fail(LE_ERR_ARG, "mixture normalisers n_k must be finite and > 0");
+    le::MixParams m{};
+    m.kmix = n;
+    m.c0 = rho;
+    for (int k = 1; k <= n; ++k) {
+        const double ak = (double)n / (double)k;
+        m.a[k - 1] = ak;
+        m.b[k - 1] = std::log((1.0 - rho) / n) - std::lgamma(ak + 1.0) - ak * std::log(nk[k - 1]);
+    }
+    mixp = m;
+    drop_graphs();   // the captured rounds hold the previous density
+    return LE_OK;
+}
+
+// n_k = mean |p_k|^2 over the walkers at their current positions
+int le_engine::calibrate_mixture()
+{
+    le::SweepArgs a{};
+    a.pos = pos;
+    a.N = sys.particles_number;
+    a.walkers = mc.walkers;
+    a.R = R;
+    a.mix.kmix = mix_L;
+    a.mix_pk = mix_pk;
+    HIPCHK(le::launch_mix_init(a, prec, stream));
+    std::vector<double2> pk((size_t)mc.walkers * le::kMixMax);
+    HIPCHK(hipMemcpyAsync(pk.data(), mix_pk, sizeof(double2) * pk.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<double> nk(mix_L, 0.0);
+    for (int w = 0; w < mc.walkers; ++w)
+        for (int k = 0; k < mix_L; ++k) {
+            const double2 v = pk[(size_t)w * le::kMixMax + k];
+            nk[k] += v.x * v.x + v.y * v.y;
+        }
+    for (auto& v : nk) v /= mc.walkers;
+    return set_mixture(nk.data(), mix_L, mc.mixture_rho);
+}
+
+void le_engine::mix_features(le::FeatureArgs& f) const
+{
+    if (!mixture) return;
+    f.mix = mixp;
+    f.mix_row = mix_row;
+    f.mix_acc = mix_acc;
 }
 
 // Exact centre-of-mass heat bath + global rotation (+ dilation of the relative coordinates at
@@ -1033,8 +1154,16 @@ extern "C" int le_burnin(le_engine* e, int sweeps)
         if (rc) return rc;
     }
     if (sweeps <= 0) return LE_OK;
-    int rc = e->run_sweeps(sweeps, e->sweep_counter, e->pos);
+    int first = sweeps;
+    if (e->mixture && e->mixp.kmix == 0) first = sweeps / 2;   // |Psi_L|^2 first, then w |Psi_L|^2
+    int rc = e->run_sweeps(first, e->sweep_counter, e->pos);
     if (rc) return rc;
+    if (first < sweeps) {
+        rc = e->calibrate_mixture();
+        if (rc) return rc;
+        rc = e->run_sweeps(sweeps - first, e->sweep_counter + (unsigned long long)first, e->pos);
+        if (rc) return rc;
+    }
     e->sweep_counter += (unsigned long long)sweeps;
     e->sweeps_done += sweeps;
     e->st.sweeps = e->sweeps_done;
@@ -1120,12 +1249,21 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
 
 int le_engine::close_block(long long samples)
 {
+    // blocks are means over the samples, or for the mixture estimator sums over the sum of 1 / w
+    double norm = (double)samples;
+    if (mixture) {
+        HIPCHK(hipMemcpyAsync(mix_acc_h, mix_acc, sizeof(double) * wrows, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        norm = 0.0;
+        for (long r = 0; r < wrows; ++r) norm += mix_acc_h[r];
+        HIPCHK(le::launch_zero(mix_acc, wrows, stream));
+    }
     le::ExtractArgs x{};
     x.items = items_d;
     x.nitems = (int)items.size();
     x.total = rec_elems;
     x.ldc = ldx;
-    x.scale = samples > 0 ? 1.0 / (double)samples : 0.0;
+    x.scale = norm > 0.0 ? 1.0 / norm : 0.0;
     x.out = rec_d;
     HIPCHK(le::launch_extract(x, stream));
     HIPCHK(hipMemcpyAsync(rec_h, rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
@@ -1134,6 +1272,7 @@ int le_engine::close_block(long long samples)
     rec.emplace_back(rec_h, rec_h + 2 * rec_elems);
     block_grams = 0;
     blk_samples.push_back(samples);
+    blk_norm.push_back(norm);
     return LE_OK;
 }
 
@@ -1340,6 +1479,10 @@ extern "C" int le_sample(le_engine* e, int samples)
         if (rc) return rc;
     }
     if (e->rowwise) return e->sample_rowwise(samples);
+    if (e->mixture && e->mixp.kmix == 0) {
+        int rc = e->calibrate_mixture();
+        if (rc) return rc;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(unsigned) * e->mc.walkers, e->stream));
     le::FeatureArgs f{};
@@ -1366,6 +1509,7 @@ extern "C" int le_sample(le_engine* e, int samples)
     f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
+    e->mix_features(f);
     const int tw = std::min(e->mc.trace_walkers, e->mc.walkers);
     long long kept = 0;
     // Rounds alternate between two sample buffers: the sampler stream fills buffer b while
@@ -1542,12 +1686,13 @@ static int record_slice(const le_engine* e, int block, long off, long n, double*
         std::memcpy(out, e->rec[block].data() + 2 * off, sizeof(double) * 2 * n);
         return LE_OK;
     }
-    long long tot = 0;
-    for (auto c : e->blk_samples) tot += c;
+    // blocks weigh by their samples (mixture: by their sums of 1 / w)
+    double tot = 0;
+    for (auto c : e->blk_norm) tot += c;
     for (long q = 0; q < 2 * n; ++q) {
         double s = 0;
         for (int b = 0; b < nb; ++b) {
-            const double w = tot ? (double)e->blk_samples[b] / (double)tot : 1.0 / nb;
+            const double w = tot > 0.0 ? e->blk_norm[b] / tot : 1.0 / nb;
             s += w * e->rec[b][2 * off + q];
         }
         out[q] = s;
@@ -1713,6 +1858,8 @@ extern "C" int le_reset_moments(le_engine* e)
     if (!e) return fail(LE_ERR_ARG, "null engine");
     e->rec.clear();
     e->blk_samples.clear();
+    e->blk_norm.clear();
+    if (e->mixture) HIPCHK(le::launch_zero(e->mix_acc, e->wrows, e->stream));
     e->block_grams = 0;
     if (e->rowwise) HIPCHK(le::launch_zero(e->row_acc, 5 * e->basis.block_elems(), e->stream));
     HIPCHK(le::launch_zero(e->cpool, (long)e->cpool_elems, e->stream));
@@ -1732,6 +1879,8 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
 {
     if (!e || !pos || n <= 0) return fail(LE_ERR_ARG, "null argument or n <= 0");
     if (e->rowwise) return fail(LE_ERR_STATE, "fixed samples do not apply to the row-wise estimator's chains");
+    if (e->mixture && e->mixp.kmix == 0)
+        return fail(LE_ERR_STATE, "fixed samples of a mixture engine need its density: call le_set_mixture first");
     const int N = e->npos();
     const long chunk = (long)e->spl * e->mc.walkers;
     double* stage = nullptr;
@@ -1760,6 +1909,7 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
     f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
+    e->mix_features(f);
     int rc = LE_OK;
     for (long s0 = 0; s0 < n && rc == LE_OK; s0 += chunk) {
         const long m = std::min(chunk, n - s0);
@@ -2192,6 +2342,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         !le::sweep2_reg_supported(mc->lanes_per_walker, sys->particles_number))
         return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil((N - 1) / lanes) 13 or 25");
     if (dL < 0 || dL > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
+    if (mc->estimator != LE_EST_PLAIN) return fail(LE_ERR_ARG, "the occupation engine takes the plain estimator only");
     auto e = new le_engine();
     e->sys = *sys;
     e->mc = *mc;
@@ -2287,7 +2438,7 @@ extern "C" int le_set_proposal(le_engine* e, int kind)
     }
     if (kind != LE_PROPOSAL_LEARNED) return fail(LE_ERR_ARG, "unknown proposal");
     if (!e->table) return fail(LE_ERR_STATE, "no trained proposal (le_train_proposal or le_set_proposal_params)");
-    if (e->prec != LE_PREC_FP32 || e->two_sided || e->rowwise)
+    if (e->prec != LE_PREC_FP32 || e->two_sided || e->rowwise || e->mixture)
         return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine with the plain estimator");
     e->proposal = kind;
     return LE_OK;

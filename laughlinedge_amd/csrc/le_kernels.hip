@@ -120,7 +120,125 @@ __device__ __forceinline__ double row_propose(const RowState& st, double xo, dou
     return log_abs2_row(st, prop);
 }
 
-template <int G, typename Real, bool ROW = false>
+// Mixture density w |Psi_L|^2 (MixParams): lane l of a walker's G lanes keeps the running p_k
+// for k = l + 1 + G j (j < M).  A move z -> z' changes them by u'^k - u^k (u = z / R): the lanes
+// take u^(l+1) from an inclusive prefix product over the group (log2 G shuffle steps) and step
+// on by u^G.  Each lane sums its terms exp(a_k ln|p_k|^2 + b_k); the group sum (the same in every
+// lane: butterfly over commutative additions) plus c0 is w, and log A gains ln w' - ln w.  FP32
+// evaluates a term as exp2(a_k log2|p_k|^2 + b_k / ln 2); FP64 rounds every product on its own,
+// as the FP64 CPU restatement does.  The running p_k start each launch from launch_mix_init.
+template <int G, typename Real, int KM = kMixMax>
+struct MixLane {
+    static constexpr int M = (KM + G - 1) / G;
+    Real pr[M], pi[M];       // running p_k
+    Real ak[M], bk[M];       // exponent and offset of each term (FP32: offset in log2 units)
+    Real c0;
+    int kmix;
+    Real lw;                 // ln w at the current positions
+};
+
+template <typename Real>
+__device__ __forceinline__ Real mix_term(Real a, Real b, Real re, Real im)
+{
+#pragma clang fp contract(off)
+    const Real q = re * re + im * im;
+    if constexpr (sizeof(Real) == 4)
+        return __builtin_amdgcn_exp2f(a * __builtin_amdgcn_logf(q) + b);
+    else
+        return q > 0.0 ? exp(a * log(q) + b) : 0.0;
+}
+
+template <typename Real>
+__device__ __forceinline__ Real mix_ln(Real w)
+{
+    if constexpr (sizeof(Real) == 4)
+        return __builtin_amdgcn_logf(w) * 0.69314718055994531f;
+    else
+        return log(w);
+}
+
+template <int G, typename Real, int KM>
+__device__ __forceinline__ void mix_load(MixLane<G, Real, KM>& m, const SweepArgs& a, int walker, int lane)
+{
+    constexpr int M = MixLane<G, Real, KM>::M;
+    m.kmix = a.mix.kmix;
+    m.c0 = (Real)a.mix.c0;
+    Real local = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const int k = lane + 1 + G * j;
+        if (k <= m.kmix) {
+            const double2 p = a.mix_pk[(size_t)walker * kMixMax + (k - 1)];
+            m.pr[j] = (Real)p.x;
+            m.pi[j] = (Real)p.y;
+            m.ak[j] = (Real)a.mix.a[k - 1];
+            m.bk[j] = sizeof(Real) == 4 ? (Real)(a.mix.b[k - 1] * 1.4426950408889634) : (Real)a.mix.b[k - 1];
+            local += mix_term<Real>(m.ak[j], m.bk[j], m.pr[j], m.pi[j]);
+        } else {
+            m.pr[j] = m.pi[j] = m.ak[j] = m.bk[j] = 0;
+        }
+    }
+    m.lw = mix_ln<Real>(group_sum<G, Real>(local) + m.c0);
+}
+
+// ln w at z' when the walker's particle moves from zo to zn; qr, qi receive the proposed p_k
+template <int G, typename Real, int KM>
+__device__ __forceinline__ Real mix_propose(const MixLane<G, Real, KM>& m, Real xo, Real yo, Real xn, Real yn,
+                                            double R, int lane, Real (&qr)[MixLane<G, Real, KM>::M],
+                                            Real (&qi)[MixLane<G, Real, KM>::M])
+{
+#pragma clang fp contract(off)
+    constexpr int M = MixLane<G, Real, KM>::M;
+    Real aor, aoi, anr, ani;
+    if constexpr (sizeof(Real) == 4) {
+        const float ir = (float)(1.0 / R);
+        aor = xo * ir; aoi = yo * ir; anr = xn * ir; ani = yn * ir;
+    } else {
+        aor = xo / R; aoi = yo / R; anr = xn / R; ani = yn / R;
+    }
+    Real por = aor, poi = aoi, pnr = anr, pni = ani;
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const Real tor = __shfl_up(por, d, G), toi = __shfl_up(poi, d, G);
+        const Real tnr = __shfl_up(pnr, d, G), tni = __shfl_up(pni, d, G);
+        if (lane >= d) {
+            const Real o = por * tor - poi * toi;
+            poi = por * toi + poi * tor;
+            por = o;
+            const Real n = pnr * tnr - pni * tni;
+            pni = pnr * tni + pni * tnr;
+            pnr = n;
+        }
+    }
+    Real gor = 0, goi = 0, gnr = 0, gni = 0;
+    if constexpr (M > 1) {
+        gor = __shfl(por, G - 1, G); goi = __shfl(poi, G - 1, G);
+        gnr = __shfl(pnr, G - 1, G); gni = __shfl(pni, G - 1, G);
+    }
+    Real local = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j > 0) {
+            const Real o = por * gor - poi * goi;
+            poi = por * goi + poi * gor;
+            por = o;
+            const Real n = pnr * gnr - pni * gni;
+            pni = pnr * gni + pni * gnr;
+            pnr = n;
+        }
+        const int k = lane + 1 + G * j;
+        if (k <= m.kmix) {
+            qr[j] = (m.pr[j] - por) + pnr;
+            qi[j] = (m.pi[j] - poi) + pni;
+            local += mix_term<Real>(m.ak[j], m.bk[j], qr[j], qi[j]);
+        } else {
+            qr[j] = qi[j] = 0;
+        }
+    }
+    return mix_ln<Real>(group_sum<G, Real>(local) + m.c0);
+}
+
+template <int G, typename Real, bool ROW = false, bool MIX = false>
 __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 {
     using V2 = typename V2T<Real>::T;
@@ -172,6 +290,8 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
         }
         row.lm = log_abs2_row(row, row.pk);
     }
+    MixLane<G, Real> mix;
+    if constexpr (MIX) mix_load(mix, a, walker, lane);
 
     for (int sw = 0; sw < a.nsweeps; ++sw) {
         const unsigned long long sidx = sweep_first + (unsigned long long)sw;
@@ -285,6 +405,12 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
                     lm_new = row_propose(row, (double)zo.x, (double)zo.y, (double)xn, (double)yn, a.R, prop);
                     logA = (Real)((double)logA + (lm_new - row.lm));
                 }
+                Real mqr[MixLane<G, Real>::M], mqi[MixLane<G, Real>::M];
+                Real lw_new = 0;
+                if constexpr (MIX) {
+                    lw_new = mix_propose(mix, zo.x, zo.y, xn, yn, a.R, lane, mqr, mqi);
+                    logA += lw_new - mix.lw;
+                }
                 bool acc;
                 if constexpr (sizeof(Real) == 4) {
                     acc = (logA >= 0.f) || (lnu_move < logA);
@@ -296,6 +422,16 @@ __global__ __launch_bounds__(64) void k_sweep(SweepArgs a)
 #pragma unroll
                         for (int q = 0; q < kMaxFactors; ++q) row.pk[q] = prop[q];
                         row.lm = lm_new;
+                    }
+                }
+                if constexpr (MIX) {
+                    if (acc) {
+#pragma unroll
+                        for (int j = 0; j < MixLane<G, Real>::M; ++j) {
+                            mix.pr[j] = mqr[j];
+                            mix.pi[j] = mqi[j];
+                        }
+                        mix.lw = lw_new;
                     }
                 }
                 V2 nz;
@@ -337,9 +473,11 @@ constexpr int reg_sweep_waves()
          : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 100 ? 5 : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 128 ? 4 : 3;
 }
 
-template <int G, int S, typename Real>
+template <int G, int S, typename Real, int MIXK = 0>   // MIXK: 0, or the largest mixture k held
 __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
 {
+    constexpr bool MIX = MIXK > 0;
+    constexpr int KM = MIX ? MIXK : 1;
     using V2 = typename V2T<Real>::T;
     constexpr int WPB = 64 / G;
     constexpr int C = LogOps<Real>::chunk;
@@ -365,6 +503,8 @@ __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
     const unsigned long long sweep_first = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     unsigned accepted = 0;
+    MixLane<G, Real, KM> mix;
+    if constexpr (MIX) mix_load(mix, a, walker, lane);
 
     for (int sw = 0; sw < a.nsweeps; ++sw) {
         const unsigned long long sidx = sweep_first + (unsigned long long)sw;
@@ -440,7 +580,13 @@ __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
                 else
                     lsum += LogOps<Real>::lg(pn) - LogOps<Real>::lg(pd);
                 lsum = group_sum<G, Real>(lsum);
-                const Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zox, zox, zoy * zoy), mfac * lsum);
+                Real logA = mad((Real)-0.5, mad(xn, xn, yn * yn) - mad(zox, zox, zoy * zoy), mfac * lsum);
+                Real mqr[MixLane<G, Real, KM>::M], mqi[MixLane<G, Real, KM>::M];
+                Real lw_new = 0;
+                if constexpr (MIX) {
+                    lw_new = mix_propose(mix, zox, zoy, xn, yn, a.R, lane, mqr, mqi);
+                    logA += lw_new - mix.lw;
+                }
                 bool acc;
                 if constexpr (sizeof(Real) == 4) {
                     acc = (logA >= 0.f) || (__shfl(lnu, t, G) < logA);
@@ -450,6 +596,16 @@ __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
                 if (own) {
                     px[s] = acc ? xn : zox;
                     py[s] = acc ? yn : zoy;
+                }
+                if constexpr (MIX) {
+                    if (acc) {
+#pragma unroll
+                        for (int j = 0; j < MixLane<G, Real, KM>::M; ++j) {
+                            mix.pr[j] = mqr[j];
+                            mix.pi[j] = mqi[j];
+                        }
+                        mix.lw = lw_new;
+                    }
                 }
                 accepted += acc ? 1u : 0u;
             }
@@ -652,7 +808,35 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
         if (q % kLps == sub && q < 2 * a.K) ps[q] = acc[q];
     const long row = a.row0 + w;
     Real* W = reinterpret_cast<Real*>(a.W);
-    if (sub == 0) W[row] = (Real)acc[QV];
+    if (a.mix.kmix > 0) {
+        // mixture estimator: w = c0 + sum_k exp(a_k ln|p_k|^2 + b_k), lane sub taking k = sub + 1 + 4 r
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < KMAX / kLps; ++r) {
+            double re = acc[2 * kLps * r], im = acc[2 * kLps * r + 1];
+#pragma unroll
+            for (int c = 1; c < kLps; ++c)
+                if (sub == c) {
+                    re = acc[2 * (kLps * r + c)];
+                    im = acc[2 * (kLps * r + c) + 1];
+                }
+            const int k = kLps * r + sub;
+            if (k < a.mix.kmix) {
+#pragma clang fp contract(off)
+                const double q = re * re + im * im;
+                if (q > 0.0) t += exp(a.mix.a[k] * log(q) + a.mix.b[k]);
+            }
+        }
+        t = group_sum<kLps, double>(t);
+        const double iw = 1.0 / (a.mix.c0 + t);
+        if (sub == 0) {
+            W[row] = (Real)(acc[QV] * iw);
+            W[(size_t)a.mix_row * a.wld + row] = (Real)iw;
+            a.mix_acc[row] += iw;
+        }
+    } else if (sub == 0) {
+        W[row] = (Real)acc[QV];
+    }
     if constexpr (HARM) {
 #pragma unroll
         for (int q = 0; q < 2 * NH; ++q)
@@ -1776,31 +1960,31 @@ hipError_t launch_init_positions(void* pos, int prec, int N, int walkers, double
     return hipGetLastError();
 }
 
-template <int G, typename Real, bool ROW = false>
+template <int G, typename Real, bool ROW = false, bool MIX = false>
 static hipError_t sweep_g(const SweepArgs& a, hipStream_t s)
 {
     const size_t lds = sweep_lds_bytes(sizeof(Real) == 8 ? kFP64 : kFP32, G, a.N);
     constexpr int WPB = 64 / G;
     dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sweep<G, Real, ROW>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep<G, Real, ROW, MIX>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_sweep<G, Real, ROW>), grid, dim3(64), lds, s, a);
+    hipLaunchKernelGGL((k_sweep<G, Real, ROW, MIX>), grid, dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-template <typename Real, bool ROW = false>
+template <typename Real, bool ROW = false, bool MIX = false>
 static hipError_t sweep_dispatch(const SweepArgs& a, int G, hipStream_t s)
 {
     switch (G) {
-    case 2: return sweep_g<2, Real, ROW>(a, s);
-    case 4: return sweep_g<4, Real, ROW>(a, s);
-    case 8: return sweep_g<8, Real, ROW>(a, s);
-    case 16: return sweep_g<16, Real, ROW>(a, s);
-    case 32: return sweep_g<32, Real, ROW>(a, s);
-    case 64: return sweep_g<64, Real, ROW>(a, s);
+    case 2: return sweep_g<2, Real, ROW, MIX>(a, s);
+    case 4: return sweep_g<4, Real, ROW, MIX>(a, s);
+    case 8: return sweep_g<8, Real, ROW, MIX>(a, s);
+    case 16: return sweep_g<16, Real, ROW, MIX>(a, s);
+    case 32: return sweep_g<32, Real, ROW, MIX>(a, s);
+    case 64: return sweep_g<64, Real, ROW, MIX>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1812,7 +1996,14 @@ static hipError_t sweep_reg_gs(const SweepArgs& a, hipStream_t s)
 {
     constexpr int WPB = 64 / G;
     dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
-    hipLaunchKernelGGL((k_sweep_reg<G, S, Real>), grid, dim3(64), 0, s, a);
+    // mixture: hold 16 terms when they suffice (N = 200 at 8 lanes: 124 VGPRs and 4 waves per
+    // SIMD, against 131 and 3 with room for 24)
+    if (a.mix.kmix > 16)
+        hipLaunchKernelGGL((k_sweep_reg<G, S, Real, kMixMax>), grid, dim3(64), 0, s, a);
+    else if (a.mix.kmix > 0)
+        hipLaunchKernelGGL((k_sweep_reg<G, S, Real, 16>), grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sweep_reg<G, S, Real>), grid, dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1837,13 +2028,69 @@ hipError_t launch_sweep_reg(const SweepArgs& a, int prec, int G, hipStream_t s)
 {
     if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
     if (!sweep_reg_supported(G, a.N)) return hipErrorInvalidValue;
+    if (a.mix.kmix > 0 && (a.mix.kmix > kMixMax || !a.mix_pk)) return hipErrorInvalidValue;
     return prec == kFP64 ? sweep_reg_dispatch<double>(a, G, s) : sweep_reg_dispatch<float>(a, G, s);
 }
 
 hipError_t launch_sweep(const SweepArgs& a, int prec, int G, hipStream_t s)
 {
     if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
+    if (a.mix.kmix > 0) {
+        if (a.mix.kmix > kMixMax || !a.mix_pk) return hipErrorInvalidValue;
+        return prec == kFP64 ? sweep_dispatch<double, false, true>(a, G, s) : sweep_dispatch<float, false, true>(a, G, s);
+    }
     return prec == kFP64 ? sweep_dispatch<double>(a, G, s) : sweep_dispatch<float>(a, G, s);
+}
+
+// p_k, k = 1..kmix, of every walker's positions for the mixture chains: 4 lanes per walker
+// (particles j = sub + 4 s), FP64 powers by repeated multiplication, a 2-step butterfly
+template <typename Real>
+__global__ __launch_bounds__(256) void k_mix_init(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long w = gt / kLps;
+    const int sub = (int)(gt % kLps);
+    if (w >= a.walkers) return;                  // the 4 lanes of a walker leave together
+    const int kmix = a.mix.kmix;
+    double pr[kMixMax], pi[kMixMax];
+#pragma unroll
+    for (int k = 0; k < kMixMax; ++k) pr[k] = pi[k] = 0.0;
+    const V2* P = reinterpret_cast<const V2*>(a.pos) + (size_t)w * a.N;
+    for (int j = sub; j < a.N; j += kLps) {
+        const V2 q = P[j];
+        const double ur = (double)q.x / a.R, ui = (double)q.y / a.R;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < kMixMax; ++k) {
+            if (k < kmix) {
+                pr[k] += qr;
+                pi[k] += qi;
+                const double t = qr * ur - qi * ui;
+                qi = qr * ui + qi * ur;
+                qr = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kMixMax; ++k) {
+        pr[k] = group_sum<kLps, double>(pr[k]);
+        pi[k] = group_sum<kLps, double>(pi[k]);
+        if (k % kLps == sub && k < kmix) a.mix_pk[(size_t)w * kMixMax + k] = make_double2(pr[k], pi[k]);
+    }
+}
+
+hipError_t launch_mix_init(const SweepArgs& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    if (a.mix.kmix < 1 || a.mix.kmix > kMixMax || !a.mix_pk) return hipErrorInvalidValue;
+    const long threads = (long)a.walkers * kLps;
+    dim3 grid((unsigned)((threads + 255) / 256));
+    if (prec == kFP64)
+        hipLaunchKernelGGL(k_mix_init<double>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_mix_init<float>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1984,6 +2231,7 @@ template <typename Real>
 static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
 {
     if (a.npw > kMaxPow || !a.ps || a.ps_ld < 2 * a.K) return hipErrorInvalidValue;
+    if (a.mix.kmix > 0 && (a.mix.kmix > a.K || !a.mix_acc || a.mix_row < 1 + 2L * a.nh)) return hipErrorInvalidValue;
     // harmonic accumulators are FP64 registers: hold only as many pairs as the call has
     if (a.nh > kMaxHarmonics) return hipErrorInvalidValue;
     if (a.nh == 0) return features_h<Real, 0>(a, s);

@@ -13,6 +13,16 @@ struct Potential {
     double R;
 };
 
+// Mixture importance density w(z) |Psi_L|^2 with w = c0 + sum_{k=1..kmix} exp(a_k ln|p_k|^2 + b_k),
+// p_k = sum_i (z_i / R)^k.  kmix = 0: no mixture.
+constexpr int kMixMax = 24;
+struct MixParams {
+    int kmix;
+    double c0;
+    double a[kMixMax];
+    double b[kMixMax];
+};
+
 struct SweepArgs {
     void* pos;               // [walkers][N] float2 or double2
     unsigned* accepted;      // [walkers] accepted-move counters (added to)
@@ -29,6 +39,10 @@ struct SweepArgs {
     const uint32_t* row_fac;         // [D][8] packed (k | r << 8) factors of every state
     double2* row_pk;                 // [chains][8] running p_k of the row state's factors
     double R;                        // power sums of z / R
+    // mixture density (launch_sweep / launch_sweep_reg with mix.kmix > 0): the chains sample
+    // w |Psi_L|^2, starting from the p_k in mix_pk [walkers][kMixMax] (launch_mix_init)
+    MixParams mix;
+    double2* mix_pk;
 };
 
 // Row-wise accumulation (the reference's estimator): per row f (a basis state of sector s, d
@@ -77,6 +91,11 @@ struct FeatureArgs {
     const long long* trace_base;   // optional device value added to trace_row0 (graph replays)
     double* ps;              // [walkers][ps_ld] scratch: Re/Im p_1..p_K per sample
     int ps_ld;
+    // mixture estimator (mix.kmix > 0): row weights V / w (weight row 0) and 1 / w (weight row
+    // mix_row), and 1 / w added to mix_acc[row]
+    MixParams mix;
+    long mix_row;
+    double* mix_acc;
 };
 
 // Two-sided chain features (occupation numbers): positions [walkers][N+1]; bulk power
@@ -197,6 +216,8 @@ hipError_t launch_sweep_row(const SweepArgs& a, int prec, int lanes_per_walker, 
 hipError_t launch_row_init(const SweepArgs& a, int prec, hipStream_t s);
 hipError_t launch_row_accum(const RowAccumArgs& a, int prec, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
+// mixture chains: a.mix_pk[w][k - 1] = p_k (k <= a.mix.kmix) of walker w's positions, in FP64
+hipError_t launch_mix_init(const SweepArgs& a, int prec, hipStream_t s);
 // exact centre-of-mass heat bath + global rotation of every walker (a.sweep0 + *a.sweep_base
 // selects the random numbers)
 hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s);

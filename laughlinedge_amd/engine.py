@@ -36,11 +36,13 @@ EXPORTED = [
     "le_evolve", "le_engine_create_occupation", "le_get_occupation", "le_train_params_default",
     "le_train_proposal", "le_set_proposal", "le_get_proposal_params", "le_set_proposal_params",
     "le_proposal_loss_grad", "le_set_thin", "le_removal_weights", "le_step_search",
+    "le_set_mixture", "le_get_mixture",
 ]
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
 EST_PLAIN = 0
 EST_ROWWISE = 1
+EST_MIXTURE = 2
 N_MLP_PARAMS = 4546
 
 
@@ -92,7 +94,8 @@ class MCParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("lanes_per_walker", ctypes.c_int),
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
-                ("sweep_registers", ctypes.c_int), ("estimator", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int)]
+                ("sweep_registers", ctypes.c_int), ("estimator", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int),
+                ("mixture_rho", ctypes.c_double)]
 
 
 class Observables(ctypes.Structure):
@@ -250,6 +253,23 @@ class Engine:
 
     def burnin(self, sweeps):
         _check(lib().le_burnin(self._h, int(sweeps)))
+
+    def set_mixture(self, nk, rho):
+        """Mixture density of a LE_EST_MIXTURE engine from its normalisers n_k (k = 1..L)."""
+        v = np.ascontiguousarray(nk, dtype=np.float64)
+        _check(lib().le_set_mixture(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    ctypes.c_int(len(v)), ctypes.c_double(rho)))
+
+    def get_mixture(self):
+        """(L, c0, a, b) of the density in use (L = 0 when not set)."""
+        L = ctypes.c_int()
+        c0 = ctypes.c_double()
+        a = np.zeros(32)
+        b = np.zeros(32)
+        _check(lib().le_get_mixture(self._h, ctypes.byref(L), ctypes.byref(c0),
+                                    a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    b.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return L.value, c0.value, a[:L.value].copy(), b[:L.value].copy()
 
     def sample(self, samples):
         _check(lib().le_sample(self._h, int(samples)))

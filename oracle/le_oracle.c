@@ -17,9 +17,10 @@
  *   - DSF weights |(Vc - i Vs)[0,1:] . v_j|^2 .................. modules/eigensolver/generalized_eigenproblem.cpp:54-95
  *   - cos/sin(dL theta) excitation sums ........................ modules/edge_mc/em_markovchain.cu:101-117
  *
- * The plain |Psi_L|^2 estimator (S = <conj(p_l) p_m>, H = <conj(p_l) V p_m>) has no
- * counterpart in the reference; it is the estimator the GPU path implements and is
- * restated here from its definition.
+ * The plain |Psi_L|^2 estimator (S = <conj(p_l) p_m>, H = <conj(p_l) V p_m>) and the mixture
+ * estimator (chains on w |Psi_L|^2, S and H reweighted by 1 / w) have no counterpart in the
+ * reference; they are estimators the GPU path implements and are restated here from their
+ * definitions.
  *
  * Random numbers: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the same
  * counter layout as the GPU sampler, Box-Muller normals in FP64.
@@ -34,6 +35,10 @@
  *   or_plain_run ............ <sum r^2> = 2 (L0 + N); eigenvalues vs exact N = 3
  *   or_plain_positions ...... the same chain code as or_plain_run
  *   or_rowwise_sector ....... exact normalised metric and spectrum at N = 3
+ *   or_mixture_run .......... equal to or_plain_run when every tail term vanishes; exact S, H
+ *                             and spectra at N = 3
+ *   or_mixture_weights ...... the AM-GM bound |p_lambda|^2 <= C w on configurations deep in
+ *                             the tails
  *   or_geneig, or_dsf_weights scipy.linalg.eigh; residual, S-orthonormality
  *   or_evolve ............... closed form with F = 0; dense matrix exponential for a
  *                             piecewise-constant drive
@@ -269,18 +274,42 @@ int or_features(const or_system *sys, const or_basis *b, const double *pos, doub
 /* ------------------------------------------------------------------------ */
 /* Metropolis on |p_f Psi_L|^2 (p_f = 1 gives the plain |Psi_L|^2 chain).     */
 /* ------------------------------------------------------------------------ */
+/* Mixture importance density w(z) |Psi_L|^2 with
+ *   w = c0 + sum_{k=1..kmix} exp(a_k ln|p_k|^2 + b_k)
+ * (an estimator of this build with no counterpart in the reference).  With a_k = L/k for the
+ * largest sector L, every |p_lambda|^2 of every sector up to L is at most a constant times w
+ * (weighted AM-GM over the parts of lambda), so the reweighted integrands p_l* p_m / w are
+ * bounded; S = sum(p_l* p_m / w) / sum(1 / w) over the chain. */
+typedef struct or_mixture {
+    int kmix;
+    double c0;
+    double a[32];
+    double b[32];
+} or_mixture;
+
 typedef struct chain {
     int N, m;
     double *x, *y;          /* positions */
     const or_basis *b;
     int f;                  /* sampling state, -1 for the plain chain */
     double R;
+    const or_mixture *mx;   /* mixture density (f = -1), or NULL */
 } chain;
+
+static double mix_log_w(const or_mixture *mx, const cplx *pk)
+{
+    double w = mx->c0;
+    for (int k = 1; k <= mx->kmix; ++k) {
+        double q = creal(pk[k]) * creal(pk[k]) + cimag(pk[k]) * cimag(pk[k]);
+        if (q > 0.0) w += exp(mx->a[k - 1] * log(q) + mx->b[k - 1]);
+    }
+    return log(w);
+}
 
 /* power sums of the chain's current positions (used to start the incremental update) */
 static void chain_power_sums(const chain *c, cplx *pk)
 {
-    int K = c->b ? c->b->K : 0;
+    int K = c->mx ? c->mx->kmix : (c->b ? c->b->K : 0);
     for (int k = 0; k <= K; ++k) pk[k] = 0;
     pk[0] = c->N;
     for (int j = 0; j < c->N; ++j) {
@@ -300,7 +329,7 @@ static double log_abs2(cplx v) { return log(creal(v) * creal(v) + cimag(v) * cim
 static int sweep(chain *c, uint64_t seed, uint32_t walker, uint64_t sweep_idx, double step, cplx *cur_pk)
 {
     int acc = 0;
-    int K = (c->f >= 0) ? c->b->K : 0;
+    int K = (c->f >= 0) ? c->b->K : (c->mx ? c->mx->kmix : 0);
     cplx prop[64];
     for (int i = 0; i < c->N; ++i) {
         uint32_t r[4];
@@ -323,11 +352,17 @@ static int sweep(chain *c, uint64_t seed, uint32_t walker, uint64_t sweep_idx, d
             prop[0] = cur_pk[0];
             for (int k = 1; k <= K; ++k) { prop[k] = cur_pk[k] - qo + qn; qo *= uo; qn *= un; }
             logA += log_abs2(monomial(c->b, c->f, prop)) - log_abs2(monomial(c->b, c->f, cur_pk));
+        } else if (c->mx) {
+            cplx uo = (xo + I * yo) / c->R, un = (xn + I * yn) / c->R;
+            cplx qo = uo, qn = un;
+            prop[0] = cur_pk[0];
+            for (int k = 1; k <= K; ++k) { prop[k] = cur_pk[k] - qo + qn; qo *= uo; qn *= un; }
+            logA += mix_log_w(c->mx, prop) - mix_log_w(c->mx, cur_pk);
         }
         double u = u01(r[2]);
         if (logA >= 0.0 || log(u) < logA) {
             c->x[i] = xn; c->y[i] = yn;
-            if (c->f >= 0) for (int k = 0; k <= K; ++k) cur_pk[k] = prop[k];
+            if (c->f >= 0 || c->mx) for (int k = 0; k <= K; ++k) cur_pk[k] = prop[k];
             ++acc;
         }
     }
@@ -375,7 +410,7 @@ int or_plain_run(const or_system *sys, const or_basis *b, int walkers, int burni
         double r2_l = 0.0;
 #pragma omp for schedule(dynamic, 1)
         for (int wk = 0; wk < walkers; ++wk) {
-            chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), b, -1, sys->R};
+            chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), b, -1, sys->R, NULL};
             cplx pk[64];
             init_positions(&c, seed, (uint32_t)wk);
             uint64_t sw = 0;
@@ -429,7 +464,7 @@ int or_plain_positions(const or_system *sys, int walkers, int burnin, int sample
     long acc = 0;
 #pragma omp parallel for reduction(+:acc) schedule(dynamic, 1)
     for (int wk = 0; wk < walkers; ++wk) {
-        chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), NULL, -1, sys->R};
+        chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), NULL, -1, sys->R, NULL};
         cplx pk[64];
         init_positions(&c, seed, (uint32_t)wk);
         uint64_t sw = 0;
@@ -442,6 +477,101 @@ int or_plain_positions(const or_system *sys, int walkers, int burnin, int sample
         free(c.x); free(c.y);
     }
     if (acceptance) *acceptance = (double)acc / ((double)walkers * samples * thin * sys->N);
+    return 0;
+}
+
+/* w(z) of n configurations (n x N x 2) for the mixture density mx */
+int or_mixture_weights(const or_system *sys, const or_mixture *mx, const double *pos, long n, double *w)
+{
+    if (mx->kmix < 1 || mx->kmix > 31) return -1;
+    cplx pk[32];
+    for (long s = 0; s < n; ++s) {
+        power_sums(sys->N, sys->R, pos + (size_t)s * sys->N * 2, mx->kmix, pk);
+        w[s] = exp(mix_log_w(mx, pk));
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixture chains on w |Psi_L|^2 (or_mixture above): sums over walkers x     */
+/* samples of p_l* p_m / w and V p_l* p_m / w (S, H, block layout of         */
+/* or_plain_run) and of 1 / w (wsum); w from the power sums of each kept     */
+/* configuration.  The chains start from init_positions and update the      */
+/* running p_k per move like the row-wise chains.                           */
+/* ------------------------------------------------------------------------ */
+int or_mixture_run(const or_system *sys, const or_basis *b, const or_mixture *mx, int walkers, int burnin,
+                   int samples, int thin, double step, uint64_t seed, int threads, double *S, double *H,
+                   double *wsum, double *acceptance, double *final_pos)
+{
+    int nb = or_block_size(b);
+    memset(S, 0, sizeof(double) * 2 * nb);
+    memset(H, 0, sizeof(double) * 2 * nb);
+    if (mx->kmix < 1 || mx->kmix > 31 || mx->kmix > b->K) return -1;
+    long acc_total = 0;
+    double ws_total = 0.0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel
+    {
+        double *Sl = calloc(2 * (size_t)nb, sizeof(double));
+        double *Hl = calloc(2 * (size_t)nb, sizeof(double));
+        double *buf = malloc(sizeof(double) * 2 * sys->N);
+        double *pl = malloc(sizeof(double) * 2 * b->D);
+        cplx *fk = malloc(sizeof(cplx) * (b->K + 1));
+        long acc_l = 0;
+        double ws_l = 0.0;
+#pragma omp for schedule(dynamic, 1)
+        for (int wk = 0; wk < walkers; ++wk) {
+            chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), b, -1, sys->R, mx};
+            cplx pk[64];
+            init_positions(&c, seed, (uint32_t)wk);
+            chain_power_sums(&c, pk);
+            uint64_t sw = 0;
+            for (int t = 0; t < burnin; ++t) sweep(&c, seed, (uint32_t)wk, sw++, step, pk);
+            for (int s = 0; s < samples; ++s) {
+                for (int t = 0; t < thin; ++t) acc_l += sweep(&c, seed, (uint32_t)wk, sw++, step, pk);
+                for (int i = 0; i < sys->N; ++i) { buf[2 * i] = c.x[i]; buf[2 * i + 1] = c.y[i]; }
+                double V;
+                or_features(sys, b, buf, pl, &V);
+                power_sums(sys->N, sys->R, buf, b->K, fk);
+                const double iw = exp(-mix_log_w(mx, fk));
+                ws_l += iw;
+                for (int sec = 0; sec < b->nsec; ++sec) {
+                    int o = b->so[sec], d = b->so[sec + 1] - o;
+                    size_t bo = 2 * (size_t)sector_block_offset(b, sec);
+                    for (int a = 0; a < d; ++a) {
+                        double ar = pl[2 * (o + a)], ai = -pl[2 * (o + a) + 1];
+                        for (int q = 0; q < d; ++q) {
+                            double br = pl[2 * (o + q)], bi = pl[2 * (o + q) + 1];
+                            double xr = (ar * br - ai * bi) * iw, xi = (ar * bi + ai * br) * iw;
+                            Sl[bo + 2 * (a * d + q)] += xr;
+                            Sl[bo + 2 * (a * d + q) + 1] += xi;
+                            Hl[bo + 2 * (a * d + q)] += V * xr;
+                            Hl[bo + 2 * (a * d + q) + 1] += V * xi;
+                        }
+                    }
+                }
+            }
+            if (final_pos)
+                for (int i = 0; i < sys->N; ++i) {
+                    final_pos[((size_t)wk * sys->N + i) * 2] = c.x[i];
+                    final_pos[((size_t)wk * sys->N + i) * 2 + 1] = c.y[i];
+                }
+            free(c.x); free(c.y);
+        }
+#pragma omp critical
+        {
+            for (size_t q = 0; q < 2 * (size_t)nb; ++q) { S[q] += Sl[q]; H[q] += Hl[q]; }
+            acc_total += acc_l;
+            ws_total += ws_l;
+        }
+        free(Sl); free(Hl); free(buf); free(pl); free(fk);
+    }
+    if (wsum) *wsum = ws_total;
+    if (acceptance) *acceptance = (double)acc_total / ((double)walkers * samples * thin * sys->N);
     return 0;
 }
 
@@ -488,7 +618,7 @@ int or_rowwise_sector(const or_system *sys, const or_basis *b, int sec, int walk
             long la = 0;
 #pragma omp for schedule(dynamic, 1)
             for (int wk = 0; wk < walkers; ++wk) {
-                chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), b, o + f, sys->R};
+                chain c = {sys->N, sys->m, malloc(sizeof(double) * sys->N), malloc(sizeof(double) * sys->N), b, o + f, sys->R, NULL};
                 /* distinct stream per row: walker id offset by row */
                 uint32_t wid = (uint32_t)(wk + (f + 1) * walkers);
                 init_positions(&c, seed, wid);

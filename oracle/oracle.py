@@ -39,6 +39,21 @@ class _Basis(ctypes.Structure):
                 ("so", ctypes.POINTER(ctypes.c_int))]
 
 
+class _Mixture(ctypes.Structure):
+    _fields_ = [("kmix", ctypes.c_int), ("c0", ctypes.c_double),
+                ("a", ctypes.c_double * 32), ("b", ctypes.c_double * 32)]
+
+
+def mixture_coefficients(L, nk, rho=0.5):
+    """Mixture w = rho + sum_k c_k (|p_k|^2 / n_k)^(L / k), k = 1..L, with
+    c_k = (1 - rho) / (L Gamma(L / k + 1)) (each term has mean (1 - rho) / L when |p_k|^2 / n_k
+    is a unit exponential), as (kmix, c0, a, b) with term_k = exp(a_k ln|p_k|^2 + b_k)."""
+    from math import lgamma, log
+    a = [L / k for k in range(1, L + 1)]
+    b = [log((1.0 - rho) / L) - lgamma(a[k - 1] + 1.0) - a[k - 1] * log(nk[k - 1]) for k in range(1, L + 1)]
+    return L, float(rho), a, b
+
+
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
@@ -116,6 +131,38 @@ class Oracle:
         Sb = [x / n for x in self.split_blocks(S)]
         Hb = [x / n for x in self.split_blocks(H)]
         return Sb, Hb, acc.value, r2.value, fp
+
+    def mixture_weights(self, mix, pos):
+        """w(z) of each configuration (pos: n x N x 2) for mix = (kmix, c0, a, b)"""
+        kmix, c0, a, b = mix
+        mx = _Mixture(int(kmix), float(c0), (ctypes.c_double * 32)(*a), (ctypes.c_double * 32)(*b))
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        w = np.zeros(pos.shape[0])
+        if lib().or_mixture_weights(ctypes.byref(self.sys), ctypes.byref(mx), _dp(pos),
+                                    ctypes.c_long(pos.shape[0]), _dp(w)) != 0:
+            raise ValueError("mixture: kmix must be in 1..31")
+        return w
+
+    def mixture_run(self, mix, walkers, burnin, samples, step, seed, thin=1, threads=0, want_pos=False):
+        """Chains on w |Psi_L|^2 (mix = (kmix, c0, a, b)); returns S, H blocks normalised by
+        sum 1/w, that sum, the acceptance and optionally the final positions."""
+        kmix, c0, a, b = mix
+        mx = _Mixture(int(kmix), float(c0), (ctypes.c_double * 32)(*a), (ctypes.c_double * 32)(*b))
+        S = np.zeros(2 * self.nblock)
+        H = np.zeros(2 * self.nblock)
+        acc = ctypes.c_double()
+        ws = ctypes.c_double()
+        fp = np.zeros((walkers, self.N, 2)) if want_pos else None
+        rc = lib().or_mixture_run(ctypes.byref(self.sys), ctypes.byref(self.cb), ctypes.byref(mx),
+                                  ctypes.c_int(walkers), ctypes.c_int(burnin), ctypes.c_int(samples),
+                                  ctypes.c_int(thin), ctypes.c_double(step), ctypes.c_uint64(seed),
+                                  ctypes.c_int(threads), _dp(S), _dp(H), ctypes.byref(ws), ctypes.byref(acc),
+                                  _dp(fp) if want_pos else None)
+        if rc != 0:
+            raise ValueError("mixture: kmix must be in 1..min(31, K)")
+        Sb = [x / ws.value for x in self.split_blocks(S)]
+        Hb = [x / ws.value for x in self.split_blocks(H)]
+        return Sb, Hb, ws.value, acc.value, fp
 
     def plain_positions(self, walkers, burnin, samples, step, seed, thin=1):
         out = np.zeros((walkers, samples, self.N, 2))

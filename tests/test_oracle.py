@@ -233,6 +233,78 @@ def test_rowwise_estimator_small_n(oracle_lib):
     assert 0.2 < acc < 0.9
 
 
+def test_mixture_chain_without_tail_terms_is_the_plain_chain(oracle_lib):
+    """w = c0 (every tail term underflows to 0): the mixture chain makes the plain chain's moves
+    and its 1/w-weighted, sum(1/w)-normalised S and H are the plain means."""
+    N, m = 8, 3
+    o = oracle_lib.Oracle(N, m, [1, 2, 3], w=0.05, eps=1e-4)
+    mix = (3, 0.5, [3.0, 1.5, 1.0], [-1e300] * 3)
+    Sp, Hp, acc_p, _, pos_p = o.plain_run(16, 20, 40, 0.8, 77, want_pos=True)
+    Sm, Hm, ws, acc_m, pos_m = o.mixture_run(mix, 16, 20, 40, 0.8, 77, want_pos=True)
+    assert acc_m == acc_p and np.array_equal(pos_m, pos_p)
+    assert ws == pytest.approx(16 * 40 / 0.5, rel=1e-15)
+    for a, b in zip(Sm + Hm, Sp + Hp):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
+def test_mixture_density_bounds_every_monomial(oracle_lib):
+    """Weighted AM-GM: with t_k = c_k (|p_k|^2 / n_k)^(L/k) and w = rho + sum_k t_k, every
+    partition lambda of every sector up to L has |p_lambda|^2 / w <= prod_k n_k^(r_k)
+    max(1 / rho, max_k (k r_k / L) / c_k).  Checked on configurations far into the tails (the
+    plain estimator's integrand |p_1|^(2 L) is unbounded there)."""
+    N, m, L = 12, 3, 6
+    o = oracle_lib.Oracle(N, m, list(range(1, L + 1)))
+    rng = np.random.default_rng(5)
+    nk = rng.uniform(0.2, 2.0, size=L)
+    rho = 0.3
+    kmix, c0, a, b = oracle_lib.mixture_coefficients(L, nk, rho)
+    ck = np.exp(np.array(b) + np.array(a) * np.log(nk))
+    worst = 0.0
+    for trial in range(400):
+        pos = rng.normal(size=(N, 2)) * o.R * rng.uniform(0.05, 3.0)
+        if trial % 4 == 0:
+            pos[:, 0] += o.R * rng.uniform(0.5, 4.0)          # centre of mass far out: |p_1| large
+        w = o.mixture_weights((kmix, c0, a, b), pos[None])[0]
+        pl, _ = o.features(np.ascontiguousarray(pos.reshape(-1)))
+        for t, (lam, pairs) in enumerate(o.basis.states):
+            r = dict(pairs)
+            bound = np.prod([nk[k - 1] ** q for k, q in r.items()]) * max(
+                [1.0 / rho] + [(k * q / L) / ck[k - 1] for k, q in r.items()])
+            ratio = abs(pl[t]) ** 2 / w / bound
+            assert lam <= L and ratio <= 1.0 + 1e-9, (trial, st, ratio)
+            worst = max(worst, ratio)
+    assert worst > 1e-3          # the configurations reach the bound's scale
+
+
+def test_mixture_chain_vs_exact_small_n(oracle_lib):
+    """N = 3, m = 3, dL = 1, 2, 3: S and H of the mixture chain (sums of p_a* p_b / w over sums
+    of 1 / w) and the spectra agree with the exact expansion."""
+    N, m, sectors = 3, 3, [1, 2, 3]
+    w, eps = 0.05, 2e-3
+    o = oracle_lib.Oracle(N, m, sectors, w=w, eps=eps)
+    Se, He = exact_blocks(N, m, o.basis.states, w, eps)
+    Se, He = np.array(Se, dtype=complex), np.array(He, dtype=complex)
+    pos, _ = o.plain_positions(256, 200, 20, 0.9, 3)
+    u = (pos[..., 0] + 1j * pos[..., 1]) / o.R
+    nk = [float(np.mean(np.abs((u ** k).sum(-1)) ** 2)) for k in range(1, 4)]
+    mix = oracle_lib.mixture_coefficients(3, nk, 0.5)
+    runs = [o.mixture_run(mix, 128, 300, 1000, 0.9, 500 + seed) for seed in range(6)]
+    off = 0
+    for s, d in enumerate(o.dims):
+        Sx, Hx = Se[off:off + d, off:off + d], He[off:off + d, off:off + d]
+        off += d
+        Ss = np.array([r[0][s] for r in runs])
+        Hs = np.array([r[1][s] for r in runs])
+        for est, ex in ((Ss, Sx), (Hs, Hx)):
+            err = est.std(0, ddof=1) / np.sqrt(len(runs))
+            dev = np.abs(est.mean(0) - ex)
+            assert np.all(dev <= 5 * np.abs(err) + 1e-9 * np.abs(ex).max()), (s, dev.max(), np.abs(err).max())
+        ev = np.array([oracle_lib.geneig(Ss[i], Hs[i]) for i in range(len(runs))])
+        exact = scipy.linalg.eigh(Hx, Sx, eigvals_only=True)
+        err = ev.std(0, ddof=1) / np.sqrt(len(runs))
+        assert np.all(np.abs(ev.mean(0) - exact) < 5 * err + 1e-9), (s, ev.mean(0), exact, err)
+
+
 def _block_system(rng, dims, E0=7.0, spread=0.3):
     D = sum(dims)
     M = np.zeros((D, D), complex)

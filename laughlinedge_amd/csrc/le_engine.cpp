@@ -180,7 +180,9 @@ struct le_engine {
     int mix_L = 0;                   // largest sector dL: the terms k = 1..L
     int set_mixture(const double* nk, int n, double rho);
     int calibrate_mixture();
-    void mix_features(le::FeatureArgs& f) const;
+    // mixture weights in the features; walkers: the rows are the walkers' current samples, whose
+    // p_k start the next sweep launch
+    void mix_features(le::FeatureArgs& f, bool walkers) const;
     int gram_spread = 8;             // XCDs sharing one row split's Gram tiles (1, 2, 4 or 8)
     bool need_spread_tune = false;   // gram_xcd_spread = 0 and no choice cached for this layout yet
     long long block_grams = 0;       // Gram launches into the open block (the tuning needs none yet)
@@ -232,8 +234,9 @@ struct le_engine {
     const void* weight_ptr(int wid, int buf = 0) const;
     long item_offset(int wid, int s1, int s2) const;
     int autotune_lanes();
+    // mix_fresh: mix_pk already holds the p_k of `positions` (written by the last features launch)
     int run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
-                   const unsigned long long* sweep_base = nullptr);
+                   const unsigned long long* sweep_base = nullptr, bool mix_fresh = false);
     int sample_graph(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
     int capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hipGraphExec_t* out);
     int symmetry(unsigned long long sweep0, const unsigned long long* sweep_base);
@@ -857,7 +860,7 @@ extern "C" int le_basis_state(const le_engine* e, int state, int* dL, int* pairs
 }
 
 int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* positions,
-                          const unsigned long long* sweep_base)
+                          const unsigned long long* sweep_base, bool mix_fresh)
 {
     le::SweepArgs a{};
     a.sweep_base = sweep_base;
@@ -903,7 +906,7 @@ int le_engine::run_sweeps(int nsweeps, unsigned long long sweep0, void* position
         for (int done = 0; done < nsweeps; done += kMixChunk) {
             a.sweep0 = sweep0 + (unsigned long long)done;
             a.nsweeps = std::min(kMixChunk, nsweeps - done);
-            HIPCHK(le::launch_mix_init(a, prec, stream));
+            if (done > 0 || !mix_fresh) HIPCHK(le::launch_mix_init(a, prec, stream));
             if (map_reg(G)) HIPCHK(le::launch_sweep_reg(a, prec, map_lanes(G), stream));
             else HIPCHK(le::launch_sweep(a, prec, G, stream));
         }
@@ -962,12 +965,13 @@ int le_engine::calibrate_mixture()
     return set_mixture(nk.data(), mix_L, mc.mixture_rho);
 }
 
-void le_engine::mix_features(le::FeatureArgs& f) const
+void le_engine::mix_features(le::FeatureArgs& f, bool walkers) const
 {
     if (!mixture) return;
     f.mix = mixp;
     f.mix_row = mix_row;
     f.mix_acc = mix_acc;
+    f.mix_pk_out = walkers ? mix_pk : nullptr;
 }
 
 // Exact centre-of-mass heat bath + global rotation (+ dilation of the relative coordinates at
@@ -1318,7 +1322,7 @@ int le_engine::capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hi
     int rc = LE_OK;
     hipError_t h = hipSuccess;
     for (int q = 0; q < spl && rc == LE_OK && h == hipSuccess; ++q) {
-        rc = run_sweeps(mc.thin, (unsigned long long)q * (unsigned long long)mc.thin, pos, bases_d);
+        rc = run_sweeps(mc.thin, (unsigned long long)q * (unsigned long long)mc.thin, pos, bases_d, mixture);
         if (rc == LE_OK) rc = symmetry((unsigned long long)(q + 1) * (unsigned long long)mc.thin, bases_d);
         f.row0 = (long)q * mc.walkers;
         f.trace = traced ? trace_d : nullptr;
@@ -1509,7 +1513,18 @@ extern "C" int le_sample(le_engine* e, int samples)
     f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
-    e->mix_features(f);
+    e->mix_features(f, true);
+    if (e->mixture) {
+        // the first sweeps start from the walkers' p_k; every later one from its features
+        le::SweepArgs a{};
+        a.pos = e->pos;
+        a.N = e->sys.particles_number;
+        a.walkers = e->mc.walkers;
+        a.R = e->R;
+        a.mix = e->mixp;
+        a.mix_pk = e->mix_pk;
+        HIPCHK(le::launch_mix_init(a, e->prec, e->stream));
+    }
     const int tw = std::min(e->mc.trace_walkers, e->mc.walkers);
     long long kept = 0;
     // Rounds alternate between two sample buffers: the sampler stream fills buffer b while
@@ -1550,7 +1565,7 @@ extern "C" int le_sample(le_engine* e, int samples)
             hipEvent_t e0 = ev(), e1 = ev(), e2 = ev();
             if (!e0 || !e1 || !e2) return fail(LE_ERR_HIP, "event creation failed");
             HIPCHK(hipEventRecord(e0, e->stream));
-            int rc = e->run_sweeps(e->mc.thin, e->sweep_counter, e->pos);
+            int rc = e->run_sweeps(e->mc.thin, e->sweep_counter, e->pos, nullptr, e->mixture);
             if (rc) return rc;
             e->sweep_counter += (unsigned long long)e->mc.thin;
             rc = e->symmetry(e->sweep_counter, nullptr);
@@ -1909,7 +1924,7 @@ extern "C" int le_accumulate_positions(le_engine* e, const double* pos, long n)
     f.npw = e->npw;
     f.invR = 1.0 / e->R;
     f.pot = {e->sys.conf_w, e->sys.conf_eps, e->sys.conf_strength, e->sys.conf_exponent, e->R};
-    e->mix_features(f);
+    e->mix_features(f, false);
     int rc = LE_OK;
     for (long s0 = 0; s0 < n && rc == LE_OK; s0 += chunk) {
         const long m = std::min(chunk, n - s0);

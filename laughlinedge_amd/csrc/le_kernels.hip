@@ -125,8 +125,9 @@ __device__ __forceinline__ double row_propose(const RowState& st, double xo, dou
 // take u^(l+1) from an inclusive prefix product over the group (log2 G shuffle steps) and step
 // on by u^G.  Each lane sums its terms exp(a_k ln|p_k|^2 + b_k); the group sum (the same in every
 // lane: butterfly over commutative additions) plus c0 is w, and log A gains ln w' - ln w.  FP32
-// evaluates a term as exp2(a_k log2|p_k|^2 + b_k / ln 2); FP64 rounds every product on its own,
-// as the FP64 CPU restatement does.  The running p_k start each launch from launch_mix_init.
+// evaluates a term as exp2(a_k log2|p_k|^2 + b_k / ln 2) with fused multiply-adds; FP64 rounds
+// every product on its own, as the FP64 CPU restatement does.  The running p_k of a launch start
+// from launch_mix_init or from the p_k the features of the walkers' last sample wrote.
 template <int G, typename Real, int KM = kMixMax>
 struct MixLane {
     static constexpr int M = (KM + G - 1) / G;
@@ -137,15 +138,32 @@ struct MixLane {
     Real lw;                 // ln w at the current positions
 };
 
-template <typename Real>
-__device__ __forceinline__ Real mix_term(Real a, Real b, Real re, Real im)
+__device__ __forceinline__ float mix_term(float a, float b, float re, float im)
+{
+    return __builtin_amdgcn_exp2f(a * __builtin_amdgcn_logf(re * re + im * im) + b);
+}
+
+__device__ __forceinline__ double mix_term(double a, double b, double re, double im)
 {
 #pragma clang fp contract(off)
-    const Real q = re * re + im * im;
-    if constexpr (sizeof(Real) == 4)
-        return __builtin_amdgcn_exp2f(a * __builtin_amdgcn_logf(q) + b);
-    else
-        return q > 0.0 ? exp(a * log(q) + b) : 0.0;
+    const double q = re * re + im * im;
+    return q > 0.0 ? exp(a * log(q) + b) : 0.0;
+}
+
+// (ar, ai) *= (br, bi): FP32 with fused multiply-adds, FP64 with every product rounded
+__device__ __forceinline__ void mix_cmul(float& ar, float& ai, float br, float bi)
+{
+    const float r = ar * br - ai * bi;
+    ai = ar * bi + ai * br;
+    ar = r;
+}
+
+__device__ __forceinline__ void mix_cmul(double& ar, double& ai, double br, double bi)
+{
+#pragma clang fp contract(off)
+    const double r = ar * br - ai * bi;
+    ai = ar * bi + ai * br;
+    ar = r;
 }
 
 template <typename Real>
@@ -173,7 +191,7 @@ __device__ __forceinline__ void mix_load(MixLane<G, Real, KM>& m, const SweepArg
             m.pi[j] = (Real)p.y;
             m.ak[j] = (Real)a.mix.a[k - 1];
             m.bk[j] = sizeof(Real) == 4 ? (Real)(a.mix.b[k - 1] * 1.4426950408889634) : (Real)a.mix.b[k - 1];
-            local += mix_term<Real>(m.ak[j], m.bk[j], m.pr[j], m.pi[j]);
+            local += mix_term(m.ak[j], m.bk[j], m.pr[j], m.pi[j]);
         } else {
             m.pr[j] = m.pi[j] = m.ak[j] = m.bk[j] = 0;
         }
@@ -187,7 +205,6 @@ __device__ __forceinline__ Real mix_propose(const MixLane<G, Real, KM>& m, Real 
                                             double R, int lane, Real (&qr)[MixLane<G, Real, KM>::M],
                                             Real (&qi)[MixLane<G, Real, KM>::M])
 {
-#pragma clang fp contract(off)
     constexpr int M = MixLane<G, Real, KM>::M;
     Real aor, aoi, anr, ani;
     if constexpr (sizeof(Real) == 4) {
@@ -197,17 +214,15 @@ __device__ __forceinline__ Real mix_propose(const MixLane<G, Real, KM>& m, Real 
         aor = xo / R; aoi = yo / R; anr = xn / R; ani = yn / R;
     }
     Real por = aor, poi = aoi, pnr = anr, pni = ani;
+    // (lane exchanges through ds_bpermute: a DPP row-shift scan with u^G by squaring measured 15%
+    // slower in the VALU-bound register sweep)
 #pragma unroll
     for (int d = 1; d < G; d <<= 1) {
         const Real tor = __shfl_up(por, d, G), toi = __shfl_up(poi, d, G);
         const Real tnr = __shfl_up(pnr, d, G), tni = __shfl_up(pni, d, G);
         if (lane >= d) {
-            const Real o = por * tor - poi * toi;
-            poi = por * toi + poi * tor;
-            por = o;
-            const Real n = pnr * tnr - pni * tni;
-            pni = pnr * tni + pni * tnr;
-            pnr = n;
+            mix_cmul(por, poi, tor, toi);
+            mix_cmul(pnr, pni, tnr, tni);
         }
     }
     Real gor = 0, goi = 0, gnr = 0, gni = 0;
@@ -219,18 +234,14 @@ __device__ __forceinline__ Real mix_propose(const MixLane<G, Real, KM>& m, Real 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         if (j > 0) {
-            const Real o = por * gor - poi * goi;
-            poi = por * goi + poi * gor;
-            por = o;
-            const Real n = pnr * gnr - pni * gni;
-            pni = pnr * gni + pni * gnr;
-            pnr = n;
+            mix_cmul(por, poi, gor, goi);
+            mix_cmul(pnr, pni, gnr, gni);
         }
         const int k = lane + 1 + G * j;
         if (k <= m.kmix) {
             qr[j] = (m.pr[j] - por) + pnr;
             qi[j] = (m.pi[j] - poi) + pni;
-            local += mix_term<Real>(m.ak[j], m.bk[j], qr[j], qi[j]);
+            local += mix_term(m.ak[j], m.bk[j], qr[j], qi[j]);
         } else {
             qr[j] = qi[j] = 0;
         }
@@ -737,7 +748,8 @@ constexpr int kMaxPow = 128;
 constexpr int kMaxStatesLds = 4096;
 constexpr int kLps = 4;          // lanes per sample in k_psums
 
-template <typename Real, int KMAX, int NH>   // NH: harmonic pairs held (0, or at least a.nh)
+// NH: harmonic pairs held (0, or at least a.nh); MIX: mixture weights (a.mix.kmix > 0)
+template <typename Real, int KMAX, int NH, bool MIX = false>
 __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
 {
     constexpr bool HARM = NH > 0;
@@ -808,7 +820,7 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
         if (q % kLps == sub && q < 2 * a.K) ps[q] = acc[q];
     const long row = a.row0 + w;
     Real* W = reinterpret_cast<Real*>(a.W);
-    if (a.mix.kmix > 0) {
+    if constexpr (MIX) {
         // mixture estimator: w = c0 + sum_k exp(a_k ln|p_k|^2 + b_k), lane sub taking k = sub + 1 + 4 r
         double t = 0.0;
 #pragma unroll
@@ -829,6 +841,12 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
         }
         t = group_sum<kLps, double>(t);
         const double iw = 1.0 / (a.mix.c0 + t);
+        if (a.mix_pk_out) {
+#pragma unroll
+            for (int k = 0; k < KMAX && k < kMixMax; ++k)
+                if (k % kLps == sub && k < a.mix.kmix)
+                    a.mix_pk_out[(size_t)w * kMixMax + k] = make_double2(acc[2 * k], acc[2 * k + 1]);
+        }
         if (sub == 0) {
             W[row] = (Real)(acc[QV] * iw);
             W[(size_t)a.mix_row * a.wld + row] = (Real)iw;
@@ -2209,14 +2227,19 @@ static hipError_t features_h(const FeatureArgs& a, hipStream_t s)
 {
     const long threads = (long)a.walkers * kLps;
     dim3 grid_ps((unsigned)((threads + 255) / 256));
-    if (a.K <= 8)
-        hipLaunchKernelGGL((k_psums<Real, 8, NH>), grid_ps, dim3(256), 0, s, a);
-    else if (a.K <= 16)
-        hipLaunchKernelGGL((k_psums<Real, 16, NH>), grid_ps, dim3(256), 0, s, a);
-    else if (a.K <= 24)
-        hipLaunchKernelGGL((k_psums<Real, 24, NH>), grid_ps, dim3(256), 0, s, a);
-    else
+    const bool mix = NH == 0 && a.mix.kmix > 0;
+    if (a.K <= 8) {
+        if (mix) hipLaunchKernelGGL((k_psums<Real, 8, 0, true>), grid_ps, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_psums<Real, 8, NH>), grid_ps, dim3(256), 0, s, a);
+    } else if (a.K <= 16) {
+        if (mix) hipLaunchKernelGGL((k_psums<Real, 16, 0, true>), grid_ps, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_psums<Real, 16, NH>), grid_ps, dim3(256), 0, s, a);
+    } else if (a.K <= 24) {
+        if (mix) hipLaunchKernelGGL((k_psums<Real, 24, 0, true>), grid_ps, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_psums<Real, 24, NH>), grid_ps, dim3(256), 0, s, a);
+    } else {
         return hipErrorInvalidValue;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // enough waves to fill the chip, each looping over samples
@@ -2231,7 +2254,9 @@ template <typename Real>
 static hipError_t features_t(const FeatureArgs& a, hipStream_t s)
 {
     if (a.npw > kMaxPow || !a.ps || a.ps_ld < 2 * a.K) return hipErrorInvalidValue;
-    if (a.mix.kmix > 0 && (a.mix.kmix > a.K || !a.mix_acc || a.mix_row < 1 + 2L * a.nh)) return hipErrorInvalidValue;
+    // mixture weights: S and H only (no harmonic sums), p_k up to the largest term
+    if (a.mix.kmix > 0 && (a.mix.kmix > a.K || a.mix.kmix > kMixMax || !a.mix_acc || a.nh > 0 || a.mix_row < 1))
+        return hipErrorInvalidValue;
     // harmonic accumulators are FP64 registers: hold only as many pairs as the call has
     if (a.nh > kMaxHarmonics) return hipErrorInvalidValue;
     if (a.nh == 0) return features_h<Real, 0>(a, s);

@@ -96,6 +96,8 @@ struct FeatureArgs {
     MixParams mix;
     long mix_row;
     double* mix_acc;
+    double2* mix_pk_out;     // optional [walkers][kMixMax]: p_k (k <= mix.kmix) of each sample, the
+                             // mixture chains' starting values for their next sweep launch
 };
 
 // Two-sided chain features (occupation numbers): positions [walkers][N+1]; bulk power

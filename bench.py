@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 N, M_INV = 200, 3
 SECTORS = list(range(1, 17))
 W_CONF, EPS_CONF = 0.05, 5e-5
+MIX_RHO = 0.5              # mixture estimator: weight of the constant term of w
 WALKERS = 32768
 SEED = 42
 PEAK_FP32 = 157.3e12       # MI355X FP32 dense (MFMA; VALU counting packed FMA)
@@ -421,10 +422,11 @@ def compact(out, full_path):
         c["forms"] = cf
     if "rows" in out:
         c["rows"] = {k: {kk: _r(vv) for kk, vv in v.items() if isinstance(vv, (int, float))} for k, v in out["rows"].items()}
-        rw = out["rows"].get("rowwise_N200_dL1-16", {}).get("lowest_eigenvalues")
-        if rw:
-            c["rows"]["rowwise_N200_dL1-16"]["lowest_dL16"] = rw["lowest"]["16"]
-            c["rows"]["rowwise_N200_dL1-16"]["max_seed_deviation_sigma"] = rw["max_seed_deviation_sigma"]
+        for name in ("rowwise_N200_dL1-16", "mixture_N200_dL1-16"):
+            rw = out["rows"].get(name, {}).get("lowest_eigenvalues")
+            if rw:
+                c["rows"][name]["lowest_dL16"] = rw["lowest"]["16"]
+                c["rows"][name]["max_seed_deviation_sigma"] = rw["max_seed_deviation_sigma"]
     if "cpu_baseline" in out:
         cb = out["cpu_baseline"]
         c["cpu_baseline"] = {k: _r(v, 5) for k, v in cb.items() if k != "legs"}
@@ -572,6 +574,20 @@ def rows():
             "note": "one step = one sweep of every chain + its monomials + the per-row g/f sums; a full "
                     "S,H sample-update needs one sample of every one of the D rows"}
     out["rowwise_N200_dL1-16"]["lowest_eigenvalues"] = rowwise_spectra()
+    # the mixture estimator on the same workload (register sweep, FP32 tiles)
+    log("rows: mixture estimator N=200 dL=1..16")
+    steps = 200
+    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=steps + 8, seed=SEED,
+                tune_sweeps=50, step=0.5, estimator=2, mixture_rho=MIX_RHO, lanes_per_walker=8,
+                sweep_registers=1) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(300)
+        wall, ms_gram = _timed(e, 8, steps)
+        out["mixture_N200_dL1-16"] = {
+            "sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "gram_ms_per_step": ms_gram / steps, "acceptance": e.stats()["acceptance"], "rho": MIX_RHO}
+    out["mixture_N200_dL1-16"]["lowest_eigenvalues"] = mixture_spectra()
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
     steps = 40
@@ -586,6 +602,40 @@ def rows():
                                       "orbitals": lcut, "gram_ms_per_step": ms_gram / steps,
                                       "lanes_per_walker": e.info.lanes_per_walker, "acceptance": e.stats()["acceptance"]}
     return out
+
+
+def mixture_spectra(seeds=(1, 2), steps=2048):
+    """Lowest eigenvalue of every sector dL = 1..16 (N = 200) from the mixture estimator (walkers
+    on w |Psi_L|^2, S and H reweighted by 1 / w), two seeds of 32768 walkers x `steps` samples in
+    16 blocks: the values, their batch sigma, the jackknife values (leading 1/n bias removed) and
+    the seed-to-seed deviation in combined batch sigma."""
+    from laughlinedge_amd import Engine, geneig
+    from laughlinedge_amd.stats import batch_sigma, jackknife_eigenvalues
+    runs = []
+    for seed in seeds:
+        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, estimator=2, mixture_rho=MIX_RHO,
+                    tune_sweeps=50, step=0.5, seed=seed, samples=steps, n_blocks=16, lanes_per_walker=8,
+                    sweep_registers=1) as e:
+            e.init_walkers()
+            e.tune_step()
+            e.burnin(600)
+            for _ in range(16):
+                e.sample(steps // 16)
+            sp = [e.spectra(b) for b in range(16)]
+            mean = e.spectra()
+            blocks = [e.moments(b) for b in range(16)]
+        r = {}
+        for s, dl in enumerate(SECTORS):
+            jk, jks, _ = jackknife_eigenvalues([bl[s] for bl in blocks], geneig)
+            r[str(dl)] = (float(mean[s][0]), float(batch_sigma(np.array([b[s][0] for b in sp]))), float(jk[0]),
+                          float(jks[0]))
+        runs.append(r)
+    dev = {k: abs(runs[0][k][0] - runs[1][k][0]) / float(np.hypot(runs[0][k][1], runs[1][k][1])) for k in runs[0]}
+    return {"seeds": list(seeds), "samples": WALKERS * steps, "rho": MIX_RHO,
+            "lowest": {k: [round(runs[0][k][0], 4), round(runs[0][k][1], 4)] for k in runs[0]},
+            "lowest_jackknife": {k: [round(runs[0][k][2], 4), round(runs[0][k][3], 4)] for k in runs[0]},
+            "seed_deviation_sigma": {k: round(v, 2) for k, v in dev.items()},
+            "max_seed_deviation_sigma": round(max(dev.values()), 2)}
 
 
 def rowwise_spectra(seeds=(1, 2), steps=512):

@@ -62,6 +62,16 @@ int main()
     std::printf("ROWWISE");
     for (double e : evr) std::printf(" %.12e", e);
     std::printf("\nROWSTEP %.9e\n", pr.randomStep);
+    // the mixture estimator: one population on w |Psi_L|^2, S and H reweighted by 1 / w
+    edgeMC::monteCarloParameters pm = pr;
+    pm.setGPU_threads_blocks(64, 8);
+    pm.seed = 2468;
+    pm.randomStep = 0.8f;                // the row-wise run wrote its tuned step back into pr
+    pm.estimator = LE_EST_MIXTURE;
+    std::vector<double> evm = edgeMC::spectrum_compute(pm, 4, false, 0);
+    std::printf("MIXTURE");
+    for (double e : evm) std::printf(" %.12e", e);
+    std::printf("\nMIXSTEP %.9e\n", pm.randomStep);
     std::printf("DONE\n");
     return 0;
 }

@@ -385,6 +385,16 @@ def test_cpp_shell_driver(oracle_lib, tmp_path):
     assert np.allclose(np.diag(M_r).real, 1.0, atol=1e-12)
     assert np.allclose(evr, refr, rtol=1e-9)
     assert np.allclose(evr, ev, rtol=0.02)     # same sector, other estimator and chains
+    # the mixture estimator through the shell: the Python path with the same parameters
+    evm = np.array([float(x) for x in lines["MIXTURE"]])
+    with Engine(10, 3, [4], w=0.05, eps=1e-4, walkers=512, samples=400, burnin_sweeps=200, seed=2468,
+                tune_sweeps=100, step=float(np.float32(0.8)), estimator=2) as e:
+        e.run()
+        refm = e.spectra()[0]
+        assert e.get_mixture()[0] == 4
+        assert abs(e.stats()["step"] - float(lines["MIXSTEP"][0])) < 1e-6 * e.stats()["step"]
+    assert np.allclose(evm, refm, rtol=1e-9)
+    assert np.allclose(evm, ev, rtol=0.02)
     # DSF: the shell's weights vs the oracle's rotation of the same blocks
     dsf = np.array([float(x) for x in lines["DSF"]]).reshape(-1, 2)
     with Engine(10, 3, [0, 3], w=0.05, eps=1e-4, walkers=512, samples=400, burnin_sweeps=200, seed=1234,

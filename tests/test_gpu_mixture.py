@@ -200,3 +200,26 @@ def test_mixture_headline_seed_consistency():
     assert np.all(dev < 3.5), dev
     low = np.abs(m1[:4] - mp[:4]) / np.hypot(s1[:4], sp_[:4])
     assert np.all(low < 3.5), low
+
+
+def test_mixture_graph_replays_match_eager():
+    """Sampling rounds captured as hipGraphs (their sweeps start from the p_k the previous
+    sample's features wrote) give the same chains and moments as eager rounds: N = 100, 8 lanes
+    in registers, 4099 walkers (a partial last workgroup), 3 samples per round, 7 rounds and a
+    partial one."""
+    from laughlinedge_amd import Engine
+    print("case mixture graphs vs eager N=100", flush=True)
+    out = []
+    for graphs in (0, 1):
+        with Engine(100, 3, [2, 5, 8], w=0.05, eps=5e-5, walkers=4099, samples=23, seed=5, tune_sweeps=0, step=0.9,
+                    estimator=2, graphs=graphs, samples_per_launch=3, lanes_per_walker=8, sweep_registers=1) as e:
+            e.init_walkers()
+            e.burnin(40)
+            e.sample(23)
+            out.append((e.positions(), e.moments(), e.stats()["acceptance"], e.get_mixture()))
+    (p0, m0, a0, x0), (p1, m1, a1, x1) = out
+    assert x0[0] == 8 and np.array_equal(x0[2], x1[2]) and np.array_equal(x0[3], x1[3])
+    assert np.array_equal(p0, p1) and a0 == a1
+    for (S0, H0), (S1, H1) in zip(m0, m1):
+        assert np.abs(S0 - S1).max() <= 1e-12 * np.abs(S0).max()
+        assert np.abs(H0 - H1).max() <= 1e-12 * np.abs(H0).max()

@@ -144,12 +144,12 @@ def compare_spectra(spec, sig, ref_spec, ref_sig):
             "lowest_per_sector_dev": lowest}
 
 
-def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1, sym=0):
+def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1, sym=0, estimator=0):
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
                   trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs,
-                  lanes_per_walker=-1, symmetry_moves=sym)
+                  lanes_per_walker=-1, symmetry_moves=sym, estimator=estimator, mixture_rho=MIX_RHO)
 
 
 def timed_sampling(eng, steps, nblk):
@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--trace-walkers", type=int, default=512)
     ap.add_argument("--form-steps", type=int, default=96, help="kept samples per (form, thinning)")
     ap.add_argument("--thins", default="1,2,4,8")
+    ap.add_argument("--estimator", choices=("plain", "mixture"), default="plain",
+                    help="headline estimator: plain |Psi_L|^2 walkers (default) or the mixture w |Psi_L|^2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
@@ -210,7 +212,8 @@ def main():
         del a, b
 
     steps, warm, tw = args.steps, args.warmup, min(args.trace_walkers, args.walkers)
-    eng = make_engine(args.walkers, 0, steps + warm + ERR_BLOCKS * ERR_BLOCK_SAMPLES, 1, tw)
+    est = 2 if args.estimator == "mixture" else 0
+    eng = make_engine(args.walkers, 0, steps + warm + ERR_BLOCKS * ERR_BLOCK_SAMPLES, 1, tw, estimator=est)
     info, dims = eng.info, eng.dims
     sum_d2 = sum(d * d for d in dims)
     log(f"N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} walkers={args.walkers} "
@@ -266,6 +269,7 @@ def main():
         "data": "synthetic (seeded Metropolis walkers on |Psi_L|^2, no dataset)",
         "config": {"workload": "laughlin-edge N=200 nu=1/3 dL=1..16 grouped (D=914), V=0.025 r^2+5e-5 r^4",
                    "walkers": args.walkers, "seed": SEED, "burnin_sweeps": args.burnin, "thin": 1,
+                   "estimator": args.estimator,
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
                    "sweep_in_registers": bool(eng.info.sweep_in_registers),
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,

@@ -583,14 +583,15 @@ def rows():
     steps = 200
     with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=steps + 8, seed=SEED,
                 tune_sweeps=50, step=0.5, estimator=2, mixture_rho=MIX_RHO, lanes_per_walker=8,
-                sweep_registers=1) as e:
+                sweep_registers=1, symmetry_moves=1) as e:
         e.init_walkers()
         e.tune_step()
         e.burnin(300)
         wall, ms_gram = _timed(e, 8, steps)
         out["mixture_N200_dL1-16"] = {
             "sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
-            "gram_ms_per_step": ms_gram / steps, "acceptance": e.stats()["acceptance"], "rho": MIX_RHO}
+            "gram_ms_per_step": ms_gram / steps, "acceptance": e.stats()["acceptance"], "rho": MIX_RHO,
+            "moves": "single-particle sweeps + centre-of-mass random walk and rotation per kept sample"}
     out["mixture_N200_dL1-16"]["lowest_eigenvalues"] = mixture_spectra()
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
@@ -610,16 +611,17 @@ def rows():
 
 def mixture_spectra(seeds=(1, 2), steps=2048):
     """Lowest eigenvalue of every sector dL = 1..16 (N = 200) from the mixture estimator (walkers
-    on w |Psi_L|^2, S and H reweighted by 1 / w), two seeds of 32768 walkers x `steps` samples in
-    16 blocks: the values, their batch sigma, the jackknife values (leading 1/n bias removed) and
-    the seed-to-seed deviation in combined batch sigma."""
+    on w |Psi_L|^2 with its centre-of-mass random walk and rotation after every kept sample, S and H
+    reweighted by 1 / w), two seeds of 32768 walkers x `steps` samples in 16 blocks: the values,
+    their batch sigma, the jackknife values (leading 1/n bias removed) and the seed-to-seed
+    deviation in combined batch sigma."""
     from laughlinedge_amd import Engine, geneig
     from laughlinedge_amd.stats import batch_sigma, jackknife_eigenvalues
     runs = []
     for seed in seeds:
         with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, estimator=2, mixture_rho=MIX_RHO,
                     tune_sweeps=50, step=0.5, seed=seed, samples=steps, n_blocks=16, lanes_per_walker=8,
-                    sweep_registers=1) as e:
+                    sweep_registers=1, symmetry_moves=1) as e:
             e.init_walkers()
             e.tune_step()
             e.burnin(600)

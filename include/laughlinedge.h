@@ -106,7 +106,10 @@ typedef struct le_mc_params {
                                   its exact Gaussian and rotate by a uniform angle; 2: also redraw
                                   the scale of the coordinates relative to the centre of mass from
                                   its exact Gamma law (all exact moves of |Psi_L|^2, edge engine
-                                  only); 0 (default): single-particle moves only, as the reference */
+                                  only); 0 (default): single-particle moves only, as the reference.
+                                  With LE_EST_MIXTURE, 1 is a centre-of-mass random walk (variance
+                                  4 / N per component) accepted with the |Psi_L|^2 and w ratios,
+                                  then a uniform rotation (2 is refused) */
     int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (edge
                                   engine: lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25,
                                   plain or learned proposal; occupation engine: ceil((N - 1) / lanes)
@@ -132,7 +135,7 @@ typedef struct le_mc_params {
                                   By weighted AM-GM every |p_lambda|^2 of every sector is at most
                                   a constant times w, so the reweighted integrands are bounded
                                   (the plain estimator's |p_1|^(2 dL) tails are not).  S and H
-                                  only (no harmonics, user weights, symmetry moves or learned
+                                  only (no harmonics, user weights, dilation moves or learned
                                   proposal); register or LDS sweeps */
     int gram_xcd_spread;       /* XCDs that share one row split's S/H Gram tiles: 1, 2, 4 or 8
                                   (8: consecutive workgroups take consecutive tiles, spreading every

@@ -700,8 +700,8 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->estimator != LE_EST_PLAIN && mc->estimator != LE_EST_ROWWISE && mc->estimator != LE_EST_MIXTURE)
         return fail(LE_ERR_ARG, "estimator must be LE_EST_PLAIN, LE_EST_ROWWISE or LE_EST_MIXTURE");
     if (mc->estimator == LE_EST_MIXTURE) {
-        if (mc->symmetry_moves || (obs && (obs->n_harmonics || obs->n_user_weights)))
-            return fail(LE_ERR_ARG, "the mixture estimator takes no symmetry moves or extra observables");
+        if (mc->symmetry_moves > 1 || (obs && (obs->n_harmonics || obs->n_user_weights)))
+            return fail(LE_ERR_ARG, "the mixture estimator takes no dilation moves or extra observables");
         if (!(mc->mixture_rho > 0.0 && mc->mixture_rho < 1.0)) return fail(LE_ERR_ARG, "mixture_rho must be in (0, 1)");
         int L = 0;
         for (int i = 0; i < n_sectors; ++i) L = std::max(L, sectors[i]);
@@ -988,6 +988,13 @@ int le_engine::symmetry(unsigned long long sweep0, const unsigned long long* swe
     a.m = sys.inverse_filling;
     a.walkers = mc.walkers;
     a.dilate = mc.symmetry_moves >= 2;
+    if (mixture) {
+        if (mixp.kmix == 0) return LE_OK;   // before the density is set the chains are plain
+        a.mix = mixp;
+        a.R = R;
+        HIPCHK(le::launch_symmetry_mix(a, prec, stream));
+        return LE_OK;
+    }
     HIPCHK(le::launch_symmetry(a, prec, stream));
     return LE_OK;
 }

@@ -714,6 +714,120 @@ __global__ __launch_bounds__(64) void k_symmetry(SweepArgs a)
     }
 }
 
+// Exact moves of the mixture density w |Psi_L|^2, 16 lanes per walker (4 walkers per wavefront).
+// Translating every particle by d leaves the Jastrow factor unchanged and multiplies |Psi_L|^2
+// by exp(-(2 d.sum z + N |d|^2) / 2), so a random-walk proposal z -> z + d, d Gaussian of variance
+// 4 / N per component (twice the spread of the centre of mass under |Psi_L|^2), is accepted with
+// that factor times w(z + d) / w(z); then a rotation z -> e^{i phi} z with uniform phi, which
+// leaves every |p_k| and so w unchanged.  The walkers then cross between the bulk and the |p_1|
+// tail terms of w in a few moves instead of diffusing there by single-particle steps.  Two
+// passes over the particles (lane l takes j = l + 16 t) sum the FP64 power sums of z and of
+// z + d (one butterfly per p_k); lane k evaluates term k + 1 of w, so each lane takes one log
+// and one exp per configuration.  Every lane of a walker holds the same sums and takes the same
+// decision.  Random numbers: Philox block (walker, sweep_lo, sweep_hi16 | tag 6 << 16,
+// 0xfffffff1).
+constexpr int kMixLanes = 16;
+
+template <typename Real, int KM>
+__device__ __forceinline__ double mix_ln_w_pass(const SweepArgs& a, const typename V2T<Real>::T* P, int sub, double dx,
+                                                double dy, double* sx, double* sy)
+{
+    double pr[KM], pi[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) pr[k] = pi[k] = 0.0;
+    double ax = 0.0, ay = 0.0;
+    for (int j = sub; j < a.N; j += kMixLanes) {
+        const auto q = P[j];
+        const double x = (double)q.x + dx, y = (double)q.y + dy;
+        ax += x;
+        ay += y;
+        const double ur = x / a.R, ui = y / a.R;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+            pr[k] += qr;
+            pi[k] += qi;
+            const double o = qr * ur - qi * ui;
+            qi = qr * ui + qi * ur;
+            qr = o;
+        }
+    }
+    if (sx) {
+        *sx = group_sum<kMixLanes, double>(ax);
+        *sy = group_sum<kMixLanes, double>(ay);
+    }
+    // lane sub keeps p_{sub + 1} (and p_{sub + 17} when KM > 16) and evaluates its terms
+    double t = 0.0;
+#pragma unroll
+    for (int base = 0; base < KM; base += kMixLanes) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int k = base; k < base + kMixLanes && k < KM; ++k) {
+            const double sr = group_sum<kMixLanes, double>(pr[k]), si = group_sum<kMixLanes, double>(pi[k]);
+            if (k - base == sub) {
+                re = sr;
+                im = si;
+            }
+        }
+        const int k = base + sub;
+        const double q = re * re + im * im;
+        if (k < a.mix.kmix && q > 0.0) t += exp(a.mix.a[k] * log(q) + a.mix.b[k]);
+    }
+    return log(a.mix.c0 + group_sum<kMixLanes, double>(t));
+}
+
+template <typename Real, int KM>
+__global__ __launch_bounds__(64) void k_symmetry_mix(SweepArgs a)
+{
+    using V2 = typename V2T<Real>::T;
+    constexpr int WPB = 64 / kMixLanes;
+    const int walker = blockIdx.x * WPB + (int)threadIdx.x / kMixLanes;
+    const int sub = (int)threadIdx.x % kMixLanes;
+    if (walker >= a.walkers) return;            // the 16 lanes of a walker leave together
+    const int N = a.N;
+    V2* P = reinterpret_cast<V2*>(a.pos) + (size_t)walker * N;
+    const unsigned long long sidx = a.sweep0 + (a.sweep_base ? *a.sweep_base : 0ull);
+    const uint32_t c1 = (uint32_t)sidx, c2 = (uint32_t)((sidx >> 32) & 0xffffu);
+    uint32_t r[4];
+    philox((uint32_t)walker, c1, c2 | (6u << 16), 0xfffffff1u, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), r);
+    double g1, g2;
+    gauss2(r[0], r[1], g1, g2);
+    const double sd = 2.0 / sqrt((double)N);
+    const double dx = g1 * sd, dy = g2 * sd;
+    double sx, sy;
+    const double lw0 = mix_ln_w_pass<Real, KM>(a, P, sub, 0.0, 0.0, &sx, &sy);
+    const double lw1 = mix_ln_w_pass<Real, KM>(a, P, sub, dx, dy, nullptr, nullptr);
+    // N (|Z + d|^2 - |Z|^2) = 2 d.sum z + N |d|^2
+    const double logA = -0.5 * (2.0 * (dx * sx + dy * sy) + N * (dx * dx + dy * dy)) + (lw1 - lw0);
+    const bool acc = logA >= 0.0 || log(u01d(r[2])) < logA;
+    double sphi, cphi;
+    sincos(6.283185307179586 * u01d(r[3]), &sphi, &cphi);
+    for (int j = sub; j < N; j += kMixLanes) {
+        const V2 q = P[j];
+        const double x = (double)q.x + (acc ? dx : 0.0), y = (double)q.y + (acc ? dy : 0.0);
+        V2 o;
+        o.x = (Real)(cphi * x - sphi * y);
+        o.y = (Real)(sphi * x + cphi * y);
+        P[j] = o;
+    }
+}
+
+hipError_t launch_symmetry_mix(const SweepArgs& a, int prec, hipStream_t s)
+{
+    if (a.walkers <= 0) return hipSuccess;
+    if (a.mix.kmix < 1 || a.mix.kmix > kMixMax) return hipErrorInvalidValue;
+    constexpr int WPB = 64 / kMixLanes;
+    dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    if (a.mix.kmix <= 16) {
+        if (prec == kFP64) hipLaunchKernelGGL((k_symmetry_mix<double, 16>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_symmetry_mix<float, 16>), grid, dim3(64), 0, s, a);
+    } else {
+        if (prec == kFP64) hipLaunchKernelGGL((k_symmetry_mix<double, kMixMax>), grid, dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_symmetry_mix<float, kMixMax>), grid, dim3(64), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s)
 {
     if (a.walkers <= 0) return hipSuccess;

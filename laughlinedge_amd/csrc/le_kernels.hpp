@@ -223,6 +223,9 @@ hipError_t launch_mix_init(const SweepArgs& a, int prec, hipStream_t s);
 // exact centre-of-mass heat bath + global rotation of every walker (a.sweep0 + *a.sweep_base
 // selects the random numbers)
 hipError_t launch_symmetry(const SweepArgs& a, int prec, hipStream_t s);
+// mixture chains (a.mix): centre-of-mass random walk accepted with the |Psi_L|^2 and w ratios,
+// then a uniform rotation
+hipError_t launch_symmetry_mix(const SweepArgs& a, int prec, hipStream_t s);
 // two-sided chain: a.N is the number of particles per state (positions hold N + 1)
 hipError_t launch_sweep2(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 // two-sided chain with the N - 1 bulk particles in registers: 8 or 16 lanes, ceil((N - 1) / lanes)

@@ -36,7 +36,7 @@
  *   or_plain_positions ...... the same chain code as or_plain_run
  *   or_rowwise_sector ....... exact normalised metric and spectrum at N = 3
  *   or_mixture_run .......... equal to or_plain_run when every tail term vanishes; exact S, H
- *                             and spectra at N = 3
+ *                             and spectra at N = 3, also with the centre-of-mass moves
  *   or_mixture_weights ...... the AM-GM bound |p_lambda|^2 <= C w on configurations deep in
  *                             the tails
  *   or_geneig, or_dsf_weights scipy.linalg.eigh; residual, S-orthonormality
@@ -492,6 +492,42 @@ int or_mixture_weights(const or_system *sys, const or_mixture *mx, const double 
     return 0;
 }
 
+/* Exact move of the mixture density: every particle translated by d (Gaussian, variance 4 / N
+ * per component; |Psi_L|^2 changes by exp(-N (|Z + d|^2 - |Z|^2) / 2), Z the centre of mass),
+ * accepted with that factor times w(z + d) / w(z), then a rotation by a uniform angle (w and
+ * |Psi_L|^2 unchanged).  Philox block (walker, sweep_lo, sweep_hi16 | 6 << 16, 0xfffffff1):
+ * normals from words 0, 1, the acceptance from word 2, the angle from word 3. */
+static int mixture_com_move(chain *c, uint64_t seed, uint32_t walker, uint64_t sweep_idx)
+{
+    int N = c->N;
+    uint32_t r[4];
+    rng4(seed, 0xfffffff1u, sweep_idx, walker, 6, r);
+    double g1, g2;
+    box_muller(r[0], r[1], &g1, &g2);
+    double sd = 2.0 / sqrt((double)N);
+    double dx = g1 * sd, dy = g2 * sd;
+    double cx = 0.0, cy = 0.0;
+    for (int i = 0; i < N; ++i) { cx += c->x[i]; cy += c->y[i]; }
+    cx /= N; cy /= N;
+    cplx p0[32], p1[32];
+    for (int k = 0; k <= c->mx->kmix; ++k) p0[k] = p1[k] = 0;
+    for (int i = 0; i < N; ++i) {
+        cplx u0 = (c->x[i] + I * c->y[i]) / c->R, u1 = ((c->x[i] + dx) + I * (c->y[i] + dy)) / c->R;
+        cplx q0 = u0, q1 = u1;
+        for (int k = 1; k <= c->mx->kmix; ++k) { p0[k] += q0; p1[k] += q1; q0 *= u0; q1 *= u1; }
+    }
+    double zx = cx + dx, zy = cy + dy;
+    double logA = -0.5 * N * ((zx * zx + zy * zy) - (cx * cx + cy * cy)) + (mix_log_w(c->mx, p1) - mix_log_w(c->mx, p0));
+    int acc = logA >= 0.0 || log(u01(r[2])) < logA;
+    double phi = 6.283185307179586 * u01(r[3]), cp = cos(phi), sp = sin(phi);
+    for (int i = 0; i < N; ++i) {
+        double x = c->x[i] + (acc ? dx : 0.0), y = c->y[i] + (acc ? dy : 0.0);
+        c->x[i] = cp * x - sp * y;
+        c->y[i] = sp * x + cp * y;
+    }
+    return acc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Mixture chains on w |Psi_L|^2 (or_mixture above): sums over walkers x     */
 /* samples of p_l* p_m / w and V p_l* p_m / w (S, H, block layout of         */
@@ -500,8 +536,8 @@ int or_mixture_weights(const or_system *sys, const or_mixture *mx, const double 
 /* running p_k per move like the row-wise chains.                           */
 /* ------------------------------------------------------------------------ */
 int or_mixture_run(const or_system *sys, const or_basis *b, const or_mixture *mx, int walkers, int burnin,
-                   int samples, int thin, double step, uint64_t seed, int threads, double *S, double *H,
-                   double *wsum, double *acceptance, double *final_pos)
+                   int samples, int thin, double step, uint64_t seed, int threads, int com_moves, double *S,
+                   double *H, double *wsum, double *acceptance, double *final_pos)
 {
     int nb = or_block_size(b);
     memset(S, 0, sizeof(double) * 2 * nb);
@@ -533,6 +569,10 @@ int or_mixture_run(const or_system *sys, const or_basis *b, const or_mixture *mx
             for (int t = 0; t < burnin; ++t) sweep(&c, seed, (uint32_t)wk, sw++, step, pk);
             for (int s = 0; s < samples; ++s) {
                 for (int t = 0; t < thin; ++t) acc_l += sweep(&c, seed, (uint32_t)wk, sw++, step, pk);
+                if (com_moves) {
+                    mixture_com_move(&c, seed, (uint32_t)wk, sw);
+                    chain_power_sums(&c, pk);   /* the next sweep starts from the moved walker's p_k */
+                }
                 for (int i = 0; i < sys->N; ++i) { buf[2 * i] = c.x[i]; buf[2 * i + 1] = c.y[i]; }
                 double V;
                 or_features(sys, b, buf, pl, &V);

@@ -143,9 +143,11 @@ class Oracle:
             raise ValueError("mixture: kmix must be in 1..31")
         return w
 
-    def mixture_run(self, mix, walkers, burnin, samples, step, seed, thin=1, threads=0, want_pos=False):
+    def mixture_run(self, mix, walkers, burnin, samples, step, seed, thin=1, threads=0, want_pos=False,
+                    com_moves=False):
         """Chains on w |Psi_L|^2 (mix = (kmix, c0, a, b)); returns S, H blocks normalised by
-        sum 1/w, that sum, the acceptance and optionally the final positions."""
+        sum 1/w, that sum, the acceptance and optionally the final positions.  com_moves: the
+        exact centre-of-mass random walk and rotation after every kept sample's sweeps."""
         kmix, c0, a, b = mix
         mx = _Mixture(int(kmix), float(c0), (ctypes.c_double * 32)(*a), (ctypes.c_double * 32)(*b))
         S = np.zeros(2 * self.nblock)
@@ -156,7 +158,8 @@ class Oracle:
         rc = lib().or_mixture_run(ctypes.byref(self.sys), ctypes.byref(self.cb), ctypes.byref(mx),
                                   ctypes.c_int(walkers), ctypes.c_int(burnin), ctypes.c_int(samples),
                                   ctypes.c_int(thin), ctypes.c_double(step), ctypes.c_uint64(seed),
-                                  ctypes.c_int(threads), _dp(S), _dp(H), ctypes.byref(ws), ctypes.byref(acc),
+                                  ctypes.c_int(threads), ctypes.c_int(1 if com_moves else 0), _dp(S), _dp(H),
+                                  ctypes.byref(ws), ctypes.byref(acc),
                                   _dp(fp) if want_pos else None)
         if rc != 0:
             raise ValueError("mixture: kmix must be in 1..min(31, K)")

@@ -5,10 +5,12 @@ reweighted integrand is bounded, so the heavy tails that bias the plain estimato
 spectra are gone.
 
 - FP64 chains follow the FP64 CPU restatement (or_mixture_run) move for move on the same Philox
-  streams (same final positions and acceptance; S and H to rounding).
+  streams (same final positions and acceptance; S and H to rounding), also with the exact
+  centre-of-mass random walk and rotation after every kept sample.
 - Fixed samples: the weights 1 / w and V / w of the feature kernel and the sum(1 / w)
   normalisation against the oracle's w (or_mixture_weights) at FP32, FP64 and BF16 hi/mid/lo.
-- N = 3: S, H and the spectra of dL = 1..3 agree with the exact expansion at FP32 and FP64.
+- N = 3: S, H and the spectra of dL = 1..3 agree with the exact expansion at FP32 and FP64, and
+  with the mixture's exact centre-of-mass random walk and rotation after every kept sample.
 - N = 200, dL 1..16 (the headline configuration's sectors) in harmonic confinement, where
   H = w (L0 + dL + N) S holds exactly in every sector: every entry of H - w (L0 + dL + N) S is
   consistent with zero within its block error, and the lowest eigenvalue of every sector is
@@ -24,14 +26,15 @@ def _rel_fro(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_mixture_fp64_chains_match_oracle(oracle_lib):
+@pytest.mark.parametrize("sym", [0, 1])
+def test_mixture_fp64_chains_match_oracle(oracle_lib, sym):
     from laughlinedge_amd import Engine
     N, m, sectors, W = 10, 3, [1, 2, 3, 4], 64
-    print("case mixture FP64 chains vs oracle N=10 dL=1..4", flush=True)
+    print(f"case mixture FP64 chains vs oracle N=10 dL=1..4 centre-of-mass moves={sym}", flush=True)
     o = oracle_lib.Oracle(N, m, sectors, w=0.05, eps=1e-4)
     nk = [0.35, 0.7, 1.0, 1.3]
     with Engine(N, m, sectors, w=0.05, eps=1e-4, walkers=W, precision=1, estimator=2, tune_sweeps=0, step=0.5,
-                seed=1234, samples=40, lanes_per_walker=2) as e:
+                seed=1234, samples=40, lanes_per_walker=2, symmetry_moves=sym) as e:
         e.set_mixture(nk, 0.4)
         L, c0, a, b = e.get_mixture()
         ref = oracle_lib.mixture_coefficients(4, nk, 0.4)
@@ -43,7 +46,8 @@ def test_mixture_fp64_chains_match_oracle(oracle_lib):
         mom = e.moments()
         pos = e.positions()
         acc = e.stats()["acceptance"]
-    So, Ho, ws, acc_o, pos_o = o.mixture_run((L, c0, list(a), list(b)), W, 20, 40, 0.5, 1234, want_pos=True)
+    So, Ho, ws, acc_o, pos_o = o.mixture_run((L, c0, list(a), list(b)), W, 20, 40, 0.5, 1234, want_pos=True,
+                                             com_moves=bool(sym))
     same = np.abs(pos - pos_o).max(axis=(1, 2)) < 1e-9
     print(f"chains identical to the oracle's: {same.mean():.3f}; acceptance {acc:.6f} vs {acc_o:.6f}")
     assert same.all()
@@ -81,23 +85,23 @@ def test_mixture_fixed_sample_weights(oracle_lib, prec, tol):
         assert es < tol and eh < tol, (s, es, eh)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_mixture_exact_small_n(prec):
+@pytest.mark.parametrize("prec,sym", [(0, 0), (1, 0), (0, 1)])
+def test_mixture_exact_small_n(prec, sym):
     import scipy.linalg
     from laughlinedge_amd import Engine
     from oracle.exact_lll import exact_blocks
     from oracle.partitions import Basis
     N, m, sectors = 3, 3, [1, 2, 3]
     w, eps = 0.05, 2e-3
-    nb = 8
-    print(f"case mixture exact N=3 precision={prec}", flush=True)
+    nb = 16     # 8 blocks give t(7)-distributed errors: 4-5 sigma entries turn up every few runs
+    print(f"case mixture exact N=3 precision={prec} centre-of-mass moves={sym}", flush=True)
     with Engine(N, m, sectors, w=w, eps=eps, walkers=4096, precision=prec, estimator=2, tune_sweeps=40, step=0.9,
-                seed=41, samples=nb * 200, n_blocks=nb) as e:
+                seed=41, samples=nb * 100, n_blocks=nb, symmetry_moves=sym) as e:
         e.init_walkers()
         e.tune_step()
         e.burnin(400)
         for _ in range(nb):
-            e.sample(200)
+            e.sample(100)
         blocks = [e.moments(q) for q in range(nb)]
         mean = e.moments()
         spec_b = [e.spectra(q) for q in range(nb)]
@@ -171,7 +175,8 @@ def test_mixture_harmonic_identity_n200():
 
 def test_mixture_headline_seed_consistency():
     """The headline workload (N = 200, m = 3, V = 0.025 r^2 + 5e-5 r^4, dL = 1..16, 32768 walkers,
-    FP32 tiles): two seeds of 3.4e7 samples each give lowest eigenvalues that agree within 3.5
+    FP32 tiles; the mixture chains with their centre-of-mass moves): two seeds of 3.4e7 samples
+    each give lowest eigenvalues that agree within 3.5
     combined batch sigma in every sector, so the batch sigma is an honest error bar; at
     dL <= 4, where the plain estimator's integrands are light-tailed, both estimators agree."""
     from laughlinedge_amd import Engine
@@ -183,7 +188,7 @@ def test_mixture_headline_seed_consistency():
     for est, seed in ((2, 1), (2, 2), (0, 1)):
         with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=32768, precision=0, estimator=est, mixture_rho=0.5,
                     tune_sweeps=50, step=0.5, seed=seed, samples=steps, n_blocks=nb, lanes_per_walker=8,
-                    sweep_registers=1) as e:
+                    sweep_registers=1, symmetry_moves=1 if est == 2 else 0) as e:
             e.init_walkers()
             e.tune_step()
             e.burnin(600)

@@ -276,9 +276,11 @@ def test_mixture_density_bounds_every_monomial(oracle_lib):
     assert worst > 1e-3          # the configurations reach the bound's scale
 
 
-def test_mixture_chain_vs_exact_small_n(oracle_lib):
+@pytest.mark.parametrize("com", [False, True])
+def test_mixture_chain_vs_exact_small_n(oracle_lib, com):
     """N = 3, m = 3, dL = 1, 2, 3: S and H of the mixture chain (sums of p_a* p_b / w over sums
-    of 1 / w) and the spectra agree with the exact expansion."""
+    of 1 / w) and the spectra agree with the exact expansion, also with the exact centre-of-mass
+    random walk and rotation after every kept sample."""
     N, m, sectors = 3, 3, [1, 2, 3]
     w, eps = 0.05, 2e-3
     o = oracle_lib.Oracle(N, m, sectors, w=w, eps=eps)
@@ -288,7 +290,7 @@ def test_mixture_chain_vs_exact_small_n(oracle_lib):
     u = (pos[..., 0] + 1j * pos[..., 1]) / o.R
     nk = [float(np.mean(np.abs((u ** k).sum(-1)) ** 2)) for k in range(1, 4)]
     mix = oracle_lib.mixture_coefficients(3, nk, 0.5)
-    runs = [o.mixture_run(mix, 128, 300, 1000, 0.9, 500 + seed) for seed in range(6)]
+    runs = [o.mixture_run(mix, 128, 300, 1000, 0.9, 500 + seed, com_moves=com) for seed in range(6)]
     off = 0
     for s, d in enumerate(o.dims):
         Sx, Hx = Se[off:off + d, off:off + d], He[off:off + d, off:off + d]

@@ -609,7 +609,7 @@ def rows():
     return out
 
 
-def mixture_spectra(seeds=(1, 2), steps=2048):
+def mixture_spectra(seeds=(1, 2), steps=8192):
     """Lowest eigenvalue of every sector dL = 1..16 (N = 200) from the mixture estimator (walkers
     on w |Psi_L|^2 with its centre-of-mass random walk and rotation after every kept sample, S and H
     reweighted by 1 / w), two seeds of 32768 walkers x `steps` samples in 16 blocks: the values,

@@ -533,6 +533,7 @@ def rows():
     - the FP64 evolution of those eight kicks (dynamics.cpp:286-290): M dC/dt = -i (H + F(t) U_k) C,
       F = 1e-3 exp(-(t - 20)^2 / 50), 0 <= t <= 100, saved every 0.1, one observable;
     - the DSF at dL = 16 (em_dsf.cu:113-134): N = 200, sectors {0, 16}, harmonic 16, 32768 walkers;
+    - the fused step kernel (fused_step = 1) on the headline workload;
     - the occupation numbers of the dL = 0 state (gf_occupation_numbers.cu:16-239): N = 200,
       two-sided chain of 201 coordinates, 598 orbitals, 32768 walkers.
     Algorithmic FLOP per sample-update follow the Gram count of the headline (8 m n per
@@ -593,6 +594,27 @@ def rows():
             "gram_ms_per_step": ms_gram / steps, "acceptance": e.stats()["acceptance"], "rho": MIX_RHO,
             "moves": "single-particle sweeps + centre-of-mass random walk and rotation per kept sample"}
     out["mixture_N200_dL1-16"]["lowest_eigenvalues"] = mixture_spectra()
+    # the fused step kernel on the headline workload (one launch per kept sample)
+    log("rows: fused step N=200 dL=1..16")
+    steps = 200
+    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=steps + 8, seed=SEED,
+                tune_sweeps=50, step=0.5, fused_step=1) as e:
+        e.init_walkers()
+        e.tune_step()
+        e.burnin(300)
+        e.sample(8)
+        st0 = e.stats()
+        t0 = time.perf_counter()
+        e.sample(steps)
+        wall = time.perf_counter() - t0
+        st1 = e.stats()
+        out["fused_step_N200_dL1-16"] = {
+            "sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "device_ms_per_step": (st1["ms_step"] - st0["ms_step"]) / steps,
+            "acceptance": st1["acceptance"],
+            "note": "one launch per kept sample: register sweeps, power sums and monomials of the sample and "
+                    "the S/H Gram tiles of the previous one as work items (fused_step = 1); the headline "
+                    "runs the same work as separate kernels on two streams"}
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
     steps = 40

@@ -145,6 +145,18 @@ typedef struct le_mc_params {
                                   full-size launch and keep the fastest (cached per sector layout) */
     double mixture_rho;        /* LE_EST_MIXTURE: weight c0 of the constant term of w (0 < rho < 1,
                                   default 0.5; see le_set_mixture) */
+    int fused_step;            /* 1: le_sample runs one fused launch per kept sample (no reference
+                                  counterpart; em_spectrum.cu:139-208 also fuses its sampler in one
+                                  kernel): the sweeps, power sums and monomials of the sample and
+                                  the S/H Gram tiles of the previous one as work items of one
+                                  launch, so that VALU-bound sweeps and MFMA-bound tiles share the
+                                  CUs.  Same chains and samples as the separate kernels.  Needs the
+                                  plain estimator, FP32, particles in registers at 8 lanes per
+                                  walker (ceil(N / 8) = 13 or 25), K <= 16, S and H only (no
+                                  harmonics or user weights), no symmetry moves and the plain
+                                  proposal; le_sample fails with LE_ERR_ARG otherwise.
+                                  0 (default): separate kernels on two streams */
+    int fused_rows;            /* rows per split of the fused Gram items (a multiple of 32; 0: 2048) */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -213,6 +225,9 @@ typedef struct le_run_stats {
     double ms_gram_launch_min; /* shortest full-size Gram launch (samples_per_launch x walkers rows)
                                   of le_sample so far, 0 before the first; the last launch of each
                                   call runs with no sampling beside it, so this is the kernel alone */
+    double ms_step;            /* device time of the fused launches (fused_step = 1), the closing
+                                  Gram launch of each le_sample call included */
+    long long step_launches;   /* fused launches so far */
 } le_run_stats;
 
 void le_mc_params_default(le_mc_params* p);

@@ -228,6 +228,10 @@ struct le_engine {
     };
     std::vector<SampleGraph> graphs;
     unsigned long long* bases_d = nullptr;   // [0] sweep base, [1] trace row base
+    // fused steps (mc.fused_step): the work queues of launch_step, zero between launches
+    unsigned* step_q = nullptr;
+    int fused_check() const;
+    int sample_fused(int samples, le::FeatureArgs f, int tw);
 
     ~le_engine();
     int setup(const le_observables* obs);
@@ -324,6 +328,7 @@ le_engine::~le_engine()
     (void)hipFree(eig_work);
     (void)hipFree(mix_pk);
     (void)hipFree(mix_acc);
+    (void)hipFree(step_q);
     if (mix_acc_h) (void)hipHostFree(mix_acc_h);
     for (auto e : ev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -353,6 +358,8 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->estimator = LE_EST_PLAIN;
     p->gram_xcd_spread = 0;
     p->mixture_rho = 0.5;
+    p->fused_step = 0;
+    p->fused_rows = 0;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -637,6 +644,10 @@ int le_engine::setup(const le_observables* obs)
         trace_cap_rows = (long)std::max(1, mc.samples) * tw;
         HIPCHK(dalloc(&trace_d, (size_t)trace_cap_rows * 9));
     }
+    if (mc.fused_step) {
+        HIPCHK(dalloc(&step_q, (size_t)le::kStepQueueWords));
+        HIPCHK(hipMemsetAsync(step_q, 0, sizeof(unsigned) * le::kStepQueueWords, stream));
+    }
     if (rowwise) {
         const long be = basis.block_elems();
         std::vector<int> rs(std::max(1, D)), so(basis.sector_offset.begin(), basis.sector_offset.end());
@@ -710,6 +721,14 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->estimator == LE_EST_ROWWISE &&
         (mc->symmetry_moves || mc->sweep_registers || (obs && (obs->n_harmonics || obs->n_user_weights))))
         return fail(LE_ERR_ARG, "the row-wise estimator takes no symmetry moves, register sweeps or extra observables");
+    if (mc->fused_step != 0 && mc->fused_step != 1) return fail(LE_ERR_ARG, "fused_step must be 0 or 1");
+    if (mc->fused_rows < 0 || mc->fused_rows % 32) return fail(LE_ERR_ARG, "fused_rows must be 0 or a positive multiple of 32");
+    if (mc->fused_step &&
+        (mc->estimator != LE_EST_PLAIN || mc->precision != LE_PREC_FP32 || mc->symmetry_moves ||
+         (mc->lanes_per_walker != 0 && mc->lanes_per_walker != -1 && mc->lanes_per_walker != 8) ||
+         (obs && (obs->n_harmonics || obs->n_user_weights))))
+        return fail(LE_ERR_ARG, "fused_step needs the plain estimator, FP32, no symmetry moves, 8 lanes per walker "
+                                "and S, H only");
     auto e = new le_engine();
     e->sys = *sys;
     e->mc = *mc;
@@ -765,6 +784,16 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     e->need_spread_tune = mc->gram_xcd_spread == 0;
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
+    if (mc->fused_step) {
+        // the fused kernel's sweep: particles in registers at 8 lanes per walker
+        if (!le::step_supported(8, sys->particles_number, e->basis.K)) {
+            delete e;
+            return fail(LE_ERR_ARG, "fused_step needs ceil(N / 8) = 13 or 25 and power sums up to K <= 16");
+        }
+        e->G = 8 | kRegSweep;
+        e->mc.lanes_per_walker = 8;
+        e->mc.sweep_registers = 1;
+    }
     *out = e;
     return LE_OK;
 }
@@ -1481,6 +1510,104 @@ int le_engine::sample_rowwise(int samples)
     return LE_OK;
 }
 
+// What the fused kernel leaves out, checked at every le_sample (the proposal can change after
+// creation).
+int le_engine::fused_check() const
+{
+    if (proposal != LE_PROPOSAL_PLAIN) return fail(LE_ERR_ARG, "fused_step runs the plain proposal only");
+    if (wfn || n_user || nh || weights_gram || two_sided || rowwise || mixture)
+        return fail(LE_ERR_ARG, "fused_step accumulates S and H of the plain estimator only");
+    if (jobs.size() != 1 || jobs[0].nw != 2 || batches.size() != 1 || gram_prec != le::kFP32 || prec != LE_PREC_FP32)
+        return fail(LE_ERR_ARG, "fused_step needs one FP32 S/H Gram job");
+    if (G != (8 | kRegSweep) || !step_q) return fail(LE_ERR_STATE, "fused_step engine without its sweep mapping");
+    return LE_OK;
+}
+
+// Kept samples by fused launches (launch_step): launch q sweeps the walkers and writes sample q
+// into buffer q mod 2 while its Gram items accumulate sample q - 1 from the other buffer; a
+// plain Gram launch closes the call with the last sample.  Each sample fills rows 0..walkers-1
+// of its buffer.
+int le_engine::sample_fused(int samples, le::FeatureArgs f, int tw)
+{
+    int rc = fused_check();
+    if (rc) return rc;
+    const Job& j = jobs[0];
+    le::StepArgs a{};
+    a.sw.pos = pos;
+    a.sw.accepted = acc;
+    a.sw.seed = mc.seed;
+    a.sw.N = sys.particles_number;
+    a.sw.m = sys.inverse_filling;
+    a.sw.walkers = mc.walkers;
+    a.sw.nsweeps = mc.thin;
+    a.sw.step = step;
+    a.units = (mc.walkers + 63) / 64;
+    a.queue = step_q;
+    a.g.ldx = ldx;
+    a.g.nw = 2;
+    a.g.tiles = j.tiles;
+    a.g.ntiles = j.ntiles;
+    a.g.C0 = j.C[0];
+    a.g.C1 = j.C[1];
+    a.g.ldc = ldx;
+    const long rps = mc.fused_rows > 0 ? mc.fused_rows : 2048;
+    a.g.rows_per_split = rps;
+    a.g.splits = (int)((mc.walkers + rps - 1) / rps);
+    f.row0 = 0;
+    evused = 0;
+    hipEvent_t e0 = next_event(), e1 = next_event();
+    if (!e0 || !e1) return fail(LE_ERR_HIP, "event creation failed");
+    HIPCHK(hipEventRecord(e0, stream));
+    for (int q = 0; q < samples; ++q) {
+        const int b = q & 1;
+        a.sw.sweep0 = sweep_counter;
+        a.f = f;
+        a.f.X = Xb[b];
+        a.f.W = Wbb[b];
+        if (trace_d && trace_rows + tw <= trace_cap_rows) {
+            a.f.trace = trace_d;
+            a.f.trace_walkers = tw;
+            a.f.trace_row0 = trace_rows;
+            trace_rows += tw;
+        } else {
+            a.f.trace = nullptr;
+            a.f.trace_walkers = 0;
+        }
+        a.g.rows = q > 0 ? mc.walkers : 0;
+        a.g.X = Xb[1 - b];
+        a.g.w0 = weight_ptr(j.wid[0], 1 - b);
+        a.g.w1 = weight_ptr(j.wid[1], 1 - b);
+        HIPCHK(le::launch_step(a, 8, stream));
+        sweep_counter += (unsigned long long)mc.thin;
+        st.step_launches += 1;
+        st.sweep_launches += 1;
+        if (q > 0) st.gram_launches += 1;
+    }
+    rc = gram_rows(mc.walkers, (samples - 1) & 1, stream);
+    if (rc) return rc;
+    st.gram_launches += 1;
+    ++block_grams;
+    HIPCHK(hipEventRecord(e1, stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    st.ms_step += ms;
+    sweeps_done += (long long)samples * mc.thin;
+    const long long nsamp = (long long)samples * mc.walkers;
+    HIPCHK(hipMemcpyAsync(acc_h, acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream));
+    rc = close_block(nsamp);
+    if (rc) return rc;
+    double tot = 0;
+    for (int w = 0; w < mc.walkers; ++w) tot += acc_h[w];
+    acc_sampling += (unsigned long long)tot;
+    acc_moves += (long long)samples * mc.thin * npos() * mc.walkers;
+    st.acceptance = (double)acc_sampling / (double)std::max<long long>(1, acc_moves);
+    st.samples += nsamp;
+    st.sweeps = sweeps_done;
+    st.step = step;
+    return LE_OK;
+}
+
 extern "C" int le_sample(le_engine* e, int samples)
 {
     if (!e) return fail(LE_ERR_ARG, "null engine");
@@ -1533,6 +1660,13 @@ extern "C" int le_sample(le_engine* e, int samples)
         HIPCHK(le::launch_mix_init(a, e->prec, e->stream));
     }
     const int tw = std::min(e->mc.trace_walkers, e->mc.walkers);
+    if (e->mc.fused_step) {
+        int rc = e->sample_fused(samples, f, tw);
+        if (rc) return rc;
+        const auto t1 = std::chrono::steady_clock::now();
+        e->st.ms_sample_total += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        return LE_OK;
+    }
     long long kept = 0;
     // Rounds alternate between two sample buffers: the sampler stream fills buffer b while
     // the Gram stream accumulates the other one; events order the reuse of each buffer.
@@ -2459,6 +2593,7 @@ extern "C" int le_set_proposal(le_engine* e, int kind)
         return LE_OK;
     }
     if (kind != LE_PROPOSAL_LEARNED) return fail(LE_ERR_ARG, "unknown proposal");
+    if (e->mc.fused_step) return fail(LE_ERR_ARG, "fused_step runs the plain proposal only");
     if (!e->table) return fail(LE_ERR_STATE, "no trained proposal (le_train_proposal or le_set_proposal_params)");
     if (e->prec != LE_PREC_FP32 || e->two_sided || e->rowwise || e->mixture)
         return fail(LE_ERR_ARG, "the learned proposal needs an FP32 edge engine with the plain estimator");

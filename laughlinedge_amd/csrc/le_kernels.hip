@@ -484,8 +484,10 @@ constexpr int reg_sweep_waves()
          : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 100 ? 5 : (2 * S * (int)(sizeof(Real) / 4) + 40) <= 128 ? 4 : 3;
 }
 
+// The sweeps of the 64 / G walkers of wavefront unit `unit` (walkers unit * 64 / G ..), lane
+// `tid` (0..63) of the wavefront: the body of k_sweep_reg, shared with the fused step kernel.
 template <int G, int S, typename Real, int MIXK = 0>   // MIXK: 0, or the largest mixture k held
-__global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
+__device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit, int tid)
 {
     constexpr bool MIX = MIXK > 0;
     constexpr int KM = MIX ? MIXK : 1;
@@ -494,10 +496,11 @@ __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
     constexpr int C = LogOps<Real>::chunk;
     constexpr int FULL = (S / C) * C;   // slots covered by whole chunks
     const int N = a.N;                  // (S - 1) G < N <= S G, checked by the launcher
-    const int lane = threadIdx.x % G;
-    const int grp = threadIdx.x / G;
-    const int walker = blockIdx.x * WPB + grp;
-    if (walker >= a.walkers) return;    // whole groups leave; shuffles stay within a group
+    const int lane = tid % G;
+    const int grp = tid / G;
+    const long wl = unit * WPB + grp;
+    if (wl >= a.walkers) return;        // whole groups leave; shuffles stay within a group
+    const int walker = (int)wl;
     V2* gpos = reinterpret_cast<V2*>(a.pos) + (size_t)walker * N;
     Real px[S], py[S];
     static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
@@ -633,6 +636,12 @@ __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
         }
     });
     if (lane == 0) a.accepted[walker] += accepted;
+}
+
+template <int G, int S, typename Real, int MIXK = 0>
+__global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
+{
+    sweep_reg_walkers<G, S, Real, MIXK>(a, blockIdx.x, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -863,17 +872,15 @@ constexpr int kMaxStatesLds = 4096;
 constexpr int kLps = 4;          // lanes per sample in k_psums
 
 // NH: harmonic pairs held (0, or at least a.nh); MIX: mixture weights (a.mix.kmix > 0)
+// Lane `sub` of the kLps lanes of sample w (w < a.walkers; the lanes of a sample are
+// consecutive and aligned): the body of k_psums, shared with the fused step kernel.
 template <typename Real, int KMAX, int NH, bool MIX = false>
-__global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
+__device__ __forceinline__ void psums_sample(const FeatureArgs& a, long w, int sub)
 {
     constexpr bool HARM = NH > 0;
     constexpr int NQ = 2 * KMAX + 1 + 2 * NH;   // Re/Im p_1..p_K, V, cos/sin sums
     constexpr int QV = 2 * KMAX;
     constexpr int QH = 2 * KMAX + 1;
-    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long w = gt / kLps;
-    const int sub = (int)(gt % kLps);
-    if (w >= a.walkers) return;                  // the 4 lanes of a sample leave together
     const int nh = HARM ? a.nh : 0;
     double acc[NQ];
 #pragma unroll
@@ -983,6 +990,78 @@ __global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
     }
 }
 
+template <typename Real, int KMAX, int NH, bool MIX = false>
+__global__ __launch_bounds__(256) void k_psums(FeatureArgs a)
+{
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long w = gt / kLps;
+    if (w >= a.walkers) return;                  // the 4 lanes of a sample leave together
+    psums_sample<Real, KMAX, NH, MIX>(a, w, (int)(gt % kLps));
+}
+
+// Monomials of sample w by one wavefront (lane 0..63): pkw (32 double2) and ptw (kMaxPow
+// entries) are the wavefront's LDS scratch; the product plans come from LDS (plan_lds) or from
+// global memory.  The body of k_monomials, shared with the fused step kernel.
+template <typename Real>
+__device__ __forceinline__ void monomials_sample(const FeatureArgs& a, long w, int lane, double2* pkw,
+                                                 typename V2T<Real>::T* ptw, const unsigned long long* plan,
+                                                 const uint32_t* cols, bool plan_lds)
+{
+    using C2 = typename V2T<Real>::T;
+    const int D = a.D;
+    const double* ps = a.ps + (size_t)w * a.ps_ld;
+    if (lane < a.K) pkw[lane] = make_double2(ps[2 * lane], ps[2 * lane + 1]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // power table: entry e = p_k^r, one lane per entry (binary powering)
+    for (int e = lane; e < a.npw; e += 64) {
+        const int k = a.pw_k[e];
+        unsigned r = (unsigned)a.pw_r[e];
+        const double2 b0 = pkw[k - 1];
+        double br = b0.x, bi = b0.y, pr = 1.0, pi = 0.0;
+        while (r) {
+            if (r & 1u) {
+                const double t = pr * br - pi * bi;
+                pi = pr * bi + pi * br;
+                pr = t;
+            }
+            r >>= 1;
+            if (r) {
+                const double t = br * br - bi * bi;
+                bi = 2.0 * br * bi;
+                br = t;
+            }
+        }
+        ptw[e] = C2{(Real)pr, (Real)pi};
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const long row = a.row0 + w;
+    Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
+#pragma unroll 2
+    for (int t = lane; t < D; t += 64) {
+        const unsigned long long pl = plan_lds ? plan[t] : a.prod[t];
+        const uint32_t cw = plan_lds ? cols[t] : ((uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16));
+        const int cnt = (int)(pl & 0xfull);
+        Real vr = 1, vi = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q < cnt) {
+                const C2 f = ptw[(pl >> (4 + 7 * q)) & 0x7full];
+                const Real tr = vr * f.x - vi * f.y;
+                vi = vr * f.y + vi * f.x;
+                vr = tr;
+            }
+        }
+        const int c = (int)(cw & 0xffffu);
+        X[c] = vr;
+        X[c + (int)(cw >> 16)] = vi;
+    }
+    __builtin_amdgcn_wave_barrier();   // pk / ptab are reused by the next sample
+}
+
 template <typename Real>
 __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
 {
@@ -1005,59 +1084,8 @@ __global__ __launch_bounds__(256) void k_monomials(FeatureArgs a)
     double2* pkw = pk + wv * 32;
     C2* ptw = ptab + wv * kMaxPow;
     const int waves = gridDim.x * 4;
-    for (int w = blockIdx.x * 4 + wv; w < a.walkers; w += waves) {
-        const double* ps = a.ps + (size_t)w * a.ps_ld;
-        if (lane < a.K) pkw[lane] = make_double2(ps[2 * lane], ps[2 * lane + 1]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // power table: entry e = p_k^r, one lane per entry (binary powering)
-        for (int e = lane; e < a.npw; e += 64) {
-            const int k = a.pw_k[e];
-            unsigned r = (unsigned)a.pw_r[e];
-            const double2 b0 = pkw[k - 1];
-            double br = b0.x, bi = b0.y, pr = 1.0, pi = 0.0;
-            while (r) {
-                if (r & 1u) {
-                    const double t = pr * br - pi * bi;
-                    pi = pr * bi + pi * br;
-                    pr = t;
-                }
-                r >>= 1;
-                if (r) {
-                    const double t = br * br - bi * bi;
-                    bi = 2.0 * br * bi;
-                    br = t;
-                }
-            }
-            ptw[e] = C2{(Real)pr, (Real)pi};
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const long row = a.row0 + w;
-        Real* X = reinterpret_cast<Real*>(a.X) + (size_t)row * a.ldx;
-#pragma unroll 2
-        for (int t = lane; t < D; t += 64) {
-            const unsigned long long pl = plan_lds ? plan[t] : a.prod[t];
-            const uint32_t cw = plan_lds ? cols[t] : ((uint32_t)a.col_re[t] | ((uint32_t)a.col_dim[t] << 16));
-            const int cnt = (int)(pl & 0xfull);
-            Real vr = 1, vi = 0;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                if (q < cnt) {
-                    const C2 f = ptw[(pl >> (4 + 7 * q)) & 0x7full];
-                    const Real tr = vr * f.x - vi * f.y;
-                    vi = vr * f.y + vi * f.x;
-                    vr = tr;
-                }
-            }
-            const int c = (int)(cw & 0xffffu);
-            X[c] = vr;
-            X[c + (int)(cw >> 16)] = vi;
-        }
-        __builtin_amdgcn_wave_barrier();   // pk / ptab are reused by the next sample
-    }
+    for (int w = blockIdx.x * 4 + wv; w < a.walkers; w += waves)
+        monomials_sample<Real>(a, w, lane, pkw, ptw, plan, cols, plan_lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1148,8 +1176,12 @@ constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 // latency: N = 200, dL 1..16, 131072 rows of whole tiles ran at 0.87 of the FP32 MFMA peak in
 // isolation, against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).  The FP64 form
 // targets 5 waves per SIMD: at 6 its two column operands spilled inside the loop.
+//
+// gram_tile is the work of one workgroup of 512 threads (all of them call it): tile `tile_idx`
+// over row split `split` for job `job_idx`; k_gram maps its workgroups onto it, the fused step
+// kernel takes (tile, split) items from a queue.
 template <typename Real, int NW>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram(GramArgs g)
+__device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int split, int job_idx)
 {
     using acc_t = typename Mfma<Real>::acc_t;
     constexpr bool kF32 = sizeof(Real) == 4;
@@ -1161,8 +1193,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
     __shared__ __align__(16) Real xw[kPre ? NW : 1][kPre ? kKC : 1][kPre ? kLdsStride : 1];
     __shared__ Real vs[kPre ? 1 : NW][kKC];
 
-    int tile_idx, split;
-    if (!gram_block(g, tile_idx, split)) return;
     const Tile tile = g.tiles[tile_idx];
     const long r_begin = (long)split * g.rows_per_split;
     long r_end = r_begin + g.rows_per_split;
@@ -1191,7 +1221,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
         }
 
     const Real* X = reinterpret_cast<const Real*>(g.X);
-    const GramJob job = g.jobs ? g.jobs[blockIdx.y] : GramJob{g.w0, g.w1, g.C0, g.C1};
+    const GramJob job = g.jobs ? g.jobs[job_idx] : GramJob{g.w0, g.w1, g.C0, g.C1};
     const Real* wp[2] = {reinterpret_cast<const Real*>(job.w0), reinterpret_cast<const Real*>(job.w1)};
     // FP32 operand A of weight q: the row side itself when the weight is 1, else its weighted copy
     const Real(*aq[2])[kLdsStride];
@@ -1304,6 +1334,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
                 const long col = tile.j0 + co[a] + l16;
                 atomicAdd(&Cs[q][row * g.ldc + col], v);
             }
+}
+
+template <typename Real, int NW>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram(GramArgs g)
+{
+    int tile_idx, split;
+    if (!gram_block(g, tile_idx, split)) return;
+    gram_tile<Real, NW>(g, tile_idx, split, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2405,6 +2443,195 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
     } else {
         if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2>), grid, dim3(512), 0, s, a);
         else hipLaunchKernelGGL((k_gram<float, 1>), grid, dim3(512), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused sampling step: one launch per kept sample holds the sweeps, power sums and monomials of
+// every walker (into the current sample's rows) and the S/H Gram tiles of the previous
+// sample's rows, as work items: sweep units (the register sweeps of 8 wavefronts' walkers,
+// then their FP64 power sums and monomials) and Gram items (one 64x64 tile over one row split).
+// Each workgroup (512 threads, at most 128 VGPRs so that two share a CU at 4 wavefronts per
+// SIMD) takes the next item of its queue from an atomic counter, so items run in queue order
+// whatever order the dispatcher starts workgroups in, and the dispatcher backfills a CU as soon
+// as a workgroup ends.  A CU then holds sweep workgroups, bound by the VALU, and Gram
+// workgroups, bound by the matrix cores, side by side; on gfx950 the two overlap (a probe of
+// FMA-only and MFMA-only workgroups in one launch ran in the time of the FMA part alone).
+// Within a queue, sweep and Gram items alternate while both remain, so that the long sweep
+// items start early and short Gram items fill the end.  (A persistent loop over items made
+// the compiler keep values live across the loop: 166 VGPRs spilled at the 128 cap.)
+// The chains, power sums and monomials are those of the separate kernels (same device code,
+// same random numbers; the power sums in the quads of k_psums); the Gram sums differ from a
+// separate Gram launch only in where the row splits fall.
+// ---------------------------------------------------------------------------
+constexpr int kStepPlanMax = 992;    // product plans held in LDS (larger D reads them from memory; 64 KB of LDS in all)
+
+// Power sums of the fused kernel: 8 lanes per sample, lanes 0..3 of the 8 holding Re/Im p_k for
+// k < KMID and V, lanes 4..7 the k >= KMID, each quad over the particles j = sub + 4 s as in
+// k_psums (same powers, same sums, same quad reduction: the same values), with half the
+// accumulators of a lane (k_psums holds all of them: 152 VGPRs at K = 16).
+template <int KMAX>
+__device__ __forceinline__ void step_psums(const FeatureArgs& a, long w, int sub8)
+{
+    constexpr int KMID = KMAX / 2;
+    constexpr int NQ = KMAX + 1;                 // the larger half: Re/Im of KMID powers, and V
+    const bool hi = sub8 >= kLps;
+    const int sub = sub8 & (kLps - 1);
+    const int k0 = hi ? KMID : 0;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int j = sub; j < a.N; j += kLps) {
+        const float2 p = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
+        const double x = p.x, y = p.y;
+        const double r2 = x * x + y * y;
+        if (!hi) acc[KMAX] += potential_r2(a.pot, r2);
+        const double ur = x * a.invR, ui = y * a.invR;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            if (k < KMID) {
+                if (!hi) {
+                    acc[2 * k] += qr;
+                    acc[2 * k + 1] += qi;
+                }
+            } else if (hi) {
+                acc[2 * (k - KMID)] += qr;
+                acc[2 * (k - KMID) + 1] += qi;
+            }
+            const double t = qr * ur - qi * ui;
+            qi = qr * ui + qi * ur;
+            qr = t;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = group_sum<kLps, double>(acc[q]);
+    double* ps = a.ps + (size_t)w * a.ps_ld;
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) {
+        const int qq = q + 2 * k0;
+        if (q % kLps == sub && qq < 2 * a.K) ps[qq] = acc[q];
+    }
+    if (hi) return;
+    const long row = a.row0 + w;
+    if (sub == 0) reinterpret_cast<float*>(a.W)[row] = (float)acc[KMAX];
+    if (a.trace && w < a.trace_walkers && sub == 0) {
+        const long long tb = a.trace_base ? *a.trace_base : 0ll;
+        float* tr = a.trace + ((size_t)(a.trace_row0 + tb) + (size_t)w) * 9;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tr[q] = (q / 2 < a.K) ? (float)acc[q] : 0.f;
+        tr[8] = (float)acc[KMAX];
+    }
+}
+
+// Items of queue q: sweep units u = q + 8 j (u < units) and Gram items (tile, split) with
+// split = q + 8 j' (split < splits); in queue order, item i < 2 m (m = the smaller count) is a
+// sweep item for even i, a Gram item for odd i, and the rest are of the larger kind.
+struct StepQueue {
+    int ns, ng, m, total;
+};
+
+__device__ __host__ inline StepQueue step_queue(int q, int units, long grows, int splits, int ntiles)
+{
+    StepQueue r;
+    r.ns = units > q ? (units - q + 7) / 8 : 0;
+    r.ng = (grows > 0 && splits > q) ? ((splits - q + 7) / 8) * ntiles : 0;
+    r.m = r.ns < r.ng ? r.ns : r.ng;
+    r.total = r.ns + r.ng;
+    return r;
+}
+
+template <int G, int S, int KMAX>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_step(StepArgs st)
+{
+    static_assert(G == 8, "sweep units are 8 wavefronts of 8-lane walkers");
+    __shared__ __align__(16) double2 mpk[8][32];
+    __shared__ __align__(16) float2 mtab[8][kMaxPow];
+    __shared__ unsigned long long mplan[kStepPlanMax];
+    __shared__ uint32_t mcols[kStepPlanMax];
+    __shared__ int item_s;
+    constexpr int NWU = 8 * (64 / G);            // walkers per sweep unit
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const FeatureArgs& f = st.f;
+    const int q = blockIdx.x & 7;
+    const int nwg_q = ((int)gridDim.x - q + 7) / 8;   // workgroups of queue q (one fetch each)
+    const StepQueue Q = step_queue(q, st.units, st.g.rows, st.g.splits, st.g.ntiles);
+    if (tid == 0) item_s = (int)atomicAdd(&st.queue[q], 1u);
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(item_s);
+    if (i < Q.total) {
+        bool sweep;
+        int j;
+        if (i < 2 * Q.m) {
+            sweep = !(i & 1);
+            j = i >> 1;
+        } else {
+            sweep = Q.ns > Q.ng;
+            j = Q.m + (i - 2 * Q.m);
+        }
+        if (sweep) {
+            const long u = q + 8L * j;
+            const bool plan_lds = f.D <= kStepPlanMax;
+            if (plan_lds)
+                for (int t = tid; t < f.D; t += 512) {
+                    mplan[t] = f.prod[t];
+                    mcols[t] = (uint32_t)f.col_re[t] | ((uint32_t)f.col_dim[t] << 16);
+                }
+            sweep_reg_walkers<G, S, float>(st.sw, u * 8 + wave, lane);
+            __syncthreads();   // the unit's positions are written
+            const long w0 = u * NWU;
+            {
+                const long w = w0 + tid / 8;   // 8 lanes per sample, 64 samples per unit
+                if (w < f.walkers) step_psums<KMAX>(f, w, tid % 8);
+            }
+            __syncthreads();   // their power sums are written
+            for (int k = wave; k < NWU; k += 8) {
+                const long w = w0 + k;
+                if (w < f.walkers) monomials_sample<float>(f, w, lane, mpk[wave], mtab[wave], mplan, mcols, plan_lds);
+            }
+        } else {
+            gram_tile<float, 2>(st.g, j % st.g.ntiles, q + 8 * (j / st.g.ntiles), 0);
+        }
+    }
+    // the last workgroup of a queue to finish clears it for the next launch
+    if (tid == 0 && atomicAdd(&st.queue[8 + q], 1u) == (unsigned)(nwg_q - 1)) {
+        atomicExch(&st.queue[q], 0u);
+        atomicExch(&st.queue[8 + q], 0u);
+    }
+}
+
+bool step_supported(int G, int N, int K)
+{
+    const int S = (N + G - 1) / G;
+    return G == 8 && (S == 13 || S == 25) && K >= 1 && K <= 16;
+}
+
+hipError_t launch_step(const StepArgs& a, int G, hipStream_t s)
+{
+    if (!step_supported(G, a.sw.N, a.f.K) || !a.queue) return hipErrorInvalidValue;
+    if (a.sw.nsweeps < 1 || a.sw.walkers != a.f.walkers || a.sw.mix.kmix > 0 || a.f.mix.kmix > 0 || a.f.nh > 0 ||
+        a.f.npw > kMaxPow || !a.f.ps || a.f.ps_ld < 2 * a.f.K || a.f.pos != a.sw.pos || a.f.pos_is_double)
+        return hipErrorInvalidValue;
+    const long units = ((long)a.sw.walkers + 63) / 64;
+    if (a.units != units) return hipErrorInvalidValue;
+    if (a.g.rows > 0 && (a.g.nw != 2 || a.g.jobs || a.g.ntiles < 1 || a.g.rows_per_split < 1 ||
+                         a.g.rows_per_split % kKC || (long)a.g.splits * a.g.rows_per_split < a.g.rows))
+        return hipErrorInvalidValue;
+    // one workgroup per item of the longest queue, for each of the 8 queues
+    int most = 0;
+    for (int q = 0; q < 8; ++q) most = std::max(most, step_queue(q, a.units, a.g.rows, a.g.splits, a.g.ntiles).total);
+    if (most == 0) return hipSuccess;
+    const dim3 grid((unsigned)(8 * most));
+    const int S = (a.sw.N + G - 1) / G;
+    const bool k8 = a.f.K <= 8;
+    if (S == 25) {
+        if (k8) hipLaunchKernelGGL((k_step<8, 25, 8>), grid, dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_step<8, 25, 16>), grid, dim3(512), 0, s, a);
+    } else {
+        if (k8) hipLaunchKernelGGL((k_step<8, 13, 8>), grid, dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_step<8, 13, 16>), grid, dim3(512), 0, s, a);
     }
     return hipGetLastError();
 }

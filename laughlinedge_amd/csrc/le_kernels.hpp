@@ -165,6 +165,22 @@ struct GramArgs {
 // Workgroups of a Gram launch: gram_grid(ntiles, splits, xcd_spread)
 long gram_grid(int ntiles, int splits, int xcd_spread);
 
+// One fused sampling step (launch_step): the sweeps, power sums and monomials of every walker
+// (sw, f: the current sample's rows) and the S/H Gram tiles of the previous sample's rows (g,
+// two weights, g.jobs unused; g.rows = 0 for none), as work items of one launch.  Queue q
+// (q < 8) serves the workgroups b with b mod 8 = q: sweep units u = q + 8 j (8 wavefronts,
+// 8 x 64 / G walkers) and Gram items (tile, split) with split = q + 8 j'.  queue[q] counts the
+// items taken, queue[8 + q] the workgroups that finished; both are zero at launch, and the last
+// workgroup of a queue to finish clears them.
+constexpr int kStepQueueWords = 16;
+struct StepArgs {
+    SweepArgs sw;
+    FeatureArgs f;
+    GramArgs g;
+    int units;
+    unsigned* queue;         // [kStepQueueWords]
+};
+
 // Many weights on one small block: C_q[r0 + i][c0 + j] += sum_s w_q(s) X[s][r0 + i] X[s][c0 + j]
 // for i < nr, j < nc (both <= 64), q < nweights, with weight q's samples at W + q wld and its
 // accumulator jobs[q / 2].C0 (q even) or .C1 (q odd).
@@ -235,6 +251,10 @@ hipError_t launch_sweep2_reg(const SweepArgs& a, int prec, int lanes_per_walker,
 hipError_t launch_features2(const Features2Args& a, int prec, hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, int prec, hipStream_t s);
 hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s);
+// fused steps: FP32 plain chains with particles in registers at 8 lanes per walker
+// (ceil(N / 8) = 13 or 25), power sums up to K <= 16, no harmonics, trace allowed
+bool step_supported(int lanes_per_walker, int N, int K);
+hipError_t launch_step(const StepArgs& a, int lanes_per_walker, hipStream_t s);
 hipError_t launch_gram_weights(const WeightGramArgs& a, int prec, hipStream_t s);
 hipError_t launch_extract(const ExtractArgs& a, hipStream_t s);
 hipError_t launch_geneig(const double* S, const double* H, double* work, const GenEigProblem* probs,

@@ -95,7 +95,7 @@ class MCParams(ctypes.Structure):
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
                 ("sweep_registers", ctypes.c_int), ("estimator", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int),
-                ("mixture_rho", ctypes.c_double)]
+                ("mixture_rho", ctypes.c_double), ("fused_step", ctypes.c_int), ("fused_rows", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -118,7 +118,8 @@ class RunStats(ctypes.Structure):
                 ("ms_sweep", ctypes.c_double), ("ms_features", ctypes.c_double),
                 ("ms_gram", ctypes.c_double), ("ms_sample_total", ctypes.c_double),
                 ("gram_launches", ctypes.c_longlong), ("sweep_launches", ctypes.c_longlong),
-                ("ms_sampler_graph", ctypes.c_double), ("ms_gram_launch_min", ctypes.c_double)]
+                ("ms_sampler_graph", ctypes.c_double), ("ms_gram_launch_min", ctypes.c_double),
+                ("ms_step", ctypes.c_double), ("step_launches", ctypes.c_longlong)]
 
 
 _lib = None

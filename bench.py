@@ -85,6 +85,25 @@ def tau_pk(taus):
     return max(taus[k] for k in PK_NAMES)
 
 
+def mixture_kish(eng):
+    """Kish factor (sum 1/w)^2 / (n sum 1/w^2) of the mixture weights 1/w over the walkers' current
+    configurations: w = c0 + sum_k exp(a_k ln|p_k|^2 + b_k), p_k = sum_i (z_i / R)^k."""
+    L, c0, a, b = eng.get_mixture()
+    if L == 0:
+        return 1.0
+    pos = eng.positions()
+    z = (pos[..., 0] + 1j * pos[..., 1]) / np.sqrt(2.0 * M_INV * N)
+    w = np.full(z.shape[0], c0)
+    zk = np.ones_like(z)
+    for k in range(L):
+        zk = zk * z
+        q = np.abs(zk.sum(axis=1)) ** 2
+        with np.errstate(divide="ignore"):
+            w += np.where(q > 0, np.exp(a[k] * np.log(np.maximum(q, 1e-300)) + b[k]), 0.0)
+    iw = 1.0 / w
+    return float(iw.sum() ** 2 / (len(iw) * np.sum(iw * iw)))
+
+
 def taus_of(eng, n_samples, tw):
     from laughlinedge_amd.stats import tau_int
     tr = eng.trace()
@@ -329,6 +348,11 @@ def main():
         eg.burnin(min(args.burnin, 100))
         eg.sample(8)
         wall_eager, de = timed_sampling(eg, ksteps, 1)
+        # the sweep alone: 20 sweeps of every walker in one launch, nothing beside it (le_sample
+        # returned with both streams synchronised)
+        t_sw = time.perf_counter()
+        eg.burnin(20)
+        ms_sweep_alone = 1e3 * (time.perf_counter() - t_sw) / 20
         eg.close()
         eager_updates = args.walkers * ksteps
         log(f"eager: {eager_updates / wall_eager:.4e} sample-updates/s")
@@ -339,7 +363,11 @@ def main():
                         "bound": "valu",
                         "achieved_tflops": sweep_rate / 1e12, "peak_tflops": PEAK_FP32 / 1e12,
                         "frac": sweep_rate / PEAK_FP32, "flop_per_walker_sweep": 12 * N * (N - 1),
-                        "ms_per_sweep": de["ms_sweep"] / ksteps},
+                        "ms_per_sweep": de["ms_sweep"] / ksteps,
+                        "ms_per_sweep_alone": ms_sweep_alone,
+                        "frac_alone": 12.0 * N * (N - 1) * args.walkers / (ms_sweep_alone * 1e-3) / PEAK_FP32,
+                        "note": "ms_per_sweep and frac: eager sampling, beside the Gram stream's launches; "
+                                "alone: one launch of 20 sweeps (le_burnin), wall time"},
             "features": {"kernel": "k_psums<float,16> + k_monomials<float>", "bound": "hbm",
                          "ms_per_step": de["ms_features"] / ksteps,
                          "sample_matrix_gb_per_s": stream_bytes / (de["ms_features"] * 1e-3) / 1e9,
@@ -400,7 +428,9 @@ def compact(out, full_path):
     if "kernels" in out:
         k = out["kernels"]
         c["sampler"] = {"kernel": k["sampler"]["kernel"], "ms_per_sweep": _r(k["sampler"]["ms_per_sweep"]),
-                        "frac_fp32_valu": _r(k["sampler"]["frac"])}
+                        "frac_fp32_valu": _r(k["sampler"]["frac"]),
+                        "ms_per_sweep_alone": _r(k["sampler"]["ms_per_sweep_alone"]),
+                        "frac_fp32_valu_alone": _r(k["sampler"]["frac_alone"])}
         c["features_ms_per_step"] = _r(k["features"]["ms_per_step"])
     if "precision" in out:
         p = out["precision"]
@@ -422,6 +452,15 @@ def compact(out, full_path):
                 row["lowest_max_sigma_jk"] = _r(max(jk["lowest_per_sector_dev"].values()) if jk["lowest_per_sector_dev"] else None, 3)
                 row["lowest_max_sigma_batch"] = _r(max(bt["lowest_per_sector_dev"].values()) if bt["lowest_per_sector_dev"] else None, 3)
                 row["within_1sigma_jk"] = _r(jk["frac_within_1_sigma"], 3)
+            if "spectra_vs_mixture_fp64" in r:
+                jk = r["spectra_vs_mixture_fp64_jackknife"]
+                bt = r["spectra_vs_mixture_fp64"]
+                row["vs_mixture_fp64"] = {
+                    "lowest_max_sigma_batch": _r(max(bt["lowest_per_sector_dev"].values()) if bt["lowest_per_sector_dev"] else None, 3),
+                    "lowest_max_sigma_jk": _r(max(jk["lowest_per_sector_dev"].values()) if jk["lowest_per_sector_dev"] else None, 3),
+                    "within_1sigma_batch": _r(bt["frac_within_1_sigma"], 3),
+                    "within_2sigma_batch": _r(bt["frac_within_2_sigma"], 3),
+                    "max_sigma_batch": _r(bt["max_dev_in_combined_sigma"] if "max_dev_in_combined_sigma" in bt else None, 3)}
             cf[name] = row
         c["forms"] = cf
     if "rows" in out:
@@ -711,17 +750,21 @@ def forms(args, thins):
     tw = min(args.trace_walkers, args.walkers)
     fs = args.form_steps
     nblk = 32
-    table = (("plain_fp64", 1, False, 0.5, 0), ("plain_fp32", 0, False, 0.5, 0), ("plain_fp32_wide", 0, False, 0.3, 0),
-             ("plain_bf16x6_wide", 3, False, 0.3, 0), ("plain_fp32_wide_cmrot", 0, False, 0.3, 1),
-             ("plain_fp32_wide_sym", 0, False, 0.3, 2), ("plain_bf16x6_wide_sym", 3, False, 0.3, 2),
-             ("learned_fp32", 0, True, 0.5, 0), ("learned_fp32_sym", 0, True, 0.5, 2))
-    for name, prec, learned, target, sym in table:
-        eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target, sym=sym)
+    # (name, precision, learned, target acceptance, symmetry moves, estimator)
+    table = (("plain_fp64", 1, False, 0.5, 0, 0), ("plain_fp32", 0, False, 0.5, 0, 0),
+             ("plain_fp32_wide", 0, False, 0.3, 0, 0), ("plain_bf16x6_wide", 3, False, 0.3, 0, 0),
+             ("plain_fp32_wide_cmrot", 0, False, 0.3, 1, 0), ("plain_fp32_wide_sym", 0, False, 0.3, 2, 0),
+             ("plain_bf16x6_wide_sym", 3, False, 0.3, 2, 0), ("learned_fp32", 0, True, 0.5, 0, 0),
+             ("learned_fp32_sym", 0, True, 0.5, 2, 0), ("mixture_fp64", 1, False, 0.5, 0, 2),
+             ("mixture_fp32", 0, False, 0.5, 0, 2), ("mixture_fp32_cmrot", 0, False, 0.5, 1, 2))
+    for name, prec, learned, target, sym, est in table:
+        eng = make_engine(args.walkers, prec, (len(thins) + 1) * (fs + 4), nblk, tw, target, sym=sym, estimator=est)
         eng.init_walkers()
         step = eng.tune_step()
         eng.burnin(args.burnin)
         r = {"precision": ["fp32", "fp64", "bf16x3", "bf16x6"][prec], "step": step, "target_acceptance": target,
-             "symmetry_moves": ["none", "centre of mass + rotation", "centre of mass + rotation + dilation"][sym]}
+             "symmetry_moves": ["none", "centre of mass + rotation", "centre of mass + rotation + dilation"][sym],
+             "estimator": ["plain", "rowwise", "mixture"][est]}
         if learned:
             t0 = time.perf_counter()
             rep = eng.train_proposal(steps=2000, lr=1e-3, batch=4096, n_configs=100000, snapshot_sweeps=2, seed=99)
@@ -735,14 +778,18 @@ def forms(args, thins):
             taus = taus_of(eng, fs, tw)
             sh = {k: v for k, v in taus.items() if k not in PK_NAMES}
             tmax, tmax_sh = max(taus.values()), tau_sh(taus)
-            ess = args.walkers * fs / (2.0 * tmax) / wall
-            ess_sh = args.walkers * fs / (2.0 * tmax_sh) / wall
+            # mixture chains: the reweighted estimates keep kish = (sum 1/w)^2 / (n sum 1/w^2) of the
+            # samples (importance weights 1/w of the walkers' current configurations)
+            kish = mixture_kish(eng) if est == 2 else 1.0
+            ess = kish * args.walkers * fs / (2.0 * tmax) / wall
+            ess_sh = kish * args.walkers * fs / (2.0 * tmax_sh) / wall
             runs.append({"thin": thin, "ms_per_kept_sample": 1e3 * wall / fs,
                          "ms_sampler_per_kept_sample": (d["ms_sweep"] + d["ms_features"] + d["ms_sampler_graph"]) / fs,
                          "tau_max": tmax, "tau_max_obs": max(taus, key=taus.get),
                          "tau_max_sh": tmax_sh, "tau_max_sh_obs": max(sh, key=sh.get),
                          "tau_re_im_pk": tau_pk(taus), "tau_V": taus["V"], "tau_p1sq": taus["|p1|^2"],
-                         "ess_per_s": ess, "ess_per_s_sh": ess_sh, "acceptance": eng.stats()["acceptance"]})
+                         "ess_per_s": ess, "ess_per_s_sh": ess_sh, "kish": kish,
+                         "acceptance": eng.stats()["acceptance"]})
             log(f"form {name} thin {thin}: {1e3 * wall / fs:.3f} ms/sample tau_max {tmax:.2f} (S/H {tmax_sh:.2f}) "
                 f"ESS/s {ess:.3e} (S/H {ess_sh:.3e})")
         best = max(runs, key=lambda x: x["ess_per_s"])
@@ -763,12 +810,17 @@ def forms(args, thins):
         eng.close()
         res[name] = r
     ref = res["plain_fp64"]
+    mref = res["mixture_fp64"]
     for name, *_ in table[1:]:
         r = res[name]
         r["ess_ratio_vs_plain_fp64"] = r["ess_per_s"] / ref["ess_per_s"]
         r["ess_sh_ratio_vs_plain_fp64"] = r["ess_per_s_sh"] / ref["ess_per_s_sh"]
         r["spectra_vs_plain_fp64"] = compare_spectra(r["_spec"], r["_sig"], ref["_spec"], ref["_sig"])
         r["spectra_vs_plain_fp64_jackknife"] = compare_spectra(r["_jspec"], r["_jsig"], ref["_jspec"], ref["_jsig"])
+        if name.startswith("mixture_") and name != "mixture_fp64":
+            r["spectra_vs_mixture_fp64"] = compare_spectra(r["_spec"], r["_sig"], mref["_spec"], mref["_sig"])
+            r["spectra_vs_mixture_fp64_jackknife"] = compare_spectra(r["_jspec"], r["_jsig"], mref["_jspec"],
+                                                                     mref["_jsig"])
     for name, *_ in table:
         for k in ("_spec", "_sig", "_jspec", "_jsig"):
             res[name].pop(k)
@@ -781,7 +833,10 @@ def forms(args, thins):
                    "each accumulation on identical samples is in 'precision'; "
                    "spectra_vs_plain_fp64_jackknife compares the delete-one-block jackknife estimates "
                    "(the O(1/samples) low bias of the noisy Rayleigh-Ritz values removed) with their "
-                   "jackknife sigma, over the same 32 blocks")
+                   "jackknife sigma, over the same 32 blocks; the mixture_* forms sample w |Psi_L|^2, "
+                   "whose reweighted S and H integrands are bounded, so their batch sigma is honest: "
+                   "spectra_vs_mixture_fp64 compares them with the FP64 form of the same estimator, and "
+                   "their ESS carries the Kish factor of the 1/w weights")
     return res
 
 

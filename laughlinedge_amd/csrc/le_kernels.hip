@@ -1461,6 +1461,13 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
         if (r0 + kKC < r_end) fetch(r0 + kKC);
 
         if (active) {
+            // column-side fragments once for both weights
+            bf16x8 bv[2][NS];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int v = 0; v < NS; ++v)
+                    bv[a][v] = *reinterpret_cast<const bf16x8*>(&bj[v][co[a] + l16][8 * k4]);
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 const int rr = ro + l16;
@@ -1469,16 +1476,13 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
                 for (int u = 0; u < NS; ++u) av[u] = *reinterpret_cast<const bf16x8*>(&ai[q][u][rr][8 * k4]);
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
-                    const int c = co[a] + l16;
                     floatx4 d = acc[q][a];
                     // smallest products first
 #pragma unroll
-                    for (int v = NS - 1; v >= 0; --v) {
-                        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&bj[v][c][8 * k4]);
+                    for (int v = NS - 1; v >= 0; --v)
 #pragma unroll
                         for (int u = NS - 1 - v; u >= 0; --u)
-                            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[u], bv, d, 0, 0, 0);
-                    }
+                            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[u], bv[a][v], d, 0, 0, 0);
                     acc[q][a] = d;
                 }
             }

@@ -292,10 +292,11 @@ def main():
                    "step": step, "lanes_per_walker": eng.info.lanes_per_walker,
                    "sweep_in_registers": bool(eng.info.sweep_in_registers),
                    "samples_per_launch": info.samples_per_launch, "gram_tiles": info.n_tiles,
-                   "gram_xcd_spread": eng.info.gram_xcd_spread,
+                   "gram_xcd_spread": eng.info.gram_xcd_spread, "gram_tile_cols": eng.info.gram_tile_cols,
                    "gram_launches_timed": int(d["gram_launches"]),
                    "sampler_rounds": "hipGraph replay"},
-        "roofline": {"bound": "mfma", "kernel": "k_gram<float, 2>",
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_gram<float, 2>" if eng.info.gram_tile_cols == 64 else "k_gram<float, 2, 32>",
                      "achieved": gram_rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": gram_rate / PEAK_FP32, "traffic": traffic,
                      "traffic_note": traffic_note,

@@ -157,6 +157,10 @@ typedef struct le_mc_params {
                                   proposal; le_sample fails with LE_ERR_ARG otherwise.
                                   0 (default): separate kernels on two streams */
     int fused_rows;            /* rows per split of the fused Gram items (a multiple of 32; 0: 2048) */
+    int gram_tile_cols;        /* Gram tile width: 64 (64x64 tiles, 512-thread workgroups) or 32 (64x32
+                                  tiles, 256-thread workgroups; FP32 and FP64 accumulation only);
+                                  0 (default): time both with each XCD placement on the first
+                                  full-size launch and keep the fastest (cached per sector layout) */
 } le_mc_params;
 
 typedef struct le_engine le_engine;
@@ -208,6 +212,9 @@ typedef struct le_engine_info {
                                   choice once the first full-size round has run) */
     long chains;               /* Markov chains held: walkers, or D x walkers for the row-wise
                                   estimator (the leading dimension of le_get/set_positions) */
+    int gram_tile_cols;        /* Gram tile width in use (64 or 32; the autotuned choice once the
+                                  first full-size round has run); n_tiles and
+                                  gram_mfma_flop_per_row describe that width's tile list */
 } le_engine_info;
 
 typedef struct le_run_stats {

@@ -78,7 +78,7 @@ int default_lanes(int N, int prec)
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, int> g_tuned_lanes;   // (N, walkers, prec, two-sided) -> G
 // Gram tile placement chosen per (rows per launch, Gram kind, sector layout)
-std::map<std::tuple<long, int, std::vector<int>, int>, int> g_tuned_spread;
+std::map<std::tuple<long, int, std::vector<int>, int>, std::pair<int, int>> g_tuned_spread;   // -> (spread, tile columns)
 
 }  // namespace
 
@@ -145,6 +145,8 @@ struct le_engine {
         std::vector<std::pair<int, int>> blocks;   // sector pairs s1 <= s2
         le::Tile* tiles = nullptr;
         int ntiles = 0;
+        le::Tile* tiles32 = nullptr;   // the same entries on 64x32 tiles
+        int ntiles32 = 0;
         double* C[2] = {nullptr, nullptr};
     };
     std::vector<Job> jobs;
@@ -184,7 +186,9 @@ struct le_engine {
     // p_k start the next sweep launch
     void mix_features(le::FeatureArgs& f, bool walkers) const;
     int gram_spread = 8;             // XCDs sharing one row split's Gram tiles (1, 2, 4 or 8)
-    bool need_spread_tune = false;   // gram_xcd_spread = 0 and no choice cached for this layout yet
+    int gram_tc = 64;                // Gram tile columns in use (64, or 32 for the FP32 / FP64 tiles)
+    double gram_flop_row32 = 0;      // MFMA FLOPs per sample row issued with 32-column tiles
+    bool need_spread_tune = false;   // XCD placement or tile width to time on the first full-size round
     long long block_grams = 0;       // Gram launches into the open block (the tuning needs none yet)
     int tune_gram_spread(int buf);
     double gram_flop_row = 0;        // MFMA FLOPs issued per sample row over all Gram jobs
@@ -307,7 +311,10 @@ le_engine::~le_engine()
         if (drained[b]) (void)hipEventDestroy(drained[b]);
     }
     if (gstream) (void)hipStreamDestroy(gstream);
-    for (auto& j : jobs) (void)hipFree(j.tiles);
+    for (auto& j : jobs) {
+        (void)hipFree(j.tiles);
+        (void)hipFree(j.tiles32);
+    }
     (void)hipFree(jobtab[0]);
     (void)hipFree(jobtab[1]);
     (void)hipFree(cpool);
@@ -360,6 +367,7 @@ extern "C" void le_mc_params_default(le_mc_params* p)
     p->mixture_rho = 0.5;
     p->fused_step = 0;
     p->fused_rows = 0;
+    p->gram_tile_cols = 0;
 }
 
 extern "C" const char* le_last_error(void) { return g_err.c_str(); }
@@ -450,14 +458,6 @@ int le_engine::setup(const le_observables* obs)
         else add_job(LE_W_USER(k), 0, 1, all);
     }
     for (auto& j : jobs) {
-        std::set<std::pair<int, int>> tl;
-        for (auto& pr : j.blocks) {
-            const int c1 = 2 * basis.sector_offset[pr.first], e1 = c1 + 2 * basis.dim(pr.first) - 1;
-            const int c2 = 2 * basis.sector_offset[pr.second], e2 = c2 + 2 * basis.dim(pr.second) - 1;
-            for (int a = c1 / 64; a <= e1 / 64; ++a)
-                for (int b = c2 / 64; b <= e2 / 64; ++b)
-                    if (a <= b) tl.insert({a, b});
-        }
         // per tile, the 16x16 blocks that hold an entry the extraction reads (one with
         // row <= column inside a requested block or its transpose), two to a wave within a row
         // block so that they share the row operand; the Gram kernels compute only these
@@ -467,36 +467,55 @@ int le_engine::setup(const le_observables* obs)
             const int qa = std::max(q0, qb0), qb = std::min(q1, qb1);
             return ra <= rb && qa <= qb && ra <= qb;
         };
-        std::vector<le::Tile> th;
-        for (auto& pr : tl) {
-            le::Tile t{};
-            t.i0 = pr.first * 64;
-            t.j0 = pr.second * 64;
-            std::vector<uint32_t> act;
-            for (int rb = 0; rb < 4; ++rb) {
-                const size_t first = act.size();
-                for (int cb = 0; cb < 4; ++cb) {
-                    const int r0 = t.i0 + 16 * rb, q0 = t.j0 + 16 * cb;
-                    for (auto& bp : j.blocks) {
-                        const int c1 = 2 * basis.sector_offset[bp.first], e1 = c1 + 2 * basis.dim(bp.first) - 1;
-                        const int c2 = 2 * basis.sector_offset[bp.second], e2 = c2 + 2 * basis.dim(bp.second) - 1;
-                        if (meets(r0, r0 + 15, q0, q0 + 15, c1, e1, c2, e2) ||
-                            meets(r0, r0 + 15, q0, q0 + 15, c2, e2, c1, e1)) {
-                            act.push_back((uint32_t)(rb << 2 | cb));
-                            break;
+        // tiles of 64 rows x TC columns (TC = 64, 32) with their active blocks
+        auto build = [&](int TC, double& flop_row) {
+            std::set<std::pair<int, int>> tset;
+            for (auto& pr : j.blocks) {
+                const int c1 = 2 * basis.sector_offset[pr.first], e1 = c1 + 2 * basis.dim(pr.first) - 1;
+                const int c2 = 2 * basis.sector_offset[pr.second], e2 = c2 + 2 * basis.dim(pr.second) - 1;
+                for (int a = c1 / 64; a <= e1 / 64; ++a)
+                    for (int b = c2 / TC; b <= e2 / TC; ++b)
+                        if (64 * a <= TC * b + TC - 1) tset.insert({a, b});
+            }
+            std::vector<le::Tile> th;
+            for (auto& pr : tset) {
+                le::Tile t{};
+                t.i0 = pr.first * 64;
+                t.j0 = pr.second * TC;
+                std::vector<uint32_t> act;
+                for (int rb = 0; rb < 4; ++rb) {
+                    const size_t first = act.size();
+                    for (int cb = 0; cb < TC / 16; ++cb) {
+                        const int r0 = t.i0 + 16 * rb, q0 = t.j0 + 16 * cb;
+                        for (auto& bp : j.blocks) {
+                            const int c1 = 2 * basis.sector_offset[bp.first], e1 = c1 + 2 * basis.dim(bp.first) - 1;
+                            const int c2 = 2 * basis.sector_offset[bp.second], e2 = c2 + 2 * basis.dim(bp.second) - 1;
+                            if (meets(r0, r0 + 15, q0, q0 + 15, c1, e1, c2, e2) ||
+                                meets(r0, r0 + 15, q0, q0 + 15, c2, e2, c1, e1)) {
+                                act.push_back((uint32_t)(rb << 2 | cb));
+                                break;
+                            }
                         }
                     }
+                    if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
                 }
-                if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
+                if (act.empty()) continue;
+                t.nwave = (int)act.size() / 2;
+                flop_row += 2.0 * 16 * 16 * (double)act.size() * j.nw;
+                for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
+                th.push_back(t);
             }
-            t.nwave = (int)act.size() / 2;
-            gram_flop_row += 2.0 * 16 * 16 * (double)act.size() * j.nw;
-            for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
-            th.push_back(t);
-        }
+            return th;
+        };
+        std::vector<le::Tile> th = build(64, gram_flop_row);
+        std::vector<le::Tile> th32 = build(32, gram_flop_row32);
         j.ntiles = (int)th.size();
         HIPCHK(dalloc(&j.tiles, (size_t)std::max(1, j.ntiles)));
         if (j.ntiles) HIPCHK(hipMemcpy(j.tiles, th.data(), j.ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
+        j.ntiles32 = (int)th32.size();
+        HIPCHK(dalloc(&j.tiles32, (size_t)std::max(1, j.ntiles32)));
+        if (j.ntiles32)
+            HIPCHK(hipMemcpy(j.tiles32, th32.data(), j.ntiles32 * sizeof(le::Tile), hipMemcpyHostToDevice));
         cpool_elems += (size_t)j.nw * ldx * ldx;
         for (int q = 0; q < j.nw; ++q)
             for (auto& pr : j.blocks) {
@@ -702,6 +721,10 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->gram_xcd_spread != 0 && mc->gram_xcd_spread != 1 && mc->gram_xcd_spread != 2 && mc->gram_xcd_spread != 4 &&
         mc->gram_xcd_spread != 8)
         return fail(LE_ERR_ARG, "gram_xcd_spread must be 0, 1, 2, 4 or 8");
+    if (mc->gram_tile_cols != 0 && mc->gram_tile_cols != 32 && mc->gram_tile_cols != 64)
+        return fail(LE_ERR_ARG, "gram_tile_cols must be 0, 32 or 64");
+    if (mc->gram_tile_cols == 32 && (mc->precision == LE_PREC_BF16X3 || mc->precision == LE_PREC_BF16X6))
+        return fail(LE_ERR_ARG, "the BF16 split Gram runs 64-column tiles only");
     if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
         !le::sweep_reg_supported(mc->lanes_per_walker, sys->particles_number))
         return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25");
@@ -781,7 +804,9 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
         return rc;
     }
     e->gram_spread = mc->gram_xcd_spread > 0 ? mc->gram_xcd_spread : 8;
-    e->need_spread_tune = mc->gram_xcd_spread == 0;
+    e->gram_tc = mc->gram_tile_cols > 0 ? mc->gram_tile_cols : 64;
+    e->need_spread_tune = mc->gram_xcd_spread == 0 ||
+                          (mc->gram_tile_cols == 0 && e->gram_prec != le::kBF16X3 && e->gram_prec != le::kBF16X6);
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     if (mc->fused_step) {
@@ -837,11 +862,12 @@ extern "C" int le_engine_get_info(const le_engine* e, le_engine_info* info)
     info->lanes_per_walker = map_lanes(e->G);
     info->sweep_in_registers = map_reg(e->G) ? 1 : 0;
     info->samples_per_launch = e->spl;
-    info->n_tiles = e->ntiles;
+    info->n_tiles = (e->gram_tc == 32 && !e->jobs.empty()) ? e->jobs[0].ntiles32 : e->ntiles;
     info->gram_xcd_spread = e->gram_spread;
+    info->gram_tile_cols = e->gram_tc;
     info->chains = e->mc.walkers;
     info->x_ld = e->ldx;
-    info->gram_mfma_flop_per_row = e->gram_flop_row;
+    info->gram_mfma_flop_per_row = e->gram_tc == 32 ? e->gram_flop_row32 : e->gram_flop_row;
     return LE_OK;
 }
 
@@ -1263,8 +1289,9 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         // ~6144 workgroups (24 per CU) per launch: with 2-3 resident 512-thread workgroups per CU
         // the last, partial round of workgroups is then a small part of the launch (at 2048
         // workgroups N = 200, dL 1..16 needed 4.02 rounds and lost ~20% to the fifth)
-        const long want_wg = 6144;
-        const long per_split = (long)std::max(1, j.ntiles) * bt.count;
+        const bool narrow = gram_tc == 32;
+        const long want_wg = narrow ? 12288 : 6144;   // the same threads with 256-thread workgroups
+        const long per_split = (long)std::max(1, narrow ? j.ntiles32 : j.ntiles) * bt.count;
         long splits = std::max<long>(1, (want_wg + per_split - 1) / per_split);
         // with the tiles of a split on K XCDs, splits come in multiples of the 8 / K groups
         const int K = gram_spread;
@@ -1277,8 +1304,9 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         g.xcd_spread = K;
         g.rows_per_split = rps;
         g.splits = (int)splits;
-        g.tiles = j.tiles;
-        g.ntiles = j.ntiles;
+        g.tiles = narrow ? j.tiles32 : j.tiles;
+        g.ntiles = narrow ? j.ntiles32 : j.ntiles;
+        g.tile_cols = gram_tc;
         g.C0 = j.C[0];
         g.C1 = j.C[1];
         g.ldc = ldx;
@@ -1383,10 +1411,11 @@ int le_engine::capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hi
     return LE_OK;
 }
 
-// Time each placement of the Gram tiles over the XCDs (gram_xcd_spread 8, 4, 2, 1: two
-// launches each on the round in buffer `buf`, the shortest counts) on the Gram stream before
-// the round's own accumulation, keep the fastest for this layout, and clear the accumulators
-// the timing launches wrote.  Once per (rows per launch, Gram kind, sector layout).
+// Time each placement of the Gram tiles over the XCDs (gram_xcd_spread 8, 4, 2, 1) with each
+// tile width (64 columns, and 32 for the FP32 / FP64 tiles), those not fixed by the Monte-Carlo
+// parameters: two launches each on the round in buffer `buf`, the shortest counts, on the Gram
+// stream before the round's own accumulation; keep the fastest for this layout and clear the
+// accumulators the timing launches wrote.  Once per (rows per launch, Gram kind, sector layout).
 int le_engine::tune_gram_spread(int buf)
 {
     need_spread_tune = false;
@@ -1394,39 +1423,53 @@ int le_engine::tune_gram_spread(int buf)
     std::vector<int> layout(basis.sectors);
     layout.push_back(n_user);
     layout.push_back(nh);
+    layout.push_back(mc.gram_xcd_spread);
+    layout.push_back(mc.gram_tile_cols);
     const auto key = std::make_tuple(rows, gram_prec, layout, (int)jobs.size());
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned_spread.find(key);
         if (it != g_tuned_spread.end()) {
-            gram_spread = it->second;
+            gram_spread = it->second.first;
+            gram_tc = it->second.second;
             return LE_OK;
         }
     }
     hipEvent_t t0 = next_event(), t1 = next_event();
     if (!t0 || !t1) return fail(LE_ERR_HIP, "event creation failed");
-    int best = 8;
+    std::vector<int> Ks, TCs;
+    if (mc.gram_xcd_spread > 0) Ks = {mc.gram_xcd_spread};
+    else Ks = {8, 4, 2, 1};
+    const bool split_form = gram_prec == le::kBF16X3 || gram_prec == le::kBF16X6;
+    if (mc.gram_tile_cols > 0) TCs = {mc.gram_tile_cols};
+    else if (split_form) TCs = {64};
+    else TCs = {64, 32};
+    int best = Ks[0], best_tc = TCs[0];
     float best_ms = 1e30f;
-    for (int K : {8, 4, 2, 1}) {
-        gram_spread = K;
-        for (int rep = 0; rep < 2; ++rep) {
-            HIPCHK(hipEventRecord(t0, gstream));
-            int rc = gram_rows(rows, buf, gstream);
-            if (rc) return rc;
-            HIPCHK(hipEventRecord(t1, gstream));
-            HIPCHK(hipEventSynchronize(t1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, t0, t1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = K;
+    for (int TC : TCs)
+        for (int K : Ks) {
+            gram_spread = K;
+            gram_tc = TC;
+            for (int rep = 0; rep < 2; ++rep) {
+                HIPCHK(hipEventRecord(t0, gstream));
+                int rc = gram_rows(rows, buf, gstream);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(t1, gstream));
+                HIPCHK(hipEventSynchronize(t1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = K;
+                    best_tc = TC;
+                }
             }
         }
-    }
     gram_spread = best;
+    gram_tc = best_tc;
     HIPCHK(le::launch_zero(cpool, (long)cpool_elems, gstream));
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned_spread[key] = best;
+    g_tuned_spread[key] = {best, best_tc};
     return LE_OK;
 }
 
@@ -2494,6 +2537,10 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
     if (mc->gram_xcd_spread != 0 && mc->gram_xcd_spread != 1 && mc->gram_xcd_spread != 2 && mc->gram_xcd_spread != 4 &&
         mc->gram_xcd_spread != 8)
         return fail(LE_ERR_ARG, "gram_xcd_spread must be 0, 1, 2, 4 or 8");
+    if (mc->gram_tile_cols != 0 && mc->gram_tile_cols != 32 && mc->gram_tile_cols != 64)
+        return fail(LE_ERR_ARG, "gram_tile_cols must be 0, 32 or 64");
+    if (mc->gram_tile_cols == 32 && (mc->precision == LE_PREC_BF16X3 || mc->precision == LE_PREC_BF16X6))
+        return fail(LE_ERR_ARG, "the BF16 split Gram runs 64-column tiles only");
     if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
         !le::sweep2_reg_supported(mc->lanes_per_walker, sys->particles_number))
         return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil((N - 1) / lanes) 13 or 25");
@@ -2523,6 +2570,7 @@ extern "C" int le_engine_create_occupation(const le_system_params* sys, const le
         return rc;
     }
     e->gram_spread = mc->gram_xcd_spread > 0 ? mc->gram_xcd_spread : 8;
+    e->gram_tc = mc->gram_tile_cols > 0 ? mc->gram_tile_cols : 64;
     e->G = mc->lanes_per_walker > 0 ? mc->lanes_per_walker : default_lanes(sys->particles_number, e->prec);
     if (mc->lanes_per_walker > 0 && mc->sweep_registers == 1) e->G |= kRegSweep;
     *out = e;

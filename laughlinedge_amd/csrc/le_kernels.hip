@@ -1177,12 +1177,17 @@ constexpr int kFlushSteps = 8;   // FP32: flush into FP64 every 8 x 32 samples
 // isolation, against 0.62 for 4 waves of 32 x 32 blocks (2 waves per SIMD).  The FP64 form
 // targets 5 waves per SIMD: at 6 its two column operands spilled inside the loop.
 //
-// gram_tile is the work of one workgroup of 512 threads (all of them call it): tile `tile_idx`
-// over row split `split` for job `job_idx`; k_gram maps its workgroups onto it, the fused step
-// kernel takes (tile, split) items from a queue.
-template <typename Real, int NW>
+// gram_tile is the work of one workgroup (all of its threads call it): tile `tile_idx` over row
+// split `split` for job `job_idx`; k_gram maps its workgroups onto it, the fused step kernel
+// takes (tile, split) items from a queue.  TC = 64: 64x64 tiles, 512 threads.  TC = 32: narrow
+// 64x32 tiles (rows [i0, i0 + 64), columns [j0, j0 + 32), eight 16x16 blocks, two per wave),
+// 256 threads, each staging 8 row-side and 4 column-side values per step; when the column side
+// lies inside the row side (j0 - i0 in [0, 32]) it is read from the row side's LDS rows.
+template <typename Real, int NW, int TC = 64>
 __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int split, int job_idx)
 {
+    static_assert(TC == 64 || TC == 32, "64- or 32-column tiles");
+    constexpr bool kNarrow = TC == 32;
     using acc_t = typename Mfma<Real>::acc_t;
     constexpr bool kF32 = sizeof(Real) == 4;
     // FP32: the row side is also staged times each weight other than 1; FP64 multiplies in the
@@ -1197,7 +1202,9 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
     const long r_begin = (long)split * g.rows_per_split;
     long r_end = r_begin + g.rows_per_split;
     if (r_end > g.rows) r_end = g.rows;
-    const bool same = tile.i0 == tile.j0;
+    // narrow tiles: column-side offset inside the row side's LDS rows, when it is there
+    const int coff = kNarrow ? tile.j0 - tile.i0 : 0;
+    const bool same = kNarrow ? (coff >= 0 && coff <= 32) : tile.i0 == tile.j0;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1228,7 +1235,7 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
 #pragma unroll
     for (int q = 0; q < 2; ++q)
         aq[q] = (q < NW && wp[q]) ? reinterpret_cast<const Real(*)[kLdsStride]>(&xw[q < NW ? q : 0][0][0]) : xi;
-    const int side = tid >> 8;                 // 0: row-side block, 1: column-side block
+    const int side = kNarrow ? 0 : tid >> 8;   // 0: row-side block, 1: column-side block
     const int rr = (tid & 255) >> 3;           // 0..31
     // columns cc .. cc + 3 and cc + 32 .. cc + 35: each store instruction of 8 lanes then fills
     // 128 contiguous bytes (all 32 banks once); 8 consecutive columns per lane hit every bank
@@ -1237,7 +1244,7 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
     const bool loader = !(side && same);
     const long col0 = side ? tile.j0 : tile.i0;
     Real(*dst)[kLdsStride] = side ? xj : xi;
-    Real buf[8], bw[NW];
+    Real buf[8], bufc[4], bw[NW];
     auto fetch = [&](long r0) {
         const long row = r0 + rr;
         const bool in = row < r_end;
@@ -1248,6 +1255,16 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) buf[e] = 0;
+            }
+        }
+        if constexpr (kNarrow) {
+            if (!same) {
+                if (in) {
+                    load4(X + (size_t)row * g.ldx + tile.j0 + cc, bufc);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) bufc[e] = 0;
+                }
             }
         }
         if constexpr (kPre) {
@@ -1270,6 +1287,12 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
             for (int e = 0; e < 4; ++e) {
                 dst[rr][cc + e] = buf[e];
                 dst[rr][cc + 32 + e] = buf[4 + e];
+            }
+        }
+        if constexpr (kNarrow) {
+            if (!same) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) xj[rr][cc + e] = bufc[e];
             }
         }
         if constexpr (kPre) {
@@ -1296,7 +1319,7 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
                 const int kr = kk + k4;
                 Real bv[2];
 #pragma unroll
-                for (int a = 0; a < 2; ++a) bv[a] = yj[kr][co[a] + l16];
+                for (int a = 0; a < 2; ++a) bv[a] = yj[kr][co[a] + l16 + (kNarrow && same ? coff : 0)];
 #pragma unroll
                 for (int q = 0; q < NW; ++q) {
                     Real av;
@@ -1336,12 +1359,15 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
             }
 }
 
-template <typename Real, int NW>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 5))) void k_gram(GramArgs g)
+// 64-column tiles: 6 (FP32) or 5 (FP64) waves per SIMD; 32-column tiles stage 12 values per
+// thread and hold 40 KB of LDS per 4 waves: 4 (FP32) or 3 (FP64)
+template <typename Real, int NW, int TC = 64>
+__global__ __launch_bounds__(TC == 64 ? 512 : 256)
+__attribute__((amdgpu_waves_per_eu(TC == 64 ? (sizeof(Real) == 4 ? 6 : 5) : (sizeof(Real) == 4 ? 4 : 3)))) void k_gram(GramArgs g)
 {
     int tile_idx, split;
     if (!gram_block(g, tile_idx, split)) return;
-    gram_tile<Real, NW>(g, tile_idx, split, blockIdx.y);
+    gram_tile<Real, NW, TC>(g, tile_idx, split, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2430,6 +2456,9 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
 {
     if (a.rows <= 0 || a.ntiles <= 0) return hipSuccess;
     if (a.splits != splits) return hipErrorInvalidValue;
+    // 32-column tiles: FP32 and FP64 tiles only
+    if (a.tile_cols != 0 && a.tile_cols != 64 && (a.tile_cols != 32 || prec == kBF16X3 || prec == kBF16X6))
+        return hipErrorInvalidValue;
     if (a.jobs ? (a.njobs < 1 || a.njobs > 65535) : a.njobs > 1) return hipErrorInvalidValue;
     dim3 grid((unsigned)gram_grid(a.ntiles, splits, a.xcd_spread), (unsigned)(a.jobs ? a.njobs : 1));
     if (prec == kBF16X3 || prec == kBF16X6) {
@@ -2440,6 +2469,14 @@ hipError_t launch_gram(const GramArgs& a, int prec, int splits, hipStream_t s)
         } else {
             if (a.nw == 2) hipLaunchKernelGGL((k_gram_bf16<2, 3>), grid, dim3(512), 0, s, a);
             else hipLaunchKernelGGL((k_gram_bf16<1, 3>), grid, dim3(512), 0, s, a);
+        }
+    } else if (a.tile_cols == 32) {
+        if (prec == kFP64) {
+            if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2, 32>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_gram<double, 1, 32>), grid, dim3(256), 0, s, a);
+        } else {
+            if (a.nw == 2) hipLaunchKernelGGL((k_gram<float, 2, 32>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_gram<float, 1, 32>), grid, dim3(256), 0, s, a);
         }
     } else if (prec == kFP64) {
         if (a.nw == 2) hipLaunchKernelGGL((k_gram<double, 2>), grid, dim3(512), 0, s, a);
@@ -2596,7 +2633,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                 if (w < f.walkers) monomials_sample<float>(f, w, lane, mpk[wave], mtab[wave], mplan, mcols, plan_lds);
             }
         } else {
-            gram_tile<float, 2>(st.g, j % st.g.ntiles, q + 8 * (j / st.g.ntiles), 0);
+            gram_tile<float, 2, 64>(st.g, j % st.g.ntiles, q + 8 * (j / st.g.ntiles), 0);
         }
     }
     // the last workgroup of a queue to finish clears it for the next launch

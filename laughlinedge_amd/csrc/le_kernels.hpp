@@ -123,10 +123,10 @@ struct Features2Args {
     int ps_ld;               // >= 2 K + 2
 };
 
-// A 64x64 upper Gram tile and the 16x16 blocks of it that hold requested entries, two per
-// wave and both in one row block: byte a of wblk[w] = (row block << 2) | column block, bit 4
-// set when the block only pads an odd count and is not written back.  Waves w >= nwave take
-// no blocks.
+// A 64x64 upper Gram tile (or a 64x32 one: column blocks 0 and 1 only) and the 16x16 blocks of
+// it that hold requested entries, two per wave and both in one row block: byte a of wblk[w] =
+// (row block << 2) | column block, bit 4 set when the block only pads an odd count and is not
+// written back.  Waves w >= nwave take no blocks.
 struct Tile {
     int i0, j0;              // column origins (multiples of 64)
     int nwave;
@@ -160,6 +160,7 @@ struct GramArgs {
     long ldc;
     int xcd_spread;          // 1, 2 or 4: the tiles of one row split run on that many XCDs;
                              // otherwise consecutive workgroups take consecutive tiles (all 8)
+    int tile_cols;           // 64 (or 0): 64x64 tiles; 32: 64x32 tiles (FP32 and FP64 only)
 };
 
 // Workgroups of a Gram launch: gram_grid(ntiles, splits, xcd_spread)

@@ -95,7 +95,8 @@ class MCParams(ctypes.Structure):
                 ("samples_per_launch", ctypes.c_int), ("n_blocks", ctypes.c_int),
                 ("trace_walkers", ctypes.c_int), ("graphs", ctypes.c_int), ("symmetry_moves", ctypes.c_int),
                 ("sweep_registers", ctypes.c_int), ("estimator", ctypes.c_int), ("gram_xcd_spread", ctypes.c_int),
-                ("mixture_rho", ctypes.c_double), ("fused_step", ctypes.c_int), ("fused_rows", ctypes.c_int)]
+                ("mixture_rho", ctypes.c_double), ("fused_step", ctypes.c_int), ("fused_rows", ctypes.c_int),
+                ("gram_tile_cols", ctypes.c_int)]
 
 
 class Observables(ctypes.Structure):
@@ -109,7 +110,7 @@ class EngineInfo(ctypes.Structure):
                 ("lanes_per_walker", ctypes.c_int), ("samples_per_launch", ctypes.c_int),
                 ("n_tiles", ctypes.c_int), ("x_ld", ctypes.c_long), ("sweep_in_registers", ctypes.c_int),
                 ("gram_mfma_flop_per_row", ctypes.c_double), ("gram_xcd_spread", ctypes.c_int),
-                ("chains", ctypes.c_long)]
+                ("chains", ctypes.c_long), ("gram_tile_cols", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):

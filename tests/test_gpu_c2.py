@@ -12,9 +12,10 @@ seed 42, V = 0.025 r^2 + 5e-5 r^4.
   walkers the mean is 2 (L0 + N) and the variance 4 (L0 + N).
 - Sampling rounds replayed from hipGraphs leave the same walkers and traces, bit for bit, as
   eager rounds, and the same moments up to the order of the FP64 atomic adds.
-- Every placement of the Gram tiles over the XCDs (gram_xcd_spread 1, 2, 4, 8) accumulates the
-  same moments up to the order of the FP64 atomic adds, and the autotuned engine reports the
-  placement it chose."""
+- Every placement of the Gram tiles over the XCDs (gram_xcd_spread 1, 2, 4, 8) and both tile
+  widths (64x64 and 64x32 tiles) accumulate the same FP64 moments up to the order of the FP64
+  atomic adds; the 64x32 tiles also meet the oracle at FP32 and FP64; the autotuned engine
+  reports the placement and width it chose."""
 import numpy as np
 import pytest
 
@@ -57,15 +58,18 @@ def test_c2_sampler_gamma_law(chain):
     assert abs(r2.var(ddof=1) / var - 1.0) < 5 * np.sqrt(2.0 / n)
 
 
-@pytest.mark.parametrize("prec,tol,etol", [(0, 1e-6, 2e-5), (3, 1e-6, 2e-5), (2, 2e-5, 2e-4), (1, 1e-12, 1e-10)])
-def test_c2_fixed_samples_vs_oracle(chain, oracle_lib, prec, tol, etol):
+@pytest.mark.parametrize("prec,tol,etol,cols", [(0, 1e-6, 2e-5, 64), (3, 1e-6, 2e-5, 64), (2, 2e-5, 2e-4, 64),
+                                                (1, 1e-12, 1e-10, 64), (0, 1e-6, 2e-5, 32), (1, 1e-12, 1e-10, 32)])
+def test_c2_fixed_samples_vs_oracle(chain, oracle_lib, prec, tol, etol, cols):
     from laughlinedge_amd import Engine
     pos, _ = chain
     n = pos.shape[0]
-    print(f"case c2 fixed samples precision={prec} ({n} configurations)", flush=True)
+    print(f"case c2 fixed samples precision={prec} tile columns={cols} ({n} configurations)", flush=True)
     o = oracle_lib.Oracle(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF)
     So, Ho, _ = o.accumulate_fixed(pos)
-    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=1024, samples=1, precision=prec) as f:
+    with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=1024, samples=1, precision=prec,
+                gram_tile_cols=cols) as f:
+        assert f.info.gram_tile_cols == cols
         f.accumulate_positions(pos)
         mom = f.moments()
         spec = f.spectra()
@@ -110,20 +114,25 @@ def test_c2_gram_placements_agree(chain):
     from laughlinedge_amd import Engine
     pos, step = chain
     print("case c2 Gram tile placements over the XCDs", flush=True)
-    ref = None
-    for K in (8, 4, 2, 1):
-        with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=1, gram_xcd_spread=K) as f:
-            f.accumulate_positions(pos)
-            mom = f.moments()
-        if ref is None:
-            ref = mom
-            continue
-        for (S0, H0), (S1, H1) in zip(ref, mom):
-            assert np.linalg.norm(S1 - S0) <= 1e-12 * np.linalg.norm(S0), K
-            assert np.linalg.norm(H1 - H0) <= 1e-12 * np.linalg.norm(H0), K
+    # FP32 tiles over the four placements (same row splits: only the FP64 atomic order differs);
+    # FP64 tiles over both widths (the 64x32 tiles split the rows elsewhere)
+    for prec, cases in ((0, ((8, 64), (4, 64), (2, 64), (1, 64))), (1, ((8, 64), (8, 32), (1, 32)))):
+        ref = None
+        for K, cols in cases:
+            with Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=WALKERS, samples=1, gram_xcd_spread=K,
+                        gram_tile_cols=cols, precision=prec) as f:
+                f.accumulate_positions(pos)
+                mom = f.moments()
+            if ref is None:
+                ref = mom
+                continue
+            for (S0, H0), (S1, H1) in zip(ref, mom):
+                assert np.linalg.norm(S1 - S0) <= 1e-12 * np.linalg.norm(S0), (prec, K, cols)
+                assert np.linalg.norm(H1 - H0) <= 1e-12 * np.linalg.norm(H0), (prec, K, cols)
     with _engine(tune_sweeps=0, step=step, lanes_per_walker=8, sweep_registers=1) as e:
         e.init_walkers()
         e.burnin(20)
         e.sample(e.info.samples_per_launch)
-        assert e.info.gram_xcd_spread in (1, 2, 4, 8)
-        print(f"autotuned placement: tiles of a row split on {e.info.gram_xcd_spread} XCDs")
+        assert e.info.gram_xcd_spread in (1, 2, 4, 8) and e.info.gram_tile_cols in (32, 64)
+        print(f"autotuned placement: tiles of a row split on {e.info.gram_xcd_spread} XCDs, "
+              f"{e.info.gram_tile_cols}-column tiles")

@@ -163,12 +163,13 @@ def compare_spectra(spec, sig, ref_spec, ref_sig):
             "lowest_per_sector_dev": lowest}
 
 
-def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1, sym=0, estimator=0):
+def make_engine(walkers, precision, samples, n_blocks, tw, target=0.5, graphs=1, sym=0, estimator=0, fused=0):
     from laughlinedge_amd import Engine
     return Engine(N, M_INV, SECTORS, w=W_CONF, eps=EPS_CONF, walkers=walkers, seed=SEED,
                   precision=precision, samples=samples, n_blocks=n_blocks, tune_sweeps=50, step=0.5,
                   trace_walkers=tw, target_acceptance=target, acceptance_tol=0.01, graphs=graphs,
-                  lanes_per_walker=-1, symmetry_moves=sym, estimator=estimator, mixture_rho=MIX_RHO)
+                  lanes_per_walker=-1, symmetry_moves=sym, estimator=estimator, mixture_rho=MIX_RHO,
+                  fused_step=fused)
 
 
 def timed_sampling(eng, steps, nblk):
@@ -181,7 +182,7 @@ def timed_sampling(eng, steps, nblk):
     wall = time.perf_counter() - t0
     st1 = eng.stats()
     return wall, {k: st1[k] - st0[k] for k in ("ms_sweep", "ms_features", "ms_gram", "gram_launches",
-                                               "sweep_launches", "ms_sampler_graph")}
+                                               "sweep_launches", "ms_sampler_graph", "ms_step", "step_launches")}
 
 
 ERR_BLOCKS, ERR_BLOCK_SAMPLES = 8, 8
@@ -199,6 +200,10 @@ def main():
     ap.add_argument("--thins", default="1,2,4,8")
     ap.add_argument("--estimator", choices=("plain", "mixture"), default="plain",
                     help="headline estimator: plain |Psi_L|^2 walkers (default) or the mixture w |Psi_L|^2")
+    ap.add_argument("--fused", action="store_true",
+                    help="headline with the fused step kernel (one launch per kept sample, fused_step = 1): "
+                         "the command a kernel-trace profile of k_step runs; its roofline block reports the "
+                         "fused launches, not a Gram kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
@@ -232,7 +237,8 @@ def main():
 
     steps, warm, tw = args.steps, args.warmup, min(args.trace_walkers, args.walkers)
     est = 2 if args.estimator == "mixture" else 0
-    eng = make_engine(args.walkers, 0, steps + warm + ERR_BLOCKS * ERR_BLOCK_SAMPLES, 1, tw, estimator=est)
+    eng = make_engine(args.walkers, 0, steps + warm + ERR_BLOCKS * ERR_BLOCK_SAMPLES, 1, tw, estimator=est,
+                      fused=int(args.fused))
     info, dims = eng.info, eng.dims
     sum_d2 = sum(d * d for d in dims)
     log(f"N={N} sectors=1..16 D={info.D} sum_d2={sum_d2} tiles={info.n_tiles} walkers={args.walkers} "
@@ -266,8 +272,10 @@ def main():
     gram_flop = 16.0 * sum_d2 * updates
     herm_flop = 8.0 * sum(x * (x + 1) for x in dims) * updates
     launches = max(1, d["gram_launches"])
-    avg_launch_ms = d["ms_gram"] / launches
-    gram_rate = gram_flop / (d["ms_gram"] * 1e-3)
+    # fused steps: the Gram tiles run inside the step launches (timed as a whole, ms_step)
+    ms_gram = d["ms_gram"] if not args.fused else d["ms_step"]
+    avg_launch_ms = ms_gram / launches
+    gram_rate = gram_flop / (ms_gram * 1e-3)
     rows_per_launch = info.samples_per_launch * args.walkers
     traffic, traffic_note = gram_traffic(rows_per_launch)
     tau_max = tau_sh(taus) if taus else None
@@ -308,9 +316,9 @@ def main():
                      "algorithmic_bytes_per_launch": rows_per_launch * (8 * info.D + 8),
                      "rows_per_launch": rows_per_launch,
                      "hermitian_flop_per_launch": herm_flop / launches,
-                     "hermitian_frac": herm_flop / (d["ms_gram"] * 1e-3) / PEAK_FP32,
+                     "hermitian_frac": herm_flop / (ms_gram * 1e-3) / PEAK_FP32,
                      "issued_mfma_flop_per_launch": info.gram_mfma_flop_per_row * updates / launches,
-                     "issued_frac": info.gram_mfma_flop_per_row * updates / (d["ms_gram"] * 1e-3) / PEAK_FP32,
+                     "issued_frac": info.gram_mfma_flop_per_row * updates / (ms_gram * 1e-3) / PEAK_FP32,
                      "flop_note": "algorithmic = 16 sum d^2 per sample (complex GEMM count over whole "
                                   "Hermitian blocks, S and H); hermitian = 8 sum d(d+1) (upper triangles "
                                   "only); the kernel issues MFMAs only for the 16x16 blocks holding "
@@ -328,6 +336,12 @@ def main():
                           "note": "FP64 Cholesky + tridiagonalization + bisection on the GPU, all sectors "
                                   "in one launch, host copies included"},
     }
+    if args.fused:
+        out["config"]["sampler_rounds"] = "fused launches, one per kept sample (fused_step = 1)"
+        out["roofline"].update({
+            "kernel": "k_step<8, 25, 16> (fused: sweeps, features and Gram tiles)", "bound": "valu+mfma",
+            "launch_time_source": "HIP events around all fused launches of the timed le_sample call "
+                                  "(the closing Gram launch included); achieved counts the Gram FLOPs only"})
     iso_ms = eng.stats()["ms_gram_launch_min"]
     if iso_ms > 0:
         iso_rate = 16.0 * sum_d2 * rows_per_launch / (iso_ms * 1e-3)

@@ -204,26 +204,40 @@ def test_scale_evolution_of_kicks(sampled):
     assert ratio < 1e-2, ratio
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5), (3, 1e-6)])
-def test_config1_fixed_samples(oracle_lib, prec, tol):
+@pytest.fixture(scope="module")
+def config1_samples(oracle_lib):
     """nu = 1/3, N = 100, V = 0.025 r^2 + 5e-5 r^4, dL = 0..12 (D = 272), 16384 walkers, seed 7:
-    65536 configurations of the GPU chain through the FP32 (or a BF16-split) accumulation and
-    through the FP64 oracle; every sector's S and H within tol relative Frobenius."""
+    1,048,576 configurations of the GPU chain (64 snapshots of the walkers, 2 sweeps apart, after
+    tuning and 200 burn-in sweeps) and their FP64 oracle sums (or_accumulate_fixed)."""
     from laughlinedge_amd import Engine
     n100, secs = 100, list(range(13))
-    print(f"case config 1 fixed samples prec={prec}", flush=True)
+    print("case config 1: 1,048,576 configurations and their FP64 oracle sums", flush=True)
     with Engine(n100, 3, secs, w=0.05, eps=5e-5, walkers=16384, samples=4, seed=7, tune_sweeps=30,
                 step=0.5) as e:
         assert e.info.D == 272
         e.init_walkers()
         e.tune_step()
+        e.burnin(200)
         pos = []
-        for _ in range(4):
-            e.burnin(50)
+        for _ in range(64):
+            e.burnin(2)
             pos.append(e.positions())
     pos = np.concatenate(pos)
+    assert pos.shape == (1 << 20, n100, 2)
     o = oracle_lib.Oracle(n100, 3, secs, w=0.05, eps=5e-5)
     So, Ho, _ = o.accumulate_fixed(pos)
+    return pos, So, Ho
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (2, 2e-5), (3, 1e-6)])
+def test_config1_fixed_samples(config1_samples, prec, tol):
+    """The configuration-1 fixed-sample comparison at its configured size: the 1,048,576
+    configurations through the FP32 (or a BF16-split) accumulation and through the FP64 oracle;
+    every sector's S and H within tol relative Frobenius."""
+    from laughlinedge_amd import Engine
+    pos, So, Ho = config1_samples
+    n100, secs = 100, list(range(13))
+    print(f"case config 1 fixed samples prec={prec}", flush=True)
     n = pos.shape[0]
     with Engine(n100, 3, secs, w=0.05, eps=5e-5, walkers=16384, samples=1, precision=prec) as f:
         f.accumulate_positions(pos)
@@ -234,7 +248,7 @@ def test_config1_fixed_samples(oracle_lib, prec, tol):
         eh = np.linalg.norm(Hg - Ho[s] / n) / np.linalg.norm(Ho[s] / n)
         worst = max(worst, es, eh)
         assert es < tol and eh < tol, (s, es, eh)
-    print(f"worst relative Frobenius error {worst:.3e}")
+    print(f"worst relative Frobenius error {worst:.3e} over {n} configurations")
 
 
 def test_dsf_harmonic_blocks_n200(oracle_lib):

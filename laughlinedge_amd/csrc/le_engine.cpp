@@ -86,6 +86,74 @@ namespace le {
 void set_error(const std::string& msg) { g_err = msg; }
 }  // namespace le
 
+// Block records.  Records of up to 64 MB live in pinned host memory: each closed block is
+// copied from the device straight into its slot (no host-side copy); slots come from chunks of
+// ~64 MB allocated once and reused after le_reset_moments.  A copy into freshly allocated
+// pageable memory cost ~1.5 ms per 4 MB record in page faults, inside every sampling call.
+// Larger records (many weights on large bases) go through the pinned staging buffer into
+// pageable vectors, as pinned chunks of their size would tie up gigabytes.
+struct RecordStore {
+    std::vector<double*> chunks;
+    std::vector<std::vector<double>> big;   // records too large for pinned slots
+    std::vector<const double*> recs;        // record b
+    size_t len = 0;                         // doubles per record
+    size_t per_chunk = 1;
+    size_t used = 0;                        // pinned slots handed out
+    bool pinned = true;
+    static constexpr size_t kChunkBytes = (size_t)64 << 20;
+    hipError_t init(size_t doubles)
+    {
+        len = std::max<size_t>(2, doubles);
+        pinned = len * sizeof(double) <= kChunkBytes;
+        per_chunk = std::max<size_t>(1, kChunkBytes / (len * sizeof(double)));
+        return pinned ? grow() : hipSuccess;
+    }
+    hipError_t grow()
+    {
+        double* c = nullptr;
+        hipError_t h = hipHostMalloc((void**)&c, sizeof(double) * len * per_chunk, hipHostMallocDefault);
+        if (h == hipSuccess) chunks.push_back(c);
+        return h;
+    }
+    // where the next record is copied from the device: its pinned slot, or the staging buffer
+    hipError_t next(double* staging, double** out)
+    {
+        if (!pinned) {
+            *out = staging;
+            return hipSuccess;
+        }
+        if (used == chunks.size() * per_chunk) {
+            hipError_t h = grow();
+            if (h != hipSuccess) return h;
+        }
+        *out = chunks[used / per_chunk] + (used % per_chunk) * len;
+        return hipSuccess;
+    }
+    // after the copy has completed
+    void push(const double* slot)
+    {
+        if (pinned) {
+            recs.push_back(slot);
+            ++used;
+        } else {
+            big.emplace_back(slot, slot + len);
+            recs.push_back(big.back().data());
+        }
+    }
+    size_t size() const { return recs.size(); }
+    const double* operator[](size_t b) const { return recs[b]; }
+    void clear()
+    {
+        recs.clear();
+        big.clear();
+        used = 0;
+    }
+    ~RecordStore()
+    {
+        for (double* c : chunks) (void)hipHostFree(c);
+    }
+};
+
 struct le_engine {
     le_system_params sys{};
     le_mc_params mc{};
@@ -200,7 +268,7 @@ struct le_engine {
     long rec_elems = 0;              // complex elements per block record
     le::ExtractItem* items_d = nullptr;
     double* rec_d = nullptr;
-    double* rec_h = nullptr;         // pinned staging of one block record (2 rec_elems doubles)
+    double* rec_h = nullptr;         // pinned staging of one block record too large for the pinned slots
     unsigned* acc_h = nullptr;       // pinned staging of the per-walker accepted-move counts
     float* trace_d = nullptr;
     long trace_cap_rows = 0;
@@ -209,7 +277,7 @@ struct le_engine {
     size_t eig_work_elems = 0;
 
     // host state: one record (all items) per closed block
-    std::vector<std::vector<double>> rec;
+    RecordStore rec;
     std::vector<long long> blk_samples;
     std::vector<double> blk_norm;    // per block: the samples, or the sum of 1 / w (mixture)
     bool walkers_ready = false;
@@ -547,6 +615,7 @@ int le_engine::setup(const le_observables* obs)
     // bounce buffer at ~2 GB/s (2 ms for the 4 MB record of N = 200, dL 1..16)
     HIPCHK(hipHostMalloc((void**)&rec_h, sizeof(double) * (size_t)std::max<long>(2, 2 * rec_elems),
                          hipHostMallocDefault));
+    if (!rowwise) HIPCHK(rec.init((size_t)2 * rec_elems));
     HIPCHK(hipHostMalloc((void**)&acc_h, sizeof(unsigned) * (size_t)std::max(1, mc.walkers), hipHostMallocDefault));
 
     // kept samples per launch round: ~128k rows, X under 1 GiB
@@ -686,6 +755,7 @@ int le_engine::setup(const le_observables* obs)
         HIPCHK(hipMemcpy(sec_bo_d, bo.data(), bo.size() * sizeof(long), hipMemcpyHostToDevice));
         if (rec_h) (void)hipHostFree(rec_h);
         HIPCHK(hipHostMalloc((void**)&rec_h, sizeof(double) * (size_t)std::max<long>(2, 5 * be), hipHostMallocDefault));
+        HIPCHK(rec.init((size_t)5 * be));
     }
     ev.resize(2 * spl + 4);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -1334,10 +1404,12 @@ int le_engine::close_block(long long samples)
     x.scale = norm > 0.0 ? 1.0 / norm : 0.0;
     x.out = rec_d;
     HIPCHK(le::launch_extract(x, stream));
-    HIPCHK(hipMemcpyAsync(rec_h, rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
+    double* slot = nullptr;
+    HIPCHK(rec.next(rec_h, &slot));
+    HIPCHK(hipMemcpyAsync(slot, rec_d, sizeof(double) * 2 * rec_elems, hipMemcpyDeviceToHost, stream));
     HIPCHK(le::launch_zero(cpool, (long)cpool_elems, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    rec.emplace_back(rec_h, rec_h + 2 * rec_elems);
+    rec.push(slot);
     block_grams = 0;
     blk_samples.push_back(samples);
     blk_norm.push_back(norm);
@@ -1406,6 +1478,12 @@ int le_engine::capture_round(le::FeatureArgs f, int buf, bool traced, int tw, hi
     h = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (h != hipSuccess) return fail(LE_ERR_HIP, hipGetErrorString(h));
+    // upload now, so that the graph's first replay (possibly inside a timed call) does not pay it
+    h = hipGraphUpload(ex, stream);
+    if (h != hipSuccess) {
+        (void)hipGraphExecDestroy(ex);
+        return fail(LE_ERR_HIP, hipGetErrorString(h));
+    }
     graphs.push_back({ex, buf, traced, mc.thin, proposal, G, spl, table_T, step, table});
     *out = ex;
     return LE_OK;
@@ -1535,10 +1613,12 @@ int le_engine::sample_rowwise(int samples)
     // close the block: raw sums to the host, accumulators cleared
     const long n5 = 5 * basis.block_elems();
     HIPCHK(hipMemcpyAsync(acc_h, acc, sizeof(unsigned) * mc.walkers, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(rec_h, row_acc, sizeof(double) * n5, hipMemcpyDeviceToHost, stream));
+    double* slot = nullptr;
+    HIPCHK(rec.next(rec_h, &slot));
+    HIPCHK(hipMemcpyAsync(slot, row_acc, sizeof(double) * n5, hipMemcpyDeviceToHost, stream));
     HIPCHK(le::launch_zero(row_acc, n5, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    rec.emplace_back(rec_h, rec_h + n5);
+    rec.push(slot);
     blk_samples.push_back((long long)samples * row_W);     // samples per row
     double tot = 0;
     for (int w = 0; w < mc.walkers; ++w) tot += acc_h[w];
@@ -1882,7 +1962,7 @@ static int record_slice(const le_engine* e, int block, long off, long n, double*
     if (nb == 0) return fail(LE_ERR_STATE, "no closed sample blocks");
     if (block >= nb || block < -1) return fail(LE_ERR_ARG, "block out of range");
     if (block >= 0) {
-        std::memcpy(out, e->rec[block].data() + 2 * off, sizeof(double) * 2 * n);
+        std::memcpy(out, e->rec[block] + 2 * off, sizeof(double) * 2 * n);
         return LE_OK;
     }
     // blocks weigh by their samples (mixture: by their sums of 1 / w)

@@ -1105,6 +1105,15 @@ template <> struct Mfma<double> {
     __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 };
 
+// Loads through a pointer known to address global memory count only in vmcnt.  A load through a
+// generic (flat) pointer counts in lgkmcnt too, so the next wait for an LDS read would also wait
+// for it, and a prefetch issued before the LDS reads of a step would stall them.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ void load4(const float* p, float* o)
 {
     const float4 a = reinterpret_cast<const float4*>(p)[0];
@@ -1244,44 +1253,48 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
     const bool loader = !(side && same);
     const long col0 = side ? tile.j0 : tile.i0;
     Real(*dst)[kLdsStride] = side ? xj : xi;
-    Real buf[8], bufc[4], bw[NW];
+    Real buf[8] = {}, bufc[4] = {}, bw[NW] = {};
+    // The prefetch loads are unconditional (a row past r_end loads row r_end - 1, and its values
+    // are zeroed when staged: bin, bwin), the weights are read through global pointers, and no
+    // register that a load fills is written on another branch.  The compiler then waits for
+    // them only where they are staged, one step later.  A conditional zero, or a weight read
+    // through a generic pointer, made it wait for the whole prefetch before the MFMA loop
+    // (4% of the launch at N = 200, dL 1..16).
+    bool bin = false, bwin = false;
     auto fetch = [&](long r0) {
         const long row = r0 + rr;
-        const bool in = row < r_end;
+        bin = row < r_end;
+        const long rc = bin ? row : r_end - 1;
         if (loader) {
-            if (in) {
-                load4(X + (size_t)row * g.ldx + col0 + cc, buf);
-                load4(X + (size_t)row * g.ldx + col0 + cc + 32, buf + 4);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) buf[e] = 0;
-            }
+            load4(X + (size_t)rc * g.ldx + col0 + cc, buf);
+            load4(X + (size_t)rc * g.ldx + col0 + cc + 32, buf + 4);
         }
         if constexpr (kNarrow) {
-            if (!same) {
-                if (in) {
-                    load4(X + (size_t)row * g.ldx + tile.j0 + cc, bufc);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) bufc[e] = 0;
-                }
-            }
+            if (!same) load4(X + (size_t)rc * g.ldx + tile.j0 + cc, bufc);
         }
         if constexpr (kPre) {
             if (!side) {
 #pragma unroll
-                for (int q = 0; q < NW; ++q) bw[q] = (in && wp[q]) ? wp[q][row] : (Real)0;
+                for (int q = 0; q < NW; ++q)
+                    if (wp[q]) bw[q] = as_global(wp[q])[rc];
             }
         } else if (tid < NW * kKC) {
             const int q = tid / kKC, k = tid % kKC;
             const long row2 = r0 + k;
-            bw[0] = (row2 < r_end) ? (wp[q] ? wp[q][row2] : (Real)1) : (Real)0;
+            bwin = row2 < r_end;
+            if (wp[q]) bw[0] = as_global(wp[q])[bwin ? row2 : r_end - 1];
         }
     };
     if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         __syncthreads();   // previous step's readers are done
+        if (!bin) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) buf[e] = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bufc[e] = 0;
+        }
         if (loader) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1308,7 +1321,7 @@ __device__ __forceinline__ void gram_tile(const GramArgs& g, int tile_idx, int s
                     }
             }
         } else if (tid < NW * kKC) {
-            vs[tid / kKC][tid % kKC] = bw[0];
+            vs[tid / kKC][tid % kKC] = bwin ? (wp[tid / kKC] ? bw[0] : (Real)1) : (Real)0;
         }
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
@@ -1450,16 +1463,24 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     const int side = wave >> 2;
     const int sc = lane, sk = (wave & 3) * 8;
     const long col = (side ? tile.j0 : tile.i0) + sc;
-    float x[8], wv[NW][8];
+    // unconditional prefetch loads (rows past r_end load row r_end - 1 and are zeroed when
+    // staged, inmask), weights through global pointers, as in gram_tile: the compiler then
+    // waits for them where they are staged, not before the MFMA loop
+    float x[8] = {}, wv[NW][8] = {};
+    uint32_t inmask = 0;
     auto fetch = [&](long r0) {
+        inmask = 0;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const long row = r0 + sk + e;
             const bool in = row < r_end;
-            x[e] = in ? X[(size_t)row * g.ldx + col] : 0.f;
+            inmask |= (uint32_t)in << e;
+            const long rc = in ? row : r_end - 1;
+            x[e] = X[(size_t)rc * g.ldx + col];
             if (!side) {
 #pragma unroll
-                for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
+                for (int q = 0; q < NW; ++q)
+                    if (wp[q]) wv[q][e] = as_global(wp[q])[rc];
             }
         }
     };
@@ -1468,12 +1489,15 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         bf16x8 t[NS];
         __syncthreads();   // the previous step's fragments have been read
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            if (!(inmask >> e & 1)) x[e] = 0.f;
         if (!side) {
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 float y[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) y[e] = x[e] * wv[q][e];
+                for (int e = 0; e < 8; ++e) y[e] = wp[q] ? x[e] * wv[q][e] : x[e];
                 split8<NS>(y, t);
 #pragma unroll
                 for (int a = 0; a < NS; ++a) *reinterpret_cast<bf16x8*>(&ai[q][a][sc][sk]) = t[a];
@@ -1601,27 +1625,38 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
     const long xbase = xside ? g.c0 : g.r0;
     const bool wq_ok = q0 + sq < g.nweights;
     const Real* wrow = W + (size_t)(wq_ok ? q0 + sq : 0) * g.wld;
-    Real wb[4], xb[8];
+    // unconditional prefetch loads at clamped rows and columns, zeroed when staged (wmask,
+    // xmask), so the compiler waits for them where they are staged and not before the MFMAs
+    Real wb[4] = {}, xb[8] = {};
+    uint32_t wmask = 0, xmask = 0;
     auto fetch = [&](long r0) {
+        wmask = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const long row = r0 + ss + e;
-            wb[e] = (wq_ok && row < r_end) ? wrow[row] : (Real)0;
+            const bool in = wq_ok && row < r_end;
+            wmask |= (uint32_t)in << e;
+            wb[e] = wrow[row < r_end ? row : r_end - 1];
         }
         const long row = r0 + xs;
-        const Real* xp = X + (size_t)(row < r_end ? row : 0) * g.ldx + xbase;
+        const bool rin = row < r_end;
+        const Real* xp = X + (size_t)(rin ? row : r_end - 1) * g.ldx + xbase;
+        xmask = 0;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xb[e] = (row < r_end && xcol + e < xn) ? xp[xcol + e] : (Real)0;
+        for (int e = 0; e < 8; ++e) {
+            xmask |= (uint32_t)(rin && xcol + e < xn) << e;
+            xb[e] = xp[xcol + e < xn ? xcol + e : xn - 1];
+        }
     };
     if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         __syncthreads();   // previous step's readers are done
 #pragma unroll
-        for (int e = 0; e < 4; ++e) wt[ss + e][sq] = wb[e];
+        for (int e = 0; e < 4; ++e) wt[ss + e][sq] = (wmask >> e & 1) ? wb[e] : (Real)0;
         Real(*xd)[kLdsStride] = xside ? xc : xr;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xd[xs][xcol + e] = xb[e];
+        for (int e = 0; e < 8; ++e) xd[xs][xcol + e] = (xmask >> e & 1) ? xb[e] : (Real)0;
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
         if (active) {

@@ -1463,24 +1463,16 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     const int side = wave >> 2;
     const int sc = lane, sk = (wave & 3) * 8;
     const long col = (side ? tile.j0 : tile.i0) + sc;
-    // unconditional prefetch loads (rows past r_end load row r_end - 1 and are zeroed when
-    // staged, inmask), weights through global pointers, as in gram_tile: the compiler then
-    // waits for them where they are staged, not before the MFMA loop
-    float x[8] = {}, wv[NW][8] = {};
-    uint32_t inmask = 0;
+    float x[8], wv[NW][8];
     auto fetch = [&](long r0) {
-        inmask = 0;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const long row = r0 + sk + e;
             const bool in = row < r_end;
-            inmask |= (uint32_t)in << e;
-            const long rc = in ? row : r_end - 1;
-            x[e] = X[(size_t)rc * g.ldx + col];
+            x[e] = in ? X[(size_t)row * g.ldx + col] : 0.f;
             if (!side) {
 #pragma unroll
-                for (int q = 0; q < NW; ++q)
-                    if (wp[q]) wv[q][e] = as_global(wp[q])[rc];
+                for (int q = 0; q < NW; ++q) wv[q][e] = (in && wp[q]) ? wp[q][row] : 1.f;
             }
         }
     };
@@ -1489,15 +1481,12 @@ __global__ __launch_bounds__(512) void k_gram_bf16(GramArgs g)
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         bf16x8 t[NS];
         __syncthreads();   // the previous step's fragments have been read
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            if (!(inmask >> e & 1)) x[e] = 0.f;
         if (!side) {
 #pragma unroll
             for (int q = 0; q < NW; ++q) {
                 float y[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) y[e] = wp[q] ? x[e] * wv[q][e] : x[e];
+                for (int e = 0; e < 8; ++e) y[e] = x[e] * wv[q][e];
                 split8<NS>(y, t);
 #pragma unroll
                 for (int a = 0; a < NS; ++a) *reinterpret_cast<bf16x8*>(&ai[q][a][sc][sk]) = t[a];
@@ -1625,38 +1614,27 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(Real
     const long xbase = xside ? g.c0 : g.r0;
     const bool wq_ok = q0 + sq < g.nweights;
     const Real* wrow = W + (size_t)(wq_ok ? q0 + sq : 0) * g.wld;
-    // unconditional prefetch loads at clamped rows and columns, zeroed when staged (wmask,
-    // xmask), so the compiler waits for them where they are staged and not before the MFMAs
-    Real wb[4] = {}, xb[8] = {};
-    uint32_t wmask = 0, xmask = 0;
+    Real wb[4], xb[8];
     auto fetch = [&](long r0) {
-        wmask = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const long row = r0 + ss + e;
-            const bool in = wq_ok && row < r_end;
-            wmask |= (uint32_t)in << e;
-            wb[e] = wrow[row < r_end ? row : r_end - 1];
+            wb[e] = (wq_ok && row < r_end) ? wrow[row] : (Real)0;
         }
         const long row = r0 + xs;
-        const bool rin = row < r_end;
-        const Real* xp = X + (size_t)(rin ? row : r_end - 1) * g.ldx + xbase;
-        xmask = 0;
+        const Real* xp = X + (size_t)(row < r_end ? row : 0) * g.ldx + xbase;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            xmask |= (uint32_t)(rin && xcol + e < xn) << e;
-            xb[e] = xp[xcol + e < xn ? xcol + e : xn - 1];
-        }
+        for (int e = 0; e < 8; ++e) xb[e] = (row < r_end && xcol + e < xn) ? xp[xcol + e] : (Real)0;
     };
     if (r_begin < r_end) fetch(r_begin);
     int steps = 0;
     for (long r0 = r_begin; r0 < r_end; r0 += kKC) {
         __syncthreads();   // previous step's readers are done
 #pragma unroll
-        for (int e = 0; e < 4; ++e) wt[ss + e][sq] = (wmask >> e & 1) ? wb[e] : (Real)0;
+        for (int e = 0; e < 4; ++e) wt[ss + e][sq] = wb[e];
         Real(*xd)[kLdsStride] = xside ? xc : xr;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xd[xs][xcol + e] = (xmask >> e & 1) ? xb[e] : (Real)0;
+        for (int e = 0; e < 8; ++e) xd[xs][xcol + e] = xb[e];
         __syncthreads();
         if (r0 + kKC < r_end) fetch(r0 + kKC);
         if (active) {

@@ -53,6 +53,60 @@ hipError_t dalloc(T** p, size_t count)
     return hipMalloc((void**)p, count * sizeof(T));
 }
 
+// Gram tiles of 64 rows x TC columns (TC = 64, 32) over the sector pairs `blocks`, sector s in
+// sample-row columns [colb[s], colb[s] + 2 d_s).  Per tile, the 16x16 blocks that hold an entry
+// the extraction reads (one with row <= column inside a requested block or its transpose), two
+// to a wave within a row block so that they share the row operand; an odd block of a row block
+// is computed again and not written.  The Gram kernels compute only these (N = 200, dL 1..16:
+// 0.76 of the products of whole tiles).  flop_row gains the MFMA FLOP per sample row of nw
+// weights.
+std::vector<le::Tile> gram_tiles(const le::Basis& basis, const std::vector<int>& colb,
+                                 const std::vector<std::pair<int, int>>& blocks, int TC, int nw, double& flop_row)
+{
+    auto meets = [](int r0, int r1, int q0, int q1, int rb0, int rb1, int qb0, int qb1) {
+        const int ra = std::max(r0, rb0), rb = std::min(r1, rb1);
+        const int qa = std::max(q0, qb0), qb = std::min(q1, qb1);
+        return ra <= rb && qa <= qb && ra <= qb;
+    };
+    std::set<std::pair<int, int>> tset;
+    for (auto& pr : blocks) {
+        const int c1 = colb[pr.first], e1 = c1 + 2 * basis.dim(pr.first) - 1;
+        const int c2 = colb[pr.second], e2 = c2 + 2 * basis.dim(pr.second) - 1;
+        for (int a = c1 / 64; a <= e1 / 64; ++a)
+            for (int b = c2 / TC; b <= e2 / TC; ++b)
+                if (64 * a <= TC * b + TC - 1) tset.insert({a, b});
+    }
+    std::vector<le::Tile> th;
+    for (auto& pr : tset) {
+        le::Tile t{};
+        t.i0 = pr.first * 64;
+        t.j0 = pr.second * TC;
+        std::vector<uint32_t> act;
+        for (int rb = 0; rb < 4; ++rb) {
+            const size_t first = act.size();
+            for (int cb = 0; cb < TC / 16; ++cb) {
+                const int r0 = t.i0 + 16 * rb, q0 = t.j0 + 16 * cb;
+                for (auto& bp : blocks) {
+                    const int c1 = colb[bp.first], e1 = c1 + 2 * basis.dim(bp.first) - 1;
+                    const int c2 = colb[bp.second], e2 = c2 + 2 * basis.dim(bp.second) - 1;
+                    if (meets(r0, r0 + 15, q0, q0 + 15, c1, e1, c2, e2) ||
+                        meets(r0, r0 + 15, q0, q0 + 15, c2, e2, c1, e1)) {
+                        act.push_back((uint32_t)(rb << 2 | cb));
+                        break;
+                    }
+                }
+            }
+            if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
+        }
+        if (act.empty()) continue;
+        t.nwave = (int)act.size() / 2;
+        flop_row += 2.0 * 16 * 16 * (double)act.size() * nw;
+        for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
+        th.push_back(t);
+    }
+    return th;
+}
+
 // Default walker mapping: the smallest power of two >= N / 16 lanes per walker (about 13-25
 // pair slots per lane), in [2, 64].  Deterministic, so a seed fixes the chain.
 // Lanes per walker without timing: the smallest power of two with at most 16 particle
@@ -187,6 +241,8 @@ struct le_engine {
     int* pwr_d = nullptr;
     int npw = 0;
     long ldx = 0;
+    // first column of sector s in a sample row; sector s takes [colb[s], colb[s] + 2 d_s)
+    std::vector<int> colb;
     // weights: row 0 = V, rows 1..2 nh = cos/sin harmonics, then user weights
     void* Wb = nullptr;
     long wrows = 0;
@@ -476,16 +532,6 @@ int le_engine::setup(const le_observables* obs)
     rsz = (prec == LE_PREC_FP64) ? sizeof(double) : sizeof(float);
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 
-    // real-embedded columns: sector s occupies [2 o_s, 2 o_s + 2 d_s): Re then Im
-    ldx = std::max<long>(64, ((2L * D + 63) / 64) * 64);
-    std::vector<int> cre(D), cdim(D);
-    for (int s = 0; s < ns; ++s) {
-        const int o = basis.sector_offset[s], d = basis.dim(s);
-        for (int t = 0; t < d; ++t) {
-            cre[o + t] = 2 * o + t;
-            cdim[o + t] = d;
-        }
-    }
     if (obs) {
         nh = obs->n_harmonics;
         for (int h = 0; h < nh; ++h) harm[h] = obs->harmonics[h];
@@ -525,58 +571,48 @@ int le_engine::setup(const le_observables* obs)
         if (k + 1 < n_user) add_job(LE_W_USER(k), LE_W_USER(k + 1), 2, all);
         else add_job(LE_W_USER(k), 0, 1, all);
     }
+    // Sample-row layout: sector s takes columns [colb[s], colb[s] + 2 d_s), Re then Im.  Three
+    // candidates: sectors packed back to back; every sector starting on a multiple of 16
+    // columns, so that no 16x16 Gram block straddles two sectors; only sectors of 16 columns
+    // or more aligned.  The one with the least Gram work is kept, counting the computed
+    // 16x16 blocks of all jobs' 64-column tiles plus 1.5 per tile for its staging
+    // (N = 200, dL 1..16: aligned, 1180 blocks in 97 tiles against 1258 in 103 packed; many
+    // small sectors stay packed).  Padding columns stay zero.
+    {
+        double best = -1.0;
+        for (int mode = 0; mode < 3; ++mode) {
+            std::vector<int> cb(ns + 1, 0);
+            for (int s = 0; s < ns; ++s) {
+                const int w = 2 * basis.dim(s);
+                int start = cb[s];
+                if (mode == 1 || (mode == 2 && w >= 16)) start = (start + 15) / 16 * 16;
+                cb[s] = start;
+                cb[s + 1] = start + w;
+            }
+            double cost = 0.0;
+            for (auto& j : jobs) {
+                double fl = 0.0;
+                const std::vector<le::Tile> th = gram_tiles(basis, cb, j.blocks, 64, 1, fl);
+                cost += fl / (2.0 * 16 * 16) + 1.5 * (double)th.size();
+            }
+            if (best < 0.0 || cost < best) {
+                best = cost;
+                colb = cb;
+            }
+        }
+    }
+    ldx = std::max<long>(64, ((long)colb[ns] + 63) / 64 * 64);
+    std::vector<int> cre(D), cdim(D);
+    for (int s = 0; s < ns; ++s) {
+        const int o = basis.sector_offset[s], d = basis.dim(s);
+        for (int t = 0; t < d; ++t) {
+            cre[o + t] = colb[s] + t;
+            cdim[o + t] = d;
+        }
+    }
     for (auto& j : jobs) {
-        // per tile, the 16x16 blocks that hold an entry the extraction reads (one with
-        // row <= column inside a requested block or its transpose), two to a wave within a row
-        // block so that they share the row operand; the Gram kernels compute only these
-        // (N = 200, dL 1..16: 0.76 of the products of whole tiles)
-        auto meets = [](int r0, int r1, int q0, int q1, int rb0, int rb1, int qb0, int qb1) {
-            const int ra = std::max(r0, rb0), rb = std::min(r1, rb1);
-            const int qa = std::max(q0, qb0), qb = std::min(q1, qb1);
-            return ra <= rb && qa <= qb && ra <= qb;
-        };
-        // tiles of 64 rows x TC columns (TC = 64, 32) with their active blocks
-        auto build = [&](int TC, double& flop_row) {
-            std::set<std::pair<int, int>> tset;
-            for (auto& pr : j.blocks) {
-                const int c1 = 2 * basis.sector_offset[pr.first], e1 = c1 + 2 * basis.dim(pr.first) - 1;
-                const int c2 = 2 * basis.sector_offset[pr.second], e2 = c2 + 2 * basis.dim(pr.second) - 1;
-                for (int a = c1 / 64; a <= e1 / 64; ++a)
-                    for (int b = c2 / TC; b <= e2 / TC; ++b)
-                        if (64 * a <= TC * b + TC - 1) tset.insert({a, b});
-            }
-            std::vector<le::Tile> th;
-            for (auto& pr : tset) {
-                le::Tile t{};
-                t.i0 = pr.first * 64;
-                t.j0 = pr.second * TC;
-                std::vector<uint32_t> act;
-                for (int rb = 0; rb < 4; ++rb) {
-                    const size_t first = act.size();
-                    for (int cb = 0; cb < TC / 16; ++cb) {
-                        const int r0 = t.i0 + 16 * rb, q0 = t.j0 + 16 * cb;
-                        for (auto& bp : j.blocks) {
-                            const int c1 = 2 * basis.sector_offset[bp.first], e1 = c1 + 2 * basis.dim(bp.first) - 1;
-                            const int c2 = 2 * basis.sector_offset[bp.second], e2 = c2 + 2 * basis.dim(bp.second) - 1;
-                            if (meets(r0, r0 + 15, q0, q0 + 15, c1, e1, c2, e2) ||
-                                meets(r0, r0 + 15, q0, q0 + 15, c2, e2, c1, e1)) {
-                                act.push_back((uint32_t)(rb << 2 | cb));
-                                break;
-                            }
-                        }
-                    }
-                    if ((act.size() - first) & 1) act.push_back(act.back() | 16u);   // computed again, not written
-                }
-                if (act.empty()) continue;
-                t.nwave = (int)act.size() / 2;
-                flop_row += 2.0 * 16 * 16 * (double)act.size() * j.nw;
-                for (int w = 0; w < t.nwave; ++w) t.wblk[w] = act[2 * w] | act[2 * w + 1] << 8;
-                th.push_back(t);
-            }
-            return th;
-        };
-        std::vector<le::Tile> th = build(64, gram_flop_row);
-        std::vector<le::Tile> th32 = build(32, gram_flop_row32);
+        std::vector<le::Tile> th = gram_tiles(basis, colb, j.blocks, 64, j.nw, gram_flop_row);
+        std::vector<le::Tile> th32 = gram_tiles(basis, colb, j.blocks, 32, j.nw, gram_flop_row32);
         j.ntiles = (int)th.size();
         HIPCHK(dalloc(&j.tiles, (size_t)std::max(1, j.ntiles)));
         if (j.ntiles) HIPCHK(hipMemcpy(j.tiles, th.data(), j.ntiles * sizeof(le::Tile), hipMemcpyHostToDevice));
@@ -605,7 +641,7 @@ int le_engine::setup(const le_observables* obs)
         for (auto& j : jobs)
             for (int q = 0; q < j.nw; ++q)
                 if (j.wid[q] == it.wid) C = j.C[q];
-        xi.push_back({C, 2 * basis.sector_offset[it.s1], basis.dim(it.s1), 2 * basis.sector_offset[it.s2],
+        xi.push_back({C, colb[it.s1], basis.dim(it.s1), colb[it.s2],
                       basis.dim(it.s2), it.off});
     }
     HIPCHK(dalloc(&items_d, std::max<size_t>(1, xi.size())));
@@ -1328,9 +1364,9 @@ int le_engine::gram_rows(long rows, int buf, hipStream_t s)
         g.wld = wrows;
         g.nweights = n_user;
         g.jobs = jobtab[buf];
-        g.r0 = 2 * basis.sector_offset[0];
+        g.r0 = colb[0];
         g.nr = 2 * basis.dim(0);
-        g.c0 = 2 * basis.sector_offset[1];
+        g.c0 = colb[1];
         g.nc = 2 * basis.dim(1);
         g.rows = rows;
         g.ldc = ldx;

@@ -1,5 +1,5 @@
 """The batched-sector configuration at full size: nu = 1/3, N = 200, dL = 1..16 accumulated
-together (D = 914, sector sizes 1..231, one grouped Gram over 103 tiles), 32768 walkers,
+together (D = 914, sector sizes 1..231, one grouped Gram over 97 tiles), 32768 walkers,
 seed 42, V = 0.025 r^2 + 5e-5 r^4.
 
 - Fixed samples: all 32768 walkers' configurations after burn-in go through the FP32, BF16
@@ -38,7 +38,7 @@ def _engine(**kw):
 def chain():
     print("case c2 N=200 dL=1..16 32768 walkers seed 42: tune, burn-in, sample", flush=True)
     e = _engine(lanes_per_walker=-1)
-    assert e.info.D == 914 and max(e.dims) == 231 and e.info.n_tiles == 103
+    assert e.info.D == 914 and max(e.dims) == 231 and e.info.n_tiles == 97
     e.init_walkers()
     e.tune_step()
     e.burnin(300)

@@ -885,8 +885,7 @@ __device__ __forceinline__ void psums_sample(const FeatureArgs& a, long w, int s
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int j = sub; j < a.N; j += kLps) {
-        double x, y;
+    auto load = [&](int j, double& x, double& y) {
         if (a.pos_is_double || sizeof(Real) == 8) {
             const double2 q = reinterpret_cast<const double2*>(a.pos)[(size_t)w * a.N + j];
             x = q.x; y = q.y;
@@ -894,20 +893,10 @@ __device__ __forceinline__ void psums_sample(const FeatureArgs& a, long w, int s
             const float2 q = reinterpret_cast<const float2*>(a.pos)[(size_t)w * a.N + j];
             x = q.x; y = q.y;
         }
-        const double r2 = x * x + y * y;
-        acc[QV] += potential_r2(a.pot, r2);
-        const double ur = x * a.invR, ui = y * a.invR;
-        double qr = ur, qi = ui;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            acc[2 * k] += qr;
-            acc[2 * k + 1] += qi;
-            const double t = qr * ur - qi * ui;
-            qi = qr * ui + qi * ur;
-            qr = t;
-        }
+    };
+    // e^{i L theta} = (z / |z|)^L by binary powering, added to the harmonic sums
+    auto harmonics = [&](double x, double y, double r2) {
         if constexpr (HARM) {
-            // e^{i L theta} = (z / |z|)^L by binary powering
             const double inv = r2 > 0.0 ? 1.0 / sqrt(r2) : 0.0;
             const double er = r2 > 0.0 ? x * inv : 1.0, ei = y * inv;
 #pragma unroll
@@ -931,6 +920,54 @@ __device__ __forceinline__ void psums_sample(const FeatureArgs& a, long w, int s
                 }
             }
         }
+    };
+    // Without harmonics, two particles j, j + kLps per pass: their power chains are
+    // independent, so the FP64 multiplies of one fill the latency of the other's.  Each sum
+    // still adds the particles' terms in the order j = sub, sub + kLps, ..., so the results are
+    // those of one particle per pass, bit for bit.
+    int j = sub;
+    for (; !HARM && j + kLps < a.N; j += 2 * kLps) {
+        double xa, ya, xb, yb;
+        load(j, xa, ya);
+        load(j + kLps, xb, yb);
+        const double r2a = xa * xa + ya * ya, r2b = xb * xb + yb * yb;
+        acc[QV] += potential_r2(a.pot, r2a);
+        acc[QV] += potential_r2(a.pot, r2b);
+        const double ura = xa * a.invR, uia = ya * a.invR;
+        const double urb = xb * a.invR, uib = yb * a.invR;
+        double qra = ura, qia = uia, qrb = urb, qib = uib;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            acc[2 * k] += qra;
+            acc[2 * k] += qrb;
+            acc[2 * k + 1] += qia;
+            acc[2 * k + 1] += qib;
+            const double ta = qra * ura - qia * uia;
+            qia = qra * uia + qia * ura;
+            qra = ta;
+            const double tb = qrb * urb - qib * uib;
+            qib = qrb * uib + qib * urb;
+            qrb = tb;
+        }
+        harmonics(xa, ya, r2a);
+        harmonics(xb, yb, r2b);
+    }
+    for (; j < a.N; j += kLps) {
+        double x, y;
+        load(j, x, y);
+        const double r2 = x * x + y * y;
+        acc[QV] += potential_r2(a.pot, r2);
+        const double ur = x * a.invR, ui = y * a.invR;
+        double qr = ur, qi = ui;
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            acc[2 * k] += qr;
+            acc[2 * k + 1] += qi;
+            const double t = qr * ur - qi * ui;
+            qi = qr * ui + qi * ur;
+            qr = t;
+        }
+        harmonics(x, y, r2);
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = group_sum<kLps, double>(acc[q]);

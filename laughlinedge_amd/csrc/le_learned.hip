@@ -378,6 +378,66 @@ __device__ __forceinline__ void near8_insert(float (&t)[8], float v)
     }
 }
 
+// Sorts 8 values ascending: Batcher's odd-even merge sort, 19 compare-exchanges (38 min/max,
+// against 128 for inserting them one by one).
+__device__ __forceinline__ void sort8(float (&v)[8])
+{
+    auto ce = [&](int i, int j) {
+        const float lo = fminf(v[i], v[j]);
+        v[j] = fmaxf(v[i], v[j]);
+        v[i] = lo;
+    };
+    ce(0, 1); ce(2, 3); ce(4, 5); ce(6, 7);
+    ce(0, 2); ce(1, 3); ce(4, 6); ce(5, 7);
+    ce(1, 2); ce(5, 6);
+    ce(0, 4); ce(1, 5); ce(2, 6); ce(3, 7);
+    ce(2, 4); ce(3, 5);
+    ce(1, 2); ce(3, 4); ce(5, 6);
+}
+
+// t = the 8 smallest of the sorted lists t and b, sorted: min(t_q, b_{7-q}) holds them as a
+// bitonic sequence, which three half-cleaners sort.
+__device__ __forceinline__ void near8_merge(float (&t)[8], const float (&b)[8])
+{
+    float c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = fminf(t[q], b[7 - q]);
+#pragma unroll
+    for (int d = 4; d >= 1; d >>= 1)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (!(q & d)) {
+                const float lo = fminf(c[q], c[q + d]), hi = fmaxf(c[q], c[q + d]);
+                c[q] = lo;
+                c[q + d] = hi;
+            }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t[q] = c[q];
+}
+
+// Slot u (of S) of a pass: value v joins the lane's sorted 8 smallest.  Slots come in batches
+// of 8, sorted and merged (70 min/max per batch; the first batch only clipped to the cap);
+// slots past the last whole batch are inserted.  The list is the sorted 8 smallest of the same
+// multiset as by insertion, so the features are the same, bit for bit.
+template <int u, int FULL8>
+__device__ __forceinline__ void near8_slot(float (&t)[8], float (&bt)[8], float v)
+{
+    if constexpr (u < FULL8) {
+        bt[u % 8] = v;
+        if constexpr (u % 8 == 7) {
+            sort8(bt);
+            if constexpr (u == 7) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) t[q] = fminf(bt[q], kDcap * kDcap);
+            } else {
+                near8_merge(t, bt);
+            }
+        }
+    } else {
+        near8_insert(t, v);
+    }
+}
+
 // One butterfly stage: with the partner lane's sorted list b, min(t_q, b_{7-q}) holds the 8
 // smallest of both lists as a bitonic sequence, which three half-cleaners sort.  Partner lanes
 // end with the same multiset, hence the same sorted list.
@@ -644,14 +704,14 @@ __global__ __launch_bounds__(64) void k_sweep_learned_reg(SweepArgs a, Table3 tb
                 const uint32_t ub = (uint32_t)__shfl((int)ubl, t, G);
                 const bool own = lane == t;
                 // pass 1: nearest distances at z
-                float near[8];
+                float near[8], bt[8];
                 near8_init(near);
                 static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
                     constexpr int u = decltype(uc)::value;
                     const float dx = zx - px[u], dy = zy - py[u];
                     bool skip = own && u == s;
                     if constexpr (u == S - 1) skip = skip || pad_last;
-                    near8_insert(near, skip ? kCap2 : dx * dx + dy * dy);
+                    near8_slot<u, FULL8>(near, bt, skip ? kCap2 : dx * dx + dy * dy);
                 });
                 const float r0 = sqrtf(zx * zx + zy * zy);
                 float f[3];
@@ -688,7 +748,7 @@ __global__ __launch_bounds__(64) void k_sweep_learned_reg(SweepArgs a, Table3 tb
                     const float A2 = ax * ax + ay * ay;
                     pn *= A2;
                     pd *= bx * bx + by * by;
-                    near8_insert(near, skip ? kCap2 : A2);
+                    near8_slot<u, FULL8>(near, bt, skip ? kCap2 : A2);
                     if constexpr ((u + 1) % 8 == 0 && u + 1 <= FULL8) {
                         lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
                         pn = 1.f;

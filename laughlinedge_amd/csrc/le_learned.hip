@@ -537,10 +537,24 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                 // pass 1: nearest distances at z (slot i and the pads hold a far point)
                 P[i] = far;
                 if (has_pad) P[pad_j] = far;
-                float near[8];
+                float near[8], bt[8];
                 near8_init(near);
                 {
+                    // whole batches of 8 slots sorted and merged, the rest inserted (the same
+                    // sorted list as by insertion)
                     int s = 0;
+                    for (; s + 8 <= S; s += 8) {
+                        float2 q[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) q[u] = P[lane + G * (s + u)];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const float dx = zo.x - q[u].x, dy = zo.y - q[u].y;
+                            bt[u] = dx * dx + dy * dy;
+                        }
+                        sort8(bt);
+                        near8_merge(near, bt);
+                    }
                     for (; s + 4 <= S; s += 4) {
                         float2 q[4];
 #pragma unroll
@@ -591,8 +605,10 @@ __global__ __launch_bounds__(64) void k_sweep_learned(SweepArgs a, Table3 tb, fl
                         const float A2 = ax * ax + ay * ay;
                         pn *= A2;
                         pd *= bx * bx + by * by;
-                        near8_insert(near, (s + u == skip_i || s + u == skip_p) ? kDcap * kDcap : A2);
+                        bt[u] = (s + u == skip_i || s + u == skip_p) ? kDcap * kDcap : A2;
                     }
+                    sort8(bt);
+                    near8_merge(near, bt);
                     lsum += __builtin_amdgcn_logf(fmaxf(pn, 1e-30f)) - __builtin_amdgcn_logf(fmaxf(pd, 1e-30f));
                     pn = 1.f;
                     pd = 1.f;

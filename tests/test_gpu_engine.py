@@ -65,6 +65,32 @@ def test_fixed_sample_large_tiles(oracle_lib, prec, tol):
         assert es < tol and eh < tol, (s, es, eh)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-6), (1, 1e-12)])
+def test_fixed_sample_mixed_layout(oracle_lib, prec, tol):
+    """The sample-row layout is chosen per sector set; N = 200, dL 0..8 (dims 1, 1, 2, 3, 5, 7,
+    11, 15, 22) takes the mixed one: the sectors of 16 real columns or more (dL 6, 7, 8) start
+    on multiples of 16, the small ones stay packed.  Its tiles compute 24 16x16 blocks against
+    28 packed and 30 with every sector aligned (all in 5 tiles).  S and H of 2048 configurations
+    match the FP64 oracle, and the raw records stay exactly Hermitian."""
+    from laughlinedge_amd import Engine
+    N, m, sectors = 200, 3, list(range(0, 9))
+    print(f"case mixed layout prec={prec} N={N} sectors=0..8", flush=True)
+    o = oracle_lib.Oracle(N, m, sectors, w=0.05, eps=5e-5)
+    rng = np.random.default_rng(17)
+    pos = _disc_positions(rng, 2048, N, o.R)
+    So, Ho, _ = o.accumulate_fixed(pos)
+    with Engine(N, m, sectors, w=0.05, eps=5e-5, walkers=256, samples=1, precision=prec) as e:
+        assert e.info.gram_mfma_flop_per_row == 24 * 16 * 16 * 2 * 2
+        e.accumulate_positions(pos)
+        mom = e.moments()
+    errs = [(_rel_fro(Sg, So[s] / 2048), _rel_fro(Hg, Ho[s] / 2048)) for s, (Sg, Hg) in enumerate(mom)]
+    print(f"prec={prec} max rel Frobenius error S {max(e[0] for e in errs):.3e} H {max(e[1] for e in errs):.3e}")
+    for s, (es, eh) in enumerate(errs):
+        assert es < tol and eh < tol, (s, es, eh)
+    for Sg, Hg in mom:
+        assert np.array_equal(Sg, Sg.conj().T) and np.array_equal(Hg, Hg.conj().T)
+
+
 def test_fp64_chain_matches_oracle(oracle_lib):
     """FP64 sampler: same Philox streams as the oracle chain -> same positions."""
     from laughlinedge_amd import Engine

@@ -111,8 +111,9 @@ typedef struct le_mc_params {
                                   4 / N per component) accepted with the |Psi_L|^2 and w ratios,
                                   then a uniform rotation (2 is refused) */
     int sweep_registers;       /* 1: keep each lane's particles in registers instead of LDS (edge
-                                  engine: lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25,
-                                  plain or learned proposal; occupation engine: ceil((N - 1) / lanes)
+                                  engine: lanes_per_walker 8 or 16 with ceil(N / lanes) in [2, 25],
+                                  13 or 25 for the mixture estimator and the learned proposal's
+                                  register kernel; occupation engine: ceil((N - 1) / lanes)
                                   13 or 25);
                                   0 (default): LDS.  lanes_per_walker = -1 times both kinds and
                                   keeps the faster */

@@ -832,8 +832,9 @@ extern "C" int le_engine_create(const le_system_params* sys, const le_mc_params*
     if (mc->gram_tile_cols == 32 && (mc->precision == LE_PREC_BF16X3 || mc->precision == LE_PREC_BF16X6))
         return fail(LE_ERR_ARG, "the BF16 split Gram runs 64-column tiles only");
     if (mc->sweep_registers == 1 && mc->lanes_per_walker > 0 &&
-        !le::sweep_reg_supported(mc->lanes_per_walker, sys->particles_number))
-        return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil(N / lanes) 13 or 25");
+        !le::sweep_reg_supported(mc->lanes_per_walker, sys->particles_number, mc->estimator == LE_EST_MIXTURE))
+        return fail(LE_ERR_ARG, "sweep_registers needs lanes_per_walker 8 or 16 with ceil(N / lanes) in [2, 25] "
+                                "(13 or 25 for the mixture estimator)");
     std::vector<int> dls(sectors, sectors + n_sectors);
     for (int dl : dls)
         if (dl < 0 || dl > 40) return fail(LE_ERR_ARG, "sector dL must be in [0, 40]");
@@ -1193,7 +1194,7 @@ int le_engine::autotune_lanes()
         cands.push_back(c);
     }
     for (int c : {8, 16})
-        if (two_sided ? le::sweep2_reg_supported(c, N) : le::sweep_reg_supported(c, N)) cands.push_back(c | kRegSweep);
+        if (two_sided ? le::sweep2_reg_supported(c, N) : le::sweep_reg_supported(c, N, mixture)) cands.push_back(c | kRegSweep);
     for (int cand : cands) {
         G = cand;
         HIPCHK(hipMemcpyAsync(scratch, pos, bytes, hipMemcpyDeviceToDevice, stream));

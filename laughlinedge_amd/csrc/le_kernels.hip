@@ -486,7 +486,10 @@ constexpr int reg_sweep_waves()
 
 // The sweeps of the 64 / G walkers of wavefront unit `unit` (walkers unit * 64 / G ..), lane
 // `tid` (0..63) of the wavefront: the body of k_sweep_reg, shared with the fused step kernel.
-template <int G, int S, typename Real, int MIXK = 0>   // MIXK: 0, or the largest mixture k held
+// EXACT: S = ceil(N / G).  Otherwise the walkers use the first SA = ceil(N / G) <= S slots
+// and the rest are skipped (wave-uniform tests): the moves, pair order, chunks and pads are
+// those of SA slots, so the chain is still the LDS kernel's.
+template <int G, int S, typename Real, int MIXK = 0, bool EXACT = true>   // MIXK: 0, or the largest mixture k held
 __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit, int tid)
 {
     constexpr bool MIX = MIXK > 0;
@@ -495,7 +498,9 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
     constexpr int WPB = 64 / G;
     constexpr int C = LogOps<Real>::chunk;
     constexpr int FULL = (S / C) * C;   // slots covered by whole chunks
-    const int N = a.N;                  // (S - 1) G < N <= S G, checked by the launcher
+    const int N = a.N;                  // (SA - 1) G < N <= SA G, SA <= S, checked by the launcher
+    const int SA = EXACT ? S : (N + G - 1) / G;
+    const int FULLA = EXACT ? FULL : (SA / C) * C;
     const int lane = tid % G;
     const int grp = tid / G;
     const long wl = unit * WPB + grp;
@@ -510,7 +515,7 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
         px[s] = q.x;
         py[s] = q.y;
     });
-    const bool pad_last = lane + G * (S - 1) >= N;   // the last slot holds no particle
+    const bool pad_last = lane + G * (SA - 1) >= N;   // the last slot used holds no particle
 
     const Real step = (Real)a.step;
     const Real mfac = (Real)a.m * LogOps<Real>::to_ln;
@@ -526,6 +531,7 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
         const uint32_t c2 = (uint32_t)((sidx >> 32) & 0xffffu);
         static_for(std::make_integer_sequence<int, S>{}, [&](auto sc) {
             constexpr int s = decltype(sc)::value;
+            if (!EXACT && s >= SA) return;
             const int base = G * s;
             // opaque counter: keeps the compiler from hoisting the sweep-invariant part of
             // Philox for every slot out of the sweep loop (S x 3 live registers)
@@ -558,12 +564,18 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
                 const Real dd = mad(dx, dx, dy * dy), tdx = dx + dx, tdy = dy + dy;
                 static_for(std::make_integer_sequence<int, S>{}, [&](auto uc) {
                     constexpr int u = decltype(uc)::value;
+                    if (!EXACT && u >= SA) return;
                     Real qx = px[u], qy = py[u];
-                    if constexpr (u == S - 1) {
-                        if (pad_last) {              // pads hold the midpoint
-                            qx = mid.x;
-                            qy = mid.y;
+                    if constexpr (EXACT) {
+                        if constexpr (u == S - 1) {
+                            if (pad_last) {          // pads hold the midpoint
+                                qx = mid.x;
+                                qy = mid.y;
+                            }
                         }
+                    } else if (u == SA - 1 && pad_last) {
+                        qx = mid.x;
+                        qy = mid.y;
                     }
                     const Real bx = zox - qx, by = zoy - qy;
                     const Real B = mad(bx, bx, by * by);
@@ -577,6 +589,7 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
                     pn *= A;
                     pd *= B;
                     if constexpr ((u + 1) % C == 0 && u + 1 <= FULL) {
+                        if (!EXACT && u + 1 > FULLA) return;
                         if constexpr (sizeof(Real) == 4) {
                             int en, ed;
                             pn = frexpf(pn, &en);
@@ -638,10 +651,10 @@ __device__ __forceinline__ void sweep_reg_walkers(const SweepArgs& a, long unit,
     if (lane == 0) a.accepted[walker] += accepted;
 }
 
-template <int G, int S, typename Real, int MIXK = 0>
+template <int G, int S, typename Real, int MIXK = 0, bool EXACT = true>
 __global__ __launch_bounds__(64) void k_sweep_reg(SweepArgs a)
 {
-    sweep_reg_walkers<G, S, Real, MIXK>(a, blockIdx.x, threadIdx.x);
+    sweep_reg_walkers<G, S, Real, MIXK, EXACT>(a, blockIdx.x, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2239,13 +2252,21 @@ static hipError_t sweep_dispatch(const SweepArgs& a, int G, hipStream_t s)
     }
 }
 
-// Register-resident sweeps: instantiated for G in {8, 16} and S = ceil(N / G) in {13, 25}
-// (N = 97..104, 193..200 at G = 8; N = 193..208, 385..400 at G = 16).
+// Register-resident sweeps: instantiated for G in {8, 16} and S in {13, 25} slots per lane.
+// S = ceil(N / G) exactly (N = 97..104, 193..200 at G = 8; N = 193..208, 385..400 at G = 16)
+// runs the exact kernels, for every estimator.  Other N with 2 <= ceil(N / G) <= 25 run the
+// S = 13 or S = 25 kernel with the unused slots skipped (plain and row-free chains only; the
+// mixture chains keep the exact fits).
 template <int G, int S, typename Real>
 static hipError_t sweep_reg_gs(const SweepArgs& a, hipStream_t s)
 {
     constexpr int WPB = 64 / G;
     dim3 grid((unsigned)((a.walkers + WPB - 1) / WPB));
+    if ((a.N + G - 1) / G != S) {
+        if (a.mix.kmix > 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_sweep_reg<G, S, Real, 0, false>), grid, dim3(64), 0, s, a);
+        return hipGetLastError();
+    }
     // mixture: hold 16 terms when they suffice (N = 200 at 8 lanes: 124 VGPRs and 4 waves per
     // SIMD, against 131 and 3 with room for 24)
     if (a.mix.kmix > 16)
@@ -2257,27 +2278,27 @@ static hipError_t sweep_reg_gs(const SweepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-bool sweep_reg_supported(int G, int N)
+bool sweep_reg_supported(int G, int N, bool mixture)
 {
     const int S = (N + G - 1) / G;
-    return (G == 8 || G == 16) && (S == 13 || S == 25);
+    if (G != 8 && G != 16) return false;
+    if (S == 13 || S == 25) return true;
+    return !mixture && S >= 2 && S <= 25;
 }
 
 template <typename Real>
 static hipError_t sweep_reg_dispatch(const SweepArgs& a, int G, hipStream_t s)
 {
     const int S = (a.N + G - 1) / G;
-    if (G == 8 && S == 13) return sweep_reg_gs<8, 13, Real>(a, s);
-    if (G == 8 && S == 25) return sweep_reg_gs<8, 25, Real>(a, s);
-    if (G == 16 && S == 13) return sweep_reg_gs<16, 13, Real>(a, s);
-    if (G == 16 && S == 25) return sweep_reg_gs<16, 25, Real>(a, s);
+    if (G == 8) return S <= 13 ? sweep_reg_gs<8, 13, Real>(a, s) : sweep_reg_gs<8, 25, Real>(a, s);
+    if (G == 16) return S <= 13 ? sweep_reg_gs<16, 13, Real>(a, s) : sweep_reg_gs<16, 25, Real>(a, s);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_sweep_reg(const SweepArgs& a, int prec, int G, hipStream_t s)
 {
     if (a.nsweeps <= 0 || a.walkers <= 0) return hipSuccess;
-    if (!sweep_reg_supported(G, a.N)) return hipErrorInvalidValue;
+    if (!sweep_reg_supported(G, a.N, a.mix.kmix > 0)) return hipErrorInvalidValue;
     if (a.mix.kmix > 0 && (a.mix.kmix > kMixMax || !a.mix_pk)) return hipErrorInvalidValue;
     return prec == kFP64 ? sweep_reg_dispatch<double>(a, G, s) : sweep_reg_dispatch<float>(a, G, s);
 }

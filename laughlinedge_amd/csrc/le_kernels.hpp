@@ -267,7 +267,9 @@ hipError_t launch_set_bases(unsigned long long* bases, unsigned long long sweep_
 
 size_t sweep_lds_bytes(int prec, int lanes_per_walker, int N);
 // register-resident sweeps (k_sweep_reg): the (G, ceil(N / G)) pairs that are instantiated
-bool sweep_reg_supported(int lanes_per_walker, int N);
+// register sweeps: G in {8, 16} with ceil(N / G) in {13, 25}, or (not for the mixture chains)
+// any ceil(N / G) in [2, 25]
+bool sweep_reg_supported(int lanes_per_walker, int N, bool mixture = false);
 hipError_t launch_sweep_reg(const SweepArgs& a, int prec, int lanes_per_walker, hipStream_t s);
 size_t sweep2_lds_bytes(int prec, int lanes_per_walker, int N);
 bool sweep_supported(int lanes_per_walker);

@@ -703,13 +703,15 @@ def test_config0_full_run_matches_oracle(oracle_lib):
         assert np.allclose(spec[s], or_geneig(So[s], Ho[s]), rtol=1e-9)
 
 
-@pytest.mark.parametrize("prec,N,G", [(0, 200, 8), (0, 200, 16), (0, 100, 8), (1, 200, 16)])
+@pytest.mark.parametrize("prec,N,G", [(0, 200, 8), (0, 200, 16), (0, 100, 8), (1, 200, 16),
+                                      (0, 150, 8), (0, 60, 8), (1, 150, 16)])
 def test_register_sweep_matches_lds(prec, N, G):
     """k_sweep_reg (each lane's particles in registers) reproduces k_sweep (particles in LDS)
     at the same lanes per walker: bit-identical positions and acceptance after burn-in and
     sampling, and the same moments (to 1e-12: the FP64 atomics of the Gram accumulation add
     workgroup results in a run-dependent order); 1003 walkers leave the last workgroup partly
-    empty."""
+    empty.  N = 150 at 8 lanes (19 slots) and N = 60 (8 slots) run the 25- and 13-slot kernels
+    with their unused slots skipped, N = 150 at 16 lanes (10 slots) the 13-slot FP64 kernel."""
     from laughlinedge_amd import Engine
     print(f"case register sweep vs LDS prec={prec} N={N} G={G}", flush=True)
     out = []

@@ -208,6 +208,9 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--no-rows", action="store_true", help="skip the timings of the other entry points")
+    ap.add_argument("--rows", default="all",
+                    help="comma list of rows to time: kicks (excitation response and its evolution), dsf, rowwise, "
+                         "mixture, fused, occupation; default all")
     ap.add_argument("--no-eager", action="store_true", help="skip the eager per-kernel split run")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed headline run (implies every --no-* flag): the command whose "
@@ -397,7 +400,7 @@ def main():
     if not args.no_forms:
         out["forms"] = forms(args, [int(x) for x in args.thins.split(",")])
     if not args.no_rows:
-        out["rows"] = rows()
+        out["rows"] = rows(set(args.rows.split(",")) if args.rows != "all" else None)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(value)
@@ -577,7 +580,7 @@ def _rows_kicks(helper_path):
     return out
 
 
-def rows():
+def rows(only=None):
     """Throughput of the other entry points at their working sizes, each a few seconds:
     - excitation response (em_edgeResponse.tpp:129-164): m = 5, N = 400, dL 0..20 (D = 2714,
       sectors up to 627 states), 65536 walkers; S, H and eight edge kicks
@@ -595,11 +598,29 @@ def rows():
     from laughlinedge_amd import Engine
     from laughlinedge_amd.engine import OccupationEngine
     out = {}
+    want = lambda name: only is None or name in only   # noqa: E731
     helper_path = os.path.join(ROOT, "tests", "cpp", "libuserweights.so")
-    if os.path.exists(helper_path):
+    if not want("kicks"):
+        pass
+    elif os.path.exists(helper_path):
         out.update(_rows_kicks(helper_path))
     else:
         out["excitation_response_m5_N400_dL0-20"] = {"skipped": f"{helper_path} is not built"}
+    if want("dsf"):
+        _row_dsf(out)
+    if want("rowwise"):
+        _row_rowwise(out)
+    if want("mixture"):
+        _row_mixture(out)
+    if want("fused"):
+        _row_fused(out)
+    if want("occupation"):
+        _row_occupation(out)
+    return out
+
+
+def _row_dsf(out):
+    from laughlinedge_amd import Engine
     # DSF at dL = 16
     log("rows: DSF dL=16 N=200")
     steps = 40
@@ -611,6 +632,10 @@ def rows():
         wall, ms_gram = _timed(e, 4, steps)
         out["dsf_N200_dL16"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
                                 "D": e.info.D, "gram_ms_per_step": ms_gram / steps}
+
+
+def _row_rowwise(out):
+    from laughlinedge_amd import Engine
     # the reference's row-wise estimator on the same workload
     log("rows: row-wise estimator N=200 dL=1..16")
     steps, W = 16, 36
@@ -633,6 +658,10 @@ def rows():
             "note": "one step = one sweep of every chain + its monomials + the per-row g/f sums; a full "
                     "S,H sample-update needs one sample of every one of the D rows"}
     out["rowwise_N200_dL1-16"]["lowest_eigenvalues"] = rowwise_spectra()
+
+
+def _row_mixture(out):
+    from laughlinedge_amd import Engine
     # the mixture estimator on the same workload (register sweep, FP32 tiles)
     log("rows: mixture estimator N=200 dL=1..16")
     steps = 200
@@ -648,6 +677,10 @@ def rows():
             "gram_ms_per_step": ms_gram / steps, "acceptance": e.stats()["acceptance"], "rho": MIX_RHO,
             "moves": "single-particle sweeps + centre-of-mass random walk and rotation per kept sample"}
     out["mixture_N200_dL1-16"]["lowest_eigenvalues"] = mixture_spectra()
+
+
+def _row_fused(out):
+    from laughlinedge_amd import Engine
     # the fused step kernel on the headline workload (one launch per kept sample)
     log("rows: fused step N=200 dL=1..16")
     steps = 200
@@ -669,6 +702,10 @@ def rows():
             "note": "one launch per kept sample: register sweeps, power sums and monomials of the sample and "
                     "the S/H Gram tiles of the previous one as work items (fused_step = 1); the headline "
                     "runs the same work as separate kernels on two streams"}
+
+
+def _row_occupation(out):
+    from laughlinedge_amd.engine import OccupationEngine
     # occupation numbers, dL = 0
     log("rows: occupation numbers dL=0 N=200")
     steps = 40
@@ -682,7 +719,6 @@ def rows():
         out["occupation_N200_dL0"] = {"sample_updates_per_s": WALKERS * steps / wall, "ms_per_step": 1e3 * wall / steps,
                                       "orbitals": lcut, "gram_ms_per_step": ms_gram / steps,
                                       "lanes_per_walker": e.info.lanes_per_walker, "acceptance": e.stats()["acceptance"]}
-    return out
 
 
 def mixture_spectra(seeds=(1, 2), steps=8192):

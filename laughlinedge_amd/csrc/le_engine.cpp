@@ -1565,6 +1565,9 @@ int le_engine::tune_gram_spread(int buf)
         for (int K : Ks) {
             gram_spread = K;
             gram_tc = TC;
+            // two timed launches per candidate; one when a launch takes over 10 ms (the
+            // excitation response's 2714-state Gram: ~140 ms each), where the launch-to-launch
+            // spread is small against the differences between candidates
             for (int rep = 0; rep < 2; ++rep) {
                 HIPCHK(hipEventRecord(t0, gstream));
                 int rc = gram_rows(rows, buf, gstream);
@@ -1578,6 +1581,7 @@ int le_engine::tune_gram_spread(int buf)
                     best = K;
                     best_tc = TC;
                 }
+                if (ms > 10.f) break;
             }
         }
     gram_spread = best;

@@ -6,8 +6,8 @@ Workload (N=200 Laughlin droplet, nu=1/3, all sectors dL=1..16 at once, D=914):
 A step is one Metropolis sweep of every walker followed by one kept sample per walker,
 accumulated into every sector's S and H (one "sample-update" per walker and step).
 
-The last stdout line is ONE compact JSON object (a few KB); the full result goes to
---full-out (default gpurun_out/bench_full.json).  value = S,H sample-updates/s of the
+The last stdout line is ONE compact JSON object (a few KB); the full result goes to the
+JSON file that --full-out names.  value = S,H sample-updates/s of the
 production form (FP32 sampler, FP32 MFMA accumulation, one sweep per kept sample), timed as
 one le_sample call of --steps kept samples per walker.  Alongside:
   roofline      the Gram accumulation kernel (MFMA bound), average launch time from HIP

@@ -100,23 +100,32 @@ __global__ void k_hermitian_part(double2* W, int D, int nu)
     Wu[(size_t)j * D + i] = {w.x, -w.y};
 }
 
+// A 16-byte load marked nontemporal: W is streamed once per stage and is larger than the
+// caches, so it should not evict y and k.
+__device__ __forceinline__ double2 load_stream(const double2* p)
+{
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+    return double2{v.x, v.y};
+}
+
 // k_u += -i F e^{i lam t} W_u y_u for Hermitian W_u, reading only its upper 64 x 64 tiles.
 // One workgroup per tile (I, J), I <= J, and perturbation u: it adds W_IJ y_J to the rows of
 // tile row I and, for I < J, W_IJ^H y_I to the rows of tile row J, so every off-diagonal
 // element is read once for two products (half the bytes of a dense product, which bounds
-// this kernel: nu D^2 complex values per stage).  The tile is staged in LDS with coalesced
-// row loads (16 in flight per thread); the row sums take lane = row (row stride 65 elements,
-// so 8 lanes' 16-byte reads hit distinct banks), the column sums lane = column, each over a
-// 16-element quarter per wave, and the quarters meet in LDS.  D = 2714, 8 perturbations:
-// 0.109 ms per stage against 0.172 ms for the dense product (one wavefront per row), and
-// 0.170 ms when each wave kept its 16 row sums in registers (206 VGPRs, 2 waves per SIMD,
-// load and reduction phases not overlapped).
+// this kernel: nu D^2 complex values per stage).  The tile passes through LDS in two 32-row
+// halves: both halves' coalesced row loads are issued at once (16 per thread), and the second
+// half stays in registers while the first is summed.  The row sums take 32 lanes per half,
+// each over 8 columns (row stride 65 elements, so 16 lanes' 16-byte reads hit distinct
+// banks); the column sums take lane = column, each wave over 8 rows of each half, kept in
+// registers across the halves; the partial sums meet in LDS.  39 KB of LDS per workgroup lets
+// four share a CU.
 __global__ __launch_bounds__(256) void k_stage_hemv(const double2* W, const double2* y, double2* kout, double F,
                                                     double t, const double* lam, int D, int nt)
 {
-    __shared__ double2 sW[64][65];
-    __shared__ double2 yI[64], yJ[64], phI[64], phJ[64];
-    __shared__ double2 part[4][64];
+    __shared__ double2 sW[32][65];
+    __shared__ double2 yI[64], yJ[64];
+    __shared__ double2 part[256];
     const int u = blockIdx.y;
     int I = 0, rest = (int)blockIdx.x;
     while (rest >= nt - I) {
@@ -126,81 +135,96 @@ __global__ __launch_bounds__(256) void k_stage_hemv(const double2* W, const doub
     const int J = I + rest;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double2* Wu = W + (size_t)u * D * D;
-    {
-        const int col = J * 64 + lane;
-        double2 v[16];
+    const int col = J * 64 + lane;
+    double2 v0[8], v1[8];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int row = I * 64 + 4 * i + wv;
-            v[i] = (row < D && col < D) ? Wu[(size_t)row * D + col] : double2{0.0, 0.0};
-        }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) sW[4 * i + wv][lane] = v[i];
-        // y of the tile's row and column blocks, -i F e^{i lam t} of their rows
-        const int j = ((wv & 1) ? J : I) * 64 + lane;
-        const bool ok = j < D;
-        if (wv < 2) {
-            (wv ? yJ : yI)[lane] = ok ? y[(size_t)u * D + j] : double2{0.0, 0.0};
-        } else {
-            double s = 0.0, c = 0.0;
-            if (ok) sincos(lam[j] * t, &s, &c);
-            (wv == 3 ? phJ : phI)[lane] = {F * s, -F * c};
-        }
+    for (int i = 0; i < 8; ++i) {
+        const int row = I * 64 + 4 * i + wv;
+        v0[i] = (row < D && col < D) ? load_stream(Wu + (size_t)row * D + col) : double2{0.0, 0.0};
     }
-    __syncthreads();
-    {
-        double sr = 0.0, si = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int c = 16 * wv + k;
-            const double2 w = sW[lane][c], x = yJ[c];
-            sr += w.x * x.x - w.y * x.y;
-            si += w.x * x.y + w.y * x.x;
-        }
-        part[wv][lane] = {sr, si};
+    for (int i = 0; i < 8; ++i) {
+        const int row = I * 64 + 32 + 4 * i + wv;
+        v1[i] = (row < D && col < D) ? load_stream(Wu + (size_t)row * D + col) : double2{0.0, 0.0};
     }
-    __syncthreads();
+    // -i F e^{i lam t} in wave 0's registers: of column `col` and of row I * 64 + lane (the
+    // row whose sum this lane adds: half lane / 32, row lane % 32 of it)
+    double2 phI = {0.0, 0.0}, phJ = {0.0, 0.0};
     if (wv == 0) {
-        const int row = I * 64 + lane;
-        if (row < D) {
-            double xr = 0.0, xi = 0.0;
+        const int ri = I * 64 + lane;
+        double s = 0.0, c = 0.0;
+        if (ri < D) sincos(lam[ri] * t, &s, &c);
+        phI = {F * s, -F * c};
+        s = 0.0;
+        c = 0.0;
+        if (col < D) sincos(lam[col] * t, &s, &c);
+        phJ = {F * s, -F * c};
+    } else if (wv < 3) {
+        // y of the tile's row and column blocks
+        const int j = (wv == 2 ? J : I) * 64 + lane;
+        (wv == 2 ? yJ : yI)[lane] = j < D ? y[(size_t)u * D + j] : double2{0.0, 0.0};
+    }
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                xr += part[w][lane].x;
-                xi += part[w][lane].y;
+    for (int i = 0; i < 8; ++i) sW[4 * i + wv][lane] = v0[i];
+    __syncthreads();
+    double cr = 0.0, ci = 0.0;   // column sum over this wave's 8 rows of each half
+    for (int hf = 0; hf < 2; ++hf) {
+        const int r0 = I * 64 + 32 * hf;
+        if (r0 >= D) break;   // workgroup-uniform; the previous pass ended at a barrier
+        {
+            const int r = tid & 31, c0 = 8 * (tid >> 5);
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double2 w = sW[r][c0 + k], x = yJ[c0 + k];
+                sr += w.x * x.x - w.y * x.y;
+                si += w.x * x.y + w.y * x.x;
             }
-            const double2 p = phI[lane];
-            atomicAdd(&kout[(size_t)u * D + row].x, p.x * xr - p.y * xi);
-            atomicAdd(&kout[(size_t)u * D + row].y, p.x * xi + p.y * xr);
+            part[tid] = {sr, si};
         }
+        if (I != J) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int r = 8 * wv + k;
+                const double2 w = sW[r][lane], x = yI[32 * hf + r];
+                cr += w.x * x.x + w.y * x.y;   // conj(w) y_r
+                ci += w.x * x.y - w.y * x.x;
+            }
+        }
+        __syncthreads();   // part[] complete; every read of this half in sW done
+        if (wv == 0 && (lane >> 5) == hf) {
+            const int row = r0 + (lane & 31);
+            if (row < D) {
+                double xr = 0.0, xi = 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    xr += part[32 * q + (lane & 31)].x;
+                    xi += part[32 * q + (lane & 31)].y;
+                }
+                const double2 p = phI;
+                atomicAdd(&kout[(size_t)u * D + row].x, p.x * xr - p.y * xi);
+                atomicAdd(&kout[(size_t)u * D + row].y, p.x * xi + p.y * xr);
+            }
+        }
+        if (hf == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sW[4 * i + wv][lane] = v1[i];
+        }
+        __syncthreads();   // the second half visible; part[] free again
     }
     if (I == J) return;   // workgroup-uniform
-    __syncthreads();      // part[] is reused
-    {
-        double sr = 0.0, si = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const int r = 16 * wv + k;
-            const double2 w = sW[r][lane], x = yI[r];
-            sr += w.x * x.x + w.y * x.y;   // conj(w) y_r
-            si += w.x * x.y - w.y * x.x;
-        }
-        part[wv][lane] = {sr, si};
-    }
+    part[tid] = {cr, ci};
     __syncthreads();
-    if (wv == 0) {
-        const int col = J * 64 + lane;
-        if (col < D) {
-            double xr = 0.0, xi = 0.0;
+    if (wv == 0 && col < D) {
+        double xr = 0.0, xi = 0.0;
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                xr += part[w][lane].x;
-                xi += part[w][lane].y;
-            }
-            const double2 p = phJ[lane];
-            atomicAdd(&kout[(size_t)u * D + col].x, p.x * xr - p.y * xi);
-            atomicAdd(&kout[(size_t)u * D + col].y, p.x * xi + p.y * xr);
+        for (int w = 0; w < 4; ++w) {
+            xr += part[64 * w + lane].x;
+            xi += part[64 * w + lane].y;
         }
+        const double2 p = phJ;
+        atomicAdd(&kout[(size_t)u * D + col].x, p.x * xr - p.y * xi);
+        atomicAdd(&kout[(size_t)u * D + col].y, p.x * xi + p.y * xr);
     }
 }
 

@@ -254,9 +254,10 @@ __global__ void k_rk_combine(const double2* b, const double2* ks, const double* 
 }
 
 // Observation at time t, in two passes.  k_phase: a_u[m] = e^{-i lam_m t} b_u[m] for every
-// perturbation u (one sincos per m) and the norms out[u][nrho] = a_u^H a_u.  k_observe: one
-// wavefront per row j of R_l reads that row once and forms (R_l a_u)_j for eight u at a
-// time; out[u][l] += conj(a_u[j]) (R_l a_u)_j.
+// perturbation u (one sincos per m) and the norms out[u][nrho] = a_u^H a_u.  k_observe:
+// wavefronts loop over (row j, chunk of `chunk` columns, a multiple of 64) tasks of R_l; each
+// task reads that part of the row once and forms its part of (R_l a_u)_j for eight u at a time,
+// times a_u[j]; a workgroup's sums meet in LDS and are added to out[u][l] atomically.
 __global__ __launch_bounds__(256) void k_phase(const double2* b, const double* lam, double t, int D, int nu,
                                                int nrho, double2* a, double2* out)
 {
@@ -278,49 +279,66 @@ __global__ __launch_bounds__(256) void k_phase(const double2* b, const double* l
 }
 
 __global__ __launch_bounds__(256) void k_observe(const double2* a, const double2* R, int D, int nu, int nrho,
-                                                 double2* out)
+                                                 int chunk, double2* out)
 {
     constexpr int U = 8;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (wave >= D * nrho) return;
-    const int j = wave % D, l = wave / D;
-    const double2* rr = R + ((size_t)l * D + j) * D;
-    for (int u0 = 0; u0 < nu; u0 += U) {
-        double sr[U], si[U];
+    __shared__ double red[4][2 * U];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nchunk = (D + chunk - 1) / chunk;
+    const long ntask = (long)D * nchunk;   // (row, chunk) pairs of one R_l
+    const long nwave = (long)gridDim.x * 4;
+    for (int l = 0; l < nrho; ++l) {
+        for (int u0 = 0; u0 < nu; u0 += U) {
+            double qr[U], qi[U];   // this wavefront's part of out[u][l], in every lane
 #pragma unroll
-        for (int k = 0; k < U; ++k) sr[k] = si[k] = 0.0;
-        for (int m = lane; m < D; m += 64) {
-            const double2 r = rr[m];
+            for (int k = 0; k < U; ++k) qr[k] = qi[k] = 0.0;
+            for (long task = (long)blockIdx.x * 4 + wv; task < ntask; task += nwave) {
+                const int part = (int)(task % nchunk), j = (int)(task / nchunk);
+                const int m1 = min(D, (part + 1) * chunk);
+                const double2* rr = R + ((size_t)l * D + j) * D;
+                double sr[U], si[U];
 #pragma unroll
-            for (int k = 0; k < U; ++k) {
-                if (u0 + k < nu) {
-                    const double2 am = a[(size_t)(u0 + k) * D + m];
-                    sr[k] += r.x * am.x - r.y * am.y;
-                    si[k] += r.x * am.y + r.y * am.x;
+                for (int k = 0; k < U; ++k) sr[k] = si[k] = 0.0;
+                for (int m = part * chunk + lane; m < m1; m += 64) {
+                    const double2 r = rr[m];
+#pragma unroll
+                    for (int k = 0; k < U; ++k) {
+                        if (u0 + k < nu) {
+                            const double2 am = a[(size_t)(u0 + k) * D + m];
+                            sr[k] += r.x * am.x - r.y * am.y;
+                            si[k] += r.x * am.y + r.y * am.x;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < U; ++k) {
+                    for (int off = 32; off > 0; off >>= 1) {
+                        sr[k] += __shfl_xor(sr[k], off, 64);
+                        si[k] += __shfl_xor(si[k], off, 64);
+                    }
+                    if (u0 + k < nu) {
+                        const double2 aj = a[(size_t)(u0 + k) * D + j];
+                        const z2 q = zcmul({aj.x, aj.y}, {sr[k], si[k]});
+                        qr[k] += q.r;
+                        qi[k] += q.i;
+                    }
                 }
             }
-        }
+            // the workgroup's four parts meet in LDS; one atomic per (u, Re/Im) and workgroup
+            if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < U; ++k) {
-            for (int off = 32; off > 0; off >>= 1) {
-                sr[k] += __shfl_xor(sr[k], off, 64);
-                si[k] += __shfl_xor(si[k], off, 64);
-            }
-        }
-        if (lane < U && u0 + lane < nu) {
-            double xr = 0, xi = 0;
-#pragma unroll
-            for (int k = 0; k < U; ++k)
-                if (k == lane) {
-                    xr = sr[k];
-                    xi = si[k];
+                for (int k = 0; k < U; ++k) {
+                    red[wv][2 * k] = qr[k];
+                    red[wv][2 * k + 1] = qi[k];
                 }
-            const int u = u0 + lane;
-            const double2 aj = a[(size_t)u * D + j];
-            const z2 q = zcmul({aj.x, aj.y}, {xr, xi});
-            atomicAdd(&out[(size_t)u * (nrho + 1) + l].x, q.r);
-            atomicAdd(&out[(size_t)u * (nrho + 1) + l].y, q.i);
+            }
+            __syncthreads();
+            if (threadIdx.x < 2 * U && u0 + (int)(threadIdx.x >> 1) < nu) {
+                const int k = threadIdx.x >> 1;
+                const double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+                atomicAdd(&out[(size_t)(u0 + k) * (nrho + 1) + l].x + (threadIdx.x & 1), v);
+            }
+            __syncthreads();   // red[] is reused
         }
     }
 }
@@ -377,9 +395,14 @@ hipError_t launch_observe(const double* b, const double* R, const double* lam, d
                        nrho, (double2*)a, (double2*)out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nrho == 0) return e;
-    const long waves = (long)D * nrho;
-    hipLaunchKernelGGL(k_observe, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (const double2*)a,
-                       (const double2*)R, D, nu, nrho, (double2*)out);
+    // (row, 640-column chunk) tasks of R_l spread over 1024 workgroups of four wavefronts (four
+    // per CU): enough wavefronts in flight to cover the row loads' latency, and one atomic add
+    // per sum and workgroup instead of one per sum and row (one per row, on the same 2 nu
+    // addresses, took 0.21 ms per call at D = 2714, nu = 8; this takes 0.10 ms, 512
+    // workgroups 0.13 ms)
+    const int chunk = 640;
+    hipLaunchKernelGGL(k_observe, dim3(1024u), dim3(256), 0, s, (const double2*)a,
+                       (const double2*)R, D, nu, nrho, chunk, (double2*)out);
     return hipGetLastError();
 }
 
